@@ -1,0 +1,211 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py's cpu_baseline leg.  The product library never uses it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int32)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.orc_lame.argtypes = [ctypes.c_int, _d, _d, _d, _d]
+        L.orc_prism_geometry.argtypes = [_d, _d, _d]
+        L.orc_inv3.argtypes = [_d, _d]
+        L.orc_elements.argtypes = [ctypes.c_int, _d, _d, _d, _d, _d, _d, _d, ctypes.c_double,
+                                   ctypes.c_int, _d, _d, _d, _d]
+        L.orc_assemble_dense.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d, _d, _d, _d,
+                                         _d, _u8, _i, ctypes.c_int, _d, _d, ctypes.c_double,
+                                         ctypes.c_int, _d, _d, _d]
+        L.orc_csr_threshold.argtypes = [_d, ctypes.c_int, _i, _i, _d, ctypes.c_int]
+        L.orc_csr_threshold.restype = ctypes.c_int
+        L.orc_dense_solve_transposed.argtypes = [_d, ctypes.c_int, _d, _d]
+        L.orc_dense_solve_transposed.restype = ctypes.c_int
+        L.orc_dnrm2.argtypes = [_d, ctypes.c_int]
+        L.orc_dnrm2.restype = ctypes.c_double
+        L.orc_visc_surfaces.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _i, _i, _u8, _d]
+        step_args = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d, _d, _d, _d, _d, _u8, _i,
+                     ctypes.c_int, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_nr_step_dense.argtypes = step_args + [ctypes.c_int, _d, _d, _i]
+        L.orc_nr_step_dense.restype = ctypes.c_int
+        L.orc_nr_step_sparse.argtypes = step_args + [ctypes.c_double, ctypes.c_int, _d, _d, _i, _i]
+        L.orc_nr_step_sparse.restype = ctypes.c_int
+        L.orc_nr_iteration_sparse.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d, _d, _d,
+                                              _d, _d, _u8, _d, _d, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, _d, _d, _d]
+        L.orc_nr_iteration_sparse.restype = ctypes.c_int
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+def lame(E, nu):
+    E = _f64(E); nu = _f64(np.broadcast_to(nu, E.shape))
+    lam = np.empty_like(E); mu = np.empty_like(E)
+    lib().orc_lame(E.size, _p(E, _d), _p(nu, _d), _p(lam, _d), _p(mu, _d))
+    return lam, mu
+
+
+def inv3(A):
+    A = _f64(A).reshape(9)
+    X = np.empty(9)
+    lib().orc_inv3(_p(A, _d), _p(X, _d))
+    return X.reshape(3, 3)
+
+
+def inv3_many(M):
+    M = _f64(M).reshape(-1, 9)
+    return np.stack([inv3(m).reshape(9) for m in M])
+
+
+def prism_geometry(ref18):
+    ref18 = _f64(ref18).reshape(18)
+    inv = np.empty(54); det = np.empty(6)
+    lib().orc_prism_geometry(_p(ref18, _d), _p(inv, _d), _p(det, _d))
+    return inv.reshape(6, 3, 3), det
+
+
+def elements(pos, refpos, invFg, lam, mu, eta, vel=None, dt=1.0, tangent_mode=1):
+    """Per-element forces (ge, gv: n_el x 18) and tangents (Ke, Kv: n_el x 18 x 18)."""
+    pos = _f64(pos).reshape(-1, 18); ne = pos.shape[0]
+    refpos = _f64(refpos).reshape(ne, 18); invFg = _f64(invFg).reshape(ne, 9)
+    lam = _f64(lam); mu = _f64(mu); eta = _f64(eta)
+    vel = None if vel is None else _f64(vel).reshape(ne, 18)
+    ge = np.empty((ne, 18)); gv = np.empty((ne, 18))
+    Ke = np.empty((ne, 18, 18)); Kv = np.empty((ne, 18, 18))
+    lib().orc_elements(ne, _p(pos, _d), _p(refpos, _d), _p(invFg, _d), _p(lam, _d), _p(mu, _d),
+                       _p(eta, _d), _p(vel, _d), float(dt), int(tangent_mode), _p(ge, _d),
+                       _p(gv, _d), _p(Ke, _d), _p(Kv, _d))
+    return ge, gv, Ke, Kv
+
+
+class Problem:
+    """Arrays of one tissue in the oracle's (and the C ABI's) layout."""
+
+    def __init__(self, conn, refpos, invFg, lam, mu, eta, ext_visc, visc_surf, fixed=None,
+                 pairs=None):
+        self.conn = _i32(conn).reshape(-1, 6)
+        self.n_el = self.conn.shape[0]
+        self.refpos = _f64(refpos).reshape(self.n_el, 18)
+        self.invFg = _f64(invFg).reshape(self.n_el, 9)
+        self.lam = _f64(lam); self.mu = _f64(mu); self.eta = _f64(eta)
+        self.ext_visc = _f64(ext_visc).reshape(-1, 3)
+        self.n_nodes = self.ext_visc.shape[0]
+        self.visc_surf = _f64(visc_surf)
+        self.fixed = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
+        self.pairs = (np.zeros((0, 2), np.int32) if pairs is None else _i32(pairs).reshape(-1, 2))
+
+    def _common(self):
+        return (self.n_nodes, self.n_el, _p(self.conn, _i), _p(self.refpos, _d),
+                _p(self.invFg, _d), _p(self.lam, _d), _p(self.mu, _d), _p(self.eta, _d),
+                _p(self.ext_visc, _d), _p(self.visc_surf, _d), _p(self.fixed, _u8),
+                _p(self.pairs, _i), self.pairs.shape[0])
+
+    def assemble_dense(self, uk, un, dt, tangent_mode=1):
+        n = 3 * self.n_nodes
+        uk = _f64(uk).reshape(n); un = _f64(un).reshape(n)
+        K = np.empty((n, n)); g = np.empty(n); gi = np.empty(n)
+        lib().orc_assemble_dense(*self._common(), _p(uk, _d), _p(un, _d), float(dt),
+                                 int(tangent_mode), _p(K, _d), _p(g, _d), _p(gi, _d))
+        return K, g, gi
+
+    def nr_step_dense(self, un, dt, tol=1e-8, maxit=20, tangent_mode=1):
+        n = 3 * self.n_nodes
+        un = _f64(un).reshape(n)
+        out = np.empty(n); du = np.zeros(maxit + 2); conv = np.zeros(1, np.int32)
+        it = lib().orc_nr_step_dense(*self._common(), _p(un, _d), float(dt), float(tol),
+                                     int(maxit), int(tangent_mode), _p(out, _d), _p(du, _d),
+                                     _p(conv, _i))
+        if it < 0:
+            raise RuntimeError("oracle dense solve failed (singular matrix)")
+        return out, du[:it], bool(conv[0])
+
+    def nr_step_sparse(self, un, dt, tol=1e-8, maxit=20, krylov_rtol=1e-12, krylov_maxit=20000):
+        n = 3 * self.n_nodes
+        un = _f64(un).reshape(n)
+        out = np.empty(n); du = np.zeros(maxit + 2); kit = np.zeros(maxit + 2, np.int32)
+        conv = np.zeros(1, np.int32)
+        it = lib().orc_nr_step_sparse(*self._common(), _p(un, _d), float(dt), float(tol),
+                                      int(maxit), float(krylov_rtol), int(krylov_maxit),
+                                      _p(out, _d), _p(du, _d), _p(kit, _i), _p(conv, _i))
+        if it < 0:
+            raise RuntimeError("oracle BiCGStab breakdown")
+        return out, du[:it], kit[:it], bool(conv[0])
+
+    def nr_iteration_sparse(self, uk, un, dt, krylov_rtol, krylov_maxit):
+        n = 3 * self.n_nodes
+        uk = _f64(uk).reshape(n); un = _f64(un).reshape(n)
+        du = np.empty(n); ta = np.zeros(1); ts = np.zeros(1)
+        k = lib().orc_nr_iteration_sparse(
+            self.n_nodes, self.n_el, _p(self.conn, _i), _p(self.refpos, _d), _p(self.invFg, _d),
+            _p(self.lam, _d), _p(self.mu, _d), _p(self.eta, _d), _p(self.ext_visc, _d),
+            _p(self.visc_surf, _d), _p(self.fixed, _u8), _p(uk, _d), _p(un, _d), float(dt),
+            float(krylov_rtol), int(krylov_maxit), _p(du, _d), _p(ta, _d), _p(ts, _d))
+        return du, k, float(ta[0]), float(ts[0])
+
+
+def csr_threshold(K):
+    K = _f64(K); n = K.shape[0]
+    nnz = lib().orc_csr_threshold(_p(K, _d), n, None, None, None, 0)
+    ia = np.empty(n + 1, np.int32); ja = np.empty(nnz, np.int32); a = np.empty(nnz)
+    lib().orc_csr_threshold(_p(K, _d), n, _p(ia, _i), _p(ja, _i), _p(a, _d), nnz)
+    return ia, ja, a
+
+
+def dense_solve_transposed(K, b):
+    K = _f64(K); b = _f64(b); x = np.empty_like(b)
+    if lib().orc_dense_solve_transposed(_p(K, _d), K.shape[0], _p(b, _d), _p(x, _d)) != 0:
+        raise RuntimeError("singular")
+    return x
+
+
+def dnrm2(x):
+    x = _f64(x)
+    return lib().orc_dnrm2(_p(x, _d), x.size)
+
+
+def visc_surfaces(n_nodes, conn, u, elem_placement, elem_type, spans):
+    conn = _i32(conn); u = _f64(u).reshape(-1)
+    ep = _i32(elem_placement); et = _i32(elem_type)
+    sp = np.ascontiguousarray(spans, dtype=np.uint8)
+    A = np.empty(n_nodes)
+    lib().orc_visc_surfaces(n_nodes, conn.shape[0], _p(conn, _i), _p(u, _d), _p(ep, _i),
+                            _p(et, _i), _p(sp, _u8), _p(A, _d))
+    return A

@@ -1,0 +1,181 @@
+"""Host-side tissue bookkeeping and synthetic prism meshes.
+
+Element placement and type follow the reference's rules so that the exposed
+surface areas (the node "viscosity surfaces" that scale external drag) are the
+ones the reference would compute:
+  ShapeBase::setTissuePlacement  TissueFolding/SourceCode/ShapeBase.cpp:905-980
+  ShapeBase::setTissueType       TissueFolding/SourceCode/ShapeBase.cpp:1001-1037
+Node placement codes: 0 basal, 1 apical, 2 mid-line, 3 lateral, 4 apical bulk.
+Node type codes: 0 columnar, 1 peripodial, 2 linker.
+
+The synthetic generators lay nodes out layer by layer (all nodes of the basal
+layer first), the numbering the reference's own meshes use (for example
+ToolBox/MeshGeneration/2DEllipse/smallRectangle.mesh), and order every prism
+as the reference expects after Prism::checkRotationConsistency3D
+(Prism.cpp:369-417): basal triangle 0,1,2 counter-clockwise seen from the
+apical side, apical triangle 3,4,5 above it.
+"""
+import numpy as np
+
+
+def element_placement(conn, node_placement):
+    """tissuePlacement and spansWholeTissue per element (ShapeBase.cpp:905-980)."""
+    npl = np.asarray(node_placement)[np.asarray(conn)]
+    has_ap = (npl == 1).any(1)
+    has_bulk = (npl == 4).any(1)
+    has_bas = (npl == 0).any(1)
+    has_lat = (npl == 3).any(1)
+    place = np.full(len(npl), 2, dtype=np.int32)
+    spans = np.zeros(len(npl), dtype=np.uint8)
+    place[~has_lat & ~has_ap & ~has_bas & has_bulk] = 4
+    place[~has_lat & ~has_ap & has_bas] = 0
+    place[~has_lat & has_ap & ~has_bas] = 1
+    both = ~has_lat & has_ap & has_bas
+    place[both] = 2
+    spans[both] = 1
+    place[has_lat] = 3
+    return place, spans
+
+
+def element_type(conn, node_type):
+    """tissueType per element (ShapeBase.cpp:1001-1037)."""
+    nt = np.asarray(node_type)[np.asarray(conn)]
+    t = np.zeros(len(nt), dtype=np.int32)
+    t[(nt == 1).any(1)] = 1
+    t[(nt == 2).any(1)] = 2
+    return t
+
+
+def _extrude(xy, tris, thickness, n_layers, boundary):
+    n2 = xy.shape[0]
+    z = np.linspace(0.0, thickness, n_layers + 1)
+    nodes = np.zeros(((n_layers + 1) * n2, 3))
+    for k in range(n_layers + 1):
+        nodes[k * n2:(k + 1) * n2, :2] = xy
+        nodes[k * n2:(k + 1) * n2, 2] = z[k]
+    placement = np.full((n_layers + 1) * n2, 2, dtype=np.int32)
+    placement[:n2] = 0
+    placement[n_layers * n2:] = 1
+    circ = np.tile(boundary.astype(np.int32), n_layers + 1)
+    conn = np.zeros((n_layers * len(tris), 6), dtype=np.int32)
+    for k in range(n_layers):
+        blk = conn[k * len(tris):(k + 1) * len(tris)]
+        blk[:, :3] = tris + k * n2
+        blk[:, 3:] = tris + (k + 1) * n2
+    layer = np.repeat(np.arange(n_layers), len(tris)).astype(np.int32)
+    refpos = nodes[conn].reshape(-1, 18).copy()
+    return dict(nodes=nodes, conn=conn, refpos=refpos, node_placement=placement,
+                node_type=np.zeros(len(nodes), dtype=np.int32), at_circumference=circ,
+                layer=layer, n_layers=n_layers, n2d=n2)
+
+
+def _ccw(xy, tris):
+    a, b, c = xy[tris[:, 0]], xy[tris[:, 1]], xy[tris[:, 2]]
+    cross = (b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0])
+    t = tris.copy()
+    flip = cross < 0
+    t[flip, 1], t[flip, 2] = tris[flip, 2], tris[flip, 1]
+    return t
+
+
+def rectangle(lx=10.0, ly=5.0, side=1.0, thickness=2.0, n_layers=2):
+    """Regular right-triangle grid on an lx x ly rectangle, extruded in z."""
+    nx = int(round(lx / side)); ny = int(round(ly / side))
+    xs = np.linspace(0, lx, nx + 1); ys = np.linspace(0, ly, ny + 1)
+    X, Y = np.meshgrid(xs, ys, indexing="xy")
+    xy = np.stack([X.ravel(), Y.ravel()], 1)
+    idx = np.arange((nx + 1) * (ny + 1)).reshape(ny + 1, nx + 1)
+    tris = []
+    for j in range(ny):
+        for i in range(nx):
+            p0, p1, p2, p3 = idx[j, i], idx[j, i + 1], idx[j + 1, i], idx[j + 1, i + 1]
+            tris.append([p0, p1, p3])
+            tris.append([p0, p3, p2])
+    tris = _ccw(xy, np.array(tris, dtype=np.int32))
+    boundary = (np.isclose(xy[:, 0], 0) | np.isclose(xy[:, 0], lx) |
+                np.isclose(xy[:, 1], 0) | np.isclose(xy[:, 1], ly))
+    m = _extrude(xy, tris, thickness, n_layers, boundary)
+    m["kind"] = "rectangle"
+    return m
+
+
+def ellipse(a=100.0, b=70.0, side=2.0, thickness=20.0, n_layers=4):
+    """Equilateral lattice clipped to an ellipse (x/a)^2 + (y/b)^2 <= 1, extruded in z.
+
+    Rows of points at y = j h, h = side*sqrt(3)/2, odd rows shifted by side/2.
+    Every lattice triangle whose three corners lie inside the ellipse is kept;
+    unused points are dropped.  Deterministic for given sizes.
+    """
+    h = side * np.sqrt(3.0) / 2.0
+    nj = int(np.floor(b / h))
+    rows = []
+    for j in range(-nj, nj + 1):
+        y = j * h
+        off = 0.5 * side if (j % 2) else 0.0
+        ni = int(np.floor(a / side)) + 1
+        xs = off + side * np.arange(-ni, ni + 1)
+        rows.append((j, y, xs))
+    pid = {}
+    pts = []
+    for j, y, xs in rows:
+        for i, x in enumerate(xs):
+            if (x / a) ** 2 + (y / b) ** 2 <= 1.0:
+                pid[(j, i)] = len(pts)
+                pts.append((x, y))
+    xy = np.array(pts)
+    ni_of = {j: len(xs) for j, _, xs in rows}
+    tris = []
+    for j, y, xs in rows:
+        if j == nj:
+            continue
+        odd = (j % 2) != 0
+        for i in range(len(xs)):
+            # next row index whose x is +side/2 relative to (j, i)
+            ir = i if not odd else i + 1
+            il = ir - 1
+            cand = [((j, i), (j, i + 1), (j + 1, ir)), ((j, i), (j + 1, ir), (j + 1, il))]
+            for t in cand:
+                if all(k in pid for k in t) and all(0 <= k[1] < ni_of[k[0]] for k in t):
+                    tris.append([pid[k] for k in t])
+    tris = np.array(tris, dtype=np.int32)
+    used = np.unique(tris)
+    remap = -np.ones(len(xy), dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    xy = xy[used]
+    tris = remap[tris].astype(np.int32)
+    tris = _ccw(xy, tris)
+    # boundary points: edges used by one triangle only
+    e = np.sort(np.concatenate([tris[:, [0, 1]], tris[:, [1, 2]], tris[:, [2, 0]]]), 1)
+    key = e[:, 0].astype(np.int64) * len(xy) + e[:, 1]
+    uniq, cnt = np.unique(key, return_counts=True)
+    bnd_edges = uniq[cnt == 1]
+    boundary = np.zeros(len(xy), dtype=bool)
+    boundary[(bnd_edges // len(xy)).astype(np.int64)] = True
+    boundary[(bnd_edges % len(xy)).astype(np.int64)] = True
+    m = _extrude(xy, tris, thickness, n_layers, boundary)
+    m["kind"] = "ellipse"
+    m["axes"] = (a, b)
+    return m
+
+
+def gaussian_random_field(points, mean, sd, corr_len, seed, n_modes=256):
+    """Stationary Gaussian random field with a squared-exponential covariance
+    sd^2 exp(-r^2 / (2 corr_len^2)), sampled by random Fourier features."""
+    rng = np.random.default_rng(seed)
+    d = points.shape[1]
+    k = rng.normal(0.0, 1.0 / corr_len, size=(n_modes, d))
+    phase = rng.uniform(0, 2 * np.pi, size=n_modes)
+    f = np.cos(points @ k.T + phase) @ np.ones(n_modes)
+    return mean + sd * np.sqrt(2.0 / n_modes) * f
+
+
+def growth_increment(rate_per_hour_xy, dt):
+    """In-plane isotropic growth increment diag(exp(r dt), exp(r dt), 1) with r
+    in 1/s, as ShapeBase::calculateFgFromRates (ShapeBase.cpp:471-505) builds it."""
+    r = np.asarray(rate_per_hour_xy, dtype=np.float64) / 3600.0
+    g = np.exp(r * dt)
+    inc = np.zeros((len(g), 3, 3))
+    inc[:, 0, 0] = g
+    inc[:, 1, 1] = g
+    inc[:, 2, 2] = 1.0
+    return inc
