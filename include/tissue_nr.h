@@ -1,0 +1,191 @@
+/*
+ * tissue_nr.h -- C ABI of the MI355X implicit Newton-Raphson step on 6-node
+ * prism meshes (libtissue_nr.so).
+ *
+ * The library replaces, for one tissue at a time, the hot path of
+ * TissueOrigami's Simulation::updateStepNR (TissueFolding/SourceCode/
+ * Simulation.cpp:6605-6746): the element loop of
+ * NewtonRaphsonSolver::calculateForcesAndJacobianMatrixNR (NewtonRaphsonSolver.cpp:221-241)
+ * over ShapeBase/Prism (ShapeBase.cpp:2356-3285, Prism.cpp:766-886), the
+ * scatter into the system Jacobian (NewtonRaphsonSolver.cpp:342-360), the
+ * boundary conditions (NewtonRaphsonSolver.cpp:128-219, 362-457) and the
+ * Pardiso solve of solveForDeltaU (NewtonRaphsonSolver.cpp:461-763).
+ *
+ * Conventions (all arrays are caller-owned host memory, copied in or out
+ * during the call; no pointer is retained):
+ *   - DOF index = 3 * node + dim (NewtonRaphsonSolver.cpp:107-110);
+ *     positions and forces are 3 * n_nodes doubles, node-major.
+ *   - conn: n_prisms x 6 int32 node ids in the reference's prism order
+ *     (basal triangle 0,1,2 then apical 3,4,5, counter-clockwise seen from
+ *     the apical side, as left by Prism::checkRotationConsistency3D,
+ *     Prism.cpp:369-417).
+ *   - ref_pos: n_prisms x 18 reference (stress-free) node positions,
+ *     ReferenceShapeBase::Positions.
+ *   - Matrices are row-major.
+ * Every entry point returns a tf_status; no entry point calls exit().
+ * A handle is used by one host thread at a time and owns one HIP stream;
+ * distinct handles may run concurrently.
+ */
+#ifndef TISSUE_NR_H
+#define TISSUE_NR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum tf_status {
+  TF_OK = 0,
+  TF_ERR_HIP = 1,            /* a HIP runtime call failed */
+  TF_ERR_NAN = 2,            /* NaN/Inf in the residual or the increment */
+  TF_ERR_KRYLOV = 3,         /* BiCGStab breakdown or iteration cap */
+  TF_ERR_NOT_CONVERGED = 4,  /* NR hit maxit: result still written, as the
+                                reference accepts the iterate
+                                (Simulation.cpp:6724-6727) */
+  TF_ERR_ARG = 5,            /* invalid argument or call order */
+  TF_ERR_NO_DEVICE = 6       /* no usable gfx950 device */
+} tf_status;
+
+typedef enum tf_precision {
+  TF_PREC_FP64 = 0,   /* all-FP64 step */
+  TF_PREC_MIXED = 1   /* FP32 tangent blocks, FP32 SpMV and block-Jacobi,
+                         FP64 residual, det F / log J and FP64 refinement */
+} tf_precision;
+
+#define TF_MAX_NR_ITERS 32
+
+typedef struct tf_step_stats {
+  int32_t status;                          /* tf_status of the step */
+  int32_t nr_iterations;                   /* NR iterations run */
+  int32_t converged;                       /* 1 if |du|_2 <= tol was reached */
+  int32_t krylov_iterations_total;
+  int32_t krylov_iterations[TF_MAX_NR_ITERS];
+  double du_norms[TF_MAX_NR_ITERS];        /* |du|_2 per NR iteration
+                                              (NewtonRaphsonSolver.cpp:815) */
+  double ms_element;                       /* per-prism kernel */
+  double ms_assembly;                      /* slot-map reduction + BCs */
+  double ms_krylov;                        /* linear solves */
+  double ms_total;                         /* whole step, device time */
+} tf_step_stats;
+
+typedef struct tf_handle tf_handle;
+
+/* Creates a solver for one mesh: uploads connectivity and reference shape,
+ * builds the block pattern of the Jacobian and the element -> block slot map.
+ * Replaces NewtonRaphsonSolver::NewtonRaphsonSolver (NewtonRaphsonSolver.cpp:16-41)
+ * and the per-element geometry set-up of Prism::Prism (Prism.cpp:9-251). */
+tf_status tf_nr_create(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,
+                       const double* ref_pos, tf_handle** out);
+
+/* Per-prism Young's modulus E, Poisson ratio nu and internal viscosity eta;
+ * per-node external viscosity (n_nodes x 3); fixed DOF mask (n_nodes x 3,
+ * Node::FixedPos, may be NULL); slave/master DOF pairs (n_pairs x 2,
+ * NewtonRaphsonSolver::slaveMasterList, may be NULL).  Lame parameters follow
+ * Prism::setElasticProperties (Prism.cpp:464-515).  Replaces the physical
+ * property state read by ShapeBase (ShapeBase.h:129-143) and Node (Node.h:21-58). */
+tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu,
+                             const double* eta, const double* ext_visc,
+                             const uint8_t* fixed, const int32_t* slave_master,
+                             int32_t n_pairs);
+
+/* Inverse growth deformation gradient per prism (n_prisms x 9, row-major),
+ * ShapeBase::InvFg as left by ShapeBase::growShapeByFg (ShapeBase.cpp:1452-1505). */
+tf_status tf_nr_set_growth(tf_handle* h, const double* inv_Fg);
+
+/* Exposed surface area per node (Node::viscositySurface), computed at the
+ * start of the step by Simulation::updateNodeViscositySurfaces
+ * (Simulation.cpp:7598-7621). */
+tf_status tf_nr_set_visc_surface(tf_handle* h, const double* area);
+
+/* Computes the node viscosity surfaces on the device from positions u
+ * (3 x n_nodes) and per-prism tissue placement / type / spans-whole-tissue
+ * flags (ShapeBase.cpp:905-1037), then stores them as tf_nr_set_visc_surface
+ * would.  Replaces Simulation::updateNodeViscositySurfaces with
+ * Prism::calculateApicalArea/BasalArea (Prism.cpp:1485-1556). */
+tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u,
+                                    const int32_t* elem_placement,
+                                    const int32_t* elem_type, const uint8_t* spans,
+                                    double* area_out);
+
+/* One implicit time step: NR iterations from u_n until |du|_2 <= tol or
+ * maxit (the reference uses tol 1e-8 and maxit 20,
+ * NewtonRaphsonSolver.cpp:22, Simulation.cpp:6615).  Replaces
+ * Simulation::updateStepNR.  krylov_rtol is the relative residual of each
+ * linear solve (0 selects 1e-10).  u_out receives the positions. */
+tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int32_t maxit,
+                     int32_t precision_mode, double krylov_rtol, double* u_out,
+                     tf_step_stats* stats);
+
+/* One assembly at positions uk (previous-step positions un, time step dt):
+ * the per-prism kernel, the slot-map reduction and the boundary conditions.
+ * g_out (3 x n_nodes, may be NULL) receives gSum after fixing and binding
+ * (NewtonRaphsonSolver::gSum); internal_out (may be NULL) receives the
+ * internal nodal forces of non-fixed DOFs (the SystemForces of Save_Force,
+ * ShapeBase::writeInternalForcesTogeAndgv, ShapeBase.cpp:2362-2387).
+ * Replaces calculateForcesAndJacobianMatrixNR + writeForcesTogeAndgvInternal
+ * + writeImplicitElementalKToJacobian + calculateExternalViscousForcesForNR +
+ * addImplicitKViscousExternalToJacobian + calculateSumOfInternalForces +
+ * calcutateFixedK (NewtonRaphsonSolver.cpp:195-457). */
+tf_status tf_nr_assemble(tf_handle* h, const double* uk, const double* un, double dt,
+                         double* g_out, double* internal_out);
+
+/* Solves J x = b with the Jacobian of the last tf_nr_assemble (J = K^T, the
+ * transposed solve of solveWithPardiso, NewtonRaphsonSolver.cpp:716-723;
+ * binding handled as N^T J N + I_slave, NewtonRaphsonSolver.cpp:128-193).
+ * Returns TF_ERR_KRYLOV if rtol was not reached within maxit iterations. */
+tf_status tf_nr_solve(tf_handle* h, const double* b, double* x, double rtol, int32_t maxit,
+                      int32_t* iterations);
+
+/* Structural CSR pattern of K (scalar rows, ascending columns, 0-based), the
+ * superset of constructiaForPardiso's thresholded pattern
+ * (NewtonRaphsonSolver.cpp:766-787).  ia has 3*n_nodes+1 entries; pass
+ * ja == NULL to query *nnz first. */
+tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz);
+
+/* The assembled K of the last tf_nr_assemble in the reference orientation,
+ * as constructiaForPardiso builds it: entries with |K_ij| > threshold (the
+ * reference uses 1e-13), row-major, ascending columns, 0-based.  Binding
+ * pairs are applied as calculateBoundKWithSlavesMasterDoF does.  Pass
+ * ja == NULL to query *nnz first. */
+tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja, double* a,
+                        int64_t* nnz);
+
+/* Per-prism outputs of the last tf_nr_assemble, for element-level parity:
+ * elastic and internal viscous element forces (n_prisms x 18,
+ * ElementalElasticSystemForces / ElementalInternalViscousSystemForces before
+ * fixed-DOF masking) and the 18 x 18 tangents TriPointKe / TriPointKv in the
+ * reference's storage orientation (n_prisms x 324).  Any output may be NULL. */
+tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke, double* Kv);
+
+/* Device-time accounting of the hot kernels, for measurement.  The reference
+ * has no timers (no counterpart); these exist for the roofline figures.
+ * tf_nr_profile(h, 1) resets the counters and brackets every SpMV launch
+ * with HIP events on the handle's stream; tf_nr_kernel_times reads them. */
+typedef struct tf_kernel_times {
+  double spmv_ms[2];          /* summed SpMV durations: [0] FP64, [1] FP32 */
+  int64_t spmv_launches[2];
+  double element_ms;          /* per-prism kernel, summed over assemblies */
+  double assembly_ms;         /* slot-map reduction + residual, summed */
+  int64_t assemblies;
+} tf_kernel_times;
+
+tf_status tf_nr_profile(tf_handle* h, int32_t enable);
+tf_status tf_nr_kernel_times(tf_handle* h, tf_kernel_times* out);
+
+/* Size of the Jacobian pattern: stored 3x3 blocks (n_blocks, the structural
+ * nonzeros / 9) and padded SELL-64 slots.  No reference counterpart. */
+tf_status tf_nr_pattern_info(tf_handle* h, int64_t* n_blocks, int64_t* n_slots);
+
+/* Releases every device buffer of the handle (NewtonRaphsonSolver::~NewtonRaphsonSolver,
+ * NewtonRaphsonSolver.cpp:43-71). */
+tf_status tf_nr_destroy(tf_handle* h);
+
+/* Human-readable text of the last error on this thread. */
+const char* tf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TISSUE_NR_H */

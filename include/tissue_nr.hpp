@@ -1,0 +1,221 @@
+// tissue_nr.hpp -- C++ drop-in surface over the C ABI of tissue_nr.h.
+//
+// Keeps the names of the reference's NR path so that a caller written against
+// TissueOrigami's Node / ShapeBase / Prism / NewtonRaphsonSolver
+// (TissueFolding/SourceCode/Node.h:15-87, ShapeBase.h:301-324, Prism.h:15-67,
+// NewtonRaphsonSolver.h:22-104) can drive the device step with the same loop
+// shape as Simulation::updateStepNR (Simulation.cpp:6605-6746):
+//
+//   solver.setMatricesToZeroAtTheBeginningOfIteration(false);
+//   solver.constructUnMatrix(Nodes);
+//   solver.initialteUkMatrix();
+//   while (!converged) {
+//     solver.calculateDisplacementMatrix(dt);
+//     solver.calculateForcesAndJacobianMatrixNR(Nodes, Elements, dt);   // + scatter, BCs
+//     solver.solveForDeltaU();
+//     converged = solver.checkConvergenceViaDeltaU();
+//     solver.updateUkInIteration();
+//     ...
+//   }
+//
+// or in one call: solver.updateStepNR(Nodes, Elements, dt).  The Jacobian never
+// leaves the device; un, uk, deltaU and gSum are host vectors as the
+// reference's gsl matrices were.  Header-only; link with -ltissue_nr.
+#ifndef TISSUE_NR_HPP
+#define TISSUE_NR_HPP
+
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tissue_nr.h"
+
+namespace tissue_nr {
+
+inline void check(tf_status s) {
+  if (s != TF_OK) throw std::runtime_error(std::string("tissue_nr: ") + tf_last_error());
+}
+
+// Node state read by the NR step (Node.h:21-58).
+struct Node {
+  int Id = 0;
+  std::array<double, 3> Position{};
+  std::array<bool, 3> FixedPos{};
+  std::array<double, 3> externalViscosity{};
+  double viscositySurface = 0.0;
+  int tissuePlacement = 2;
+  int tissueType = 0;
+};
+
+// Element state read by the NR step (ShapeBase.h:53-69, 129-143).
+class ShapeBase {
+ public:
+  virtual ~ShapeBase() = default;
+  std::array<int, 6> NodeIds{};
+  double ReferencePositions[6][3] = {};
+  double E = 10.0, v = 0.3, internalViscosity = 0.0;
+  double InvFg[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  bool IsAblated = false;
+  int getNodeNumber() const { return 6; }
+  const int* getNodeIds() const { return NodeIds.data(); }
+  // ShapeBase::updatePositions (ShapeBase.cpp:3279-3285): element positions
+  // follow the nodes after every NR update.
+  void updatePositions(const std::vector<std::unique_ptr<Node>>& Nodes) {
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 3; ++j) Positions[i][j] = Nodes[NodeIds[i]]->Position[j];
+  }
+  double Positions[6][3] = {};
+};
+
+// 6-node wedge (Prism.h:15-67).
+class Prism : public ShapeBase {
+ public:
+  Prism(const int* ids, const std::vector<std::unique_ptr<Node>>& Nodes) {
+    for (int i = 0; i < 6; ++i) NodeIds[i] = ids[i];
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 3; ++j) ReferencePositions[i][j] = Nodes[ids[i]]->Position[j];
+    updatePositions(Nodes);
+  }
+};
+
+class NewtonRaphsonSolver {
+ public:
+  // NewtonRaphsonSolver(int nDim, int nNodes) (NewtonRaphsonSolver.cpp:16-41);
+  // the mesh is bound at the first call that sees the elements.
+  NewtonRaphsonSolver(int nDim, int nNodes) : nDim(nDim), nNodes(nNodes) {
+    if (nDim != 3) throw std::invalid_argument("only 3D prism meshes");
+    un.assign(3 * (size_t)nNodes, 0.0);
+    uk = un;
+    deltaU = un;
+    gSum = un;
+    displacementPerDt = un;
+  }
+  ~NewtonRaphsonSolver() { tf_nr_destroy(h); }
+  NewtonRaphsonSolver(const NewtonRaphsonSolver&) = delete;
+  NewtonRaphsonSolver& operator=(const NewtonRaphsonSolver&) = delete;
+
+  double threshold = 1E-8;                    // NewtonRaphsonSolver.cpp:22
+  std::vector<double> un, uk, deltaU, gSum, displacementPerDt;
+  std::vector<std::vector<int>> slaveMasterList;
+  bool boundNodesWithSlaveMasterDefinition = false;
+  int precision = TF_PREC_FP64;
+  double krylovTolerance = 1e-10;
+  int lastKrylovIterations = 0;
+
+  void setMatricesToZeroAtTheBeginningOfIteration(bool) {
+    std::fill(gSum.begin(), gSum.end(), 0.0);
+    std::fill(deltaU.begin(), deltaU.end(), 0.0);
+  }
+  void constructUnMatrix(const std::vector<std::unique_ptr<Node>>& Nodes) {
+    for (int i = 0; i < nNodes; ++i)
+      for (int j = 0; j < 3; ++j) un[3 * i + j] = Nodes[i]->Position[j];
+  }
+  void initialteUkMatrix() { uk = un; }
+  void calculateDisplacementMatrix(double dt) {
+    for (size_t i = 0; i < uk.size(); ++i) displacementPerDt[i] = (uk[i] - un[i]) * (1.0 / dt);
+  }
+  // Element loop, scatter, external viscosity, fixed DOF and binding: the
+  // device counterpart of calculateForcesAndJacobianMatrixNR +
+  // writeForcesTogeAndgvInternal + writeImplicitElementalKToJacobian +
+  // calculateExternalViscousForcesForNR + addImplicitKViscousExternalToJacobian +
+  // calculateSumOfInternalForces + calcutateFixedK + calculateBoundKWithSlavesMasterDoF.
+  void calculateForcesAndJacobianMatrixNR(const std::vector<std::unique_ptr<Node>>& Nodes,
+                                          const std::vector<std::unique_ptr<ShapeBase>>& Elements,
+                                          double dt) {
+    bind(Nodes, Elements);
+    check(tf_nr_assemble(h, uk.data(), un.data(), dt, gSum.data(), nullptr));
+  }
+  // solveForDeltaU (NewtonRaphsonSolver.cpp:461-488) with the device Krylov
+  // solver in place of Pardiso; slaves follow their masters.
+  void solveForDeltaU() {
+    int32_t it = 0;
+    check(tf_nr_solve(h, gSum.data(), deltaU.data(), krylovTolerance, 50000, &it));
+    lastKrylovIterations = it;
+    for (auto& p : slaveMasterList) deltaU[p[0]] = deltaU[p[1]];
+  }
+  bool checkConvergenceViaDeltaU() {
+    double s = 0.0;
+    for (double d : deltaU) s += d * d;
+    return !(std::sqrt(s) > threshold);
+  }
+  void updateUkInIteration() {
+    for (size_t i = 0; i < uk.size(); ++i) uk[i] = uk[i] + deltaU[i];
+  }
+  void updateNodePositionsNR(const std::vector<std::unique_ptr<Node>>& Nodes) const {
+    for (int i = 0; i < nNodes; ++i)
+      for (int j = 0; j < 3; ++j) Nodes[i]->Position[j] = uk[3 * i + j];
+  }
+
+  // The whole of Simulation::updateStepNR in one device call.  Returns the
+  // step statistics; node and element positions are updated in place.
+  tf_step_stats updateStepNR(const std::vector<std::unique_ptr<Node>>& Nodes,
+                             const std::vector<std::unique_ptr<ShapeBase>>& Elements, double dt,
+                             int maxIteration = 20) {
+    bind(Nodes, Elements);
+    constructUnMatrix(Nodes);
+    tf_step_stats st;
+    const tf_status s = tf_nr_step(h, un.data(), dt, threshold, maxIteration, precision,
+                                   krylovTolerance, uk.data(), &st);
+    if (s != TF_OK && s != TF_ERR_NOT_CONVERGED) check(s);
+    updateNodePositionsNR(Nodes);
+    for (auto& e : Elements) e->updatePositions(Nodes);
+    return st;
+  }
+
+  // Uploads the per-step inputs (material, growth, viscosity surfaces, fixing,
+  // binding) from the node and element objects.
+  void bind(const std::vector<std::unique_ptr<Node>>& Nodes,
+            const std::vector<std::unique_ptr<ShapeBase>>& Elements) {
+    const int ne = (int)Elements.size();
+    if (!h) {
+      std::vector<int32_t> conn(6 * (size_t)ne);
+      std::vector<double> ref(18 * (size_t)ne);
+      for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 6; ++a) {
+          conn[6 * e + a] = Elements[e]->NodeIds[a];
+          for (int j = 0; j < 3; ++j) ref[18 * e + 3 * a + j] = Elements[e]->ReferencePositions[a][j];
+        }
+      check(tf_nr_create(nNodes, ne, conn.data(), ref.data(), &h));
+    }
+    std::vector<double> E(ne), nu(ne), eta(ne), iFg(9 * (size_t)ne), ev(3 * (size_t)nNodes),
+        A(nNodes);
+    std::vector<uint8_t> fixed(3 * (size_t)nNodes);
+    for (int e = 0; e < ne; ++e) {
+      E[e] = Elements[e]->E;
+      nu[e] = Elements[e]->v;
+      eta[e] = Elements[e]->internalViscosity;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) iFg[9 * e + 3 * i + j] = Elements[e]->InvFg[i][j];
+    }
+    for (int i = 0; i < nNodes; ++i) {
+      A[i] = Nodes[i]->viscositySurface;
+      for (int j = 0; j < 3; ++j) {
+        ev[3 * i + j] = Nodes[i]->externalViscosity[j];
+        fixed[3 * i + j] = Nodes[i]->FixedPos[j] ? 1 : 0;
+      }
+    }
+    std::vector<int32_t> pairs;
+    for (auto& p : slaveMasterList) {
+      pairs.push_back(p[0]);
+      pairs.push_back(p[1]);
+    }
+    check(tf_nr_set_material(h, E.data(), nu.data(), eta.data(), ev.data(), fixed.data(),
+                             pairs.empty() ? nullptr : pairs.data(), (int32_t)(pairs.size() / 2)));
+    check(tf_nr_set_growth(h, iFg.data()));
+    check(tf_nr_set_visc_surface(h, A.data()));
+  }
+
+  tf_handle* handle() const { return h; }
+
+ private:
+  int nDim, nNodes;
+  tf_handle* h = nullptr;
+};
+
+}  // namespace tissue_nr
+
+#endif  // TISSUE_NR_HPP

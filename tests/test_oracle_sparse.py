@@ -1,0 +1,34 @@
+"""Pins the oracle's block-sparse path (used as the CPU baseline at sizes the
+dense path cannot hold) against its dense transposed-LU path."""
+import numpy as np
+import pytest
+
+import problems
+
+
+@pytest.mark.parametrize("visc,fixed", [(False, False), (True, True)])
+def test_sparse_step_matches_dense_step(orc, visc, fixed):
+    p = problems.perturbed_problem(seed=7, visc=visc, fixed=fixed)
+    lam, mu = orc.lame(p["E"], p["nu"])
+    P = orc.Problem(p["mesh"]["conn"], p["refpos"], p["invFg"], lam, mu, p["eta"], p["ext_visc"],
+                    p["area"], p["fixed"])
+    ud, dud, cd = P.nr_step_dense(p["un"], p["dt"])
+    us, dus, kit, cs = P.nr_step_sparse(p["un"], p["dt"], krylov_rtol=1e-13)
+    assert cd and cs
+    assert len(dud) == len(dus)
+    assert np.abs(us - ud).max() <= 1e-10 * np.abs(ud).max()
+    assert np.all(kit > 0)
+
+
+def test_sparse_step_with_binding_matches_dense(orc):
+    p = problems.perturbed_problem(seed=8, visc=True)
+    nn = len(p["mesh"]["nodes"])
+    pairs = np.array([[3 * 7, 3 * 2], [3 * 7 + 1, 3 * 2 + 1], [3 * (nn - 1) + 2, 3 * 5 + 2]],
+                     np.int32)
+    lam, mu = orc.lame(p["E"], p["nu"])
+    P = orc.Problem(p["mesh"]["conn"], p["refpos"], p["invFg"], lam, mu, p["eta"], p["ext_visc"],
+                    p["area"], None, pairs)
+    ud, _, cd = P.nr_step_dense(p["un"], p["dt"])
+    us, _, _, cs = P.nr_step_sparse(p["un"], p["dt"], krylov_rtol=1e-13)
+    assert cd and cs
+    assert np.abs(us - ud).max() <= 1e-10 * np.abs(ud).max()

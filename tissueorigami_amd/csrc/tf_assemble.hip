@@ -1,0 +1,333 @@
+// Deterministic slot-map assembly of the system Jacobian and residual.
+//
+// The reference scatters every element's 18x18 blocks into a dense K, all
+// viscous blocks first and then all elastic ones, each pass in element order
+// (NewtonRaphsonSolver::writeImplicitElementalKToJacobian,
+// NewtonRaphsonSolver.cpp:350-360, ShapeBase.cpp:2935-2983), then adds the
+// external-viscosity diagonal (NewtonRaphsonSolver.cpp:399-423) and clears
+// fixed rows and columns (calcutateFixedK, NewtonRaphsonSolver.cpp:195-219).
+// Here one thread owns one 3x3 block of J = K^T (the operator of the
+// transposed Pardiso solve) and gathers its contributions through a
+// precomputed list sorted by element id, so the sum of every entry is formed
+// in exactly the reference's order and the result is bitwise reproducible.
+// The same thread applies the external viscosity and the fixed-DOF masks and,
+// for diagonal blocks, writes the block-Jacobi inverse.
+#include "tf_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace tf {
+namespace {
+
+__device__ inline void inv3_dev(const double A[9], double X[9]) {
+  // partial-pivot LU, then L U X = P I
+  double LU[9];
+  int perm[3] = {0, 1, 2};
+#pragma unroll
+  for (int q = 0; q < 9; ++q) LU[q] = A[q];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    int piv = j;
+    double mx = fabs(LU[3 * j + j]);
+#pragma unroll
+    for (int i = j + 1; i < 3; ++i)
+      if (fabs(LU[3 * i + j]) > mx) { mx = fabs(LU[3 * i + j]); piv = i; }
+    if (piv != j) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double t = LU[3 * j + k]; LU[3 * j + k] = LU[3 * piv + k]; LU[3 * piv + k] = t;
+      }
+      int t = perm[j]; perm[j] = perm[piv]; perm[piv] = t;
+    }
+    if (LU[3 * j + j] != 0.0) {
+#pragma unroll
+      for (int i = j + 1; i < 3; ++i) {
+        LU[3 * i + j] /= LU[3 * j + j];
+#pragma unroll
+        for (int k = j + 1; k < 3; ++k) LU[3 * i + k] -= LU[3 * i + j] * LU[3 * j + k];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double y[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = (perm[i] == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= LU[3 * i + k] * y[k];
+      y[i] = s;
+    }
+#pragma unroll
+    for (int i = 2; i >= 0; --i) {
+      double s = y[i];
+#pragma unroll
+      for (int k = i + 1; k < 3; ++k) s -= LU[3 * i + k] * X[3 * k + c];
+      X[3 * i + c] = s / LU[3 * i + i];
+    }
+  }
+}
+
+template <typename KT>
+__device__ inline void gather_blocks(const KT* __restrict__ Kel, const int32_t* __restrict__ src,
+                                     int beg, int end, double J[9]) {
+  for (int t = beg; t < end; ++t) {
+    const KT* b = Kel + (size_t)src[t] * 9;
+    double k[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) k[q] = (double)b[q];
+    // J(R, C)[p][q] = K(C, R)[q][p]
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) J[3 * p + q] = J[3 * p + q] + k[3 * q + p];
+  }
+}
+
+template <bool VISC, bool MIXED>
+__global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
+  const int64_t n_slots = a.pat.n_slots;
+  const int n_rows = a.pat.n_slices * kSlice;
+  // one thread per (padded) block row; its slots are base + k*64 + lane, so
+  // consecutive lanes touch consecutive slots (coalesced planar stores)
+  for (int R = blockIdx.x * kBlock + threadIdx.x; R < n_rows; R += gridDim.x * kBlock) {
+    const int sl = R / kSlice, lane = R % kSlice;
+    const int64_t sbase = a.pat.slice_base[sl];
+    const int width = (int)((a.pat.slice_base[sl + 1] - sbase) / kSlice);
+    const int row_len = (R < a.n_nodes) ? a.pat.row_len[R] : 0;
+    for (int kk = 0; kk < width; ++kk) {
+    const int64_t slot = sbase + (int64_t)kk * kSlice + lane;
+    if (kk >= row_len) {  // padding slot: zero block
+      if (MIXED) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a.val32[(size_t)q * n_slots + slot] = 0.0f;
+      }
+      if (a.val) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a.val[(size_t)q * n_slots + slot] = 0.0;
+      }
+      continue;
+    }
+    const int C = a.pat.col[slot];
+    double J[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) J[q] = 0.0;
+    const int beg = a.pat.contrib_ptr[slot], end = a.pat.contrib_ptr[slot + 1];
+    if (MIXED) {
+      if (VISC) gather_blocks(a.Kv32, a.pat.contrib_src, beg, end, J);
+      gather_blocks(a.Ke32, a.pat.contrib_src, beg, end, J);
+    } else {
+      if (VISC) gather_blocks(a.Kv, a.pat.contrib_src, beg, end, J);
+      gather_blocks(a.Ke, a.pat.contrib_src, beg, end, J);
+    }
+    {
+      if (R == C) {
+        // external viscous drag (NewtonRaphsonSolver.cpp:414-420)
+        const double spdt = a.area[R] / a.dt;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) J[4 * j] = spdt * a.ext_visc[3 * R + j] + J[4 * j];
+      }
+      if (a.dof_flags) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const bool fr = a.dof_flags[3 * R + p] & 1, fc = a.dof_flags[3 * C + q] & 1;
+            if (fr || fc) J[3 * p + q] = (R == C && p == q) ? 1.0 : 0.0;
+          }
+      }
+      if (R == C) {
+        double M[9], X[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) M[q] = J[q];
+        if (a.dof_flags) {
+#pragma unroll
+          for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+              if ((a.dof_flags[3 * R + p] | a.dof_flags[3 * R + q]) & 2)
+                M[3 * p + q] = (p == q) ? 1.0 : 0.0;
+        }
+        inv3_dev(M, X);
+        if (MIXED) {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) a.minv32[9 * R + q] = (float)X[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a.minv[9 * R + q] = X[q];
+      }
+    }
+    if (MIXED) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) a.val32[(size_t)q * n_slots + slot] = (float)J[q];
+    }
+    if (a.val) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) a.val[(size_t)q * n_slots + slot] = J[q];
+    }
+    }
+  }
+}
+
+// Node residual: gSum = ge + gvInternal + gvExternal (+ gExt = 0), fixed DOF
+// zeroed (NewtonRaphsonSolver.cpp:362-397, 441-457, 209); element forces
+// are summed in element order (ShapeBase.cpp:2362-2376).  internal receives
+// SystemForces (elastic + viscous per element, non-fixed DOF only).
+__global__ __launch_bounds__(kBlock) void node_force_kernel(AssembleArgs a) {
+  const double rdt = 1.0 / a.dt;
+  for (int R = blockIdx.x * kBlock + threadIdx.x; R < a.n_nodes; R += gridDim.x * kBlock) {
+    const int beg = a.pat.node_ptr[R], end = a.pat.node_ptr[R + 1];
+    double ge[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0}, sf[3] = {0.0, 0.0, 0.0};
+    bool fx[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) fx[i] = a.dof_flags ? (a.dof_flags[3 * R + i] & 1) : false;
+    for (int t = beg; t < end; ++t) {
+      const int s = a.pat.node_src[t];  // e*6 + a
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double fe = fx[i] ? 0.0 : a.Fe[3 * (size_t)s + i];
+        const double fv = fx[i] ? 0.0 : a.Fv[3 * (size_t)s + i];
+        ge[i] = ge[i] + fe;
+        gv[i] = gv[i] + fv;
+        if (!fx[i]) sf[i] = sf[i] + (a.Fe[3 * (size_t)s + i] + a.Fv[3 * (size_t)s + i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double sv = a.area[R] * a.ext_visc[3 * R + i];
+      const double disp = (a.uk[3 * R + i] - a.un[3 * R + i]) * rdt;
+      const double gve = -(sv * disp);
+      double g = ge[i] + gv[i] + gve;
+      g = g + 0.0;
+      if (fx[i]) g = 0.0;
+      a.g[3 * R + i] = g;
+      if (a.internal) a.internal[3 * R + i] = sf[i];
+    }
+  }
+}
+
+// Slave/master binding of the residual, pair by pair in list order
+// (NewtonRaphsonSolver.cpp:148-158).  One thread: the list is short and the
+// reference order is sequential.
+__global__ void bind_forces_kernel(const int32_t* pairs, int n_pairs, double* g) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int p = 0; p < n_pairs; ++p) {
+    const int s = pairs[2 * p], m = pairs[2 * p + 1];
+    const double slaveForce = g[s];
+    double masterForce = g[m];
+    masterForce += slaveForce;
+    g[s] = 0.0;
+    g[m] = masterForce;
+  }
+}
+
+// equateSlaveDisplacementsToMasters (NewtonRaphsonSolver.cpp:185-193).
+__global__ void bind_increment_kernel(const int32_t* pairs, int n_pairs, double* du) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int p = 0; p < n_pairs; ++p) du[pairs[2 * p]] = du[pairs[2 * p + 1]];
+}
+
+// Exposed-surface area of one triangle of a prism (Prism::calculateApicalArea /
+// calculateBasalArea, Prism.cpp:1485-1556).
+__device__ double tri_area(const double* u, int n0, int n1, int n2) {
+  const double Threshold = 1E-5;
+  double s1[3], s2[3], Side1 = 0, Side2 = 0, costet = 0, Area = 0;
+  for (int i = 0; i < 3; ++i) {
+    s1[i] = u[3 * n1 + i] - u[3 * n0 + i];
+    s2[i] = u[3 * n2 + i] - u[3 * n0 + i];
+    costet += s1[i] * s2[i];
+    Side1 += s1[i] * s1[i];
+    Side2 += s2[i] * s2[i];
+  }
+  if (Side1 > Threshold && Side2 > Threshold) {
+    Side1 = sqrt(Side1);
+    Side2 = sqrt(Side2);
+    costet /= (Side1 * Side2);
+    const double sinTetSq = 1 - costet * costet;
+    double sintet = 0;
+    if (sinTetSq > Threshold) sintet = sqrt(sinTetSq);
+    Area = Side1 * Side2 * sintet / 2.0;
+  }
+  return Area;
+}
+
+// Node viscosity surface: for each node, the exposed faces of its prisms in
+// prism order (Simulation::updateNodeViscositySurfaces, Simulation.cpp:7598-7621;
+// ShapeBase::assignViscositySurfaceAreaToNodes, ShapeBase.cpp:3764-3775;
+// Prism::assignExposedSurfaceAreaIndices, Prism.cpp:1650-1709).
+__global__ __launch_bounds__(kBlock) void visc_surface_kernel(
+    int32_t n_nodes, const int32_t* conn, const double* u, const int32_t* node_ptr,
+    const int32_t* node_src, const int32_t* placement, const int32_t* type,
+    const uint8_t* spans, double* area) {
+  for (int R = blockIdx.x * kBlock + threadIdx.x; R < n_nodes; R += gridDim.x * kBlock) {
+    double A = 0.0;
+    for (int t = node_ptr[R]; t < node_ptr[R + 1]; ++t) {
+      const int s = node_src[t];
+      const int e = s / 6, k = s % 6;
+      const int tt = type[e], tp = placement[e];
+      if (tt != 0 && tt != 1) continue;
+      const bool has_apical = (tp == 1) || (tp == 2 && spans[e]);
+      const bool has_basal = (tp == 0) || (tp == 2 && spans[e]);
+      const int* c = conn + 6 * e;
+      const bool node_is_apical_face = (tt == 1) ? (k < 3) : (k >= 3);
+      if (has_apical && node_is_apical_face) {
+        const double ar = (tt == 1) ? tri_area(u, c[0], c[1], c[2]) : tri_area(u, c[3], c[4], c[5]);
+        A += ar / 3;
+      }
+      if (has_basal && !node_is_apical_face) {
+        const double ar = (tt == 1) ? tri_area(u, c[3], c[4], c[5]) : tri_area(u, c[0], c[1], c[2]);
+        A += ar / 3;
+      }
+    }
+    area[R] = A;
+  }
+}
+
+int grid_for(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 65535 * 4) g = 65535 * 4;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st) {
+  const int g1 = grid_for((int64_t)a.pat.n_slices * kSlice);
+  if (a.any_visc) {
+    if (mixed) hipLaunchKernelGGL((assemble_kernel<true, true>), dim3(g1), dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL((assemble_kernel<true, false>), dim3(g1), dim3(kBlock), 0, st, a);
+  } else {
+    if (mixed) hipLaunchKernelGGL((assemble_kernel<false, true>), dim3(g1), dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL((assemble_kernel<false, false>), dim3(g1), dim3(kBlock), 0, st, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(node_force_kernel, dim3(grid_for(a.n_nodes)), dim3(kBlock), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_bind_forces(const int32_t* pairs, int n_pairs, double* g, hipStream_t st) {
+  if (n_pairs == 0) return hipSuccess;
+  hipLaunchKernelGGL(bind_forces_kernel, dim3(1), dim3(64), 0, st, pairs, n_pairs, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_bind_increment(const int32_t* pairs, int n_pairs, double* du, hipStream_t st) {
+  if (n_pairs == 0) return hipSuccess;
+  hipLaunchKernelGGL(bind_increment_kernel, dim3(1), dim3(64), 0, st, pairs, n_pairs, du);
+  return hipGetLastError();
+}
+
+hipError_t launch_visc_surface(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,
+                               const double* u, const int32_t* node_ptr,
+                               const int32_t* node_src, const int32_t* placement,
+                               const int32_t* type, const uint8_t* spans, double* area,
+                               hipStream_t st) {
+  (void)n_prisms;
+  hipLaunchKernelGGL(visc_surface_kernel, dim3(grid_for(n_nodes)), dim3(kBlock), 0, st, n_nodes,
+                     conn, u, node_ptr, node_src, placement, type, spans, area);
+  return hipGetLastError();
+}
+
+}  // namespace tf

@@ -1,0 +1,530 @@
+// Per-prism kernel: shape functions at the 6 Gauss points, Neo-Hookean elastic
+// and viscous nodal forces, 18x18 elastic and viscous tangents.
+//
+// Follows, per Gauss point, Prism::calculateCurrNodalForces (Prism.cpp:766-886)
+// with the ShapeBase helpers (ShapeBase.cpp:2448-2845); the elastic tangent
+// uses the exact closed form of ShapeBase::calculateElasticKIntegral1 +
+// calculateElasticKIntegral2 (ShapeBase.cpp:3010-3095): because
+// Fe C^-1 Fe^T = I the push-forward of the Lagrangian tensor D81 reduces to
+// Je^-1 [lambda d_ij d_kl + mu' (d_ik d_jl + d_il d_jk)], mu' = mu - lambda ln Je.
+// The viscous tangent follows ShapeBase::calculateImplicitKViscous
+// (ShapeBase.cpp:2868-2933, 3097-3209) and is stored transposed as the
+// reference stores it.
+//
+// Work decomposition: one workgroup handles kEPB prisms.
+//   phase 1: one thread per (prism, Gauss point) computes the kinematics and
+//            stresses and leaves grad N, sigma and the scalar factors in LDS;
+//   phase 2: one thread per (prism, 3x3 block) sums its block over the Gauss
+//            points and writes 9 contiguous values (coalesced across lanes);
+//   phase 3: one thread per (prism, node) sums the nodal forces.
+// Every floating-point operation is written in the order of the CPU oracle's
+// restatement and compiled without contraction, so the FP64 outputs agree
+// with the oracle to the last bit up to the device log().
+#include <type_traits>
+
+#include "tf_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace tf {
+namespace {
+
+constexpr int kEPB = 32;              // prisms per workgroup
+constexpr int kThreads = kEPB * 6;    // 192
+
+struct ElemConst {
+  double sfd[6][3][6];  // shape function derivatives per Gauss point
+  double w;             // Gauss weight (1/6 for all six points)
+};
+
+struct GPData {
+  double dN[6][3];
+  double sig[9];
+  double detF, detdXde, cl, cm;
+};
+
+struct GPVisc {
+  double sv[9];
+  double P[9];
+  double eta;
+};
+
+__device__ inline double det3(const double M[3][3]) {
+  double det = 0.0;
+  det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]);
+  det -= M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]);
+  det += M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+  return det;
+}
+
+// LU with partial pivoting, then L U X = P I (the GSL algorithm the reference
+// calls at ShapeBase.cpp:3546-3561).
+__device__ inline void inv3(const double A[3][3], double X[3][3]) {
+  double LU[3][3];
+  int perm[3] = {0, 1, 2};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) LU[i][j] = A[i][j];
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    int piv = j;
+    double mx = fabs(LU[j][j]);
+#pragma unroll
+    for (int i = j + 1; i < 3; ++i)
+      if (fabs(LU[i][j]) > mx) { mx = fabs(LU[i][j]); piv = i; }
+    if (piv != j) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { double t = LU[j][k]; LU[j][k] = LU[piv][k]; LU[piv][k] = t; }
+      int t = perm[j]; perm[j] = perm[piv]; perm[piv] = t;
+    }
+    if (LU[j][j] != 0.0) {
+#pragma unroll
+      for (int i = j + 1; i < 3; ++i) {
+        LU[i][j] /= LU[j][j];
+#pragma unroll
+        for (int k = j + 1; k < 3; ++k) LU[i][k] -= LU[i][j] * LU[j][k];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double y[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double s = (perm[i] == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= LU[i][k] * y[k];
+      y[i] = s;
+    }
+#pragma unroll
+    for (int i = 2; i >= 0; --i) {
+      double s = y[i];
+#pragma unroll
+      for (int k = i + 1; k < 3; ++k) s -= LU[i][k] * X[k][c];
+      X[i][c] = s / LU[i][i];
+    }
+  }
+}
+
+__device__ inline void matmul3(const double A[3][3], const double B[3][3], double C[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s += A[i][k] * B[k][j];
+      C[i][j] = s;
+    }
+}
+
+template <bool VISC, bool MIXED>
+__global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementArgs a) {
+  __shared__ GPData sd[kEPB * 6];
+  __shared__ GPVisc svd[VISC ? kEPB * 6 : 1];
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * kEPB;
+
+  // ---------------- phase 1: one thread per (prism, Gauss point) ----------
+  {
+    const int el = tid / 6;
+    const int g = tid % 6;
+    const int e = e0 + el;
+    if (e < a.n_prisms) {
+      int nd[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) nd[k] = a.conn[6 * e + k];
+      double pos[6][3], ref[6][3], iFg[3][3];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          pos[k][i] = a.uk[3 * nd[k] + i];
+          ref[k][i] = a.refpos[18 * e + 3 * k + i];
+        }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) iFg[i][j] = a.invFg[9 * e + 3 * i + j];
+      const double lambda = a.lambda[e];
+      const double mu = a.mu[e];
+      const double(*SFD)[6] = c.sfd[g];
+      // reference geometry (Prism.cpp:715-732)
+      double dXde[3][3], InvdXde[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) s += SFD[j][k] * ref[k][i];
+          dXde[i][j] = s;
+        }
+      const double detdXde = det3(dXde);
+      inv3(dXde, InvdXde);
+      // current Jacobian, F, Fe (Prism.cpp:782-788)
+      double Jac[3][3], F[3][3], Fe[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) s += SFD[j][k] * pos[k][i];
+          Jac[i][j] = s;
+        }
+      matmul3(Jac, InvdXde, F);
+      matmul3(F, iFg, Fe);
+      double C[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) s += Fe[k][i] * Fe[k][j];
+          C[i][j] = s;
+        }
+      const double detFe = det3(Fe);
+      const double lnJ = log(detFe);
+      double InvC[3][3];
+      inv3(C, InvC);
+      double PK2[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double Iij = (i == j) ? 1.0 : 0.0;
+          PK2[i][j] = InvC[i][j] * (lambda * lnJ) + (Iij - InvC[i][j]) * mu;
+        }
+      double T1[3][3];
+      matmul3(Fe, PK2, T1);
+      const double inv_detFe = 1.0 / detFe;
+      GPData& out = sd[tid];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) s += T1[i][k] * Fe[j][k];
+          out.sig[3 * i + j] = s * inv_detFe;
+        }
+      double InvJ[3][3];
+      inv3(Jac, InvJ);
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          double s = 0.0;
+#pragma unroll
+          for (int n = 0; n < 3; ++n) s += InvJ[n][m] * SFD[n][k];
+          out.dN[k][m] = s;
+        }
+      const double detF = det3(F);
+      out.detF = detF;
+      out.detdXde = detdXde;
+      const double vol = detF * detdXde;
+      const double cc = vol / detFe;
+      out.cl = cc * lambda;
+      out.cm = cc * (mu - lambda * lnJ);
+      if (VISC) {
+        const double eta = a.eta[e];
+        GPVisc& ov = svd[tid];
+        ov.eta = eta;
+        if (eta != 0) {
+          const double rdt = 1.0 / a.dt;
+          double vel[6][3];
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+              vel[k][i] = (a.uk[3 * nd[k] + i] - a.un[3 * nd[k] + i]) * rdt;
+          // l = B v (ShapeBase.cpp:2640-2716), Voigt xx, yy, zz, xy, yz, xz
+          double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0, l4 = 0.0, l5 = 0.0;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            const double nx = out.dN[k][0], ny = out.dN[k][1], nz = out.dN[k][2];
+            l0 += nx * vel[k][0];
+            l1 += ny * vel[k][1];
+            l2 += nz * vel[k][2];
+            l3 += ny * vel[k][0];
+            l3 += nx * vel[k][1];
+            l4 += nz * vel[k][1];
+            l4 += ny * vel[k][2];
+            l5 += nz * vel[k][0];
+            l5 += nx * vel[k][2];
+          }
+          double d[9];
+          d[0] = l0; d[4] = l1; d[8] = l2;
+          d[1] = 0.5 * l3; d[7] = 0.5 * l4; d[2] = 0.5 * l5;
+          d[3] = 0.5 * l3; d[5] = 0.5 * l4; d[6] = 0.5 * l5;
+#pragma unroll
+          for (int q = 0; q < 9; ++q) ov.sv[q] = d[q] * eta;
+          // velocity gradient and (I/dt - L) (ShapeBase.cpp:2908-2916, 3097-3121)
+          double L[9];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) L[q] = 0.0;
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) L[3 * i + j] += vel[k][i] * out.dN[k][j];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              ov.P[3 * i + j] = ((i == j) ? 1.0 : 0.0) * (1.0 / a.dt) - L[3 * i + j];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) { ov.sv[q] = 0.0; ov.P[q] = 0.0; }
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  const double w = c.w;
+  // ---------------- phase 2a: elastic 3x3 blocks ---------------------------
+  // (FP32 arithmetic from the FP64 kinematics in mixed precision)
+  using T = typename std::conditional<MIXED, float, double>::type;
+  const T tw = (T)w;
+  for (int task = tid; task < kEPB * 36; task += kThreads) {
+    const int el = task / 36;
+    const int ab = task % 36;
+    const int e = e0 + el;
+    if (e >= a.n_prisms) break;
+    const int ia = ab / 6, ib = ab % 6;
+    T K[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) K[q] = (T)0;
+#pragma unroll
+    for (int g = 0; g < 6; ++g) {
+      const GPData& d = sd[el * 6 + g];
+      T Na[3], Nb[3], sg[9];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) { Na[j] = (T)d.dN[ia][j]; Nb[j] = (T)d.dN[ib][j]; }
+#pragma unroll
+      for (int q = 0; q < 9; ++q) sg[q] = (T)d.sig[q];
+      const T cl = (T)d.cl, cm = (T)d.cm;
+      const T vol = MIXED ? (T)(d.detF * d.detdXde) : (T)d.detF * (T)d.detdXde;
+      T dot = (T)0, geo = (T)0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dot += Na[j] * Nb[j];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        T s = (T)0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s += sg[3 * i + j] * Nb[j];
+        geo += Na[i] * s;
+      }
+      const T diag = cm * dot + vol * geo;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          T v = cl * Na[i] * Nb[k] + cm * Na[k] * Nb[i];
+          if (i == k) v += diag;
+          K[3 * i + k] += v * tw;
+        }
+    }
+    const size_t base = ((size_t)e * 36 + ab) * 9;
+    if (MIXED) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) a.Ke32[base + q] = (float)K[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) a.Ke[base + q] = (double)K[q];
+    }
+  }
+
+  // ---------------- phase 2b: viscous 3x3 blocks (transposed storage) -----
+  if (VISC) {
+    for (int task = tid; task < kEPB * 36; task += kThreads) {
+      const int el = task / 36;
+      const int ab = task % 36;
+      const int e = e0 + el;
+      if (e >= a.n_prisms) break;
+      const int ia = ab / 6, ib = ab % 6;
+      T K[9];  // K[3i+j] -> TriPointKv[3b+j][3a+i]
+#pragma unroll
+      for (int q = 0; q < 9; ++q) K[q] = (T)0;
+      if (svd[el * 6].eta != 0) {
+#pragma unroll 1
+        for (int g = 0; g < 6; ++g) {
+          const GPData& d = sd[el * 6 + g];
+          const GPVisc& dv = svd[el * 6 + g];
+          T Na[3], Nb[3];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) { Na[j] = (T)d.dN[ia][j]; Nb[j] = (T)d.dN[ib][j]; }
+          // B columns of nodes a and b (Voigt rows xx, yy, zz, xy, yz, xz)
+          T Ba[3][6], Bb[3][6];
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int r = 0; r < 6; ++r) { Ba[j][r] = (T)0; Bb[j][r] = (T)0; }
+          Ba[0][0] = Na[0]; Ba[0][3] = Na[1]; Ba[0][5] = Na[2];
+          Ba[1][1] = Na[1]; Ba[1][3] = Na[0]; Ba[1][4] = Na[2];
+          Ba[2][2] = Na[2]; Ba[2][4] = Na[1]; Ba[2][5] = Na[0];
+          Bb[0][0] = Nb[0]; Bb[0][3] = Nb[1]; Bb[0][5] = Nb[2];
+          Bb[1][1] = Nb[1]; Bb[1][3] = Nb[0]; Bb[1][4] = Nb[2];
+          Bb[2][2] = Nb[2]; Bb[2][4] = Nb[1]; Bb[2][5] = Nb[0];
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int r = 3; r < 6; ++r) Bb[j][r] = (T)0.5 * Bb[j][r];
+          T BB[3][3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              T s = (T)0;
+#pragma unroll
+              for (int r = 0; r < 6; ++r) s += Ba[i][r] * Bb[j][r];
+              BB[i][j] = s;
+            }
+          T P[3][3], K1[3][3], S[3][3], paren[3][3], K2[3][3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              P[i][j] = (T)dv.P[3 * i + j];
+              S[i][j] = (T)dv.sv[3 * i + j];
+              paren[i][j] = Na[i] * Nb[j] - Nb[i] * Na[j];
+            }
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              T s1 = (T)0, s2 = (T)0;
+#pragma unroll
+              for (int q = 0; q < 3; ++q) {
+                s1 += BB[i][q] * P[q][j];
+                s2 += S[i][q] * paren[q][j];
+              }
+              K1[i][j] = s1;
+              K2[i][j] = s2;
+            }
+          const T detF = (T)d.detF, detdXde = (T)d.detdXde, eta = (T)dv.eta;
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              T v1 = K1[i][j] * detF;
+              v1 *= detdXde;
+              v1 *= eta;
+              T v2 = K2[i][j] * detF;
+              v2 *= detdXde;
+              T curr = (T)0;
+              curr += v1;
+              curr += v2;
+              K[3 * i + j] += curr * tw;
+            }
+        }
+      }
+      // reference storage: TriPointKv[3b+j][3a+i] -> block (b, a), entry (j, i)
+      const size_t base = ((size_t)e * 36 + ib * 6 + ia) * 9;
+      if (MIXED) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) a.Kv32[base + 3 * j + i] = (float)K[3 * i + j];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) a.Kv[base + 3 * j + i] = (double)K[3 * i + j];
+      }
+    }
+  }
+
+  // ---------------- phase 3: nodal forces ----------------------------------
+  if (tid < kEPB * 6) {
+    const int el = tid / 6;
+    const int k = tid % 6;
+    const int e = e0 + el;
+    if (e < a.n_prisms) {
+      double te[3] = {0.0, 0.0, 0.0}, tv[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int g = 0; g < 6; ++g) {
+        const GPData& d = sd[el * 6 + g];
+        const double* s = d.sig;
+        // compact stress [xx, yy, zz, xy, zy, xz] (ShapeBase.cpp:2560-2566)
+        const double c0 = s[0], c1 = s[4], c2 = s[8], c3 = s[1], c4 = s[7], c5 = s[2];
+        double bx = d.dN[k][0] * d.detF; bx *= d.detdXde;
+        double by = d.dN[k][1] * d.detF; by *= d.detdXde;
+        double bz = d.dN[k][2] * d.detF; bz *= d.detdXde;
+        double f0 = 0.0; f0 += bx * c0; f0 += by * c3; f0 += bz * c5;
+        double f1 = 0.0; f1 += by * c1; f1 += bx * c3; f1 += bz * c4;
+        double f2 = 0.0; f2 += bz * c2; f2 += by * c4; f2 += bx * c5;
+        te[0] = te[0] + f0 * w;
+        te[1] = te[1] + f1 * w;
+        te[2] = te[2] + f2 * w;
+        if (VISC) {
+          const double* v = svd[el * 6 + g].sv;
+          const double v0 = v[0], v1 = v[4], v2 = v[8], v3 = v[1], v4 = v[7], v5 = v[2];
+          double h0 = 0.0; h0 += bx * v0; h0 += by * v3; h0 += bz * v5;
+          double h1 = 0.0; h1 += by * v1; h1 += bx * v3; h1 += bz * v4;
+          double h2 = 0.0; h2 += bz * v2; h2 += by * v4; h2 += bx * v5;
+          tv[0] = tv[0] + h0 * w;
+          tv[1] = tv[1] + h1 * w;
+          tv[2] = tv[2] + h2 * w;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        a.Fe[18 * e + 3 * k + i] = 0.0 - te[i];
+        a.Fv[18 * e + 3 * k + i] = 0.0 - tv[i];
+      }
+    }
+  }
+}
+
+ElemConst make_const() {
+  ElemConst c;
+  // Gauss rule (Prism.cpp:99-131) and shape-function derivative table
+  // (Prism.cpp:597-652), evaluated on the host with the reference's expressions.
+  const double s = 1.0 / std::pow(3.0, 0.5);
+  const double A = 1.0 / 6.0, B = 2.0 / 3.0;
+  const double pts[6][3] = {{A, A, s}, {B, A, s}, {A, B, s}, {A, A, -s}, {B, A, -s}, {A, B, -s}};
+  for (int g = 0; g < 6; ++g) {
+    const double eta = pts[g][0], nu = pts[g][1], zeta = pts[g][2];
+    const double alpha = (1 - zeta) / 2;
+    const double beta = (1 + zeta) / 2;
+    const double lam = 1 - eta - nu;
+    const double r0[6] = {-alpha, alpha, 0, -beta, beta, 0};
+    const double r1[6] = {-alpha, 0, alpha, -beta, 0, beta};
+    const double r2[6] = {-lam / 2, -eta / 2, -nu / 2, lam / 2, eta / 2, nu / 2};
+    for (int k = 0; k < 6; ++k) {
+      c.sfd[g][0][k] = r0[k];
+      c.sfd[g][1][k] = r1[k];
+      c.sfd[g][2][k] = r2[k];
+    }
+  }
+  c.w = 1.0 / 6.0;
+  return c;
+}
+
+}  // namespace
+
+hipError_t launch_element(const ElementArgs& a, int mixed, hipStream_t st) {
+  static const ElemConst c = make_const();
+  if (a.n_prisms == 0) return hipSuccess;
+  const int grid = (a.n_prisms + kEPB - 1) / kEPB;
+  if (a.any_visc) {
+    if (mixed)
+      hipLaunchKernelGGL((element_kernel<true, true>), dim3(grid), dim3(kThreads), 0, st, c, a);
+    else
+      hipLaunchKernelGGL((element_kernel<true, false>), dim3(grid), dim3(kThreads), 0, st, c, a);
+  } else {
+    if (mixed)
+      hipLaunchKernelGGL((element_kernel<false, true>), dim3(grid), dim3(kThreads), 0, st, c, a);
+    else
+      hipLaunchKernelGGL((element_kernel<false, false>), dim3(grid), dim3(kThreads), 0, st, c, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace tf

@@ -1,0 +1,169 @@
+// Internal declarations shared by the host driver (tf_nr.hip) and the device
+// code (tf_element.hip, tf_assemble.hip, tf_krylov.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/tissue_nr.h"
+
+namespace tf {
+
+// Jacobian storage: SELL-64 of 3x3 node blocks.  Slice s covers block rows
+// [64 s, 64 s + 64); every row of the slice is padded to the slice width w_s.
+// Block slot of (row R, k-th stored column of R) is
+//   slice_base[R / 64] + k * 64 + R % 64,
+// values are planar: val[c * n_slots + slot], c = 3 p + q of the 3x3 block.
+// Padding slots point at column R with zero values.
+constexpr int kSlice = 64;
+
+struct DevPattern {
+  int32_t n_slices = 0;
+  int64_t n_slots = 0;          // padded SELL size in blocks
+  int32_t* slice_base = nullptr;  // n_slices + 1 (slot offsets)
+  int32_t* col = nullptr;         // n_slots
+  int32_t* diag_slot = nullptr;   // n_nodes
+  int32_t* row_len = nullptr;     // n_nodes: stored blocks of each row
+  int32_t* contrib_ptr = nullptr; // n_slots + 1
+  int32_t* contrib_src = nullptr; // n_prisms * 36: e*36 + A*6 + B (K block A,B)
+  int32_t* node_ptr = nullptr;    // n_nodes + 1
+  int32_t* node_src = nullptr;    // n_prisms * 6: e*6 + a
+};
+
+struct Workspace {
+  // per-prism outputs (block-major, reference K orientation):
+  // elemK[(e*36 + A*6 + B)*9 + r*3 + c] = TriPointKe[3A+r][3B+c]
+  double* elemKe = nullptr;
+  double* elemKv = nullptr;
+  float* elemKe32 = nullptr;
+  float* elemKv32 = nullptr;
+  double* elemFe = nullptr;  // n_prisms * 18, unmasked -sum_gp w g
+  double* elemFv = nullptr;
+  // assembled operator
+  double* val = nullptr;    // 9 * n_slots
+  float* val32 = nullptr;   // 9 * n_slots (mixed precision)
+  double* minv = nullptr;   // 9 * n_nodes block-Jacobi inverse
+  float* minv32 = nullptr;
+  // vectors (3 * n_nodes)
+  double *un = nullptr, *uk = nullptr, *g = nullptr, *du = nullptr, *internal = nullptr;
+  double *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *ph = nullptr, *s = nullptr,
+         *sh = nullptr, *t = nullptr, *x = nullptr, *b = nullptr, *tmp = nullptr;
+  float *r32 = nullptr, *rh32 = nullptr, *p32 = nullptr, *v32 = nullptr, *ph32 = nullptr,
+        *s32 = nullptr, *sh32 = nullptr, *t32 = nullptr, *x32 = nullptr;
+  double* partials = nullptr;  // kParts * 4
+  double* scal = nullptr;      // Krylov scalars
+  int32_t* flag = nullptr;     // [0] state, [1] iterations
+  double* host_pinned = nullptr;
+  int32_t* host_flag = nullptr;
+};
+
+constexpr int kParts = 2048;  // max workgroups of a reduction grid
+constexpr int kBlock = 256;
+
+enum Scal { S_RHO = 0, S_RHO_NEW, S_ALPHA, S_OMEGA, S_BNORM, S_RNORM, S_COUNT };
+enum KState { K_RUN = 0, K_CONVERGED = 1, K_BREAKDOWN = 2 };
+
+struct Launch {
+  hipStream_t stream;
+  int grid;  // workgroups of vector kernels (multiple of 8, <= kParts)
+};
+
+// ---- per-prism kernel (tf_element.hip) ----
+struct ElementArgs {
+  int32_t n_prisms;
+  const int32_t* conn;
+  const double* refpos;
+  const double* invFg;
+  const double* lambda;
+  const double* mu;
+  const double* eta;
+  const double* uk;
+  const double* un;
+  double dt;
+  int any_visc;
+  double* Ke;
+  double* Kv;
+  float* Ke32;
+  float* Kv32;
+  double* Fe;
+  double* Fv;
+};
+hipError_t launch_element(const ElementArgs& a, int mixed, hipStream_t st);
+
+// ---- assembly (tf_assemble.hip) ----
+struct AssembleArgs {
+  int32_t n_nodes;
+  DevPattern pat;
+  const double* Ke;
+  const double* Kv;
+  const float* Ke32;
+  const float* Kv32;
+  const double* Fe;
+  const double* Fv;
+  int any_visc;
+  const double* ext_visc;
+  const double* area;
+  const uint8_t* dof_flags;  // bit0 fixed, bit1 slave
+  const double* uk;
+  const double* un;
+  double dt;
+  double* val;
+  float* val32;
+  double* minv;
+  float* minv32;
+  double* g;
+  double* internal;
+};
+hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st);
+hipError_t launch_bind_forces(const int32_t* pairs, int n_pairs, double* g, hipStream_t st);
+hipError_t launch_bind_increment(const int32_t* pairs, int n_pairs, double* du, hipStream_t st);
+hipError_t launch_visc_surface(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,
+                               const double* u, const int32_t* node_ptr,
+                               const int32_t* node_src, const int32_t* placement,
+                               const int32_t* type, const uint8_t* spans, double* area,
+                               hipStream_t st);
+
+// ---- Krylov (tf_krylov.hip) ----
+// Optional event timing of the SpMV launches ([0] FP64 operator, [1] FP32).
+constexpr int kProfPairs = 128;
+struct Prof {
+  int on = 0;
+  hipEvent_t ev[2 * kProfPairs] = {};
+  int kind[kProfPairs] = {};
+  int used = 0;
+  double ms[2] = {0.0, 0.0};
+  int64_t count[2] = {0, 0};
+};
+void prof_harvest(Prof* p);
+
+struct KrylovArgs {
+  int32_t n_nodes;
+  DevPattern pat;
+  const double* val;
+  const float* val32;
+  const double* minv;
+  const float* minv32;
+  const int32_t* pairs;
+  int32_t n_pairs;
+  Workspace* ws;
+  Launch L;
+  Prof* prof;
+};
+// Solves J x = b (FP64).  Returns the iteration count in *iters, the Krylov
+// state (K_CONVERGED/K_BREAKDOWN/K_RUN=cap reached) in *state.
+hipError_t krylov_bicgstab(const KrylovArgs& k, const double* b, double* x, double rtol,
+                           int maxit, int* iters, int* state, int sync_every);
+// Mixed precision: FP32 inner BiCGStab, FP64 residual refinement.
+hipError_t krylov_mixed(const KrylovArgs& k, const double* b, double* x, double rtol,
+                        double inner_rtol, int maxit, int* iters, int* state, int sync_every);
+// y = A x with the binding operator; FP64.
+hipError_t apply_operator(const KrylovArgs& k, const double* x, double* y);
+// |x|_2 and x_k += dx (deterministic two-level reduction), result on host.
+hipError_t norm_and_update(int32_t n, const double* dx, double* xk, Workspace* ws, Launch L,
+                           double* norm_out);
+
+void set_error(const std::string& s);
+
+}  // namespace tf

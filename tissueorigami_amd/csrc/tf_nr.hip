@@ -1,0 +1,817 @@
+// Host driver of libtissue_nr.so: the C ABI declared in include/tissue_nr.h.
+//
+// tf_nr_create builds, once per mesh, the block pattern of the Jacobian from
+// the prism connectivity and the element -> block slot map that makes the
+// assembly a deterministic gather; tf_nr_step runs the NR loop of
+// Simulation::updateStepNR (Simulation.cpp:6605-6746) on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "tf_internal.h"
+
+namespace tf {
+namespace {
+thread_local std::string g_err;
+}
+void set_error(const std::string& s) { g_err = s; }
+}  // namespace tf
+
+using namespace tf;
+
+struct tf_handle {
+  int32_t Nn = 0, Ne = 0;
+  int device = 0;
+  hipStream_t st = nullptr;
+  DevPattern pat;
+  std::vector<int32_t> h_slice_base, h_col, h_row_len, h_rowptr, h_nbr;
+  // inputs
+  int32_t* conn = nullptr;
+  double* refpos = nullptr;
+  double* invFg = nullptr;
+  double* lam = nullptr;
+  double* mu = nullptr;
+  double* eta = nullptr;
+  double* extv = nullptr;
+  double* area = nullptr;
+  uint8_t* dof_flags = nullptr;
+  int32_t* pairs = nullptr;
+  int32_t n_pairs = 0;
+  std::vector<int32_t> h_pairs;
+  bool any_visc = false, have_material = false, have_growth = false, have_area = false;
+  bool assembled = false;
+  int last_mixed = 0;
+  Workspace ws;
+  Launch L;
+  Prof prof;
+  double prof_el_ms = 0.0, prof_as_ms = 0.0;
+  int64_t prof_assemblies = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+#define TF_CHECK(call)                                                              \
+  do {                                                                              \
+    hipError_t _e = (call);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      set_error(std::string(#call) + ": " + hipGetErrorString(_e));                 \
+      return TF_ERR_HIP;                                                            \
+    }                                                                               \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(tf_handle* h, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e == hipSuccess) {
+    h->allocs.push_back(q);
+    *p = (T*)q;
+  }
+  return e;
+}
+
+tf_status check_device(int* dev) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    set_error("no HIP device visible");
+    return TF_ERR_NO_DEVICE;
+  }
+  TF_CHECK(hipGetDevice(dev));
+  hipDeviceProp_t prop;
+  TF_CHECK(hipGetDeviceProperties(&prop, *dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error(std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+    return TF_ERR_NO_DEVICE;
+  }
+  return TF_OK;
+}
+
+// Block pattern, SELL-64 layout and slot map (host side, once per mesh).
+tf_status build_pattern(tf_handle* h, const int32_t* conn) {
+  const int Nn = h->Nn, Ne = h->Ne;
+  // node -> (element, local node) incidence in element order
+  std::vector<int32_t> node_ptr(Nn + 1, 0), node_src((size_t)Ne * 6);
+  for (int64_t i = 0; i < (int64_t)Ne * 6; ++i) {
+    const int nd = conn[i];
+    if (nd < 0 || nd >= Nn) {
+      set_error("connectivity references node " + std::to_string(nd) + " outside [0, n_nodes)");
+      return TF_ERR_ARG;
+    }
+    node_ptr[nd + 1]++;
+  }
+  for (int i = 0; i < Nn; ++i) node_ptr[i + 1] += node_ptr[i];
+  {
+    std::vector<int32_t> pos(node_ptr.begin(), node_ptr.end() - 1);
+    for (int e = 0; e < Ne; ++e)
+      for (int a = 0; a < 6; ++a) node_src[pos[conn[6 * e + a]]++] = 6 * e + a;
+  }
+  // neighbours of every node (ascending), including itself
+  std::vector<int32_t>& rowptr = h->h_rowptr;
+  std::vector<int32_t>& nbr = h->h_nbr;
+  rowptr.assign(Nn + 1, 0);
+  nbr.clear();
+  nbr.reserve((size_t)Nn * 24);
+  std::vector<int32_t> cand;
+  for (int R = 0; R < Nn; ++R) {
+    cand.clear();
+    cand.push_back(R);
+    for (int t = node_ptr[R]; t < node_ptr[R + 1]; ++t) {
+      const int e = node_src[t] / 6;
+      for (int b = 0; b < 6; ++b) cand.push_back(conn[6 * e + b]);
+    }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    nbr.insert(nbr.end(), cand.begin(), cand.end());
+    rowptr[R + 1] = (int32_t)nbr.size();
+  }
+  // SELL-64
+  const int n_slices = (Nn + kSlice - 1) / kSlice;
+  std::vector<int32_t>& sb = h->h_slice_base;
+  sb.assign(n_slices + 1, 0);
+  std::vector<int32_t>& rl = h->h_row_len;
+  rl.assign(Nn, 0);
+  int64_t total = 0;
+  for (int s = 0; s < n_slices; ++s) {
+    int w = 0;
+    for (int R = s * kSlice; R < std::min(Nn, (s + 1) * kSlice); ++R) {
+      rl[R] = rowptr[R + 1] - rowptr[R];
+      w = std::max(w, rl[R]);
+    }
+    sb[s] = (int32_t)total;
+    total += (int64_t)w * kSlice;
+    if (total > INT32_MAX) {
+      set_error("mesh too large for 32-bit slot indices");
+      return TF_ERR_ARG;
+    }
+  }
+  sb[n_slices] = (int32_t)total;
+  std::vector<int32_t>& col = h->h_col;
+  col.assign(total, 0);
+  std::vector<int32_t> diag(Nn, -1);
+  for (int s = 0; s < n_slices; ++s) {
+    const int w = (sb[s + 1] - sb[s]) / kSlice;
+    for (int lane = 0; lane < kSlice; ++lane) {
+      const int R = s * kSlice + lane;
+      for (int k = 0; k < w; ++k) {
+        const int64_t slot = sb[s] + (int64_t)k * kSlice + lane;
+        if (R < Nn && k < rl[R]) {
+          col[slot] = nbr[rowptr[R] + k];
+          if (col[slot] == R) diag[R] = (int32_t)slot;
+        } else {
+          col[slot] = std::min(R, Nn - 1);
+        }
+      }
+    }
+  }
+  auto slot_of = [&](int R, int C) -> int64_t {
+    const int32_t* b = nbr.data() + rowptr[R];
+    const int32_t* e = nbr.data() + rowptr[R + 1];
+    const int32_t* p = std::lower_bound(b, e, C);
+    const int k = (int)(p - b);
+    return sb[R / kSlice] + (int64_t)k * kSlice + (R % kSlice);
+  };
+  // element block -> J slot contributions, in element order per slot
+  std::vector<int32_t> cptr(total + 1, 0);
+  std::vector<int32_t> slot_cache((size_t)Ne * 36);
+  for (int e = 0; e < Ne; ++e)
+    for (int A = 0; A < 6; ++A)
+      for (int B = 0; B < 6; ++B) {
+        const int R = conn[6 * e + B], C = conn[6 * e + A];
+        const int64_t slot = slot_of(R, C);
+        slot_cache[(size_t)e * 36 + A * 6 + B] = (int32_t)slot;
+        cptr[slot + 1]++;
+      }
+  for (int64_t i = 0; i < total; ++i) cptr[i + 1] += cptr[i];
+  std::vector<int32_t> csrc((size_t)Ne * 36);
+  {
+    std::vector<int32_t> pos(cptr.begin(), cptr.end() - 1);
+    for (int e = 0; e < Ne; ++e)
+      for (int ab = 0; ab < 36; ++ab) csrc[pos[slot_cache[(size_t)e * 36 + ab]]++] = e * 36 + ab;
+  }
+  // upload
+  DevPattern& P = h->pat;
+  P.n_slices = n_slices;
+  P.n_slots = total;
+  TF_CHECK(dalloc(h, &P.slice_base, n_slices + 1));
+  TF_CHECK(dalloc(h, &P.col, total));
+  TF_CHECK(dalloc(h, &P.diag_slot, Nn));
+  TF_CHECK(dalloc(h, &P.row_len, Nn));
+  TF_CHECK(dalloc(h, &P.contrib_ptr, total + 1));
+  TF_CHECK(dalloc(h, &P.contrib_src, (size_t)Ne * 36));
+  TF_CHECK(dalloc(h, &P.node_ptr, Nn + 1));
+  TF_CHECK(dalloc(h, &P.node_src, (size_t)Ne * 6));
+  TF_CHECK(hipMemcpy(P.slice_base, sb.data(), sizeof(int32_t) * (n_slices + 1), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P.col, col.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P.diag_slot, diag.data(), sizeof(int32_t) * Nn, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P.row_len, rl.data(), sizeof(int32_t) * Nn, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P.contrib_ptr, cptr.data(), sizeof(int32_t) * (total + 1), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P.contrib_src, csrc.data(), sizeof(int32_t) * csrc.size(), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P.node_ptr, node_ptr.data(), sizeof(int32_t) * (Nn + 1), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P.node_src, node_src.data(), sizeof(int32_t) * node_src.size(), hipMemcpyHostToDevice));
+  return TF_OK;
+}
+
+tf_status alloc_workspace(tf_handle* h) {
+  Workspace& w = h->ws;
+  const size_t n = 3 * (size_t)h->Nn;
+  TF_CHECK(dalloc(h, &w.elemKe, (size_t)h->Ne * 324));
+  TF_CHECK(dalloc(h, &w.elemFe, (size_t)h->Ne * 18));
+  TF_CHECK(dalloc(h, &w.elemFv, (size_t)h->Ne * 18));
+  TF_CHECK(dalloc(h, &w.val, (size_t)h->pat.n_slots * 9));
+  TF_CHECK(dalloc(h, &w.minv, (size_t)h->Nn * 9));
+  double** vecs[] = {&w.un, &w.uk, &w.g, &w.du, &w.internal, &w.r, &w.rh, &w.p, &w.v,
+                     &w.ph, &w.s, &w.sh, &w.t, &w.x, &w.b, &w.tmp};
+  for (double** v : vecs) TF_CHECK(dalloc(h, v, n));
+  TF_CHECK(dalloc(h, &w.partials, (size_t)kParts * 5));
+  TF_CHECK(dalloc(h, &w.scal, 16));
+  TF_CHECK(dalloc(h, &w.flag, 4));
+  TF_CHECK(hipHostMalloc((void**)&w.host_pinned, 16 * sizeof(double), hipHostMallocDefault));
+  TF_CHECK(hipHostMalloc((void**)&w.host_flag, 4 * sizeof(int32_t), hipHostMallocDefault));
+  TF_CHECK(hipMemset(w.elemFv, 0, sizeof(double) * (size_t)h->Ne * 18));
+  return TF_OK;
+}
+
+tf_status ensure_visc(tf_handle* h) {
+  if (!h->ws.elemKv) TF_CHECK(dalloc(h, &h->ws.elemKv, (size_t)h->Ne * 324));
+  return TF_OK;
+}
+
+tf_status ensure_mixed(tf_handle* h) {
+  Workspace& w = h->ws;
+  if (w.val32) return TF_OK;
+  const size_t n = 3 * (size_t)h->Nn;
+  TF_CHECK(dalloc(h, &w.elemKe32, (size_t)h->Ne * 324));
+  TF_CHECK(dalloc(h, &w.elemKv32, (size_t)h->Ne * 324));
+  TF_CHECK(dalloc(h, &w.val32, (size_t)h->pat.n_slots * 9));
+  TF_CHECK(dalloc(h, &w.minv32, (size_t)h->Nn * 9));
+  float** vecs[] = {&w.r32, &w.rh32, &w.p32, &w.v32, &w.ph32, &w.s32, &w.sh32, &w.t32, &w.x32};
+  for (float** v : vecs) TF_CHECK(dalloc(h, v, n));
+  return TF_OK;
+}
+
+KrylovArgs krylov_args(tf_handle* h) {
+  KrylovArgs k;
+  k.n_nodes = h->Nn;
+  k.pat = h->pat;
+  k.val = h->ws.val;
+  k.val32 = h->ws.val32;
+  k.minv = h->ws.minv;
+  k.minv32 = h->ws.minv32;
+  k.pairs = h->pairs;
+  k.n_pairs = h->n_pairs;
+  k.ws = &h->ws;
+  k.L = h->L;
+  k.prof = &h->prof;
+  return k;
+}
+
+// Element kernel + slot-map assembly + residual and binding of the residual.
+tf_status assemble(tf_handle* h, double dt, int mixed) {
+  if (!h->have_material || !h->have_growth || !h->have_area) {
+    set_error("tf_nr_set_material, tf_nr_set_growth and a viscosity surface must precede assembly");
+    return TF_ERR_ARG;
+  }
+  if (!(dt > 0.0)) {
+    set_error("dt must be positive");
+    return TF_ERR_ARG;
+  }
+  Workspace& w = h->ws;
+  if (mixed) {
+    tf_status s = ensure_mixed(h);
+    if (s != TF_OK) return s;
+  }
+  ElementArgs ea{h->Ne, h->conn, h->refpos, h->invFg, h->lam, h->mu, h->eta, w.uk, w.un, dt,
+                 h->any_visc ? 1 : 0, w.elemKe, w.elemKv, w.elemKe32, w.elemKv32, w.elemFe,
+                 w.elemFv};
+  TF_CHECK(hipEventRecord(h->ev[0], h->st));
+  TF_CHECK(launch_element(ea, mixed, h->st));
+  TF_CHECK(hipEventRecord(h->ev[1], h->st));
+  AssembleArgs aa;
+  aa.n_nodes = h->Nn;
+  aa.pat = h->pat;
+  aa.Ke = w.elemKe; aa.Kv = w.elemKv; aa.Ke32 = w.elemKe32; aa.Kv32 = w.elemKv32;
+  aa.Fe = w.elemFe; aa.Fv = w.elemFv;
+  aa.any_visc = h->any_visc ? 1 : 0;
+  aa.ext_visc = h->extv; aa.area = h->area; aa.dof_flags = h->dof_flags;
+  aa.uk = w.uk; aa.un = w.un; aa.dt = dt;
+  aa.val = mixed ? nullptr : w.val;
+  aa.val32 = w.val32;
+  aa.minv = w.minv; aa.minv32 = w.minv32;
+  aa.g = w.g; aa.internal = w.internal;
+  TF_CHECK(launch_assemble(aa, mixed, h->st));
+  TF_CHECK(launch_bind_forces(h->pairs, h->n_pairs, w.g, h->st));
+  TF_CHECK(hipEventRecord(h->ev[2], h->st));
+  h->assembled = true;
+  h->last_mixed = mixed;
+  return TF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tf_last_error(void) { return tf::g_err.c_str(); }
+
+tf_status tf_nr_create(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,
+                       const double* ref_pos, tf_handle** out) {
+  if (!out || n_nodes <= 0 || n_prisms <= 0 || !conn || !ref_pos) {
+    set_error("tf_nr_create: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  *out = nullptr;
+  int dev = 0;
+  tf_status s = check_device(&dev);
+  if (s != TF_OK) return s;
+  tf_handle* h = new tf_handle();
+  h->Nn = n_nodes;
+  h->Ne = n_prisms;
+  h->device = dev;
+  auto fail = [&](tf_status st) {
+    tf_nr_destroy(h);
+    return st;
+  };
+  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(TF_ERR_HIP);
+  }
+  for (auto& e : h->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(TF_ERR_HIP);
+  if ((s = build_pattern(h, conn)) != TF_OK) return fail(s);
+  if ((s = alloc_workspace(h)) != TF_OK) return fail(s);
+  const size_t Ne = n_prisms, Nn = n_nodes;
+  if (dalloc(h, &h->conn, Ne * 6) != hipSuccess || dalloc(h, &h->refpos, Ne * 18) != hipSuccess ||
+      dalloc(h, &h->invFg, Ne * 9) != hipSuccess || dalloc(h, &h->lam, Ne) != hipSuccess ||
+      dalloc(h, &h->mu, Ne) != hipSuccess || dalloc(h, &h->eta, Ne) != hipSuccess ||
+      dalloc(h, &h->extv, Nn * 3) != hipSuccess || dalloc(h, &h->area, Nn) != hipSuccess ||
+      dalloc(h, &h->dof_flags, Nn * 3) != hipSuccess) {
+    set_error("hipMalloc failed");
+    return fail(TF_ERR_HIP);
+  }
+  if (hipMemcpy(h->conn, conn, Ne * 6 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->refpos, ref_pos, Ne * 18 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(h->dof_flags, 0, Nn * 3) != hipSuccess) {
+    set_error("upload failed");
+    return fail(TF_ERR_HIP);
+  }
+  int grid = (int)std::min<int64_t>(kParts, ((int64_t)n_nodes + kBlock - 1) / kBlock);
+  grid = std::max(8, (grid + 7) / 8 * 8);
+  h->L.stream = h->st;
+  h->L.grid = grid;
+  *out = h;
+  return TF_OK;
+}
+
+tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, const double* eta,
+                             const double* ext_visc, const uint8_t* fixed,
+                             const int32_t* slave_master, int32_t n_pairs) {
+  if (!h || !E || !nu || !eta || !ext_visc || n_pairs < 0 || (n_pairs > 0 && !slave_master)) {
+    set_error("tf_nr_set_material: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  const int Ne = h->Ne, Nn = h->Nn;
+  std::vector<double> lam(Ne), mu(Ne);
+  bool any_visc = false;
+  for (int e = 0; e < Ne; ++e) {
+    // Prism::setElasticProperties (Prism.cpp:512-513)
+    const double v = nu[e];
+    lam[e] = E[e] * v / (1 + v) / (1 - 2.0 * v);
+    mu[e] = E[e] / 2.0 / (1 + v);
+    if (eta[e] != 0) any_visc = true;
+  }
+  std::vector<uint8_t> flags((size_t)Nn * 3, 0);
+  if (fixed)
+    for (size_t i = 0; i < flags.size(); ++i) flags[i] = fixed[i] ? 1 : 0;
+  h->h_pairs.assign(slave_master, slave_master + 2 * (size_t)n_pairs);
+  for (int p = 0; p < n_pairs; ++p) {
+    const int s = slave_master[2 * p], m = slave_master[2 * p + 1];
+    if (s < 0 || m < 0 || s >= 3 * Nn || m >= 3 * Nn || s == m) {
+      set_error("slave/master pair out of range");
+      return TF_ERR_ARG;
+    }
+    flags[s] |= 2;
+  }
+  TF_CHECK(hipMemcpy(h->lam, lam.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->mu, mu.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->eta, eta, sizeof(double) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->extv, ext_visc, sizeof(double) * Nn * 3, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->dof_flags, flags.data(), flags.size(), hipMemcpyHostToDevice));
+  if (n_pairs > h->n_pairs || !h->pairs) {
+    int32_t* p = nullptr;
+    TF_CHECK(dalloc(h, &p, 2 * (size_t)std::max(n_pairs, 1)));
+    h->pairs = p;
+  }
+  if (n_pairs > 0)
+    TF_CHECK(hipMemcpy(h->pairs, slave_master, sizeof(int32_t) * 2 * n_pairs, hipMemcpyHostToDevice));
+  h->n_pairs = n_pairs;
+  h->any_visc = any_visc;
+  if (any_visc) {
+    tf_status s = ensure_visc(h);
+    if (s != TF_OK) return s;
+  }
+  h->have_material = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_set_growth(tf_handle* h, const double* inv_Fg) {
+  if (!h || !inv_Fg) {
+    set_error("tf_nr_set_growth: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipMemcpy(h->invFg, inv_Fg, sizeof(double) * 9 * h->Ne, hipMemcpyHostToDevice));
+  h->have_growth = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_set_visc_surface(tf_handle* h, const double* area) {
+  if (!h || !area) {
+    set_error("tf_nr_set_visc_surface: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipMemcpy(h->area, area, sizeof(double) * h->Nn, hipMemcpyHostToDevice));
+  h->have_area = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u, const int32_t* elem_placement,
+                                    const int32_t* elem_type, const uint8_t* spans,
+                                    double* area_out) {
+  if (!h || !u || !elem_placement || !elem_type || !spans) {
+    set_error("tf_nr_update_visc_surface: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  const size_t Ne = h->Ne;
+  int32_t *pl = nullptr, *ty = nullptr;
+  uint8_t* sp = nullptr;
+  TF_CHECK(hipMalloc(&pl, Ne * sizeof(int32_t)));
+  TF_CHECK(hipMalloc(&ty, Ne * sizeof(int32_t)));
+  TF_CHECK(hipMalloc(&sp, Ne));
+  TF_CHECK(hipMemcpy(pl, elem_placement, Ne * sizeof(int32_t), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(ty, elem_type, Ne * sizeof(int32_t), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(sp, spans, Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->ws.tmp, u, sizeof(double) * 3 * h->Nn, hipMemcpyHostToDevice));
+  TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, h->ws.tmp, h->pat.node_ptr,
+                               h->pat.node_src, pl, ty, sp, h->area, h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  if (area_out)
+    TF_CHECK(hipMemcpy(area_out, h->area, sizeof(double) * h->Nn, hipMemcpyDeviceToHost));
+  TF_CHECK(hipFree(pl));
+  TF_CHECK(hipFree(ty));
+  TF_CHECK(hipFree(sp));
+  h->have_area = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_assemble(tf_handle* h, const double* uk, const double* un, double dt,
+                         double* g_out, double* internal_out) {
+  if (!h || !uk || !un) {
+    set_error("tf_nr_assemble: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  const size_t n = 3 * (size_t)h->Nn;
+  TF_CHECK(hipMemcpyAsync(h->ws.uk, uk, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  TF_CHECK(hipMemcpyAsync(h->ws.un, un, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  tf_status s = assemble(h, dt, 0);
+  if (s != TF_OK) return s;
+  if (g_out) TF_CHECK(hipMemcpyAsync(g_out, h->ws.g, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+  if (internal_out)
+    TF_CHECK(hipMemcpyAsync(internal_out, h->ws.internal, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  return TF_OK;
+}
+
+tf_status tf_nr_solve(tf_handle* h, const double* b, double* x, double rtol, int32_t maxit,
+                      int32_t* iterations) {
+  if (!h || !b || !x || maxit <= 0) {
+    set_error("tf_nr_solve: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (!h->assembled || h->last_mixed) {
+    set_error("tf_nr_solve needs a preceding FP64 tf_nr_assemble");
+    return TF_ERR_ARG;
+  }
+  const size_t n = 3 * (size_t)h->Nn;
+  TF_CHECK(hipMemcpyAsync(h->ws.b, b, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  KrylovArgs k = krylov_args(h);
+  int it = 0, state = 0;
+  TF_CHECK(krylov_bicgstab(k, h->ws.b, h->ws.x, rtol > 0 ? rtol : 1e-10, maxit, &it, &state, 16));
+  TF_CHECK(hipMemcpyAsync(x, h->ws.x, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  if (iterations) *iterations = it;
+  if (state != K_CONVERGED) {
+    set_error(state == K_BREAKDOWN ? "BiCGStab breakdown" : "BiCGStab iteration cap reached");
+    return TF_ERR_KRYLOV;
+  }
+  return TF_OK;
+}
+
+tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int32_t maxit,
+                     int32_t precision_mode, double krylov_rtol, double* u_out,
+                     tf_step_stats* stats) {
+  if (!h || !u_n || !u_out || maxit < 0 || maxit > TF_MAX_NR_ITERS - 2 ||
+      (precision_mode != TF_PREC_FP64 && precision_mode != TF_PREC_MIXED)) {
+    set_error("tf_nr_step: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  tf_step_stats st_local;
+  tf_step_stats* S = stats ? stats : &st_local;
+  std::memset(S, 0, sizeof(*S));
+  int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
+  if (mixed && h->n_pairs > 0) mixed = 0;  // bound systems use the FP64 operator
+  const size_t n = 3 * (size_t)h->Nn;
+  Workspace& w = h->ws;
+  TF_CHECK(hipMemcpyAsync(w.un, u_n, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  TF_CHECK(hipMemcpyAsync(w.uk, u_n, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
+  const int kmax = 50000;
+  int it = 0;
+  bool conv = false, hit_cap = false;
+  tf_status result = TF_OK;
+  KrylovArgs k = krylov_args(h);
+  while (!conv) {
+    tf_status s = assemble(h, dt, mixed);
+    if (s != TF_OK) return s;
+    k = krylov_args(h);
+    int kit = 0, state = 0;
+    if (mixed)
+      TF_CHECK(krylov_mixed(k, w.g, w.du, rtol, 1e-4, kmax, &kit, &state, 16));
+    else
+      TF_CHECK(krylov_bicgstab(k, w.g, w.du, rtol, kmax, &kit, &state, 16));
+    TF_CHECK(launch_bind_increment(h->pairs, h->n_pairs, w.du, h->st));
+    TF_CHECK(hipEventRecord(h->ev[3], h->st));
+    double dnorm = 0.0;
+    TF_CHECK(norm_and_update(h->Nn, w.du, w.uk, &w, h->L, &dnorm));
+    float t_el = 0, t_as = 0, t_kr = 0;
+    TF_CHECK(hipEventElapsedTime(&t_el, h->ev[0], h->ev[1]));
+    TF_CHECK(hipEventElapsedTime(&t_as, h->ev[1], h->ev[2]));
+    TF_CHECK(hipEventElapsedTime(&t_kr, h->ev[2], h->ev[3]));
+    S->ms_element += t_el;
+    S->ms_assembly += t_as;
+    if (h->prof.on) {
+      h->prof_el_ms += t_el;
+      h->prof_as_ms += t_as;
+      h->prof_assemblies++;
+    }
+    S->ms_krylov += t_kr;
+    S->krylov_iterations[it] = kit;
+    S->krylov_iterations_total += kit;
+    S->du_norms[it] = dnorm;
+    if (state == K_BREAKDOWN) {
+      set_error("BiCGStab breakdown in NR iteration " + std::to_string(it));
+      result = TF_ERR_KRYLOV;
+    } else if (state != K_CONVERGED) {
+      set_error("BiCGStab iteration cap in NR iteration " + std::to_string(it));
+      result = TF_ERR_KRYLOV;
+    }
+    if (!std::isfinite(dnorm)) {
+      set_error("NaN/Inf in the NR increment at iteration " + std::to_string(it));
+      result = TF_ERR_NAN;
+    }
+    // NewtonRaphsonSolver::checkConvergenceViaDeltaU (NewtonRaphsonSolver.cpp:807-825);
+    // the increment is applied whether or not it converged (Simulation.cpp:6716-6720)
+    conv = !(dnorm > tol);
+    it++;
+    if (result != TF_OK) break;
+    if (!conv && it > maxit) {
+      conv = true;
+      hit_cap = true;
+    }
+  }
+  S->ms_total = S->ms_element + S->ms_assembly + S->ms_krylov;
+  TF_CHECK(hipMemcpyAsync(u_out, w.uk, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  S->nr_iterations = it;
+  S->converged = (conv && !hit_cap && result == TF_OK) ? 1 : 0;
+  if (result == TF_OK && hit_cap) {
+    set_error("NR did not converge within maxit");
+    result = TF_ERR_NOT_CONVERGED;
+  }
+  S->status = result;
+  return result;
+}
+
+tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz) {
+  if (!h || !nnz) {
+    set_error("tf_csr_pattern: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  const int Nn = h->Nn;
+  const int64_t total = 9 * (int64_t)h->h_nbr.size();
+  if (!ja) {
+    *nnz = total;
+    return TF_OK;
+  }
+  if (*nnz < total) {
+    set_error("tf_csr_pattern: ja too small");
+    return TF_ERR_ARG;
+  }
+  int64_t c = 0;
+  for (int R = 0; R < Nn; ++R)
+    for (int i = 0; i < 3; ++i) {
+      if (ia) ia[3 * R + i] = (int32_t)c;
+      for (int k = h->h_rowptr[R]; k < h->h_rowptr[R + 1]; ++k)
+        for (int j = 0; j < 3; ++j) ja[c++] = 3 * h->h_nbr[k] + j;
+    }
+  if (ia) ia[3 * Nn] = (int32_t)c;
+  *nnz = c;
+  return TF_OK;
+}
+
+tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja, double* a,
+                        int64_t* nnz) {
+  if (!h || !nnz || !ia) {
+    set_error("tf_nr_get_csr: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (!h->assembled || h->last_mixed) {
+    set_error("tf_nr_get_csr needs a preceding FP64 tf_nr_assemble");
+    return TF_ERR_ARG;
+  }
+  const int Nn = h->Nn;
+  const int64_t S = h->pat.n_slots;
+  std::vector<double> val((size_t)S * 9);
+  TF_CHECK(hipMemcpy(val.data(), h->ws.val, sizeof(double) * val.size(), hipMemcpyDeviceToHost));
+  auto slot_of = [&](int R, int C) -> int64_t {
+    const int32_t* b = h->h_nbr.data() + h->h_rowptr[R];
+    const int32_t* e = h->h_nbr.data() + h->h_rowptr[R + 1];
+    const int32_t* p = std::lower_bound(b, e, C);
+    return h->h_slice_base[R / kSlice] + (int64_t)(p - b) * kSlice + (R % kSlice);
+  };
+  // K(3R+i, 3C+j) = J(3C+j, 3R+i) = J block (C, R)[j][i]
+  const int n = 3 * Nn;
+  std::vector<std::map<int, double>> rows;
+  const bool bind = h->n_pairs > 0;
+  if (bind) rows.resize(n);
+  std::vector<int32_t> tja;
+  std::vector<double> ta;
+  std::vector<int32_t> tia(n + 1, 0);
+  for (int R = 0; R < Nn; ++R)
+    for (int i = 0; i < 3; ++i) {
+      tia[3 * R + i] = (int32_t)tja.size();
+      for (int k = h->h_rowptr[R]; k < h->h_rowptr[R + 1]; ++k) {
+        const int C = h->h_nbr[k];
+        const int64_t slot = slot_of(C, R);
+        for (int j = 0; j < 3; ++j) {
+          const double v = val[(size_t)(3 * j + i) * S + slot];
+          if (bind) {
+            rows[3 * R + i][3 * C + j] = v;
+          } else if (v > threshold || v < -threshold) {
+            tja.push_back(3 * C + j);
+            ta.push_back(v);
+          }
+        }
+      }
+    }
+  tia[n] = (int32_t)tja.size();
+  if (bind) {
+    // calculateBoundKWithSlavesMasterDoF on the sparse rows (NewtonRaphsonSolver.cpp:148-180)
+    std::vector<std::map<int, double>> cols(n);
+    for (int r = 0; r < n; ++r)
+      for (auto& kv : rows[r]) cols[kv.first][r] = kv.second;
+    auto get = [&](int r, int c) -> double {
+      auto it = rows[r].find(c);
+      return it == rows[r].end() ? 0.0 : it->second;
+    };
+    auto set = [&](int r, int c, double v) {
+      rows[r][c] = v;
+      cols[c][r] = v;
+    };
+    for (int p = 0; p < h->n_pairs; ++p) {
+      const int s = h->h_pairs[2 * p], m = h->h_pairs[2 * p + 1];
+      std::vector<std::pair<int, double>> srow(rows[s].begin(), rows[s].end());
+      for (auto& kv : srow) set(m, kv.first, get(m, kv.first) + kv.second);
+      for (auto& kv : srow) set(s, kv.first, 0.0);
+      std::vector<std::pair<int, double>> scol(cols[s].begin(), cols[s].end());
+      for (auto& kv : scol) set(kv.first, m, get(kv.first, m) + kv.second);
+      for (auto& kv : scol) set(kv.first, s, 0.0);
+      set(s, s, 1.0);
+    }
+    tja.clear();
+    ta.clear();
+    for (int r = 0; r < n; ++r) {
+      tia[r] = (int32_t)tja.size();
+      for (auto& kv : rows[r])
+        if (kv.second > threshold || kv.second < -threshold) {
+          tja.push_back(kv.first);
+          ta.push_back(kv.second);
+        }
+    }
+    tia[n] = (int32_t)tja.size();
+  }
+  const int64_t total = (int64_t)tja.size();
+  std::memcpy(ia, tia.data(), sizeof(int32_t) * (n + 1));
+  if (!ja) {
+    *nnz = total;
+    return TF_OK;
+  }
+  if (*nnz < total) {
+    set_error("tf_nr_get_csr: output too small");
+    return TF_ERR_ARG;
+  }
+  std::memcpy(ja, tja.data(), sizeof(int32_t) * total);
+  if (a) std::memcpy(a, ta.data(), sizeof(double) * total);
+  *nnz = total;
+  return TF_OK;
+}
+
+tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke, double* Kv) {
+  if (!h) {
+    set_error("tf_nr_element_outputs: invalid handle");
+    return TF_ERR_ARG;
+  }
+  if (!h->assembled || h->last_mixed) {
+    set_error("tf_nr_element_outputs needs a preceding FP64 tf_nr_assemble");
+    return TF_ERR_ARG;
+  }
+  const size_t Ne = h->Ne;
+  if (ge) TF_CHECK(hipMemcpy(ge, h->ws.elemFe, sizeof(double) * Ne * 18, hipMemcpyDeviceToHost));
+  if (gv) TF_CHECK(hipMemcpy(gv, h->ws.elemFv, sizeof(double) * Ne * 18, hipMemcpyDeviceToHost));
+  std::vector<double> blk(Ne * 324);
+  auto unpack = [&](double* out) {
+    for (size_t e = 0; e < Ne; ++e)
+      for (int A = 0; A < 6; ++A)
+        for (int B = 0; B < 6; ++B)
+          for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+              out[e * 324 + (3 * A + r) * 18 + 3 * B + c] = blk[((e * 36) + A * 6 + B) * 9 + 3 * r + c];
+  };
+  if (Ke) {
+    TF_CHECK(hipMemcpy(blk.data(), h->ws.elemKe, sizeof(double) * blk.size(), hipMemcpyDeviceToHost));
+    unpack(Ke);
+  }
+  if (Kv) {
+    if (h->any_visc) {
+      TF_CHECK(hipMemcpy(blk.data(), h->ws.elemKv, sizeof(double) * blk.size(), hipMemcpyDeviceToHost));
+      unpack(Kv);
+    } else {
+      std::memset(Kv, 0, sizeof(double) * Ne * 324);
+    }
+  }
+  return TF_OK;
+}
+
+tf_status tf_nr_profile(tf_handle* h, int32_t enable) {
+  if (!h) {
+    set_error("tf_nr_profile: invalid handle");
+    return TF_ERR_ARG;
+  }
+  if (enable && !h->prof.ev[0])
+    for (auto& e : h->prof.ev) TF_CHECK(hipEventCreate(&e));
+  h->prof.on = enable ? 1 : 0;
+  h->prof.used = 0;
+  h->prof.ms[0] = h->prof.ms[1] = 0.0;
+  h->prof.count[0] = h->prof.count[1] = 0;
+  h->prof_el_ms = h->prof_as_ms = 0.0;
+  h->prof_assemblies = 0;
+  return TF_OK;
+}
+
+tf_status tf_nr_kernel_times(tf_handle* h, tf_kernel_times* out) {
+  if (!h || !out) {
+    set_error("tf_nr_kernel_times: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  prof_harvest(&h->prof);
+  for (int i = 0; i < 2; ++i) {
+    out->spmv_ms[i] = h->prof.ms[i];
+    out->spmv_launches[i] = h->prof.count[i];
+  }
+  out->element_ms = h->prof_el_ms;
+  out->assembly_ms = h->prof_as_ms;
+  out->assemblies = h->prof_assemblies;
+  return TF_OK;
+}
+
+tf_status tf_nr_pattern_info(tf_handle* h, int64_t* n_blocks, int64_t* n_slots) {
+  if (!h) {
+    set_error("tf_nr_pattern_info: invalid handle");
+    return TF_ERR_ARG;
+  }
+  if (n_blocks) *n_blocks = (int64_t)h->h_nbr.size();
+  if (n_slots) *n_slots = h->pat.n_slots;
+  return TF_OK;
+}
+
+tf_status tf_nr_destroy(tf_handle* h) {
+  if (!h) return TF_OK;
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  for (auto& e : h->prof.ev)
+    if (e) (void)hipEventDestroy(e);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->ws.host_pinned) (void)hipHostFree(h->ws.host_pinned);
+  if (h->ws.host_flag) (void)hipHostFree(h->ws.host_flag);
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+  return TF_OK;
+}
+
+}  // extern "C"

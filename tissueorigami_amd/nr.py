@@ -1,0 +1,242 @@
+"""Python binding of libtissue_nr.so (the C ABI of include/tissue_nr.h).
+
+The class mirrors the calls Simulation::updateStepNR makes on the reference's
+NewtonRaphsonSolver (TissueFolding/SourceCode/NewtonRaphsonSolver.h:51-90) at
+the granularity the C ABI offers: set-up once per mesh, per-step inputs,
+tf_nr_step for a whole implicit step, and assemble / solve / get_csr for
+testing the pieces.  There is no CPU fallback: loading fails loudly when the
+HIP library has not been built, and every call raises on a non-zero status.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtissue_nr.so")
+
+TF_OK, TF_ERR_HIP, TF_ERR_NAN, TF_ERR_KRYLOV, TF_ERR_NOT_CONVERGED, TF_ERR_ARG, TF_ERR_NO_DEVICE = range(7)
+PREC = {"fp64": 0, "mixed": 1}
+MAX_NR = 32
+
+EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_visc_surface",
+           "tf_nr_update_visc_surface", "tf_nr_step", "tf_nr_assemble", "tf_nr_solve",
+           "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_profile",
+           "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_nr_destroy", "tf_last_error"]
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("nr_iterations", ctypes.c_int32),
+                ("converged", ctypes.c_int32), ("krylov_iterations_total", ctypes.c_int32),
+                ("krylov_iterations", ctypes.c_int32 * MAX_NR),
+                ("du_norms", ctypes.c_double * MAX_NR),
+                ("ms_element", ctypes.c_double), ("ms_assembly", ctypes.c_double),
+                ("ms_krylov", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        n = self.nr_iterations
+        return dict(status=self.status, nr_iterations=n, converged=bool(self.converged),
+                    krylov_iterations_total=self.krylov_iterations_total,
+                    krylov_iterations=list(self.krylov_iterations[:n]),
+                    du_norms=list(self.du_norms[:n]), ms_element=self.ms_element,
+                    ms_assembly=self.ms_assembly, ms_krylov=self.ms_krylov,
+                    ms_total=self.ms_total)
+
+
+class KernelTimes(ctypes.Structure):
+    _fields_ = [("spmv_ms", ctypes.c_double * 2), ("spmv_launches", ctypes.c_int64 * 2),
+                ("element_ms", ctypes.c_double), ("assembly_ms", ctypes.c_double),
+                ("assemblies", ctypes.c_int64)]
+
+
+class TFError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"tissue_nr status {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int32)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+_h = ctypes.c_void_p
+
+
+def load():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with python -m tissueorigami_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    L.tf_nr_create.argtypes = [ctypes.c_int32, ctypes.c_int32, _i, _d, ctypes.POINTER(_h)]
+    L.tf_nr_set_material.argtypes = [_h, _d, _d, _d, _d, _u8, _i, ctypes.c_int32]
+    L.tf_nr_set_growth.argtypes = [_h, _d]
+    L.tf_nr_set_visc_surface.argtypes = [_h, _d]
+    L.tf_nr_update_visc_surface.argtypes = [_h, _d, _i, _i, _u8, _d]
+    L.tf_nr_step.argtypes = [_h, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                             ctypes.c_int32, ctypes.c_double, _d, ctypes.POINTER(StepStats)]
+    L.tf_nr_assemble.argtypes = [_h, _d, _d, ctypes.c_double, _d, _d]
+    L.tf_nr_solve.argtypes = [_h, _d, _d, ctypes.c_double, ctypes.c_int32, _i]
+    L.tf_csr_pattern.argtypes = [_h, _i, _i, _i64]
+    L.tf_nr_get_csr.argtypes = [_h, ctypes.c_double, _i, _i, _d, _i64]
+    L.tf_nr_element_outputs.argtypes = [_h, _d, _d, _d, _d]
+    L.tf_nr_profile.argtypes = [_h, ctypes.c_int32]
+    L.tf_nr_kernel_times.argtypes = [_h, ctypes.POINTER(KernelTimes)]
+    L.tf_nr_pattern_info.argtypes = [_h, _i64, _i64]
+    L.tf_nr_destroy.argtypes = [_h]
+    L.tf_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS[:-1]:
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _check(code):
+    if code != TF_OK:
+        raise TFError(code, load().tf_last_error().decode())
+
+
+class NewtonRaphsonSolver:
+    """One tissue on one device (a tf_handle)."""
+
+    def __init__(self, n_nodes, conn, refpos):
+        L = load()
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 6)
+        self.n_nodes = int(n_nodes)
+        self.n_prisms = self.conn.shape[0]
+        refpos = _f64(refpos).reshape(self.n_prisms, 18)
+        h = _h()
+        _check(L.tf_nr_create(self.n_nodes, self.n_prisms, _p(self.conn, _i), _p(refpos, _d),
+                              ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().tf_nr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- per-step inputs
+    def set_material(self, E, nu, eta, ext_visc, fixed=None, pairs=None):
+        E = _f64(np.broadcast_to(E, (self.n_prisms,)))
+        nu = _f64(np.broadcast_to(nu, (self.n_prisms,)))
+        eta = _f64(np.broadcast_to(eta, (self.n_prisms,)))
+        ev = _f64(np.broadcast_to(ext_visc, (self.n_nodes, 3)))
+        fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8).reshape(-1)
+        pr = None if pairs is None or len(pairs) == 0 else \
+            np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        n_pairs = 0 if pr is None else pr.shape[0]
+        _check(load().tf_nr_set_material(self._h, _p(E, _d), _p(nu, _d), _p(eta, _d),
+                                         _p(ev, _d), _p(fx, _u8), _p(pr, _i), n_pairs))
+
+    def set_growth(self, inv_Fg):
+        a = _f64(inv_Fg).reshape(self.n_prisms, 9)
+        _check(load().tf_nr_set_growth(self._h, _p(a, _d)))
+
+    def set_visc_surface(self, area):
+        a = _f64(area).reshape(self.n_nodes)
+        _check(load().tf_nr_set_visc_surface(self._h, _p(a, _d)))
+
+    def update_visc_surface(self, u, elem_placement, elem_type, spans):
+        u = _f64(u).reshape(-1)
+        pl = np.ascontiguousarray(elem_placement, dtype=np.int32)
+        ty = np.ascontiguousarray(elem_type, dtype=np.int32)
+        sp = np.ascontiguousarray(spans, dtype=np.uint8)
+        out = np.empty(self.n_nodes)
+        _check(load().tf_nr_update_visc_surface(self._h, _p(u, _d), _p(pl, _i), _p(ty, _i),
+                                                _p(sp, _u8), _p(out, _d)))
+        return out
+
+    # ---- the step
+    def step(self, u_n, dt, tol=1e-8, maxit=20, precision="fp64", krylov_rtol=1e-10,
+             allow_not_converged=False):
+        u_n = _f64(u_n).reshape(-1)
+        out = np.empty_like(u_n)
+        st = StepStats()
+        code = load().tf_nr_step(self._h, _p(u_n, _d), float(dt), float(tol), int(maxit),
+                                 PREC[precision], float(krylov_rtol), _p(out, _d),
+                                 ctypes.byref(st))
+        if code == TF_ERR_NOT_CONVERGED and allow_not_converged:
+            return out, st.as_dict()
+        _check(code)
+        return out, st.as_dict()
+
+    # ---- pieces
+    def assemble(self, uk, un, dt):
+        uk = _f64(uk).reshape(-1); un = _f64(un).reshape(-1)
+        g = np.empty_like(uk); gi = np.empty_like(uk)
+        _check(load().tf_nr_assemble(self._h, _p(uk, _d), _p(un, _d), float(dt), _p(g, _d),
+                                     _p(gi, _d)))
+        return g, gi
+
+    def solve(self, b, rtol=1e-10, maxit=20000):
+        b = _f64(b).reshape(-1)
+        x = np.empty_like(b)
+        it = np.zeros(1, np.int32)
+        _check(load().tf_nr_solve(self._h, _p(b, _d), _p(x, _d), float(rtol), int(maxit),
+                                  _p(it, _i)))
+        return x, int(it[0])
+
+    def csr_pattern(self):
+        n = 3 * self.n_nodes
+        nnz = np.zeros(1, np.int64)
+        _check(load().tf_csr_pattern(self._h, None, None, _p(nnz, _i64)))
+        ia = np.empty(n + 1, np.int32); ja = np.empty(int(nnz[0]), np.int32)
+        _check(load().tf_csr_pattern(self._h, _p(ia, _i), _p(ja, _i), _p(nnz, _i64)))
+        return ia, ja
+
+    def get_csr(self, threshold=1e-13):
+        n = 3 * self.n_nodes
+        ia = np.empty(n + 1, np.int32)
+        nnz = np.zeros(1, np.int64)
+        _check(load().tf_nr_get_csr(self._h, float(threshold), _p(ia, _i), None, None,
+                                    _p(nnz, _i64)))
+        ja = np.empty(int(nnz[0]), np.int32); a = np.empty(int(nnz[0]))
+        _check(load().tf_nr_get_csr(self._h, float(threshold), _p(ia, _i), _p(ja, _i), _p(a, _d),
+                                    _p(nnz, _i64)))
+        return ia, ja, a
+
+    def profile(self, enable=True):
+        _check(load().tf_nr_profile(self._h, 1 if enable else 0))
+
+    def kernel_times(self):
+        t = KernelTimes()
+        _check(load().tf_nr_kernel_times(self._h, ctypes.byref(t)))
+        return dict(spmv_ms_fp64=t.spmv_ms[0], spmv_ms_fp32=t.spmv_ms[1],
+                    spmv_launches_fp64=t.spmv_launches[0], spmv_launches_fp32=t.spmv_launches[1],
+                    element_ms=t.element_ms, assembly_ms=t.assembly_ms, assemblies=t.assemblies)
+
+    def pattern_info(self):
+        nb = np.zeros(1, np.int64); ns = np.zeros(1, np.int64)
+        _check(load().tf_nr_pattern_info(self._h, _p(nb, _i64), _p(ns, _i64)))
+        return int(nb[0]), int(ns[0])
+
+    def element_outputs(self):
+        ne = self.n_prisms
+        ge = np.empty((ne, 18)); gv = np.empty((ne, 18))
+        Ke = np.empty((ne, 18, 18)); Kv = np.empty((ne, 18, 18))
+        _check(load().tf_nr_element_outputs(self._h, _p(ge, _d), _p(gv, _d), _p(Ke, _d),
+                                            _p(Kv, _d)))
+        return ge, gv, Ke, Kv
