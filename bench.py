@@ -96,6 +96,9 @@ def run_steps(solver, w, u0, Fg0, n_steps, precision, profile=False):
         tot["converged"] &= st["converged"]
         tot["per_step"].append(dict(nr=st["nr_iterations"], krylov=st["krylov_iterations"],
                                     du=[float("%.3e" % d) for d in st["du_norms"]]))
+        print(f"[bench]   {precision} step: NR {st['nr_iterations']} krylov {st['krylov_iterations']} "
+              f"du {['%.2e' % d for d in st['du_norms']]} {time.perf_counter() - t0:.2f} s",
+              file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     kt = solver.kernel_times() if profile else None
     if profile:
