@@ -158,6 +158,53 @@ tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja
  * reference's storage orientation (n_prisms x 324).  Any output may be NULL. */
 tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke, double* Kv);
 
+/* Device-resident time stepping with the pre-step producers on the device.
+ * tf_nr_set_growth_rates: per-prism growth rates (n_prisms x 3, x/y/z, 1/s)
+ *   and F_g reset to identity.  Every step of tf_nr_advance applies
+ *   ShapeBase::calculateFgFromRates (ShapeBase.cpp:471-505, no rotation) and
+ *   ShapeBase::growShapeByFg (ShapeBase.cpp:1452-1505): F_g <- diag(exp(r dt)) F_g,
+ *   InvFg = inv(F_g).
+ * tf_nr_growth_tensor: reads (Fg_out) and/or sets (Fg_in) F_g (n_prisms x 9).
+ * tf_nr_set_surface_flags: per-prism placement / type / spans-whole-tissue
+ *   used to recompute the node viscosity surfaces on the device at the start
+ *   of every step (Simulation::updateNodeViscositySurfaces, Simulation.cpp:7598-7621).
+ * tf_nr_advance: n_steps implicit steps; the start positions are u0 or, when
+ *   u0 is NULL, the device-resident result of the previous call.  stats (may be
+ *   NULL) receives one entry per step; u_out (may be NULL) the final positions.
+ *   Replaces the per-step sequence of Simulation::runOneStep
+ *   (Simulation.cpp:6012-6112) restricted to growth, surfaces and updateStepNR. */
+tf_status tf_nr_set_growth_rates(tf_handle* h, const double* rates);
+tf_status tf_nr_growth_tensor(tf_handle* h, double* Fg_out, const double* Fg_in);
+tf_status tf_nr_set_surface_flags(tf_handle* h, const int32_t* elem_placement,
+                                  const int32_t* elem_type, const uint8_t* spans);
+tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double dt, double tol,
+                        int32_t maxit, int32_t precision_mode, double krylov_rtol, double* u_out,
+                        tf_step_stats* stats);
+
+/* Adjoint of n_steps implicit steps with respect to one growth parameter r_e
+ * per prism.  No counterpart in the reference (which has no derivatives); the
+ * forward is n_steps of tf_nr_step (FP64) that also keeps every step's
+ * converged positions on the device as checkpoints.
+ *   tf_adj_forward: u0 (3 x n_nodes); inv_Fg_steps (n_steps x n_prisms x 9)
+ *     is the InvFg of each step; the node viscosity surfaces of each step are
+ *     computed on the device from the step's start positions with the given
+ *     per-prism placement / type / spans flags (as tf_nr_update_visc_surface).
+ *   tf_adj_backward: given dL/du_final (3 x n_nodes) and dinv_Fg_steps, the
+ *     derivative of every step's InvFg_e with respect to r_e, returns
+ *     dL_dr (n_prisms) by the implicit-function theorem per step: one
+ *     transposed-Jacobian solve J^T mu = a per step (relative residual
+ *     krylov_rtol, 0 selects 1e-12) and hand-derived dg/dInvFg and
+ *     dg/du_{n-1} (internal viscosity, external drag and its surface areas).
+ * Slave/master binding is not supported (TF_ERR_ARG). */
+tf_status tf_adj_forward(tf_handle* h, int32_t n_steps, const double* u0,
+                         const double* inv_Fg_steps, double dt, double tol, int32_t maxit,
+                         double krylov_rtol, const int32_t* elem_placement,
+                         const int32_t* elem_type, const uint8_t* spans, double* u_final,
+                         tf_step_stats* stats);
+tf_status tf_adj_backward(tf_handle* h, const double* inv_Fg_steps, const double* dinv_Fg_steps,
+                          const double* dL_du_final, double krylov_rtol, double* dL_dr,
+                          tf_step_stats* stats);
+
 /* Device-time accounting of the hot kernels, for measurement.  The reference
  * has no timers (no counterpart); these exist for the roofline figures.
  * tf_nr_profile(h, 1) resets the counters and brackets every SpMV launch

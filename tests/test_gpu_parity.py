@@ -194,3 +194,79 @@ def test_step_is_bitwise_reproducible(nr):
     assert np.array_equal(u1, u2)
     assert st1["krylov_iterations"] == st2["krylov_iterations"]
     s.close()
+
+
+@pytest.mark.parametrize("krylov_rtol", [1e-5, 1e-10])
+def test_mixed_precision_step_matches_fp64(orc, nr, krylov_rtol):
+    """FP32 tangents / SpMV / block Jacobi with the FP64 residual converge to
+    the FP64 root: positions within 1e-8 relative of the FP64 step and of the
+    oracle's dense step."""
+    p = problems.perturbed_problem(seed=9, visc=True, fixed=True, lx=6.0, ly=5.0, layers=3)
+    s = _gpu_solver(nr, p)
+    u64, st64 = s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-10)
+    u32, st32 = s.step(p["un"], p["dt"], 1e-8, 20, "mixed", krylov_rtol)
+    uo, du, conv = _oracle_problem(orc, p).nr_step_dense(p["un"], p["dt"], 1e-8, 20)
+    assert st64["converged"] and st32["converged"] and conv
+    scale = np.abs(uo).max()
+    assert np.abs(u32 - u64).max() <= 1e-8 * scale
+    assert np.abs(u32 - uo).max() <= 1e-8 * scale
+    s.close()
+
+
+def test_mixed_c1_rectangle_matches_oracle(orc, nr):
+    c = problems.c1_config()
+    m = c["mesh"]
+    ne = len(m["conn"])
+    s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+    s.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
+    lam, mu = orc.lame(c["E"], c["nu"])
+    from tissueorigami_amd import tissue
+    inc = tissue.growth_increment(c["rate"], c["dt"])
+    u_g = m["nodes"].reshape(-1).copy()
+    u_o = u_g.copy()
+    Fg = np.tile(np.eye(3), (ne, 1, 1))
+    for step in range(10):
+        Fg = inc @ Fg
+        invFg = orc.inv3_many(Fg)
+        s.set_growth(invFg)
+        s.update_visc_surface(u_g, c["place"], c["etype"], c["spans"])
+        A_o = orc.visc_surfaces(len(m["nodes"]), m["conn"], u_o, c["place"], c["etype"],
+                                c["spans"])
+        u_g, st = s.step(u_g, c["dt"], 1e-8, 20, "mixed", 1e-5)
+        P = orc.Problem(m["conn"], m["refpos"], invFg, lam, mu, c["eta"], c["ext_visc"], A_o)
+        u_o, du, conv = P.nr_step_dense(u_o, c["dt"], 1e-8, 20)
+        assert st["converged"] and conv
+        assert np.abs(u_g - u_o).max() <= 1e-8 * np.abs(u_o).max()
+    s.close()
+
+
+def test_device_resident_advance_matches_per_step_path(orc, nr):
+    """tf_nr_advance (growth and viscosity surfaces on the device) reproduces
+    the host-driven sequence set_growth / update_visc_surface / step."""
+    c = problems.c1_config()
+    m = c["mesh"]
+    ne = len(m["conn"])
+    from tissueorigami_amd import tissue
+    rates = np.zeros((ne, 3))
+    rates[:, 0] = rates[:, 1] = c["rate"] / 3600.0
+    a = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+    a.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
+    a.set_growth_rates(rates)
+    a.set_surface_flags(c["place"], c["etype"], c["spans"])
+    u0 = m["nodes"].reshape(-1).copy()
+    ua, stats = a.advance(12, c["dt"], u0=u0, krylov_rtol=1e-12)
+    Fg_dev = a.growth_tensor()
+    b = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+    b.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
+    inc = tissue.growth_increment(c["rate"], c["dt"])
+    Fg = np.tile(np.eye(3), (ne, 1, 1))
+    ub = u0.copy()
+    for k in range(12):
+        Fg = inc @ Fg
+        b.set_growth(orc.inv3_many(Fg))
+        b.update_visc_surface(ub, c["place"], c["etype"], c["spans"])
+        ub, st = b.step(ub, c["dt"], 1e-8, 20, "fp64", 1e-12)
+        assert st["nr_iterations"] == stats[k]["nr_iterations"]
+    assert np.abs(Fg_dev - Fg.reshape(ne, 9)).max() <= 1e-15
+    assert np.abs(ua - ub).max() <= 1e-12 * np.abs(ub).max()
+    a.close(); b.close()

@@ -283,6 +283,41 @@ __global__ __launch_bounds__(kBlock) void visc_surface_kernel(
   }
 }
 
+// Pre-step growth on the device: the increment of ShapeBase::calculateFgFromRates
+// (ShapeBase.cpp:471-505, no rotation, tissue weight 1),
+// diag(exp(rx dt), exp(ry dt), exp(rz dt)), applied as ShapeBase::growShapeByFg
+// does (ShapeBase.cpp:1452-1505) with identity plastic and shape-change
+// increments: F_g <- inc F_g, then InvFg = inv(F_g) by the same LU as the
+// element kernel.
+__global__ __launch_bounds__(kBlock) void grow_kernel(int32_t n_prisms, const double* rates,
+                                                      double dt, double* Fg, double* invFg) {
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n_prisms; e += gridDim.x * kBlock) {
+    double inc[3];
+    for (int i = 0; i < 3; ++i) inc[i] = exp(rates[3 * e + i] * dt);
+    double F[9], X[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        // sum_k inc[i][k] Fg[k][j] with inc diagonal
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += ((i == k) ? inc[i] : 0.0) * Fg[9 * e + 3 * k + j];
+        F[3 * i + j] = s;
+      }
+    for (int q = 0; q < 9; ++q) Fg[9 * e + q] = F[q];
+    inv3_dev(F, X);
+    for (int q = 0; q < 9; ++q) invFg[9 * e + q] = X[q];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void inverse_kernel(int32_t n_prisms, const double* Fg,
+                                                         double* invFg) {
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n_prisms; e += gridDim.x * kBlock) {
+    double F[9], X[9];
+    for (int q = 0; q < 9; ++q) F[q] = Fg[9 * e + q];
+    inv3_dev(F, X);
+    for (int q = 0; q < 9; ++q) invFg[9 * e + q] = X[q];
+  }
+}
+
 int grid_for(int64_t n) {
   int64_t g = (n + kBlock - 1) / kBlock;
   if (g > 65535 * 4) g = 65535 * 4;
@@ -327,6 +362,19 @@ hipError_t launch_visc_surface(int32_t n_nodes, int32_t n_prisms, const int32_t*
   (void)n_prisms;
   hipLaunchKernelGGL(visc_surface_kernel, dim3(grid_for(n_nodes)), dim3(kBlock), 0, st, n_nodes,
                      conn, u, node_ptr, node_src, placement, type, spans, area);
+  return hipGetLastError();
+}
+
+hipError_t launch_grow(int32_t n_prisms, const double* rates, double dt, double* Fg,
+                       double* invFg, hipStream_t st) {
+  hipLaunchKernelGGL(grow_kernel, dim3(grid_for(n_prisms)), dim3(kBlock), 0, st, n_prisms, rates,
+                     dt, Fg, invFg);
+  return hipGetLastError();
+}
+
+hipError_t launch_inverse(int32_t n_prisms, const double* Fg, double* invFg, hipStream_t st) {
+  hipLaunchKernelGGL(inverse_kernel, dim3(grid_for(n_prisms)), dim3(kBlock), 0, st, n_prisms, Fg,
+                     invFg);
   return hipGetLastError();
 }
 

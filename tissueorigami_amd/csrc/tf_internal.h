@@ -26,6 +26,7 @@ struct DevPattern {
   int32_t* col = nullptr;         // n_slots
   int32_t* diag_slot = nullptr;   // n_nodes
   int32_t* row_len = nullptr;     // n_nodes: stored blocks of each row
+  int32_t* tslot = nullptr;       // n_slots: slot of the transposed block (C, R)
   int32_t* contrib_ptr = nullptr; // n_slots + 1
   int32_t* contrib_src = nullptr; // n_prisms * 36: e*36 + A*6 + B (K block A,B)
   int32_t* node_ptr = nullptr;    // n_nodes + 1
@@ -49,9 +50,9 @@ struct Workspace {
   // vectors (3 * n_nodes)
   double *un = nullptr, *uk = nullptr, *g = nullptr, *du = nullptr, *internal = nullptr;
   double *r = nullptr, *rh = nullptr, *p = nullptr, *v = nullptr, *ph = nullptr, *s = nullptr,
-         *sh = nullptr, *t = nullptr, *x = nullptr, *b = nullptr, *tmp = nullptr;
+         *sh = nullptr, *t = nullptr, *x = nullptr, *b = nullptr, *tmp = nullptr, *w = nullptr;
   float *r32 = nullptr, *rh32 = nullptr, *p32 = nullptr, *v32 = nullptr, *ph32 = nullptr,
-        *s32 = nullptr, *sh32 = nullptr, *t32 = nullptr, *x32 = nullptr;
+        *s32 = nullptr, *sh32 = nullptr, *t32 = nullptr, *x32 = nullptr, *w32 = nullptr;
   double* partials = nullptr;  // kParts * 4
   double* scal = nullptr;      // Krylov scalars
   int32_t* flag = nullptr;     // [0] state, [1] iterations
@@ -125,6 +126,10 @@ hipError_t launch_visc_surface(int32_t n_nodes, int32_t n_prisms, const int32_t*
                                const int32_t* type, const uint8_t* spans, double* area,
                                hipStream_t st);
 
+hipError_t launch_grow(int32_t n_prisms, const double* rates, double dt, double* Fg,
+                       double* invFg, hipStream_t st);
+hipError_t launch_inverse(int32_t n_prisms, const double* Fg, double* invFg, hipStream_t st);
+
 // ---- Krylov (tf_krylov.hip) ----
 // Optional event timing of the SpMV launches ([0] FP64 operator, [1] FP32).
 constexpr int kProfPairs = 128;
@@ -150,6 +155,7 @@ struct KrylovArgs {
   Workspace* ws;
   Launch L;
   Prof* prof;
+  int fused = 1;  // reduced-synchronisation BiCGStab (no binding)
 };
 // Solves J x = b (FP64).  Returns the iteration count in *iters, the Krylov
 // state (K_CONVERGED/K_BREAKDOWN/K_RUN=cap reached) in *state.
@@ -163,6 +169,37 @@ hipError_t apply_operator(const KrylovArgs& k, const double* x, double* y);
 // |x|_2 and x_k += dx (deterministic two-level reduction), result on host.
 hipError_t norm_and_update(int32_t n, const double* dx, double* xk, Workspace* ws, Launch L,
                            double* norm_out);
+
+// ---- adjoint (tf_adjoint.hip) ----
+struct AdjointLaunch {
+  int32_t n_prisms, n_nodes;
+  const int32_t* conn;
+  const double* refpos;
+  const double* invFg;
+  const double* dinvFg;
+  const double* lambda;
+  const double* mu;
+  const double* eta;
+  const double* ext_visc;
+  const uint8_t* dof_flags;
+  const double* uk;
+  const double* un;
+  double dt;
+  const double* m;
+  const int32_t* placement;
+  const int32_t* type;
+  const uint8_t* spans;
+  const double* area;
+  const int32_t* node_ptr;
+  const int32_t* node_src;
+  double* s_out;
+  double* b_out;
+  double* a_prev;
+};
+hipError_t launch_adjoint_element(const AdjointLaunch& l, hipStream_t st);
+hipError_t launch_transpose_operator(int64_t n_slots, const int32_t* tslot, const double* val,
+                                     double* valT, int32_t n_nodes, const double* minv,
+                                     double* minvT, hipStream_t st);
 
 void set_error(const std::string& s);
 

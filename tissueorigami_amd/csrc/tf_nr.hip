@@ -51,8 +51,22 @@ struct tf_handle {
   Prof prof;
   double prof_el_ms = 0.0, prof_as_ms = 0.0;
   int64_t prof_assemblies = 0;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<void*> allocs;
+  // device-resident time stepping: growth tensor, rates and surface flags
+  double* Fg = nullptr;
+  double* rates = nullptr;
+  int32_t *surf_place = nullptr, *surf_type = nullptr;
+  uint8_t* surf_spans = nullptr;
+  bool have_rates = false, have_flags = false, have_positions = false;
+  // adjoint state: position checkpoints of the last tf_adj_forward
+  double* adj_ckpt = nullptr;
+  int adj_cap = 0, adj_steps = 0;
+  double adj_dt = 0.0;
+  int32_t *adj_place = nullptr, *adj_type = nullptr;
+  uint8_t* adj_spans = nullptr;
+  double *adj_s = nullptr, *adj_b = nullptr, *adj_dinvFg = nullptr, *adj_valT = nullptr,
+         *adj_minvT = nullptr, *adj_a = nullptr;
 };
 
 namespace {
@@ -177,6 +191,24 @@ tf_status build_pattern(tf_handle* h, const int32_t* conn) {
     const int k = (int)(p - b);
     return sb[R / kSlice] + (int64_t)k * kSlice + (R % kSlice);
   };
+  // transpose slot map: slot of (R, k-th column C) -> slot of (C, R)
+  std::vector<int32_t> tslot(total, 0);
+  for (int R = 0; R < Nn; ++R)
+    for (int k = rowptr[R]; k < rowptr[R + 1]; ++k) {
+      const int C = nbr[k];
+      const int64_t s_rc = sb[R / kSlice] + (int64_t)(k - rowptr[R]) * kSlice + (R % kSlice);
+      tslot[s_rc] = (int32_t)slot_of(C, R);
+    }
+  for (int s = 0; s < n_slices; ++s) {
+    const int w = (sb[s + 1] - sb[s]) / kSlice;
+    for (int lane = 0; lane < kSlice; ++lane) {
+      const int R = s * kSlice + lane;
+      for (int k = (R < Nn ? rl[R] : 0); k < w; ++k) {
+        const int64_t slot = sb[s] + (int64_t)k * kSlice + lane;
+        tslot[slot] = (int32_t)slot;  // padding: zero block, maps onto itself
+      }
+    }
+  }
   // element block -> J slot contributions, in element order per slot
   std::vector<int32_t> cptr(total + 1, 0);
   std::vector<int32_t> slot_cache((size_t)Ne * 36);
@@ -203,6 +235,7 @@ tf_status build_pattern(tf_handle* h, const int32_t* conn) {
   TF_CHECK(dalloc(h, &P.col, total));
   TF_CHECK(dalloc(h, &P.diag_slot, Nn));
   TF_CHECK(dalloc(h, &P.row_len, Nn));
+  TF_CHECK(dalloc(h, &P.tslot, total));
   TF_CHECK(dalloc(h, &P.contrib_ptr, total + 1));
   TF_CHECK(dalloc(h, &P.contrib_src, (size_t)Ne * 36));
   TF_CHECK(dalloc(h, &P.node_ptr, Nn + 1));
@@ -211,6 +244,7 @@ tf_status build_pattern(tf_handle* h, const int32_t* conn) {
   TF_CHECK(hipMemcpy(P.col, col.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.diag_slot, diag.data(), sizeof(int32_t) * Nn, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.row_len, rl.data(), sizeof(int32_t) * Nn, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P.tslot, tslot.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.contrib_ptr, cptr.data(), sizeof(int32_t) * (total + 1), hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.contrib_src, csrc.data(), sizeof(int32_t) * csrc.size(), hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.node_ptr, node_ptr.data(), sizeof(int32_t) * (Nn + 1), hipMemcpyHostToDevice));
@@ -227,7 +261,7 @@ tf_status alloc_workspace(tf_handle* h) {
   TF_CHECK(dalloc(h, &w.val, (size_t)h->pat.n_slots * 9));
   TF_CHECK(dalloc(h, &w.minv, (size_t)h->Nn * 9));
   double** vecs[] = {&w.un, &w.uk, &w.g, &w.du, &w.internal, &w.r, &w.rh, &w.p, &w.v,
-                     &w.ph, &w.s, &w.sh, &w.t, &w.x, &w.b, &w.tmp};
+                     &w.ph, &w.s, &w.sh, &w.t, &w.x, &w.b, &w.tmp, &w.w};
   for (double** v : vecs) TF_CHECK(dalloc(h, v, n));
   TF_CHECK(dalloc(h, &w.partials, (size_t)kParts * 5));
   TF_CHECK(dalloc(h, &w.scal, 16));
@@ -251,7 +285,8 @@ tf_status ensure_mixed(tf_handle* h) {
   TF_CHECK(dalloc(h, &w.elemKv32, (size_t)h->Ne * 324));
   TF_CHECK(dalloc(h, &w.val32, (size_t)h->pat.n_slots * 9));
   TF_CHECK(dalloc(h, &w.minv32, (size_t)h->Nn * 9));
-  float** vecs[] = {&w.r32, &w.rh32, &w.p32, &w.v32, &w.ph32, &w.s32, &w.sh32, &w.t32, &w.x32};
+  float** vecs[] = {&w.r32, &w.rh32, &w.p32, &w.v32, &w.ph32, &w.s32, &w.sh32, &w.t32, &w.x32,
+                    &w.w32};
   for (float** v : vecs) TF_CHECK(dalloc(h, v, n));
   return TF_OK;
 }
@@ -511,36 +546,26 @@ tf_status tf_nr_solve(tf_handle* h, const double* b, double* x, double rtol, int
   return TF_OK;
 }
 
-tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int32_t maxit,
-                     int32_t precision_mode, double krylov_rtol, double* u_out,
-                     tf_step_stats* stats) {
-  if (!h || !u_n || !u_out || maxit < 0 || maxit > TF_MAX_NR_ITERS - 2 ||
-      (precision_mode != TF_PREC_FP64 && precision_mode != TF_PREC_MIXED)) {
-    set_error("tf_nr_step: invalid arguments");
-    return TF_ERR_ARG;
-  }
-  tf_step_stats st_local;
-  tf_step_stats* S = stats ? stats : &st_local;
-  std::memset(S, 0, sizeof(*S));
-  int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
-  if (mixed && h->n_pairs > 0) mixed = 0;  // bound systems use the FP64 operator
-  const size_t n = 3 * (size_t)h->Nn;
+}  // extern "C"
+
+namespace {
+
+// The NR loop of Simulation::updateStepNR on device-resident state: w.un and
+// w.uk hold u_n on entry; w.uk holds the converged iterate on exit.
+tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed, double rtol,
+                  tf_step_stats* S) {
   Workspace& w = h->ws;
-  TF_CHECK(hipMemcpyAsync(w.un, u_n, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
-  TF_CHECK(hipMemcpyAsync(w.uk, u_n, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
-  const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
   const int kmax = 50000;
   int it = 0;
   bool conv = false, hit_cap = false;
   tf_status result = TF_OK;
-  KrylovArgs k = krylov_args(h);
   while (!conv) {
     tf_status s = assemble(h, dt, mixed);
     if (s != TF_OK) return s;
-    k = krylov_args(h);
+    KrylovArgs k = krylov_args(h);
     int kit = 0, state = 0;
     if (mixed)
-      TF_CHECK(krylov_mixed(k, w.g, w.du, rtol, 1e-4, kmax, &kit, &state, 16));
+      TF_CHECK(krylov_mixed(k, w.g, w.du, rtol, std::max(rtol, 1e-6), kmax, &kit, &state, 16));
     else
       TF_CHECK(krylov_bicgstab(k, w.g, w.du, rtol, kmax, &kit, &state, 16));
     TF_CHECK(launch_bind_increment(h->pairs, h->n_pairs, w.du, h->st));
@@ -559,9 +584,11 @@ tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int
       h->prof_assemblies++;
     }
     S->ms_krylov += t_kr;
-    S->krylov_iterations[it] = kit;
+    if (it < TF_MAX_NR_ITERS) {
+      S->krylov_iterations[it] = kit;
+      S->du_norms[it] = dnorm;
+    }
     S->krylov_iterations_total += kit;
-    S->du_norms[it] = dnorm;
     if (state == K_BREAKDOWN) {
       set_error("BiCGStab breakdown in NR iteration " + std::to_string(it));
       result = TF_ERR_KRYLOV;
@@ -584,8 +611,6 @@ tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int
     }
   }
   S->ms_total = S->ms_element + S->ms_assembly + S->ms_krylov;
-  TF_CHECK(hipMemcpyAsync(u_out, w.uk, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
-  TF_CHECK(hipStreamSynchronize(h->st));
   S->nr_iterations = it;
   S->converged = (conv && !hit_cap && result == TF_OK) ? 1 : 0;
   if (result == TF_OK && hit_cap) {
@@ -594,6 +619,277 @@ tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int
   }
   S->status = result;
   return result;
+}
+
+}  // namespace
+
+extern "C" {
+
+tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int32_t maxit,
+                     int32_t precision_mode, double krylov_rtol, double* u_out,
+                     tf_step_stats* stats) {
+  if (!h || !u_n || !u_out || maxit < 0 || maxit > TF_MAX_NR_ITERS - 2 ||
+      (precision_mode != TF_PREC_FP64 && precision_mode != TF_PREC_MIXED)) {
+    set_error("tf_nr_step: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  tf_step_stats st_local;
+  tf_step_stats* S = stats ? stats : &st_local;
+  std::memset(S, 0, sizeof(*S));
+  int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
+  if (mixed && h->n_pairs > 0) mixed = 0;  // bound systems use the FP64 operator
+  const size_t n = 3 * (size_t)h->Nn;
+  Workspace& w = h->ws;
+  TF_CHECK(hipMemcpyAsync(w.un, u_n, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  TF_CHECK(hipMemcpyAsync(w.uk, u_n, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
+  tf_status result = nr_loop(h, dt, tol, maxit, mixed, rtol, S);
+  if (result != TF_OK && result != TF_ERR_NOT_CONVERGED && result != TF_ERR_KRYLOV &&
+      result != TF_ERR_NAN)
+    return result;
+  TF_CHECK(hipMemcpyAsync(u_out, w.uk, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  return result;
+}
+
+tf_status tf_nr_set_growth_rates(tf_handle* h, const double* rates) {
+  if (!h || !rates) {
+    set_error("tf_nr_set_growth_rates: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  const size_t Ne = h->Ne;
+  if (!h->Fg) {
+    TF_CHECK(dalloc(h, &h->Fg, Ne * 9));
+    TF_CHECK(dalloc(h, &h->rates, Ne * 3));
+  }
+  TF_CHECK(hipMemcpy(h->rates, rates, sizeof(double) * Ne * 3, hipMemcpyHostToDevice));
+  std::vector<double> I(Ne * 9, 0.0);
+  for (size_t e = 0; e < Ne; ++e) I[9 * e] = I[9 * e + 4] = I[9 * e + 8] = 1.0;
+  TF_CHECK(hipMemcpy(h->Fg, I.data(), sizeof(double) * Ne * 9, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->invFg, I.data(), sizeof(double) * Ne * 9, hipMemcpyHostToDevice));
+  h->have_rates = h->have_growth = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_growth_tensor(tf_handle* h, double* Fg_out, const double* Fg_in) {
+  if (!h || !h->Fg || (!Fg_out && !Fg_in)) {
+    set_error("tf_nr_growth_tensor: invalid arguments or no growth rates set");
+    return TF_ERR_ARG;
+  }
+  const size_t Ne = h->Ne;
+  if (Fg_out) TF_CHECK(hipMemcpy(Fg_out, h->Fg, sizeof(double) * Ne * 9, hipMemcpyDeviceToHost));
+  if (Fg_in) {
+    TF_CHECK(hipMemcpy(h->Fg, Fg_in, sizeof(double) * Ne * 9, hipMemcpyHostToDevice));
+    TF_CHECK(launch_inverse(h->Ne, h->Fg, h->invFg, h->st));
+    TF_CHECK(hipStreamSynchronize(h->st));
+    h->have_growth = true;
+  }
+  return TF_OK;
+}
+
+tf_status tf_nr_set_surface_flags(tf_handle* h, const int32_t* elem_placement,
+                                  const int32_t* elem_type, const uint8_t* spans) {
+  if (!h || !elem_placement || !elem_type || !spans) {
+    set_error("tf_nr_set_surface_flags: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  const size_t Ne = h->Ne;
+  if (!h->surf_place) {
+    TF_CHECK(dalloc(h, &h->surf_place, Ne));
+    TF_CHECK(dalloc(h, &h->surf_type, Ne));
+    TF_CHECK(dalloc(h, &h->surf_spans, Ne));
+  }
+  TF_CHECK(hipMemcpy(h->surf_place, elem_placement, sizeof(int32_t) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->surf_type, elem_type, sizeof(int32_t) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->surf_spans, spans, Ne, hipMemcpyHostToDevice));
+  h->have_flags = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double dt, double tol,
+                        int32_t maxit, int32_t precision_mode, double krylov_rtol, double* u_out,
+                        tf_step_stats* stats) {
+  if (!h || n_steps < 0 || maxit < 0 || maxit > TF_MAX_NR_ITERS - 2 ||
+      (precision_mode != TF_PREC_FP64 && precision_mode != TF_PREC_MIXED)) {
+    set_error("tf_nr_advance: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (!h->have_rates || !h->have_flags || (!u0 && !h->have_positions)) {
+    set_error("tf_nr_advance needs growth rates, surface flags and positions");
+    return TF_ERR_ARG;
+  }
+  const size_t n = 3 * (size_t)h->Nn;
+  Workspace& w = h->ws;
+  if (u0) TF_CHECK(hipMemcpyAsync(w.uk, u0, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  h->have_positions = true;
+  int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
+  if (mixed && h->n_pairs > 0) mixed = 0;
+  const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
+  tf_status worst = TF_OK;
+  for (int s = 0; s < n_steps; ++s) {
+    // pre-step producers on the device: growth, then the viscosity surfaces
+    // from the positions at the start of the step
+    TF_CHECK(launch_grow(h->Ne, h->rates, dt, h->Fg, h->invFg, h->st));
+    TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, w.uk, h->pat.node_ptr, h->pat.node_src,
+                                 h->surf_place, h->surf_type, h->surf_spans, h->area, h->st));
+    h->have_area = true;
+    TF_CHECK(hipMemcpyAsync(w.un, w.uk, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    tf_step_stats S;
+    std::memset(&S, 0, sizeof(S));
+    tf_status r = nr_loop(h, dt, tol, maxit, mixed, rtol, &S);
+    if (stats) stats[s] = S;
+    if (r != TF_OK && r != TF_ERR_NOT_CONVERGED) return r;
+    if (r != TF_OK) worst = r;
+  }
+  if (u_out) TF_CHECK(hipMemcpyAsync(u_out, w.uk, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  return worst;
+}
+
+tf_status tf_adj_forward(tf_handle* h, int32_t n_steps, const double* u0,
+                         const double* inv_Fg_steps, double dt, double tol, int32_t maxit,
+                         double krylov_rtol, const int32_t* elem_placement,
+                         const int32_t* elem_type, const uint8_t* spans, double* u_final,
+                         tf_step_stats* stats) {
+  if (!h || n_steps <= 0 || !u0 || !inv_Fg_steps || !u_final || !elem_placement || !elem_type ||
+      !spans || maxit < 0 || maxit > TF_MAX_NR_ITERS - 2) {
+    set_error("tf_adj_forward: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (h->n_pairs > 0) {
+    set_error("tf_adj_forward: slave/master binding is not supported by the adjoint");
+    return TF_ERR_ARG;
+  }
+  const size_t n = 3 * (size_t)h->Nn, Ne = h->Ne;
+  Workspace& w = h->ws;
+  if (h->adj_cap < n_steps + 1) {
+    if (h->adj_ckpt) TF_CHECK(hipFree(h->adj_ckpt));
+    h->adj_ckpt = nullptr;
+    TF_CHECK(hipMalloc(&h->adj_ckpt, sizeof(double) * n * (n_steps + 1)));
+    h->adj_cap = n_steps + 1;
+  }
+  if (!h->adj_place) {
+    TF_CHECK(dalloc(h, &h->adj_place, Ne));
+    TF_CHECK(dalloc(h, &h->adj_type, Ne));
+    TF_CHECK(dalloc(h, &h->adj_spans, Ne));
+    TF_CHECK(dalloc(h, &h->adj_s, Ne));
+    TF_CHECK(dalloc(h, &h->adj_b, Ne * 18));
+    TF_CHECK(dalloc(h, &h->adj_dinvFg, Ne * 9));
+    TF_CHECK(dalloc(h, &h->adj_valT, (size_t)h->pat.n_slots * 9));
+    TF_CHECK(dalloc(h, &h->adj_minvT, (size_t)h->Nn * 9));
+    TF_CHECK(dalloc(h, &h->adj_a, n));
+  }
+  TF_CHECK(hipMemcpy(h->adj_place, elem_placement, sizeof(int32_t) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->adj_type, elem_type, sizeof(int32_t) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->adj_spans, spans, Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->adj_ckpt, u0, sizeof(double) * n, hipMemcpyHostToDevice));
+  tf_step_stats total;
+  std::memset(&total, 0, sizeof(total));
+  const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
+  for (int s = 0; s < n_steps; ++s) {
+    const double* prev = h->adj_ckpt + n * s;
+    TF_CHECK(hipMemcpyAsync(h->invFg, inv_Fg_steps + 9 * Ne * s, sizeof(double) * 9 * Ne,
+                            hipMemcpyHostToDevice, h->st));
+    TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, prev, h->pat.node_ptr, h->pat.node_src,
+                                 h->adj_place, h->adj_type, h->adj_spans, h->area, h->st));
+    h->have_growth = h->have_area = true;
+    TF_CHECK(hipMemcpyAsync(w.un, prev, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    TF_CHECK(hipMemcpyAsync(w.uk, prev, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    tf_step_stats S;
+    std::memset(&S, 0, sizeof(S));
+    tf_status r = nr_loop(h, dt, tol, maxit, 0, rtol, &S);
+    if (r != TF_OK) return r;
+    TF_CHECK(hipMemcpyAsync(h->adj_ckpt + n * (s + 1), w.uk, sizeof(double) * n,
+                            hipMemcpyDeviceToDevice, h->st));
+    total.nr_iterations += S.nr_iterations;
+    total.krylov_iterations_total += S.krylov_iterations_total;
+    total.ms_element += S.ms_element;
+    total.ms_assembly += S.ms_assembly;
+    total.ms_krylov += S.ms_krylov;
+  }
+  TF_CHECK(hipMemcpyAsync(u_final, h->adj_ckpt + n * n_steps, sizeof(double) * n,
+                          hipMemcpyDeviceToHost, h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  total.converged = 1;
+  total.ms_total = total.ms_element + total.ms_assembly + total.ms_krylov;
+  h->adj_steps = n_steps;
+  h->adj_dt = dt;
+  if (stats) *stats = total;
+  return TF_OK;
+}
+
+tf_status tf_adj_backward(tf_handle* h, const double* inv_Fg_steps, const double* dinv_Fg_steps,
+                          const double* dL_du_final, double krylov_rtol, double* dL_dr,
+                          tf_step_stats* stats) {
+  if (!h || !inv_Fg_steps || !dinv_Fg_steps || !dL_du_final || !dL_dr) {
+    set_error("tf_adj_backward: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (h->adj_steps <= 0) {
+    set_error("tf_adj_backward needs a preceding tf_adj_forward");
+    return TF_ERR_ARG;
+  }
+  const size_t n = 3 * (size_t)h->Nn, Ne = h->Ne;
+  Workspace& w = h->ws;
+  const double dt = h->adj_dt;
+  const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-12;
+  std::vector<double> acc(Ne, 0.0), s_host(Ne);
+  tf_step_stats total;
+  std::memset(&total, 0, sizeof(total));
+  TF_CHECK(hipMemcpy(h->adj_a, dL_du_final, sizeof(double) * n, hipMemcpyHostToDevice));
+  hipEvent_t e0 = h->ev[4], e1 = h->ev[5];
+  for (int s = h->adj_steps - 1; s >= 0; --s) {
+    const double* prev = h->adj_ckpt + n * s;
+    const double* cur = h->adj_ckpt + n * (s + 1);
+    TF_CHECK(hipMemcpyAsync(h->invFg, inv_Fg_steps + 9 * Ne * s, sizeof(double) * 9 * Ne,
+                            hipMemcpyHostToDevice, h->st));
+    TF_CHECK(hipMemcpyAsync(h->adj_dinvFg, dinv_Fg_steps + 9 * Ne * s, sizeof(double) * 9 * Ne,
+                            hipMemcpyHostToDevice, h->st));
+    TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, prev, h->pat.node_ptr, h->pat.node_src,
+                                 h->adj_place, h->adj_type, h->adj_spans, h->area, h->st));
+    TF_CHECK(hipMemcpyAsync(w.un, prev, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    TF_CHECK(hipMemcpyAsync(w.uk, cur, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
+    tf_status r = assemble(h, dt, 0);
+    if (r != TF_OK) return r;
+    TF_CHECK(hipEventRecord(e0, h->st));
+    // J^T mu = a with the transposed operator and transposed block-Jacobi
+    TF_CHECK(launch_transpose_operator(h->pat.n_slots, h->pat.tslot, w.val, h->adj_valT, h->Nn,
+                                       w.minv, h->adj_minvT, h->st));
+    KrylovArgs k = krylov_args(h);
+    k.val = h->adj_valT;
+    k.minv = h->adj_minvT;
+    int kit = 0, state = 0;
+    TF_CHECK(krylov_bicgstab(k, h->adj_a, w.du, rtol, 50000, &kit, &state, 16));
+    if (state != K_CONVERGED) {
+      set_error("adjoint BiCGStab did not converge at step " + std::to_string(s));
+      return TF_ERR_KRYLOV;
+    }
+    AdjointLaunch l;
+    l.n_prisms = h->Ne; l.n_nodes = h->Nn; l.conn = h->conn; l.refpos = h->refpos;
+    l.invFg = h->invFg; l.dinvFg = h->adj_dinvFg; l.lambda = h->lam; l.mu = h->mu;
+    l.eta = h->eta; l.ext_visc = h->extv; l.dof_flags = h->dof_flags; l.uk = cur; l.un = prev;
+    l.dt = dt; l.m = w.du; l.placement = h->adj_place; l.type = h->adj_type;
+    l.spans = h->adj_spans; l.area = h->area; l.node_ptr = h->pat.node_ptr;
+    l.node_src = h->pat.node_src; l.s_out = h->adj_s; l.b_out = h->adj_b; l.a_prev = h->adj_a;
+    TF_CHECK(launch_adjoint_element(l, h->st));
+    TF_CHECK(hipEventRecord(e1, h->st));
+    TF_CHECK(hipMemcpyAsync(s_host.data(), h->adj_s, sizeof(double) * Ne, hipMemcpyDeviceToHost,
+                            h->st));
+    TF_CHECK(hipStreamSynchronize(h->st));
+    float ms_as = 0, ms_kr = 0;
+    TF_CHECK(hipEventElapsedTime(&ms_as, h->ev[0], h->ev[2]));
+    TF_CHECK(hipEventElapsedTime(&ms_kr, e0, e1));
+    total.ms_assembly += ms_as;
+    total.ms_krylov += ms_kr;
+    total.krylov_iterations_total += kit;
+    total.nr_iterations += 1;
+    for (size_t e = 0; e < Ne; ++e) acc[e] += s_host[e];
+  }
+  std::memcpy(dL_dr, acc.data(), sizeof(double) * Ne);
+  total.converged = 1;
+  total.ms_total = total.ms_assembly + total.ms_krylov;
+  if (stats) *stats = total;
+  return TF_OK;
 }
 
 tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz) {
@@ -805,6 +1101,7 @@ tf_status tf_nr_destroy(tf_handle* h) {
   for (auto& e : h->prof.ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
+  if (h->adj_ckpt) (void)hipFree(h->adj_ckpt);
   if (h->ws.host_pinned) (void)hipHostFree(h->ws.host_pinned);
   if (h->ws.host_flag) (void)hipHostFree(h->ws.host_flag);
   for (auto& e : h->ev)

@@ -22,7 +22,9 @@ MAX_NR = 32
 EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_visc_surface",
            "tf_nr_update_visc_surface", "tf_nr_step", "tf_nr_assemble", "tf_nr_solve",
            "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_profile",
-           "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_nr_destroy", "tf_last_error"]
+           "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
+           "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
+           "tf_nr_destroy", "tf_last_error"]
 
 
 class StepStats(ctypes.Structure):
@@ -86,6 +88,16 @@ def load():
     L.tf_nr_profile.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_kernel_times.argtypes = [_h, ctypes.POINTER(KernelTimes)]
     L.tf_nr_pattern_info.argtypes = [_h, _i64, _i64]
+    L.tf_adj_forward.argtypes = [_h, ctypes.c_int32, _d, _d, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_int32, ctypes.c_double, _i, _i, _u8, _d,
+                                 ctypes.POINTER(StepStats)]
+    L.tf_adj_backward.argtypes = [_h, _d, _d, _d, ctypes.c_double, _d, ctypes.POINTER(StepStats)]
+    L.tf_nr_set_growth_rates.argtypes = [_h, _d]
+    L.tf_nr_growth_tensor.argtypes = [_h, _d, _d]
+    L.tf_nr_set_surface_flags.argtypes = [_h, _i, _i, _u8]
+    L.tf_nr_advance.argtypes = [_h, _d, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _d,
+                                ctypes.POINTER(StepStats)]
     L.tf_nr_destroy.argtypes = [_h]
     L.tf_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
@@ -217,6 +229,63 @@ class NewtonRaphsonSolver:
         _check(load().tf_nr_get_csr(self._h, float(threshold), _p(ia, _i), _p(ja, _i), _p(a, _d),
                                     _p(nnz, _i64)))
         return ia, ja, a
+
+    # ---- device-resident time stepping
+    def set_growth_rates(self, rates_xyz_per_s):
+        r = _f64(rates_xyz_per_s).reshape(self.n_prisms, 3)
+        _check(load().tf_nr_set_growth_rates(self._h, _p(r, _d)))
+
+    def growth_tensor(self, Fg=None):
+        if Fg is None:
+            out = np.empty((self.n_prisms, 9))
+            _check(load().tf_nr_growth_tensor(self._h, _p(out, _d), None))
+            return out
+        Fg = _f64(Fg).reshape(self.n_prisms, 9)
+        _check(load().tf_nr_growth_tensor(self._h, None, _p(Fg, _d)))
+
+    def set_surface_flags(self, elem_placement, elem_type, spans):
+        pl = np.ascontiguousarray(elem_placement, dtype=np.int32)
+        ty = np.ascontiguousarray(elem_type, dtype=np.int32)
+        sp = np.ascontiguousarray(spans, dtype=np.uint8)
+        _check(load().tf_nr_set_surface_flags(self._h, _p(pl, _i), _p(ty, _i), _p(sp, _u8)))
+
+    def advance(self, n_steps, dt, u0=None, tol=1e-8, maxit=20, precision="fp64",
+                krylov_rtol=1e-10, want_positions=True):
+        """n_steps device-resident steps; returns (positions or None, per-step stats)."""
+        u0 = None if u0 is None else _f64(u0).reshape(-1)
+        out = np.empty(3 * self.n_nodes) if want_positions else None
+        st = (StepStats * max(n_steps, 1))()
+        code = load().tf_nr_advance(self._h, _p(u0, _d), int(n_steps), float(dt), float(tol),
+                                    int(maxit), PREC[precision], float(krylov_rtol), _p(out, _d),
+                                    st)
+        if code not in (TF_OK, TF_ERR_NOT_CONVERGED):
+            _check(code)
+        return out, [st[i].as_dict() for i in range(n_steps)]
+
+    # ---- adjoint (implicit-function theorem per step)
+    def adj_forward(self, u0, inv_Fg_steps, dt, elem_placement, elem_type, spans, tol=1e-8,
+                    maxit=20, krylov_rtol=1e-10):
+        u0 = _f64(u0).reshape(-1)
+        inv = _f64(inv_Fg_steps).reshape(-1, self.n_prisms, 9)
+        pl = np.ascontiguousarray(elem_placement, dtype=np.int32)
+        ty = np.ascontiguousarray(elem_type, dtype=np.int32)
+        sp = np.ascontiguousarray(spans, dtype=np.uint8)
+        out = np.empty_like(u0)
+        st = StepStats()
+        _check(load().tf_adj_forward(self._h, inv.shape[0], _p(u0, _d), _p(inv, _d), float(dt),
+                                     float(tol), int(maxit), float(krylov_rtol), _p(pl, _i),
+                                     _p(ty, _i), _p(sp, _u8), _p(out, _d), ctypes.byref(st)))
+        self._adj_inv = inv
+        return out, st.as_dict()
+
+    def adj_backward(self, dinv_Fg_steps, dL_du, krylov_rtol=1e-12):
+        dinv = _f64(dinv_Fg_steps).reshape(self._adj_inv.shape)
+        g = _f64(dL_du).reshape(-1)
+        out = np.empty(self.n_prisms)
+        st = StepStats()
+        _check(load().tf_adj_backward(self._h, _p(self._adj_inv, _d), _p(dinv, _d), _p(g, _d),
+                                      float(krylov_rtol), _p(out, _d), ctypes.byref(st)))
+        return out, st.as_dict()
 
     def profile(self, enable=True):
         _check(load().tf_nr_profile(self._h, 1 if enable else 0))
