@@ -14,6 +14,13 @@ step is timed first, then the plain FP64 step from the same state with the
 same growth; the line reports the mixed rate as `value`, the FP64 rate, their
 ratio and the relative position error of mixed against FP64.
 
+Batched sweep (reported beside the headline line, --no-batch skips it): 64
+independent elliptical tissues, prism counts log-uniform in 2 k - 40 k,
+2-6 layers, E ~ U[100, 2000] Pa per tissue, per-tissue seeded growth fields;
+all tissues in one handle (one launch per solver stage for the whole batch,
+tile size chosen by the autotuner), mixed precision; the same tissues stepped
+one after another in their own handles give the sequential rate.
+
     python bench.py [--steps K] [--warmup W] [--config c3|c2|c1] [--gpus 1]
 """
 import argparse
@@ -78,15 +85,13 @@ def invert_batch(F):
     return np.linalg.inv(F).reshape(-1, 9)
 
 
-def run_steps(solver, w, u0, Fg0, n_steps, precision, profile=False, krylov_rtol=1e-10):
+def run_steps(solver, w, u0, Fg0, n_steps, precision, krylov_rtol=1e-10):
     """n_steps device-resident implicit steps (growth, viscosity surfaces and
     the NR solve all on the device) from positions u0 and growth tensor Fg0;
     returns the final state and accumulated statistics."""
     solver.growth_tensor(Fg0)
     tot = dict(nr_iterations=0, krylov_iterations=0, ms_element=0.0, ms_assembly=0.0,
-               ms_krylov=0.0, converged=True, per_step=[])
-    if profile:
-        solver.profile(True)
+               ms_krylov=0.0, converged=True, per_step=[], fp64_fallbacks=0)
     # positions and growth state are uploaded before the timed region
     solver.advance(0, w["dt"], u0=u0, want_positions=False)
     t0 = time.perf_counter()
@@ -99,21 +104,39 @@ def run_steps(solver, w, u0, Fg0, n_steps, precision, profile=False, krylov_rtol
         for k in ("ms_element", "ms_assembly", "ms_krylov"):
             tot[k] += st[k]
         tot["converged"] &= st["converged"]
+        tot["fp64_fallbacks"] += st["fp64_fallbacks"]
         tot["per_step"].append(dict(nr=st["nr_iterations"], krylov=st["krylov_iterations"],
                                     du=[float("%.3e" % d) for d in st["du_norms"]]))
         print(f"[bench]   {precision} step: NR {st['nr_iterations']} krylov {st['krylov_iterations']} "
               f"du {['%.2e' % d for d in st['du_norms']]}", file=sys.stderr, flush=True)
-    kt = solver.kernel_times() if profile else None
-    if profile:
-        solver.profile(False)
     u, _ = solver.advance(0, w["dt"])
-    return u, solver.growth_tensor(), tot, wall, kt
+    return u, solver.growth_tensor(), tot, wall
 
 
-def spmv_bytes(n_blocks, n_nodes, real_bytes):
-    # algorithmic bytes of one SpMV: block values + block column index per
-    # stored block, one row pointer per node row, x read and y written once
-    return n_blocks * (9 * real_bytes + 4) + n_nodes * 4 + 2 * 3 * n_nodes * real_bytes
+def spmv_v_bytes(n_blocks, n_nodes, s):
+    """Algorithmic bytes of one tz_spmv_v launch (v = J ph fused with
+    s = r - alpha v, sh = M^-1 s): per stored 3x3 block its 9 values and its
+    column index; per node row ph (gathered, counted once), r and the 3x3
+    block-Jacobi inverse read, v, s and sh written."""
+    return n_blocks * (9 * s + 4) + n_nodes * s * (3 + 3 + 9 + 9)
+
+
+def spmv_t_bytes(n_blocks, n_nodes, s):
+    """tz_spmv_t (t = J sh fused with the dots (t, s), (t, t)): block values and
+    columns; per node sh (gathered once) and s read, t written."""
+    return n_blocks * (9 * s + 4) + n_nodes * s * (3 + 3 + 3)
+
+
+def profile_step(solver, w, u, Fg, precision, krylov_rtol):
+    """One extra, untimed step with per-launch event timing of the SpMVs
+    (direct launches, no graphs); returns the kernel times."""
+    solver.growth_tensor(Fg)
+    solver.profile(True)
+    solver.advance(1, w["dt"], u0=u, precision=precision, krylov_rtol=krylov_rtol,
+                   want_positions=False)
+    kt = solver.kernel_times()
+    solver.profile(False)
+    return kt
 
 
 def cpu_baseline(w, u, Fg, krylov_its_per_nr, budget_s=25.0):
@@ -145,6 +168,65 @@ def cpu_baseline(w, u, Fg, krylov_its_per_nr, budget_s=25.0):
                         f"the GPU run); Pardiso substituted by the same block-Jacobi BiCGStab"))
 
 
+def run_batch(steps, warmup, mixed_rtol, seed=4, n=64, sequential_sample=8):
+    """The batched sweep: returns a dict for the JSON line."""
+    ts = tissue.batch_ellipses(n, seed=seed)
+    dt = 60.0
+    meshes = [(len(c["mesh"]["nodes"]), c["mesh"]["conn"], c["mesh"]["refpos"]) for c in ts]
+    cat = lambda k: np.concatenate([c[k] for c in ts])
+    t0 = time.perf_counter()
+    b = nr.TissueBatch(meshes)
+    b.set_material(cat("E"), cat("nu"), cat("eta"), cat("ext_visc"))
+    b.set_growth_rates(np.concatenate([tissue.rates_xyz(c["rate"]) for c in ts]))
+    b.set_surface_flags(cat("place"), cat("etype"), cat("spans"))
+    setup_s = time.perf_counter() - t0
+    u0 = np.concatenate([c["mesh"]["nodes"].reshape(-1) for c in ts])
+    b.advance(max(warmup, 1), dt, u0=u0, precision="mixed", krylov_rtol=mixed_rtol,
+              want_positions=False)
+    spt, ms_it = b.autotune("mixed")
+    print(f"[bench] batch: {n} tissues, {b.n_prisms} prisms, autotuned {spt} slices per tile "
+          f"({ms_it:.3f} ms per solver iteration)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    _, stats = b.advance(steps, dt, None, 1e-8, 20, "mixed", mixed_rtol, want_positions=False)
+    wall = time.perf_counter() - t0
+    nr_its = sum(st["nr_iterations"] for step in stats for st in step)
+    conv = all(st["converged"] for step in stats for st in step)
+    fb = sum(st["fp64_fallbacks"] for step in stats for st in step)
+    # launches are shared: the batch's wall time per step is that of its
+    # slowest tissue; the rate counts every tissue's NR iterations
+    b.close()
+    # sequential reference: a sample of the tissues, each in its own handle,
+    # same steps, extrapolated by prism count
+    idx = np.linspace(0, n - 1, sequential_sample).astype(int)
+    seq_wall, seq_prisms = 0.0, 0
+    for i in idx:
+        c = ts[i]
+        m = c["mesh"]
+        s1 = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+        s1.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
+        s1.set_growth_rates(tissue.rates_xyz(c["rate"]))
+        s1.set_surface_flags(c["place"], c["etype"], c["spans"])
+        s1.advance(max(warmup, 1), dt, u0=m["nodes"].reshape(-1), precision="mixed",
+                   krylov_rtol=mixed_rtol, want_positions=False)
+        t1 = time.perf_counter()
+        s1.advance(steps, dt, None, 1e-8, 20, "mixed", mixed_rtol, want_positions=False)
+        seq_wall += time.perf_counter() - t1
+        seq_prisms += len(m["conn"])
+        s1.close()
+    seq_est = seq_wall * sum(len(c["mesh"]["conn"]) for c in ts) / seq_prisms
+    out = dict(workload=f"{n} ellipses, prisms log-uniform 2k-40k, 2-6 layers, E~U[100,2000] Pa",
+               tissues=n, prisms=int(sum(len(c["mesh"]["conn"]) for c in ts)),
+               nodes=int(sum(len(c["mesh"]["nodes"]) for c in ts)), seed=seed, dt_s=dt,
+               steps=steps, slices_per_tile=spt, nr_iterations=nr_its,
+               nr_iterations_per_s=nr_its / wall, time_steps_per_s=steps / wall, wall_s=wall,
+               converged=conv, fp64_fallbacks=fb, setup_s=setup_s,
+               sequential_estimate_s=seq_est, batch_speedup_vs_sequential=seq_est / wall,
+               sequential_sample=f"{sequential_sample} of the {n} tissues, scaled by prism count")
+    print(f"[bench] batch: {steps} steps {wall:.2f} s, {nr_its} NR its; sequential estimate "
+          f"{seq_est:.2f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +235,7 @@ def main():
     ap.add_argument("--config", default="c3", choices=list(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--mixed-rtol", type=float, default=1e-5,
                     help="relative residual of each FP32 linear solve (the NR loop refines)")
     ap.add_argument("--fp64-rtol", type=float, default=1e-10)
@@ -179,11 +262,11 @@ def main():
     u0 = m["nodes"].reshape(-1).copy()
     Fg0 = np.tile(np.eye(3).reshape(9), (ne, 1))
     # warm-up (mixed) from the initial state, then the timed steps continue from it
-    u_w, Fg_w, _, wall_w, _ = run_steps(solver, w, u0, Fg0, args.warmup, "mixed",
-                                        krylov_rtol=args.mixed_rtol)
+    u_w, Fg_w, _, wall_w = run_steps(solver, w, u0, Fg0, args.warmup, "mixed",
+                                     krylov_rtol=args.mixed_rtol)
     print(f"[bench] warmup {args.warmup} steps {wall_w:.2f} s", file=sys.stderr, flush=True)
-    u_m, _, tm, wall_m, kt_m = run_steps(solver, w, u_w, Fg_w, args.steps, "mixed", profile=True,
-                                         krylov_rtol=args.mixed_rtol)
+    u_m, _, tm, wall_m = run_steps(solver, w, u_w, Fg_w, args.steps, "mixed",
+                                   krylov_rtol=args.mixed_rtol)
     print(f"[bench] mixed: {tm['nr_iterations']} NR its, {tm['krylov_iterations']} Krylov its, "
           f"{wall_m:.2f} s", file=sys.stderr, flush=True)
     res = dict(metric="Newton steps/s at 1.6M prisms, mixed vs FP64 @1e-8 rel err; adjoint cost/fwd",
@@ -194,8 +277,8 @@ def main():
     res["ms_per_step"] = 1000.0 * wall_m / args.steps
     fp64 = None
     if not args.no_fp64:
-        u_d, _, td, wall_d, kt_d = run_steps(solver, w, u_w, Fg_w, args.steps, "fp64", profile=True,
-                                             krylov_rtol=args.fp64_rtol)
+        u_d, _, td, wall_d = run_steps(solver, w, u_w, Fg_w, args.steps, "fp64",
+                                       krylov_rtol=args.fp64_rtol)
         print(f"[bench] fp64: {td['nr_iterations']} NR its, {td['krylov_iterations']} Krylov its, "
               f"{wall_d:.2f} s", file=sys.stderr, flush=True)
         err = float(np.abs(u_m - u_d).max() / np.abs(u_d).max())
@@ -210,6 +293,7 @@ def main():
     res["mixed"] = dict(nr_iterations=tm["nr_iterations"], krylov_iterations=tm["krylov_iterations"],
                         wall_s=wall_m, ms_element=tm["ms_element"], ms_assembly=tm["ms_assembly"],
                         ms_krylov=tm["ms_krylov"], converged=tm["converged"],
+                        fp64_fallbacks=tm["fp64_fallbacks"],
                         time_steps_per_s=args.steps / wall_m, per_step=tm["per_step"])
     res["adjoint_cost_per_forward"] = None
     res["config"] = dict(workload=f"{args.config}: {w['cfg']['desc']}", prisms=ne, nodes=nn,
@@ -217,20 +301,35 @@ def main():
                          krylov_rtol_mixed=args.mixed_rtol, precond="node-block Jacobi",
                          krylov="BiCGStab (reduced synchronisation)",
                          growth_seed=w["seed"], setup_s=setup_s)
-    # roofline of the dominant kernel: the FP32 SpMV of the mixed solve
-    if kt_m and kt_m["spmv_launches_fp32"] > 0:
+    # roofline of the dominant kernel: the FP32 SpMV v = J ph of the mixed
+    # solve (tz_spmv_v<float>), timed with HIP events on the solver's stream
+    # in one extra profiled step from the warm state
+    kt_m = profile_step(solver, w, u_w, Fg_w, "mixed", args.mixed_rtol)
+    res["kernel_times_mixed_step"] = kt_m
+    if kt_m["spmv_launches_fp32"] > 0:
         avg_s = kt_m["spmv_ms_fp32"] / kt_m["spmv_launches_fp32"] / 1000.0
-        b = spmv_bytes(n_blocks, nn, 4)
+        b = spmv_v_bytes(n_blocks, nn, 4)
         ach = b / avg_s / 1e9
         res["roofline"] = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                               frac=ach / HBM_PEAK_GBS, traffic=None, kernel="spmv_kernel<float>",
+                               frac=ach / HBM_PEAK_GBS, traffic=None, kernel="tz_spmv_v<float>",
                                algorithmic_bytes=b, avg_launch_ms=avg_s * 1e3,
                                launches=kt_m["spmv_launches_fp32"])
-    if fp64 and kt_d and kt_d["spmv_launches_fp64"] > 0:
-        avg_s = kt_d["spmv_ms_fp64"] / kt_d["spmv_launches_fp64"] / 1000.0
-        b = spmv_bytes(n_blocks, nn, 8)
-        res["roofline_fp64_spmv"] = dict(achieved=b / avg_s / 1e9, frac=b / avg_s / 1e9 / HBM_PEAK_GBS,
-                                         avg_launch_ms=avg_s * 1e3, algorithmic_bytes=b)
+        if kt_m["spmv_t_launches_fp32"] > 0:
+            avg_t = kt_m["spmv_t_ms_fp32"] / kt_m["spmv_t_launches_fp32"] / 1000.0
+            bt = spmv_t_bytes(n_blocks, nn, 4)
+            res["roofline_spmv_t_fp32"] = dict(kernel="tz_spmv_t<float>", achieved=bt / avg_t / 1e9,
+                                               frac=bt / avg_t / 1e9 / HBM_PEAK_GBS,
+                                               avg_launch_ms=avg_t * 1e3, algorithmic_bytes=bt)
+    if fp64:
+        kt_d = profile_step(solver, w, u_w, Fg_w, "fp64", args.fp64_rtol)
+        if kt_d["spmv_launches_fp64"] > 0:
+            avg_s = kt_d["spmv_ms_fp64"] / kt_d["spmv_launches_fp64"] / 1000.0
+            b = spmv_v_bytes(n_blocks, nn, 8)
+            res["roofline_fp64_spmv"] = dict(kernel="tz_spmv_v<double>", achieved=b / avg_s / 1e9,
+                                             frac=b / avg_s / 1e9 / HBM_PEAK_GBS,
+                                             avg_launch_ms=avg_s * 1e3, algorithmic_bytes=b)
+    if not args.no_batch and args.config == "c3":
+        res["batch"] = run_batch(args.steps, args.warmup, args.mixed_rtol)
     if not args.no_cpu_baseline:
         k_per_nr = (fp64 or {}).get("krylov_iterations", tm["krylov_iterations"]) / max(
             1, (fp64 or {}).get("nr_iterations", tm["nr_iterations"]))

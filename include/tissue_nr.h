@@ -67,6 +67,9 @@ typedef struct tf_step_stats {
   double ms_assembly;                      /* slot-map reduction + BCs */
   double ms_krylov;                        /* linear solves */
   double ms_total;                         /* whole step, device time */
+  int32_t fp64_fallbacks;                  /* mixed precision: NR iterations whose
+                                              FP32 solve failed and was redone
+                                              with the FP64 operator */
 } tf_step_stats;
 
 typedef struct tf_handle tf_handle;
@@ -181,6 +184,31 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
                         int32_t maxit, int32_t precision_mode, double krylov_rtol, double* u_out,
                         tf_step_stats* stats);
 
+/* Batch of independent tissues in one handle (no counterpart in the
+ * reference, which simulates one tissue per process).  The tissues are
+ * given concatenated: nodes_per_tissue / prisms_per_tissue, conn with node
+ * ids local to each tissue, ref_pos of all prisms in order.  Every per-node
+ * array of the other entry points (positions, external viscosity, fixed DOF,
+ * surfaces) is the concatenation of the tissues' arrays; per-prism arrays
+ * likewise.  Every kernel of the step runs once per stage for the whole
+ * batch; every tissue keeps its own NR convergence test and its own Krylov
+ * scalars, so its result equals the tissue stepped alone.  Step statistics
+ * are reported per tissue: tf_nr_step fills n_tissues entries, tf_nr_advance
+ * n_steps x n_tissues.  Binding, tf_nr_assemble/solve/get_csr/csr_pattern and
+ * the adjoint take single-tissue handles only. */
+tf_status tf_batch_create(int32_t n_tissues, const int32_t* nodes_per_tissue,
+                          const int32_t* prisms_per_tissue, const int32_t* conn,
+                          const double* ref_pos, tf_handle** out);
+
+/* Tile size of the fused Krylov solver (consecutive 64-row slices per
+ * workgroup; a tile never spans two tissues).  tf_nr_autotune times 10
+ * solver iterations on the assembled operator of the given precision for
+ * each candidate size and keeps the fastest; tf_nr_set_tile_size sets it.
+ * No reference counterpart. */
+tf_status tf_nr_autotune(tf_handle* h, int32_t precision_mode, int32_t* slices_per_tile,
+                         double* ms_per_iteration);
+tf_status tf_nr_set_tile_size(tf_handle* h, int32_t slices_per_tile);
+
 /* Adjoint of n_steps implicit steps with respect to one growth parameter r_e
  * per prism.  No counterpart in the reference (which has no derivatives); the
  * forward is n_steps of tf_nr_step (FP64) that also keeps every step's
@@ -210,8 +238,12 @@ tf_status tf_adj_backward(tf_handle* h, const double* inv_Fg_steps, const double
  * tf_nr_profile(h, 1) resets the counters and brackets every SpMV launch
  * with HIP events on the handle's stream; tf_nr_kernel_times reads them. */
 typedef struct tf_kernel_times {
-  double spmv_ms[2];          /* summed SpMV durations: [0] FP64, [1] FP32 */
+  double spmv_ms[2];          /* summed durations of the SpMV v = J ph, fused with
+                                 s = r - alpha v and sh = M^-1 s: [0] FP64, [1] FP32
+                                 (the binding path's plain SpMV counts here too) */
   int64_t spmv_launches[2];
+  double spmv_t_ms[2];        /* the SpMV t = J sh fused with (t, s), (t, t) */
+  int64_t spmv_t_launches[2];
   double element_ms;          /* per-prism kernel, summed over assemblies */
   double assembly_ms;         /* slot-map reduction + residual, summed */
   int64_t assemblies;
@@ -219,6 +251,10 @@ typedef struct tf_kernel_times {
 
 tf_status tf_nr_profile(tf_handle* h, int32_t enable);
 tf_status tf_nr_kernel_times(tf_handle* h, tf_kernel_times* out);
+
+/* Replays of the Krylov iteration chunks through captured hipGraphs
+ * (default on; profiling launches directly).  No reference counterpart. */
+tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable);
 
 /* Size of the Jacobian pattern: stored 3x3 blocks (n_blocks, the structural
  * nonzeros / 9) and padded SELL-64 slots.  No reference counterpart. */

@@ -36,19 +36,28 @@ def test_library_exports_every_declared_symbol():
     assert set(nr.EXPORTS) <= exported
 
 
-def test_stats_struct_layout_matches_header(tmp_path):
-    from tissueorigami_amd import nr
+def _c_layout(tmp_path, struct, fields):
+    """sizeof and offsetof of a header struct, from gcc."""
     src = tmp_path / "sz.c"
+    args = ", ".join(["sizeof(%s)" % struct] + ["offsetof(%s, %s)" % (struct, f) for f in fields])
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "tissue_nr.h"\n'
-                   'int main(void){printf("%zu %zu %zu\\n", sizeof(tf_step_stats),'
-                   ' offsetof(tf_step_stats, du_norms), offsetof(tf_step_stats, ms_total));'
-                   'return 0;}\n')
+                   'int main(void){printf("' + " ".join(["%zu"] * (len(fields) + 1)) +
+                   '\\n", ' + args + ');return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
-    size, off_du, off_ms = map(int, subprocess.check_output([str(exe)]).split())
-    assert ctypes.sizeof(nr.StepStats) == size
-    assert nr.StepStats.du_norms.offset == off_du
-    assert nr.StepStats.ms_total.offset == off_ms
+    return list(map(int, subprocess.check_output([str(exe)]).split()))
+
+
+@pytest.mark.parametrize("struct,pyname", [("tf_step_stats", "StepStats"),
+                                           ("tf_kernel_times", "KernelTimes")])
+def test_struct_layouts_match_header(tmp_path, struct, pyname):
+    from tissueorigami_amd import nr
+    cls = getattr(nr, pyname)
+    fields = [f[0] for f in cls._fields_]
+    size, *offs = _c_layout(tmp_path, struct, fields)
+    assert ctypes.sizeof(cls) == size
+    for f, off in zip(fields, offs):
+        assert getattr(cls, f).offset == off, f
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

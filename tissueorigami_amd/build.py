@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtissue_nr.so")
 DRIVER = os.path.join(HERE, "nr_driver")
-SOURCES = ["tf_element.hip", "tf_assemble.hip", "tf_krylov.hip", "tf_adjoint.hip", "tf_nr.hip"]
+SOURCES = ["tf_element.hip", "tf_assemble.hip", "tf_krylov.hip", "tf_tiled.hip", "tf_adjoint.hip", "tf_nr.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -33,6 +33,23 @@ struct DevPattern {
   int32_t* node_src = nullptr;    // n_prisms * 6: e*6 + a
 };
 
+// Tiles of SELL slices for the fused solver: tile t covers slices
+// [tile_slice[t], tile_slice[t+1]) of tissue tile_tissue[t]; tissue T owns
+// tiles [tissue_tile[T], tissue_tile[T+1]).  grid = n_tiles rounded up to a
+// multiple of 8 (the XCD count).
+struct TileMap {
+  int32_t n_tiles = 0, grid = 0, n_tissues = 0, slices_per_tile = 0;
+  const int32_t* tile_slice = nullptr;
+  const int32_t* tile_tissue = nullptr;
+  const int32_t* tissue_tile = nullptr;
+};
+
+constexpr int kMaxTissues = 4096;
+struct TiledResult {
+  int32_t state[kMaxTissues];
+  int32_t iters[kMaxTissues];
+};
+
 struct Workspace {
   // per-prism outputs (block-major, reference K orientation):
   // elemK[(e*36 + A*6 + B)*9 + r*3 + c] = TriPointKe[3A+r][3B+c]
@@ -58,6 +75,18 @@ struct Workspace {
   int32_t* flag = nullptr;     // [0] state, [1] iterations
   double* host_pinned = nullptr;
   int32_t* host_flag = nullptr;
+  // per-tissue state of the tiled solver
+  int32_t* tstate = nullptr;   // 2 per tissue
+  double* tscal = nullptr;     // 8 per tissue
+  double* tpart = nullptr;     // 5 x tpart_stride
+  int tpart_stride = 0;
+  int32_t* tactive = nullptr;  // per tissue
+  double* tnorm = nullptr;     // per tissue
+  int32_t* host_tstate = nullptr;
+  double* host_tnorm = nullptr;
+  // captured iteration chunks of the tiled solver (hipGraph), see tf_tiled.hip
+  void* graphs = nullptr;
+  int graphs_on = 1;
 };
 
 constexpr int kParts = 2048;  // max workgroups of a reduction grid
@@ -65,6 +94,8 @@ constexpr int kBlock = 256;
 
 enum Scal { S_RHO = 0, S_RHO_NEW, S_ALPHA, S_OMEGA, S_BNORM, S_RNORM, S_COUNT };
 enum KState { K_RUN = 0, K_CONVERGED = 1, K_BREAKDOWN = 2 };
+// A Krylov residual above kDivergence |b| ends the solve as a breakdown.
+constexpr double kDivergence = 1e8;
 
 struct Launch {
   hipStream_t stream;
@@ -133,13 +164,15 @@ hipError_t launch_inverse(int32_t n_prisms, const double* Fg, double* invFg, hip
 // ---- Krylov (tf_krylov.hip) ----
 // Optional event timing of the SpMV launches ([0] FP64 operator, [1] FP32).
 constexpr int kProfPairs = 128;
+// Event timing of the SpMV launches.  kind = 2 * which + (FP32 ? 1 : 0),
+// which 0: the SpMV that forms v = J ph (with s, sh), 1: t = J sh (with dots).
 struct Prof {
   int on = 0;
   hipEvent_t ev[2 * kProfPairs] = {};
   int kind[kProfPairs] = {};
   int used = 0;
-  double ms[2] = {0.0, 0.0};
-  int64_t count[2] = {0, 0};
+  double ms[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t count[4] = {0, 0, 0, 0};
 };
 void prof_harvest(Prof* p);
 
@@ -155,20 +188,29 @@ struct KrylovArgs {
   Workspace* ws;
   Launch L;
   Prof* prof;
-  int fused = 1;  // reduced-synchronisation BiCGStab (no binding)
+  TileMap tm;
 };
 // Solves J x = b (FP64).  Returns the iteration count in *iters, the Krylov
 // state (K_CONVERGED/K_BREAKDOWN/K_RUN=cap reached) in *state.
 hipError_t krylov_bicgstab(const KrylovArgs& k, const double* b, double* x, double rtol,
                            int maxit, int* iters, int* state, int sync_every);
-// Mixed precision: FP32 inner BiCGStab, FP64 residual refinement.
-hipError_t krylov_mixed(const KrylovArgs& k, const double* b, double* x, double rtol,
-                        double inner_rtol, int maxit, int* iters, int* state, int sync_every);
-// y = A x with the binding operator; FP64.
-hipError_t apply_operator(const KrylovArgs& k, const double* x, double* y);
-// |x|_2 and x_k += dx (deterministic two-level reduction), result on host.
-hipError_t norm_and_update(int32_t n, const double* dx, double* xk, Workspace* ws, Launch L,
-                           double* norm_out);
+// Tiled fused BiCGStab on all tissues of the handle (tissues whose tactive
+// entry is 0 get x = 0); per-tissue state and iteration counts in *res.
+hipError_t tiled_bicgstab(const KrylovArgs& k, const double* b, double* x, double rtol, int maxit,
+                          int sync_every, TiledResult* res);
+// Mixed precision on all tissues: per-tissue scaled FP32 inner solves to
+// inner_rtol (at most maxit iterations in all), and FP64 residual refinement
+// while rtol < inner_rtol.  A tissue whose FP32 solve breaks down, diverges
+// or reaches maxit ends with that state.
+hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double rtol,
+                       double inner_rtol, int maxit, int sync_every, TiledResult* res);
+// x_k += dx and per-tissue |dx|_2 into norms (host).
+hipError_t tiled_norm_update(const KrylovArgs& k, const double* dx, double* xk, double* norms);
+hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int iters, float* ms);
+// Releases the captured graphs of a workspace.
+void tiled_release(Workspace* ws);
+// dst = src on the tissues whose tactive entry is 1 (others untouched).
+hipError_t tiled_copy_active(const KrylovArgs& k, const double* src, double* dst);
 
 // ---- adjoint (tf_adjoint.hip) ----
 struct AdjointLaunch {

@@ -25,7 +25,18 @@ void set_error(const std::string& s) { g_err = s; }
 using namespace tf;
 
 struct tf_handle {
-  int32_t Nn = 0, Ne = 0;
+  int32_t Nn = 0, Ne = 0;  // internal node count (tissues padded to 64-node slices) and prisms
+  // tissues of the handle: one for tf_nr_create, several for tf_batch_create
+  int32_t n_tissues = 1;
+  bool batch = false;
+  int32_t n_user = 0;                    // caller's node count (concatenated, unpadded)
+  std::vector<int32_t> user_off, int_off;  // per-tissue node offsets (user / internal)
+  std::vector<int32_t> elem_off;           // per-tissue prism offsets
+  // tiles of the fused solver
+  TileMap tm;
+  int32_t *d_tile_slice = nullptr, *d_tile_tissue = nullptr, *d_tissue_tile = nullptr;
+  int spt = 8;
+  int autotuned_spt = 0;
   int device = 0;
   hipStream_t st = nullptr;
   DevPattern pat;
@@ -291,6 +302,159 @@ tf_status ensure_mixed(tf_handle* h) {
   return TF_OK;
 }
 
+// Tiles of spt slices that never cross a tissue boundary (tissues start on
+// slice boundaries).  Rebuilding replaces the device tile table.
+tf_status build_tiles(tf_handle* h, int spt) {
+  std::vector<int32_t> ts, tt, tissue_tile(h->n_tissues + 1, 0);
+  ts.push_back(0);
+  for (int T = 0; T < h->n_tissues; ++T) {
+    const int s_beg = h->int_off[T] / kSlice;
+    const int s_end = (h->int_off[T + 1] + kSlice - 1) / kSlice;
+    tissue_tile[T] = (int32_t)tt.size();
+    for (int s = s_beg; s < s_end; s += spt) {
+      ts.back() = s;
+      ts.push_back(std::min(s + spt, s_end));
+      tt.push_back(T);
+    }
+  }
+  const int n_tiles = (int)tt.size();
+  tissue_tile[h->n_tissues] = n_tiles;
+  if (h->d_tile_slice) {
+    TF_CHECK(hipFree(h->d_tile_slice));
+    TF_CHECK(hipFree(h->d_tile_tissue));
+    TF_CHECK(hipFree(h->d_tissue_tile));
+  }
+  TF_CHECK(hipMalloc(&h->d_tile_slice, sizeof(int32_t) * (n_tiles + 1)));
+  TF_CHECK(hipMalloc(&h->d_tile_tissue, sizeof(int32_t) * std::max(n_tiles, 1)));
+  TF_CHECK(hipMalloc(&h->d_tissue_tile, sizeof(int32_t) * (h->n_tissues + 1)));
+  TF_CHECK(hipMemcpy(h->d_tile_slice, ts.data(), sizeof(int32_t) * (n_tiles + 1), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->d_tile_tissue, tt.data(), sizeof(int32_t) * n_tiles, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->d_tissue_tile, tissue_tile.data(), sizeof(int32_t) * (h->n_tissues + 1),
+                     hipMemcpyHostToDevice));
+  Workspace& w = h->ws;
+  if (w.tpart_stride < n_tiles) {
+    if (w.tpart) TF_CHECK(hipFree(w.tpart));
+    TF_CHECK(hipMalloc(&w.tpart, sizeof(double) * 5 * (size_t)n_tiles));
+    w.tpart_stride = n_tiles;
+  }
+  h->tm.n_tiles = n_tiles;
+  h->tm.grid = std::max(8, (n_tiles + 7) / 8 * 8);
+  h->tm.n_tissues = h->n_tissues;
+  h->tm.slices_per_tile = spt;
+  h->tm.tile_slice = h->d_tile_slice;
+  h->tm.tile_tissue = h->d_tile_tissue;
+  h->tm.tissue_tile = h->d_tissue_tile;
+  h->spt = spt;
+  return TF_OK;
+}
+
+tf_status alloc_tissue_state(tf_handle* h) {
+  Workspace& w = h->ws;
+  const int nt = h->n_tissues;
+  TF_CHECK(dalloc(h, &w.tstate, 2 * (size_t)nt));
+  TF_CHECK(dalloc(h, &w.tscal, 8 * (size_t)nt));
+  TF_CHECK(dalloc(h, &w.tactive, nt));
+  TF_CHECK(dalloc(h, &w.tnorm, nt));
+  TF_CHECK(hipHostMalloc((void**)&w.host_tstate, 2 * sizeof(int32_t) * nt, hipHostMallocDefault));
+  TF_CHECK(hipHostMalloc((void**)&w.host_tnorm, sizeof(double) * nt, hipHostMallocDefault));
+  std::vector<int32_t> ones(nt, 1);
+  TF_CHECK(hipMemcpy(w.tactive, ones.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice));
+  return TF_OK;
+}
+
+// user (concatenated, unpadded) per-node arrays <-> internal padded layout
+template <typename T>
+std::vector<T> to_internal(const tf_handle* h, const T* user, int comps, T fill) {
+  std::vector<T> out((size_t)h->Nn * comps, fill);
+  for (int t = 0; t < h->n_tissues; ++t) {
+    const int nu = h->user_off[t + 1] - h->user_off[t];
+    std::memcpy(out.data() + (size_t)h->int_off[t] * comps, user + (size_t)h->user_off[t] * comps,
+                sizeof(T) * nu * comps);
+  }
+  return out;
+}
+
+template <typename T>
+void from_internal(const tf_handle* h, const T* internal, T* user, int comps) {
+  for (int t = 0; t < h->n_tissues; ++t) {
+    const int nu = h->user_off[t + 1] - h->user_off[t];
+    std::memcpy(user + (size_t)h->user_off[t] * comps, internal + (size_t)h->int_off[t] * comps,
+                sizeof(T) * nu * comps);
+  }
+}
+
+// Copies a caller node vector (3 per node) to the device in the internal layout.
+tf_status upload_nodes(tf_handle* h, double* dst, const double* user) {
+  if (!h->batch) {
+    TF_CHECK(hipMemcpyAsync(dst, user, sizeof(double) * 3 * h->Nn, hipMemcpyHostToDevice, h->st));
+    return TF_OK;
+  }
+  std::vector<double> tmp = to_internal<double>(h, user, 3, 0.0);
+  TF_CHECK(hipStreamSynchronize(h->st));
+  TF_CHECK(hipMemcpy(dst, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice));
+  return TF_OK;
+}
+
+tf_status download_nodes(tf_handle* h, double* user, const double* src) {
+  if (!h->batch) {
+    TF_CHECK(hipMemcpyAsync(user, src, sizeof(double) * 3 * h->Nn, hipMemcpyDeviceToHost, h->st));
+    TF_CHECK(hipStreamSynchronize(h->st));
+    return TF_OK;
+  }
+  std::vector<double> tmp((size_t)3 * h->Nn);
+  TF_CHECK(hipStreamSynchronize(h->st));
+  TF_CHECK(hipMemcpy(tmp.data(), src, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+  from_internal<double>(h, tmp.data(), user, 3);
+  return TF_OK;
+}
+
+// Common tail of tf_nr_create and tf_batch_create: stream, pattern, slot map,
+// device buffers, tiles.  conn holds internal node ids.  Frees h on failure.
+tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos, tf_handle** out) {
+  tf_status s = TF_OK;
+  auto fail = [&](tf_status st) {
+    tf_nr_destroy(h);
+    return st;
+  };
+  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(TF_ERR_HIP);
+  }
+  for (auto& e : h->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(TF_ERR_HIP);
+  if ((s = build_pattern(h, conn)) != TF_OK) return fail(s);
+  if ((s = alloc_workspace(h)) != TF_OK) return fail(s);
+  const size_t Ne = h->Ne, Nn = h->Nn;
+  if (dalloc(h, &h->conn, Ne * 6) != hipSuccess || dalloc(h, &h->refpos, Ne * 18) != hipSuccess ||
+      dalloc(h, &h->invFg, Ne * 9) != hipSuccess || dalloc(h, &h->lam, Ne) != hipSuccess ||
+      dalloc(h, &h->mu, Ne) != hipSuccess || dalloc(h, &h->eta, Ne) != hipSuccess ||
+      dalloc(h, &h->extv, Nn * 3) != hipSuccess || dalloc(h, &h->area, Nn) != hipSuccess ||
+      dalloc(h, &h->dof_flags, Nn * 3) != hipSuccess) {
+    set_error("hipMalloc failed");
+    return fail(TF_ERR_HIP);
+  }
+  // padding nodes of a batch are fixed in every direction: identity rows
+  std::vector<uint8_t> fu((size_t)h->n_user * 3, 0);
+  std::vector<uint8_t> flags = to_internal<uint8_t>(h, fu.data(), 3, (uint8_t)1);
+  std::vector<double> zero((size_t)Nn * 3, 0.0);
+  if (hipMemcpy(h->conn, conn, Ne * 6 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->refpos, ref_pos, Ne * 18 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->dof_flags, flags.data(), Nn * 3, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->extv, zero.data(), Nn * 3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->area, zero.data(), Nn * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("upload failed");
+    return fail(TF_ERR_HIP);
+  }
+  int grid = (int)std::min<int64_t>(kParts, ((int64_t)Nn + kBlock - 1) / kBlock);
+  grid = std::max(8, (grid + 7) / 8 * 8);
+  h->L.stream = h->st;
+  h->L.grid = grid;
+  if ((s = alloc_tissue_state(h)) != TF_OK) return fail(s);
+  if ((s = build_tiles(h, 8)) != TF_OK) return fail(s);
+  *out = h;
+  return TF_OK;
+}
+
 KrylovArgs krylov_args(tf_handle* h) {
   KrylovArgs k;
   k.n_nodes = h->Nn;
@@ -304,6 +468,7 @@ KrylovArgs krylov_args(tf_handle* h) {
   k.ws = &h->ws;
   k.L = h->L;
   k.prof = &h->prof;
+  k.tm = h->tm;
   return k;
 }
 
@@ -368,39 +533,11 @@ tf_status tf_nr_create(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,
   h->Nn = n_nodes;
   h->Ne = n_prisms;
   h->device = dev;
-  auto fail = [&](tf_status st) {
-    tf_nr_destroy(h);
-    return st;
-  };
-  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
-    set_error("hipStreamCreate failed");
-    return fail(TF_ERR_HIP);
-  }
-  for (auto& e : h->ev)
-    if (hipEventCreate(&e) != hipSuccess) return fail(TF_ERR_HIP);
-  if ((s = build_pattern(h, conn)) != TF_OK) return fail(s);
-  if ((s = alloc_workspace(h)) != TF_OK) return fail(s);
-  const size_t Ne = n_prisms, Nn = n_nodes;
-  if (dalloc(h, &h->conn, Ne * 6) != hipSuccess || dalloc(h, &h->refpos, Ne * 18) != hipSuccess ||
-      dalloc(h, &h->invFg, Ne * 9) != hipSuccess || dalloc(h, &h->lam, Ne) != hipSuccess ||
-      dalloc(h, &h->mu, Ne) != hipSuccess || dalloc(h, &h->eta, Ne) != hipSuccess ||
-      dalloc(h, &h->extv, Nn * 3) != hipSuccess || dalloc(h, &h->area, Nn) != hipSuccess ||
-      dalloc(h, &h->dof_flags, Nn * 3) != hipSuccess) {
-    set_error("hipMalloc failed");
-    return fail(TF_ERR_HIP);
-  }
-  if (hipMemcpy(h->conn, conn, Ne * 6 * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(h->refpos, ref_pos, Ne * 18 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(h->dof_flags, 0, Nn * 3) != hipSuccess) {
-    set_error("upload failed");
-    return fail(TF_ERR_HIP);
-  }
-  int grid = (int)std::min<int64_t>(kParts, ((int64_t)n_nodes + kBlock - 1) / kBlock);
-  grid = std::max(8, (grid + 7) / 8 * 8);
-  h->L.stream = h->st;
-  h->L.grid = grid;
-  *out = h;
-  return TF_OK;
+  h->n_user = n_nodes;
+  h->user_off = {0, n_nodes};
+  h->int_off = {0, n_nodes};
+  h->elem_off = {0, n_prisms};
+  return finish_create(h, conn, ref_pos, out);
 }
 
 tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, const double* eta,
@@ -420,9 +557,15 @@ tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, co
     mu[e] = E[e] / 2.0 / (1 + v);
     if (eta[e] != 0) any_visc = true;
   }
-  std::vector<uint8_t> flags((size_t)Nn * 3, 0);
+  if (h->batch && n_pairs > 0) {
+    set_error("slave/master binding is not supported for a batch of tissues");
+    return TF_ERR_ARG;
+  }
+  std::vector<uint8_t> fu((size_t)h->n_user * 3, 0);
   if (fixed)
-    for (size_t i = 0; i < flags.size(); ++i) flags[i] = fixed[i] ? 1 : 0;
+    for (size_t i = 0; i < fu.size(); ++i) fu[i] = fixed[i] ? 1 : 0;
+  std::vector<uint8_t> flags = to_internal<uint8_t>(h, fu.data(), 3, (uint8_t)1);
+  std::vector<double> ev = to_internal<double>(h, ext_visc, 3, 0.0);
   h->h_pairs.assign(slave_master, slave_master + 2 * (size_t)n_pairs);
   for (int p = 0; p < n_pairs; ++p) {
     const int s = slave_master[2 * p], m = slave_master[2 * p + 1];
@@ -435,7 +578,7 @@ tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, co
   TF_CHECK(hipMemcpy(h->lam, lam.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(h->mu, mu.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(h->eta, eta, sizeof(double) * Ne, hipMemcpyHostToDevice));
-  TF_CHECK(hipMemcpy(h->extv, ext_visc, sizeof(double) * Nn * 3, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->extv, ev.data(), sizeof(double) * Nn * 3, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(h->dof_flags, flags.data(), flags.size(), hipMemcpyHostToDevice));
   if (n_pairs > h->n_pairs || !h->pairs) {
     int32_t* p = nullptr;
@@ -469,7 +612,8 @@ tf_status tf_nr_set_visc_surface(tf_handle* h, const double* area) {
     set_error("tf_nr_set_visc_surface: invalid arguments");
     return TF_ERR_ARG;
   }
-  TF_CHECK(hipMemcpy(h->area, area, sizeof(double) * h->Nn, hipMemcpyHostToDevice));
+  std::vector<double> a = to_internal<double>(h, area, 1, 0.0);
+  TF_CHECK(hipMemcpy(h->area, a.data(), sizeof(double) * h->Nn, hipMemcpyHostToDevice));
   h->have_area = true;
   return TF_OK;
 }
@@ -490,12 +634,19 @@ tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u, const int32_t
   TF_CHECK(hipMemcpy(pl, elem_placement, Ne * sizeof(int32_t), hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(ty, elem_type, Ne * sizeof(int32_t), hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(sp, spans, Ne, hipMemcpyHostToDevice));
-  TF_CHECK(hipMemcpy(h->ws.tmp, u, sizeof(double) * 3 * h->Nn, hipMemcpyHostToDevice));
+  {
+    tf_status up = upload_nodes(h, h->ws.tmp, u);
+    if (up != TF_OK) return up;
+    TF_CHECK(hipStreamSynchronize(h->st));
+  }
   TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, h->ws.tmp, h->pat.node_ptr,
                                h->pat.node_src, pl, ty, sp, h->area, h->st));
   TF_CHECK(hipStreamSynchronize(h->st));
-  if (area_out)
-    TF_CHECK(hipMemcpy(area_out, h->area, sizeof(double) * h->Nn, hipMemcpyDeviceToHost));
+  if (area_out) {
+    std::vector<double> a(h->Nn);
+    TF_CHECK(hipMemcpy(a.data(), h->area, sizeof(double) * h->Nn, hipMemcpyDeviceToHost));
+    from_internal<double>(h, a.data(), area_out, 1);
+  }
   TF_CHECK(hipFree(pl));
   TF_CHECK(hipFree(ty));
   TF_CHECK(hipFree(sp));
@@ -505,6 +656,10 @@ tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u, const int32_t
 
 tf_status tf_nr_assemble(tf_handle* h, const double* uk, const double* un, double dt,
                          double* g_out, double* internal_out) {
+  if (h && h->batch) {
+    set_error("tf_nr_assemble works on single-tissue handles only");
+    return TF_ERR_ARG;
+  }
   if (!h || !uk || !un) {
     set_error("tf_nr_assemble: invalid arguments");
     return TF_ERR_ARG;
@@ -523,6 +678,10 @@ tf_status tf_nr_assemble(tf_handle* h, const double* uk, const double* un, doubl
 
 tf_status tf_nr_solve(tf_handle* h, const double* b, double* x, double rtol, int32_t maxit,
                       int32_t* iterations) {
+  if (h && h->batch) {
+    set_error("tf_nr_solve works on single-tissue handles only");
+    return TF_ERR_ARG;
+  }
   if (!h || !b || !x || maxit <= 0) {
     set_error("tf_nr_solve: invalid arguments");
     return TF_ERR_ARG;
@@ -550,75 +709,148 @@ tf_status tf_nr_solve(tf_handle* h, const double* b, double* x, double rtol, int
 
 namespace {
 
-// The NR loop of Simulation::updateStepNR on device-resident state: w.un and
-// w.uk hold u_n on entry; w.uk holds the converged iterate on exit.
+// Mixed precision: FP32 inner solves are never asked for less than this
+// relative residual (FP64 refinement sweeps go further), and stop after
+// kMixedCap iterations, after which the tissue falls back to FP64.
+constexpr double kMixedInnerRtol = 1e-5;
+constexpr int kMixedCap = 5000;
+
+// The NR loop of Simulation::updateStepNR on device-resident state, run for
+// every tissue of the handle with the tissue's own convergence test: w.un
+// and w.uk hold u_n on entry; w.uk holds the converged iterates on exit.
+// S has one entry per tissue.
 tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed, double rtol,
                   tf_step_stats* S) {
   Workspace& w = h->ws;
+  const int nt = h->n_tissues;
   const int kmax = 50000;
-  int it = 0;
-  bool conv = false, hit_cap = false;
-  tf_status result = TF_OK;
-  while (!conv) {
+  std::vector<int32_t> act(nt, 1), it(nt, 0), cap(nt, 0), fb(nt, 0);
+  std::vector<double> norms(nt, 0.0);
+  for (int T = 0; T < nt; ++T) std::memset(&S[T], 0, sizeof(tf_step_stats));
+  static thread_local TiledResult res, fres;
+  while (true) {
+    bool any = false;
+    for (int T = 0; T < nt; ++T) any |= act[T] != 0;
+    if (!any) break;
+    TF_CHECK(hipMemcpyAsync(w.tactive, act.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
+                            h->st));
     tf_status s = assemble(h, dt, mixed);
     if (s != TF_OK) return s;
     KrylovArgs k = krylov_args(h);
-    int kit = 0, state = 0;
-    if (mixed)
-      TF_CHECK(krylov_mixed(k, w.g, w.du, rtol, std::max(rtol, 1e-6), kmax, &kit, &state, 16));
-    else
+    if (h->n_pairs > 0) {
+      int kit = 0, state = 0;
       TF_CHECK(krylov_bicgstab(k, w.g, w.du, rtol, kmax, &kit, &state, 16));
-    TF_CHECK(launch_bind_increment(h->pairs, h->n_pairs, w.du, h->st));
+      res.state[0] = state;
+      res.iters[0] = kit;
+    } else if (mixed) {
+      TF_CHECK(tiled_mixed(k, w.g, w.du, rtol, std::max(rtol, kMixedInnerRtol), kMixedCap, 16,
+                           &res));
+    } else {
+      TF_CHECK(tiled_bicgstab(k, w.g, w.du, rtol, kmax, 16, &res));
+    }
     TF_CHECK(hipEventRecord(h->ev[3], h->st));
-    double dnorm = 0.0;
-    TF_CHECK(norm_and_update(h->Nn, w.du, w.uk, &w, h->L, &dnorm));
+    TF_CHECK(hipEventSynchronize(h->ev[3]));
     float t_el = 0, t_as = 0, t_kr = 0;
     TF_CHECK(hipEventElapsedTime(&t_el, h->ev[0], h->ev[1]));
     TF_CHECK(hipEventElapsedTime(&t_as, h->ev[1], h->ev[2]));
     TF_CHECK(hipEventElapsedTime(&t_kr, h->ev[2], h->ev[3]));
-    S->ms_element += t_el;
-    S->ms_assembly += t_as;
+    // A tissue whose FP32 solve broke down, diverged or stalled is solved
+    // again with the FP64 operator; the other tissues keep their increments.
+    bool any_fb = false;
+    if (mixed && h->n_pairs == 0)
+      for (int T = 0; T < nt; ++T) {
+        fb[T] = (act[T] && res.state[T] != K_CONVERGED) ? 1 : 0;
+        any_fb |= fb[T] != 0;
+      }
+    if (any_fb) {
+      TF_CHECK(hipMemcpyAsync(w.tactive, fb.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
+                              h->st));
+      s = assemble(h, dt, 0);
+      if (s != TF_OK) return s;
+      k = krylov_args(h);
+      TF_CHECK(tiled_bicgstab(k, w.g, w.tmp, rtol, kmax, 16, &fres));
+      TF_CHECK(tiled_copy_active(k, w.tmp, w.du));
+      TF_CHECK(hipMemcpyAsync(w.tactive, act.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
+                              h->st));
+      TF_CHECK(hipEventRecord(h->ev[3], h->st));
+      TF_CHECK(hipEventSynchronize(h->ev[3]));
+      float t_fb = 0;
+      TF_CHECK(hipEventElapsedTime(&t_fb, h->ev[0], h->ev[3]));
+      t_kr += t_fb;
+      for (int T = 0; T < nt; ++T)
+        if (fb[T]) {
+          res.state[T] = fres.state[T];
+          res.iters[T] += fres.iters[T];
+          S[T].fp64_fallbacks++;
+        }
+    }
+    TF_CHECK(launch_bind_increment(h->pairs, h->n_pairs, w.du, h->st));
+    TF_CHECK(tiled_norm_update(k, w.du, w.uk, norms.data()));
     if (h->prof.on) {
       h->prof_el_ms += t_el;
       h->prof_as_ms += t_as;
       h->prof_assemblies++;
     }
-    S->ms_krylov += t_kr;
-    if (it < TF_MAX_NR_ITERS) {
-      S->krylov_iterations[it] = kit;
-      S->du_norms[it] = dnorm;
-    }
-    S->krylov_iterations_total += kit;
-    if (state == K_BREAKDOWN) {
-      set_error("BiCGStab breakdown in NR iteration " + std::to_string(it));
-      result = TF_ERR_KRYLOV;
-    } else if (state != K_CONVERGED) {
-      set_error("BiCGStab iteration cap in NR iteration " + std::to_string(it));
-      result = TF_ERR_KRYLOV;
-    }
-    if (!std::isfinite(dnorm)) {
-      set_error("NaN/Inf in the NR increment at iteration " + std::to_string(it));
-      result = TF_ERR_NAN;
-    }
-    // NewtonRaphsonSolver::checkConvergenceViaDeltaU (NewtonRaphsonSolver.cpp:807-825);
-    // the increment is applied whether or not it converged (Simulation.cpp:6716-6720)
-    conv = !(dnorm > tol);
-    it++;
-    if (result != TF_OK) break;
-    if (!conv && it > maxit) {
-      conv = true;
-      hit_cap = true;
+    for (int T = 0; T < nt; ++T) {
+      if (!act[T]) continue;
+      tf_step_stats& St = S[T];
+      St.ms_element += t_el;
+      St.ms_assembly += t_as;
+      St.ms_krylov += t_kr;
+      const double dnorm = norms[T];
+      const int kit = res.iters[T];
+      if (it[T] < TF_MAX_NR_ITERS) {
+        St.krylov_iterations[it[T]] = kit;
+        St.du_norms[it[T]] = dnorm;
+      }
+      St.krylov_iterations_total += kit;
+      tf_status result = TF_OK;
+      if (res.state[T] == K_BREAKDOWN) {
+        set_error("BiCGStab breakdown in NR iteration " + std::to_string(it[T]) +
+                  (nt > 1 ? " of tissue " + std::to_string(T) : std::string()));
+        result = TF_ERR_KRYLOV;
+      } else if (res.state[T] != K_CONVERGED) {
+        set_error("BiCGStab iteration cap in NR iteration " + std::to_string(it[T]) +
+                  (nt > 1 ? " of tissue " + std::to_string(T) : std::string()));
+        result = TF_ERR_KRYLOV;
+      }
+      if (!std::isfinite(dnorm)) {
+        set_error("NaN/Inf in the NR increment at iteration " + std::to_string(it[T]));
+        result = TF_ERR_NAN;
+      }
+      // NewtonRaphsonSolver::checkConvergenceViaDeltaU (NewtonRaphsonSolver.cpp:807-825);
+      // the increment is applied whether or not it converged (Simulation.cpp:6716-6720)
+      const bool conv = !(dnorm > tol);
+      it[T]++;
+      if (result != TF_OK) {
+        St.status = result;
+        act[T] = 0;
+      } else if (conv) {
+        St.converged = 1;
+        act[T] = 0;
+      } else if (it[T] > maxit) {
+        cap[T] = 1;
+        act[T] = 0;
+      }
     }
   }
-  S->ms_total = S->ms_element + S->ms_assembly + S->ms_krylov;
-  S->nr_iterations = it;
-  S->converged = (conv && !hit_cap && result == TF_OK) ? 1 : 0;
-  if (result == TF_OK && hit_cap) {
-    set_error("NR did not converge within maxit");
-    result = TF_ERR_NOT_CONVERGED;
+  tf_status worst = TF_OK;
+  for (int T = 0; T < nt; ++T) {
+    tf_step_stats& St = S[T];
+    St.ms_total = St.ms_element + St.ms_assembly + St.ms_krylov;
+    St.nr_iterations = it[T];
+    if (St.status == TF_OK && cap[T]) {
+      set_error("NR did not converge within maxit");
+      St.status = TF_ERR_NOT_CONVERGED;
+    }
+    if (St.status != TF_OK && (worst == TF_OK || worst == TF_ERR_NOT_CONVERGED)) worst = (tf_status)St.status;
   }
-  S->status = result;
-  return result;
+  // every tissue participates again in the next solve
+  std::vector<int32_t> ones(nt, 1);
+  TF_CHECK(hipMemcpyAsync(w.tactive, ones.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
+                          h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  return worst;
 }
 
 }  // namespace
@@ -633,22 +865,22 @@ tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int
     set_error("tf_nr_step: invalid arguments");
     return TF_ERR_ARG;
   }
-  tf_step_stats st_local;
-  tf_step_stats* S = stats ? stats : &st_local;
-  std::memset(S, 0, sizeof(*S));
+  std::vector<tf_step_stats> local(h->n_tissues);
+  tf_step_stats* S = stats ? stats : local.data();
   int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
   if (mixed && h->n_pairs > 0) mixed = 0;  // bound systems use the FP64 operator
   const size_t n = 3 * (size_t)h->Nn;
   Workspace& w = h->ws;
-  TF_CHECK(hipMemcpyAsync(w.un, u_n, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
-  TF_CHECK(hipMemcpyAsync(w.uk, u_n, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  tf_status up = upload_nodes(h, w.un, u_n);
+  if (up != TF_OK) return up;
+  TF_CHECK(hipMemcpyAsync(w.uk, w.un, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
   const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
   tf_status result = nr_loop(h, dt, tol, maxit, mixed, rtol, S);
   if (result != TF_OK && result != TF_ERR_NOT_CONVERGED && result != TF_ERR_KRYLOV &&
       result != TF_ERR_NAN)
     return result;
-  TF_CHECK(hipMemcpyAsync(u_out, w.uk, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
-  TF_CHECK(hipStreamSynchronize(h->st));
+  tf_status dn = download_nodes(h, u_out, w.uk);
+  if (dn != TF_OK) return dn;
   return result;
 }
 
@@ -720,12 +952,16 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
   }
   const size_t n = 3 * (size_t)h->Nn;
   Workspace& w = h->ws;
-  if (u0) TF_CHECK(hipMemcpyAsync(w.uk, u0, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
+  if (u0) {
+    tf_status up = upload_nodes(h, w.uk, u0);
+    if (up != TF_OK) return up;
+  }
   h->have_positions = true;
   int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
   if (mixed && h->n_pairs > 0) mixed = 0;
   const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
   tf_status worst = TF_OK;
+  std::vector<tf_step_stats> S(h->n_tissues);
   for (int s = 0; s < n_steps; ++s) {
     // pre-step producers on the device: growth, then the viscosity surfaces
     // from the positions at the start of the step
@@ -734,16 +970,67 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
                                  h->surf_place, h->surf_type, h->surf_spans, h->area, h->st));
     h->have_area = true;
     TF_CHECK(hipMemcpyAsync(w.un, w.uk, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
-    tf_step_stats S;
-    std::memset(&S, 0, sizeof(S));
-    tf_status r = nr_loop(h, dt, tol, maxit, mixed, rtol, &S);
-    if (stats) stats[s] = S;
+    tf_status r = nr_loop(h, dt, tol, maxit, mixed, rtol, S.data());
+    if (stats) std::memcpy(stats + (size_t)s * h->n_tissues, S.data(), sizeof(tf_step_stats) * h->n_tissues);
     if (r != TF_OK && r != TF_ERR_NOT_CONVERGED) return r;
     if (r != TF_OK) worst = r;
   }
-  if (u_out) TF_CHECK(hipMemcpyAsync(u_out, w.uk, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
+  if (u_out) {
+    tf_status dn = download_nodes(h, u_out, w.uk);
+    if (dn != TF_OK) return dn;
+  }
   TF_CHECK(hipStreamSynchronize(h->st));
   return worst;
+}
+
+tf_status tf_batch_create(int32_t n_tissues, const int32_t* nodes_per_tissue,
+                          const int32_t* prisms_per_tissue, const int32_t* conn,
+                          const double* ref_pos, tf_handle** out) {
+  if (!out || n_tissues <= 0 || n_tissues > kMaxTissues || !nodes_per_tissue ||
+      !prisms_per_tissue || !conn || !ref_pos) {
+    set_error("tf_batch_create: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  *out = nullptr;
+  int dev = 0;
+  tf_status s = check_device(&dev);
+  if (s != TF_OK) return s;
+  tf_handle* h = new tf_handle();
+  auto fail = [&](tf_status st) {
+    tf_nr_destroy(h);
+    return st;
+  };
+  h->batch = true;
+  h->n_tissues = n_tissues;
+  h->device = dev;
+  h->user_off.assign(n_tissues + 1, 0);
+  h->int_off.assign(n_tissues + 1, 0);
+  h->elem_off.assign(n_tissues + 1, 0);
+  for (int t = 0; t < n_tissues; ++t) {
+    if (nodes_per_tissue[t] <= 0 || prisms_per_tissue[t] <= 0) {
+      set_error("tf_batch_create: empty tissue");
+      return fail(TF_ERR_ARG);
+    }
+    h->user_off[t + 1] = h->user_off[t] + nodes_per_tissue[t];
+    // every tissue starts on a slice boundary
+    h->int_off[t + 1] = h->int_off[t] + (nodes_per_tissue[t] + kSlice - 1) / kSlice * kSlice;
+    h->elem_off[t + 1] = h->elem_off[t] + prisms_per_tissue[t];
+  }
+  h->n_user = h->user_off[n_tissues];
+  h->Nn = h->int_off[n_tissues];
+  h->Ne = h->elem_off[n_tissues];
+  std::vector<int32_t> iconn((size_t)h->Ne * 6);
+  for (int t = 0; t < n_tissues; ++t)
+    for (int e = h->elem_off[t]; e < h->elem_off[t + 1]; ++e)
+      for (int a = 0; a < 6; ++a) {
+        const int nd = conn[6 * (size_t)e + a];
+        if (nd < 0 || nd >= nodes_per_tissue[t]) {
+          set_error("tf_batch_create: prism node id outside its tissue");
+          return fail(TF_ERR_ARG);
+        }
+        iconn[6 * (size_t)e + a] = nd + h->int_off[t];
+      }
+  return finish_create(h, iconn.data(), ref_pos, out);
 }
 
 tf_status tf_adj_forward(tf_handle* h, int32_t n_steps, const double* u0,
@@ -756,8 +1043,8 @@ tf_status tf_adj_forward(tf_handle* h, int32_t n_steps, const double* u0,
     set_error("tf_adj_forward: invalid arguments");
     return TF_ERR_ARG;
   }
-  if (h->n_pairs > 0) {
-    set_error("tf_adj_forward: slave/master binding is not supported by the adjoint");
+  if (h->n_pairs > 0 || h->batch) {
+    set_error("tf_adj_forward: binding and batches are not supported by the adjoint");
     return TF_ERR_ARG;
   }
   const size_t n = 3 * (size_t)h->Nn, Ne = h->Ne;
@@ -893,6 +1180,10 @@ tf_status tf_adj_backward(tf_handle* h, const double* inv_Fg_steps, const double
 }
 
 tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz) {
+  if (h && h->batch) {
+    set_error("tf_csr_pattern works on single-tissue handles only");
+    return TF_ERR_ARG;
+  }
   if (!h || !nnz) {
     set_error("tf_csr_pattern: invalid arguments");
     return TF_ERR_ARG;
@@ -921,6 +1212,10 @@ tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz) {
 
 tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja, double* a,
                         int64_t* nnz) {
+  if (h && h->batch) {
+    set_error("tf_nr_get_csr works on single-tissue handles only");
+    return TF_ERR_ARG;
+  }
   if (!h || !nnz || !ia) {
     set_error("tf_nr_get_csr: invalid arguments");
     return TF_ERR_ARG;
@@ -1052,6 +1347,49 @@ tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke
   return TF_OK;
 }
 
+tf_status tf_nr_autotune(tf_handle* h, int32_t precision_mode, int32_t* slices_per_tile,
+                         double* ms_per_iteration) {
+  if (!h || (precision_mode != TF_PREC_FP64 && precision_mode != TF_PREC_MIXED)) {
+    set_error("tf_nr_autotune: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (!h->assembled || (precision_mode == TF_PREC_MIXED && !h->ws.val32)) {
+    set_error("tf_nr_autotune needs an assembled operator of the requested precision");
+    return TF_ERR_ARG;
+  }
+  const int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
+  const int cands[] = {2, 4, 8, 16, 32};
+  int best = h->spt;
+  double best_ms = 1e300;
+  for (int spt : cands) {
+    tf_status s = build_tiles(h, spt);
+    if (s != TF_OK) return s;
+    KrylovArgs k = krylov_args(h);
+    float ms = 0.f;
+    TF_CHECK(tiled_time_iterations(k, mixed, 2, &ms));  // warm-up
+    TF_CHECK(tiled_time_iterations(k, mixed, 10, &ms));
+    const double per = ms / 10.0;
+    if (per < best_ms) {
+      best_ms = per;
+      best = spt;
+    }
+  }
+  tf_status s = build_tiles(h, best);
+  if (s != TF_OK) return s;
+  h->autotuned_spt = best;
+  if (slices_per_tile) *slices_per_tile = best;
+  if (ms_per_iteration) *ms_per_iteration = best_ms;
+  return TF_OK;
+}
+
+tf_status tf_nr_set_tile_size(tf_handle* h, int32_t slices_per_tile) {
+  if (!h || slices_per_tile < 1 || slices_per_tile > 4096) {
+    set_error("tf_nr_set_tile_size: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  return build_tiles(h, slices_per_tile);
+}
+
 tf_status tf_nr_profile(tf_handle* h, int32_t enable) {
   if (!h) {
     set_error("tf_nr_profile: invalid handle");
@@ -1061,8 +1399,10 @@ tf_status tf_nr_profile(tf_handle* h, int32_t enable) {
     for (auto& e : h->prof.ev) TF_CHECK(hipEventCreate(&e));
   h->prof.on = enable ? 1 : 0;
   h->prof.used = 0;
-  h->prof.ms[0] = h->prof.ms[1] = 0.0;
-  h->prof.count[0] = h->prof.count[1] = 0;
+  for (int i = 0; i < 4; ++i) {
+    h->prof.ms[i] = 0.0;
+    h->prof.count[i] = 0;
+  }
   h->prof_el_ms = h->prof_as_ms = 0.0;
   h->prof_assemblies = 0;
   return TF_OK;
@@ -1078,10 +1418,23 @@ tf_status tf_nr_kernel_times(tf_handle* h, tf_kernel_times* out) {
   for (int i = 0; i < 2; ++i) {
     out->spmv_ms[i] = h->prof.ms[i];
     out->spmv_launches[i] = h->prof.count[i];
+    out->spmv_t_ms[i] = h->prof.ms[2 + i];
+    out->spmv_t_launches[i] = h->prof.count[2 + i];
   }
   out->element_ms = h->prof_el_ms;
   out->assembly_ms = h->prof_as_ms;
   out->assemblies = h->prof_assemblies;
+  return TF_OK;
+}
+
+tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable) {
+  if (!h) {
+    set_error("tf_nr_set_graphs: invalid handle");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  h->ws.graphs_on = enable ? 1 : 0;
+  if (!enable) tiled_release(&h->ws);
   return TF_OK;
 }
 
@@ -1098,10 +1451,17 @@ tf_status tf_nr_pattern_info(tf_handle* h, int64_t* n_blocks, int64_t* n_slots) 
 tf_status tf_nr_destroy(tf_handle* h) {
   if (!h) return TF_OK;
   if (h->st) (void)hipStreamSynchronize(h->st);
+  tiled_release(&h->ws);
   for (auto& e : h->prof.ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->adj_ckpt) (void)hipFree(h->adj_ckpt);
+  if (h->d_tile_slice) (void)hipFree(h->d_tile_slice);
+  if (h->d_tile_tissue) (void)hipFree(h->d_tile_tissue);
+  if (h->d_tissue_tile) (void)hipFree(h->d_tissue_tile);
+  if (h->ws.tpart) (void)hipFree(h->ws.tpart);
+  if (h->ws.host_tstate) (void)hipHostFree(h->ws.host_tstate);
+  if (h->ws.host_tnorm) (void)hipHostFree(h->ws.host_tnorm);
   if (h->ws.host_pinned) (void)hipHostFree(h->ws.host_pinned);
   if (h->ws.host_flag) (void)hipHostFree(h->ws.host_flag);
   for (auto& e : h->ev)

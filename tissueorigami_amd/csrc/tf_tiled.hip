@@ -1,0 +1,810 @@
+// Tile-based fused BiCGStab for one or many independent tissues.
+//
+// A tile is a run of consecutive SELL-64 slices of one tissue (the tiles of
+// a tissue never hold rows of another).  One workgroup works on one tile in
+// every launch, so a single launch per stage serves a whole batch of meshes of
+// uneven size.  Every tissue has its own scalars (rho, alpha, omega, |b|), its
+// own convergence flag and iteration count; its dot products are the ordered
+// sum of its tiles' partials.  Nothing a tissue computes depends on the other
+// tissues of the batch, so a tissue solved in a batch gives bitwise the result
+// of the same tissue solved alone with the same tile size.
+//
+// Per iteration (see the derivation in tf_krylov.hip): w = J^T rh once per
+// solve makes (rh, v) = (w, ph) available before the SpMV that computes v.
+//   tz_spmv_v:   convergence test, alpha = rho / (w, ph); v = J ph;
+//                s = r - alpha v; sh = M^-1 s
+//   tz_spmv_t:   t = J sh; partials (t, s), (t, t)
+//   tz_update_x: omega; x += alpha ph + omega sh; r = s - omega t;
+//                partials (rh, r), (r, r)
+//   tz_update_p: beta; p = r + beta (p - omega v); ph = M^-1 p; partial (w, ph)
+// Partial slots: 0 (rh, r), 1 (r, r), 2 (t, s), 3 (t, t), 4 (w, ph).
+// The per-tissue scalar slots are written only by the workgroup of the
+// tissue's first tile, and only slots that no workgroup of the same launch
+// reads.
+#include <cstring>
+#include <type_traits>
+
+#include "tf_internal.h"
+
+namespace tf {
+namespace {
+
+constexpr int kW = kBlock / 64;
+
+__device__ inline int xcd_tile(int bid, int grid) {
+  const int per = grid >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+
+__device__ inline void reduce_store(double* vals, int n, double* const* outs, int slot) {
+  __shared__ double red[4][kW];
+  for (int k = 0; k < n; ++k) {
+    double x = vals[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    vals[k] = x;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0)
+    for (int k = 0; k < n; ++k) red[k][wave] = vals[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < n; ++k) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kW; ++w) s += red[k][w];
+      outs[k][slot] = s;
+    }
+}
+
+// Ordered sum of part[beg..end), identical in every thread.
+__device__ inline double ordered_sum(const double* part, int beg, int end) {
+  __shared__ double red[kW];
+  __shared__ double result;
+  double s = 0.0;
+  for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) s += part[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kW; ++w) t += red[w];
+    result = t;
+  }
+  __syncthreads();
+  return result;
+}
+
+struct TC {
+  TileMap tm;
+  int32_t n_nodes;
+  const int32_t* slice_base;
+  const int32_t* col;
+  const int32_t* tslot;
+  int64_t n_slots;
+  int32_t* state;        // 2 per tissue: [K_* state, iterations]
+  double* scal;          // 8 per tissue
+  double* part;          // 5 slots x stride
+  int stride;
+  const int32_t* active; // per tissue: 0 = skip the solve (x = 0)
+  double rtol;
+};
+
+enum { TS_RHO = 0, TS_ALPHA, TS_OMEGA, TS_BNORM, TS_RNORM };
+
+__device__ inline double* TP(const TC& c, int k) { return c.part + (size_t)k * c.stride; }
+
+struct TileCtx {
+  int L, T, s0, s1, first;
+};
+
+__device__ inline bool tile_ctx(const TC& c, TileCtx& t) {
+  t.L = xcd_tile(blockIdx.x, gridDim.x);
+  if (t.L >= c.tm.n_tiles) return false;
+  t.T = c.tm.tile_tissue[t.L];
+  t.s0 = c.tm.tile_slice[t.L];
+  t.s1 = c.tm.tile_slice[t.L + 1];
+  t.first = (t.L == c.tm.tissue_tile[t.T]);
+  return true;
+}
+
+__device__ inline double tissue_sum(const TC& c, int k, int T) {
+  return ordered_sum(TP(c, k), c.tm.tissue_tile[T], c.tm.tissue_tile[T + 1]);
+}
+
+template <typename T>
+__device__ inline void minv_apply(const T* m, const T* in, T* out, int i) {
+  const T a = in[3 * i], b = in[3 * i + 1], d = in[3 * i + 2];
+  const T* M = m + 9 * (size_t)i;
+  out[3 * i] = M[0] * a + M[1] * b + M[2] * d;
+  out[3 * i + 1] = M[3] * a + M[4] * b + M[5] * d;
+  out[3 * i + 2] = M[6] * a + M[7] * b + M[8] * d;
+}
+
+template <typename T>
+__device__ inline void row_spmv(int sl, int lane, const TC& c, const T* __restrict__ val,
+                                const T* __restrict__ x, T& a0, T& a1, T& a2) {
+  const int64_t base = c.slice_base[sl];
+  const int width = (int)((c.slice_base[sl + 1] - base) >> 6);
+  a0 = 0; a1 = 0; a2 = 0;
+  const T* v0 = val + base + lane;
+  const int32_t* cp = c.col + base + lane;
+  const int64_t ns = c.n_slots;
+  for (int k = 0; k < width; ++k) {
+    const int64_t o = (int64_t)k * 64;
+    const int cc = __builtin_nontemporal_load(cp + o);
+    const T x0 = x[3 * cc], x1 = x[3 * cc + 1], x2 = x[3 * cc + 2];
+    const T* vv = v0 + o;
+    a0 += __builtin_nontemporal_load(vv) * x0 + __builtin_nontemporal_load(vv + ns) * x1 +
+          __builtin_nontemporal_load(vv + 2 * ns) * x2;
+    a1 += __builtin_nontemporal_load(vv + 3 * ns) * x0 + __builtin_nontemporal_load(vv + 4 * ns) * x1 +
+          __builtin_nontemporal_load(vv + 5 * ns) * x2;
+    a2 += __builtin_nontemporal_load(vv + 6 * ns) * x0 + __builtin_nontemporal_load(vv + 7 * ns) * x1 +
+          __builtin_nontemporal_load(vv + 8 * ns) * x2;
+  }
+}
+
+// r = rh = p = b, x = 0 (x = 0 also for tissues that are not solved).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tz_init_a(TC c, const double* b, T* r, T* rh, T* p,
+                                                    T* x) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T bi = (T)b[3 * i + d];
+      r[3 * i + d] = bi; rh[3 * i + d] = bi; p[3 * i + d] = bi; x[3 * i + d] = 0;
+    }
+}
+
+// w = J^T rh through the transpose slot map.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tz_spmv_transposed(TC c, const T* val, const T* x,
+                                                             T* y) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (!c.active[t.T]) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int sl = t.s0 + wave; sl < t.s1; sl += kW) {
+    const int64_t base = c.slice_base[sl];
+    const int width = (int)((c.slice_base[sl + 1] - base) >> 6);
+    T a0 = 0, a1 = 0, a2 = 0;
+    for (int k = 0; k < width; ++k) {
+      const int64_t o = base + (int64_t)k * 64 + lane;
+      const int cc = c.col[o];
+      const int64_t s = c.tslot[o];
+      const T x0 = x[3 * cc], x1 = x[3 * cc + 1], x2 = x[3 * cc + 2];
+      a0 += val[s] * x0 + val[3 * c.n_slots + s] * x1 + val[6 * c.n_slots + s] * x2;
+      a1 += val[c.n_slots + s] * x0 + val[4 * c.n_slots + s] * x1 + val[7 * c.n_slots + s] * x2;
+      a2 += val[2 * c.n_slots + s] * x0 + val[5 * c.n_slots + s] * x1 + val[8 * c.n_slots + s] * x2;
+    }
+    const int row = sl * 64 + lane;
+    if (row < c.n_nodes) { y[3 * row] = a0; y[3 * row + 1] = a1; y[3 * row + 2] = a2; }
+  }
+}
+
+// ph = M^-1 p; partials (rh, r) = (r, r), (r, r), (w, ph); tissue scalars reset.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tz_init_b(TC c, const T* r, const T* w, const T* p,
+                                                    T* ph, const T* minv) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) {
+    minv_apply(minv, p, ph, i);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      acc[0] += (double)r[3 * i + d] * (double)r[3 * i + d];
+      acc[2] += (double)w[3 * i + d] * (double)ph[3 * i + d];
+    }
+  }
+  acc[1] = acc[0];
+  double* outs[3] = {TP(c, 0), TP(c, 1), TP(c, 4)};
+  reduce_store(acc, 3, outs, t.L);
+  if (t.first && threadIdx.x == 0) {
+    c.scal[8 * t.T + TS_OMEGA] = 1.0;
+    c.state[2 * t.T] = c.active[t.T] ? K_RUN : K_CONVERGED;
+    c.state[2 * t.T + 1] = 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tz_spmv_v(TC c, const T* __restrict__ val,
+                                                    const T* __restrict__ ph,
+                                                    const T* __restrict__ r,
+                                                    const T* __restrict__ minv,
+                                                    T* __restrict__ v, T* __restrict__ s,
+                                                    T* __restrict__ sh) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (c.state[2 * t.T] != K_RUN) return;
+  const double rho = tissue_sum(c, 0, t.T);
+  const double rr = tissue_sum(c, 1, t.T);
+  const double wp = tissue_sum(c, 4, t.T);
+  const double rnorm = sqrt(rr);
+  const bool first_it = (c.state[2 * t.T + 1] == 0);
+  double* sc = c.scal + 8 * t.T;
+  const double bnorm = first_it ? rnorm : sc[TS_BNORM];
+  const double alpha = rho / wp;
+  // a NaN or a residual grown far past |b| (divergence, typically of an FP32
+  // solve on an ill-conditioned tissue) is a breakdown, never convergence
+  int state = K_RUN;
+  if (!isfinite(rnorm) || rnorm > kDivergence * bnorm) state = K_BREAKDOWN;
+  else if (rnorm <= c.rtol * bnorm) state = K_CONVERGED;
+  else if (!first_it && sc[TS_OMEGA] == 0.0) state = K_BREAKDOWN;
+  else if (rho == 0.0 || wp == 0.0 || !isfinite(alpha)) state = K_BREAKDOWN;
+  if (t.first && threadIdx.x == 0) {
+    if (first_it) sc[TS_BNORM] = bnorm;
+    sc[TS_RHO] = rho;
+    sc[TS_ALPHA] = alpha;
+    sc[TS_RNORM] = rnorm;
+    if (state != K_RUN) c.state[2 * t.T] = state;
+  }
+  if (state != K_RUN) return;
+  const T ta = (T)alpha;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int sl = t.s0 + wave; sl < t.s1; sl += kW) {
+    T a0, a1, a2;
+    row_spmv(sl, lane, c, val, ph, a0, a1, a2);
+    const int row = sl * 64 + lane;
+    if (row < c.n_nodes) {
+      v[3 * row] = a0; v[3 * row + 1] = a1; v[3 * row + 2] = a2;
+      s[3 * row] = r[3 * row] - ta * a0;
+      s[3 * row + 1] = r[3 * row + 1] - ta * a1;
+      s[3 * row + 2] = r[3 * row + 2] - ta * a2;
+      minv_apply(minv, s, sh, row);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tz_spmv_t(TC c, const T* __restrict__ val,
+                                                    const T* __restrict__ sh,
+                                                    const T* __restrict__ s, T* __restrict__ tv) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (c.state[2 * t.T] != K_RUN) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc[2] = {0.0, 0.0};
+  for (int sl = t.s0 + wave; sl < t.s1; sl += kW) {
+    T a0, a1, a2;
+    row_spmv(sl, lane, c, val, sh, a0, a1, a2);
+    const int row = sl * 64 + lane;
+    if (row < c.n_nodes) {
+      tv[3 * row] = a0; tv[3 * row + 1] = a1; tv[3 * row + 2] = a2;
+      acc[0] += (double)a0 * (double)s[3 * row] + (double)a1 * (double)s[3 * row + 1] +
+                (double)a2 * (double)s[3 * row + 2];
+      acc[1] += (double)a0 * (double)a0 + (double)a1 * (double)a1 + (double)a2 * (double)a2;
+    }
+  }
+  double* outs[2] = {TP(c, 2), TP(c, 3)};
+  reduce_store(acc, 2, outs, t.L);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tz_update_x(TC c, T* x, T* r, const T* rh, const T* ph,
+                                                      const T* sh, const T* s, const T* tv) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (c.state[2 * t.T] != K_RUN) return;
+  const double ts = tissue_sum(c, 2, t.T);
+  const double tt = tissue_sum(c, 3, t.T);
+  const double omega = (tt > 0.0) ? ts / tt : 0.0;
+  double* sc = c.scal + 8 * t.T;
+  const T ta = (T)sc[TS_ALPHA], tw = (T)omega;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  double acc[2] = {0.0, 0.0};
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * i + d;
+      x[k] = x[k] + ta * ph[k] + tw * sh[k];
+      const T rk = s[k] - tw * tv[k];
+      r[k] = rk;
+      acc[0] += (double)rh[k] * (double)rk;
+      acc[1] += (double)rk * (double)rk;
+    }
+  double* outs[2] = {TP(c, 0), TP(c, 1)};
+  reduce_store(acc, 2, outs, t.L);
+  if (t.first && threadIdx.x == 0) {
+    sc[TS_OMEGA] = omega;
+    c.state[2 * t.T + 1] = c.state[2 * t.T + 1] + 1;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tz_update_p(TC c, const T* r, T* p, T* ph, const T* v,
+                                                      const T* w, const T* minv) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (c.state[2 * t.T] != K_RUN) return;
+  const double rho_new = tissue_sum(c, 0, t.T);
+  const double* sc = c.scal + 8 * t.T;
+  const double rho = sc[TS_RHO], alpha = sc[TS_ALPHA], omega = sc[TS_OMEGA];
+  const double beta = (omega != 0.0 && rho != 0.0) ? (rho_new / rho) * (alpha / omega) : 0.0;
+  const T tb = (T)beta, tw = (T)omega;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  double acc[1] = {0.0};
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * i + d;
+      p[k] = r[k] + tb * (p[k] - tw * v[k]);
+    }
+    minv_apply(minv, p, ph, i);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[0] += (double)w[3 * i + d] * (double)ph[3 * i + d];
+  }
+  double* outs[1] = {TP(c, 4)};
+  reduce_store(acc, 1, outs, t.L);
+}
+
+// |dx|^2 per tile (slot 0) and xk += dx.
+__global__ __launch_bounds__(kBlock) void tz_norm_update(TC c, const double* dx, double* xk) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  double acc[1] = {0.0};
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double v = dx[3 * i + d];
+      acc[0] += v * v;
+      xk[3 * i + d] = xk[3 * i + d] + v;
+    }
+  double* outs[1] = {TP(c, 0)};
+  reduce_store(acc, 1, outs, t.L);
+}
+
+// |v|^2 per tile (slot k).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tz_sqnorm(TC c, const T* v, int slot) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  double acc[1] = {0.0};
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[0] += (double)v[3 * i + d] * (double)v[3 * i + d];
+  double* outs[1] = {TP(c, slot)};
+  reduce_store(acc, 1, outs, t.L);
+}
+
+// out[T] = ordered sum of the tissue's tile partials of slot k; one workgroup
+// per tissue.
+__global__ __launch_bounds__(kBlock) void tz_tissue_sums(TC c, int slot, double* out) {
+  const int T = blockIdx.x;
+  if (T >= c.tm.n_tissues) return;
+  const double s = tissue_sum(c, slot, T);
+  if (threadIdx.x == 0) out[T] = s;
+}
+
+// bs = b / |b_T| per tissue (0 for tissues with |b_T| = 0 or not solved).
+__global__ __launch_bounds__(kBlock) void tz_scale(TC c, const double* b, const double* norms,
+                                                   double* bs) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const double nb = norms[t.T];
+  const double f = (nb > 0.0 && c.active[t.T]) ? 1.0 / nb : 0.0;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bs[3 * i + d] = b[3 * i + d] * f;
+}
+
+// x += |b_T| z per tissue.
+__global__ __launch_bounds__(kBlock) void tz_axpy(TC c, const float* z, const double* norms,
+                                                  double* x) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const double nb = c.active[t.T] ? norms[t.T] : 0.0;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[3 * i + d] = x[3 * i + d] + (double)z[3 * i + d] * nb;
+}
+
+// r = b - J x (FP32 values promoted, FP64 accumulation), per-tile |r|^2 in slot 1.
+__global__ __launch_bounds__(kBlock) void tz_residual(TC c, const float* __restrict__ val,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ b,
+                                                      double* __restrict__ r) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc[1] = {0.0};
+  for (int sl = t.s0 + wave; sl < t.s1; sl += kW) {
+    const int64_t base = c.slice_base[sl];
+    const int width = (int)((c.slice_base[sl + 1] - base) >> 6);
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int k = 0; k < width; ++k) {
+      const int64_t o = base + (int64_t)k * 64 + lane;
+      const int cc = c.col[o];
+      const double x0 = x[3 * cc], x1 = x[3 * cc + 1], x2 = x[3 * cc + 2];
+      const int64_t ns = c.n_slots;
+      a0 += (double)val[o] * x0 + (double)val[ns + o] * x1 + (double)val[2 * ns + o] * x2;
+      a1 += (double)val[3 * ns + o] * x0 + (double)val[4 * ns + o] * x1 + (double)val[5 * ns + o] * x2;
+      a2 += (double)val[6 * ns + o] * x0 + (double)val[7 * ns + o] * x1 + (double)val[8 * ns + o] * x2;
+    }
+    const int row = sl * 64 + lane;
+    if (row < c.n_nodes) {
+      const double r0 = b[3 * row] - a0, r1 = b[3 * row + 1] - a1, r2 = b[3 * row + 2] - a2;
+      r[3 * row] = r0; r[3 * row + 1] = r1; r[3 * row + 2] = r2;
+      acc[0] += r0 * r0 + r1 * r1 + r2 * r2;
+    }
+  }
+  double* outs[1] = {TP(c, 1)};
+  reduce_store(acc, 1, outs, t.L);
+}
+
+// dst = src on the tissues being solved.
+__global__ __launch_bounds__(kBlock) void tz_copy_active(TC c, const double* src, double* dst) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (!c.active[t.T]) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dst[3 * i + d] = src[3 * i + d];
+}
+
+__global__ __launch_bounds__(kBlock) void tz_fill(TC c, double* v, double value) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) v[3 * i + d] = value;
+}
+
+TC make_tc(const KrylovArgs& k, double rtol) {
+  TC c;
+  std::memset(&c, 0, sizeof(c));  // padding too: graphs are matched by memcmp
+  c.tm = k.tm;
+  c.n_nodes = k.n_nodes;
+  c.slice_base = k.pat.slice_base;
+  c.col = k.pat.col;
+  c.tslot = k.pat.tslot;
+  c.n_slots = k.pat.n_slots;
+  c.state = k.ws->tstate;
+  c.scal = k.ws->tscal;
+  c.part = k.ws->tpart;
+  c.stride = k.ws->tpart_stride;
+  c.active = k.ws->tactive;
+  c.rtol = rtol;
+  return c;
+}
+
+template <typename T>
+struct TV {
+  T *r, *rh, *p, *v, *ph, *s, *sh, *t, *x, *w;
+  const T* val;
+  const T* minv;
+};
+
+// One chunk of `chunk` iterations (4 launches each), launched directly.
+template <typename T>
+void launch_chunk(const TC& c, const TV<T>& V, int G, hipStream_t st, int chunk, Prof* pf) {
+  auto timed = [&](int which, auto launch) {
+    const bool on = pf && pf->on && pf->used < kProfPairs;
+    if (on) {
+      pf->kind[pf->used] = 2 * which + (sizeof(T) == 4 ? 1 : 0);
+      (void)hipEventRecord(pf->ev[2 * pf->used], st);
+    }
+    launch();
+    if (on) {
+      (void)hipEventRecord(pf->ev[2 * pf->used + 1], st);
+      pf->used++;
+    }
+  };
+  for (int it = 0; it < chunk; ++it) {
+    if (pf && pf->on && pf->used + 2 > kProfPairs) {
+      (void)hipStreamSynchronize(st);
+      prof_harvest(pf);
+    }
+    timed(0, [&] {
+      hipLaunchKernelGGL(tz_spmv_v<T>, dim3(G), dim3(kBlock), 0, st, c, V.val, (const T*)V.ph,
+                         (const T*)V.r, V.minv, V.v, V.s, V.sh);
+    });
+    timed(1, [&] {
+      hipLaunchKernelGGL(tz_spmv_t<T>, dim3(G), dim3(kBlock), 0, st, c, V.val, (const T*)V.sh,
+                         (const T*)V.s, V.t);
+    });
+    hipLaunchKernelGGL(tz_update_x<T>, dim3(G), dim3(kBlock), 0, st, c, V.x, V.r, (const T*)V.rh,
+                       (const T*)V.ph, (const T*)V.sh, (const T*)V.s, (const T*)V.t);
+    hipLaunchKernelGGL(tz_update_p<T>, dim3(G), dim3(kBlock), 0, st, c, (const T*)V.r, V.p, V.ph,
+                       (const T*)V.v, (const T*)V.w, V.minv);
+  }
+}
+
+// Captured chunks: a full chunk of iterations is recorded once into a
+// hipGraph per (precision, kernel arguments, chunk length) and replayed with
+// one hipGraphLaunch.  The kernels read every per-tissue scalar and flag
+// from device memory, so a replay is the same work as the direct launches.
+// The chunk holds kernel launches only (no memset or copy nodes).
+struct ChunkGraph {
+  hipGraphExec_t exec = nullptr;
+  TC c;
+  const void* ptrs[12];
+  int chunk = 0, tsize = 0;
+  uint64_t used = 0;
+};
+constexpr int kGraphSlots = 4;
+struct GraphCache {
+  ChunkGraph g[kGraphSlots];
+  uint64_t clock = 0;
+};
+
+template <typename T>
+hipError_t run_chunk(const KrylovArgs& k, const TC& c, const TV<T>& V, int chunk) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  const int G = k.tm.grid;
+  if (!ws->graphs_on || (k.prof && k.prof->on)) {
+    launch_chunk<T>(c, V, G, st, chunk, k.prof);
+    return hipGetLastError();
+  }
+  GraphCache* gc = static_cast<GraphCache*>(ws->graphs);
+  if (!gc) {
+    gc = new GraphCache();
+    ws->graphs = gc;
+  }
+  const void* ptrs[12] = {V.r, V.rh, V.p, V.v, V.ph, V.s, V.sh, V.t, V.x, V.w, V.val, V.minv};
+  ChunkGraph* hit = nullptr;
+  for (auto& g : gc->g)
+    if (g.exec && g.tsize == (int)sizeof(T) && g.chunk == chunk &&
+        std::memcmp(&g.c, &c, sizeof(TC)) == 0 && std::memcmp(g.ptrs, ptrs, sizeof(ptrs)) == 0) {
+      hit = &g;
+      break;
+    }
+  hipError_t e;
+  if (!hit) {
+    hit = &gc->g[0];
+    for (auto& g : gc->g)
+      if (g.used < hit->used) hit = &g;
+    if (hit->exec) (void)hipGraphExecDestroy(hit->exec);
+    hit->exec = nullptr;
+    if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
+    launch_chunk<T>(c, V, G, st, chunk, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t le = hipGetLastError();
+    e = hipStreamEndCapture(st, &graph);
+    if (le != hipSuccess) e = le;
+    if (e == hipSuccess) e = hipGraphInstantiate(&hit->exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      hit->exec = nullptr;
+      return e;
+    }
+    hit->c = c;
+    std::memcpy(hit->ptrs, ptrs, sizeof(ptrs));
+    hit->chunk = chunk;
+    hit->tsize = (int)sizeof(T);
+  }
+  hit->used = ++gc->clock;
+  return hipGraphLaunch(hit->exec, st);
+}
+
+template <typename T>
+hipError_t tiled_bicgstab_impl(const KrylovArgs& k, const TV<T>& V, const double* b, double rtol,
+                               int maxit, int sync_every, TiledResult* res) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  const int G = k.tm.grid;
+  const int nt = k.tm.n_tissues;
+  TC c = make_tc(k, rtol);
+  hipLaunchKernelGGL(tz_init_a<T>, dim3(G), dim3(kBlock), 0, st, c, b, V.r, V.rh, V.p, V.x);
+  hipLaunchKernelGGL(tz_spmv_transposed<T>, dim3(G), dim3(kBlock), 0, st, c, V.val,
+                     (const T*)V.rh, V.w);
+  hipLaunchKernelGGL(tz_init_b<T>, dim3(G), dim3(kBlock), 0, st, c, (const T*)V.r,
+                     (const T*)V.w, (const T*)V.p, V.ph, V.minv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  Prof* pf = k.prof;
+  int done = 0;
+  while (true) {
+    const int chunk = std::min(sync_every, maxit - done);
+    if ((e = run_chunk<T>(k, c, V, chunk)) != hipSuccess) return e;
+    done += chunk;
+    if (done >= maxit)  // one more convergence test on the last residuals
+      hipLaunchKernelGGL(tz_spmv_v<T>, dim3(G), dim3(kBlock), 0, st, c, V.val, (const T*)V.ph,
+                         (const T*)V.r, V.minv, V.v, V.s, V.sh);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(ws->host_tstate, ws->tstate, 2 * sizeof(int32_t) * nt,
+                            hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    prof_harvest(pf);
+    bool running = false;
+    for (int q = 0; q < nt; ++q) running |= (ws->host_tstate[2 * q] == K_RUN);
+    if (!running || done >= maxit) break;
+  }
+  for (int q = 0; q < nt; ++q) {
+    res->state[q] = ws->host_tstate[2 * q];
+    res->iters[q] = ws->host_tstate[2 * q + 1];
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t tiled_bicgstab(const KrylovArgs& k, const double* b, double* x, double rtol, int maxit,
+                          int sync_every, TiledResult* res) {
+  Workspace* ws = k.ws;
+  TV<double> V{ws->r, ws->rh, ws->p, ws->v, ws->ph, ws->s, ws->sh, ws->t, x, ws->w, k.val, k.minv};
+  return tiled_bicgstab_impl<double>(k, V, b, rtol, maxit, sync_every, res);
+}
+
+hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double rtol,
+                       double inner_rtol, int maxit, int sync_every, TiledResult* res) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  const int G = k.tm.grid;
+  const int nt = k.tm.n_tissues;
+  TC c = make_tc(k, inner_rtol);
+  // x = 0 and |b_T|
+  hipLaunchKernelGGL(tz_sqnorm<double>, dim3(G), dim3(kBlock), 0, st, c, b, 1);
+  hipLaunchKernelGGL(tz_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, 1, ws->tnorm);
+  hipError_t e = hipMemsetAsync(x, 0, sizeof(double) * 3 * (size_t)k.n_nodes, st);
+  if (e != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(ws->host_tnorm, ws->tnorm, sizeof(double) * nt, hipMemcpyDeviceToHost,
+                          st)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  std::vector<double> bnorm(nt), rnorm(nt);
+  std::vector<int32_t> act(nt), act0(nt);
+  if ((e = hipMemcpy(act0.data(), ws->tactive, sizeof(int32_t) * nt, hipMemcpyDeviceToHost)) !=
+      hipSuccess)
+    return e;
+  for (int T = 0; T < nt; ++T) {
+    bnorm[T] = sqrt(ws->host_tnorm[T]);
+    rnorm[T] = bnorm[T];
+    res->state[T] = K_CONVERGED;
+    res->iters[T] = 0;
+    act[T] = act0[T] && bnorm[T] > 0.0;
+  }
+  const double* rsrc = b;
+  TV<float> V{ws->r32, ws->rh32, ws->p32, ws->v32, ws->ph32, ws->s32, ws->sh32, ws->t32, ws->x32,
+              ws->w32, k.val32, k.minv32};
+  TiledResult inner;
+  for (int outer = 0; outer < 20; ++outer) {
+    bool any = false;
+    for (int T = 0; T < nt; ++T) {
+      any |= act[T] != 0;
+      ws->host_tnorm[T] = rnorm[T];
+    }
+    if (!any) break;
+    if ((e = hipMemcpyAsync(ws->tactive, act.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
+                            st)) != hipSuccess)
+      return e;
+    if ((e = hipMemcpyAsync(ws->tnorm, ws->host_tnorm, sizeof(double) * nt, hipMemcpyHostToDevice,
+                            st)) != hipSuccess)
+      return e;
+    // scale each tissue's residual to O(1) before the inner solve rounds it to FP32
+    hipLaunchKernelGGL(tz_scale, dim3(G), dim3(kBlock), 0, st, c, rsrc, ws->tnorm, ws->b);
+    int left = maxit;
+    for (int T = 0; T < nt; ++T) left = std::min(left, maxit - res->iters[T]);
+    e = tiled_bicgstab_impl<float>(k, V, ws->b, inner_rtol, std::max(left, 1), sync_every, &inner);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(tz_axpy, dim3(G), dim3(kBlock), 0, st, c, ws->x32, ws->tnorm, x);
+    for (int T = 0; T < nt; ++T)
+      if (act[T]) {
+        res->iters[T] += inner.iters[T];
+        res->state[T] = inner.state[T];
+      }
+    if (rtol >= inner_rtol) break;  // the NR loop (FP64 residual) is the refinement
+    hipLaunchKernelGGL(tz_residual, dim3(G), dim3(kBlock), 0, st, c, k.val32, x, b, ws->r);
+    hipLaunchKernelGGL(tz_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, 1, ws->tnorm);
+    if ((e = hipMemcpyAsync(ws->host_tnorm, ws->tnorm, sizeof(double) * nt,
+                            hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    rsrc = ws->r;
+    for (int T = 0; T < nt; ++T) {
+      if (!act[T]) continue;
+      rnorm[T] = sqrt(ws->host_tnorm[T]);
+      if (!(rnorm[T] > rtol * bnorm[T])) { res->state[T] = K_CONVERGED; act[T] = 0; }
+      else if (!std::isfinite(rnorm[T]) || inner.state[T] == K_BREAKDOWN) {
+        res->state[T] = K_BREAKDOWN; act[T] = 0;
+      } else if (res->iters[T] >= maxit) { res->state[T] = K_RUN; act[T] = 0; }
+      else res->state[T] = K_RUN;
+    }
+  }
+  // restore the caller's activity mask (act0 is pageable and local: wait for the copy)
+  if ((e = hipMemcpyAsync(ws->tactive, act0.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
+                          st)) != hipSuccess)
+    return e;
+  return hipStreamSynchronize(st);
+}
+
+hipError_t tiled_copy_active(const KrylovArgs& k, const double* src, double* dst) {
+  TC c = make_tc(k, 0.0);
+  hipLaunchKernelGGL(tz_copy_active, dim3(k.tm.grid), dim3(kBlock), 0, k.L.stream, c, src, dst);
+  return hipGetLastError();
+}
+
+hipError_t tiled_norm_update(const KrylovArgs& k, const double* dx, double* xk, double* norms) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  TC c = make_tc(k, 0.0);
+  const int nt = k.tm.n_tissues;
+  hipLaunchKernelGGL(tz_norm_update, dim3(k.tm.grid), dim3(kBlock), 0, st, c, dx, xk);
+  hipLaunchKernelGGL(tz_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, 0, ws->tnorm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(ws->host_tnorm, ws->tnorm, sizeof(double) * nt, hipMemcpyDeviceToHost,
+                          st)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  for (int T = 0; T < nt; ++T) norms[T] = sqrt(ws->host_tnorm[T]);
+  return hipSuccess;
+}
+
+// Times `iters` fused iterations (FP64 or FP32 operator) on the current
+// operator, for the tile-size autotuner.
+hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int iters, float* ms) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  const int G = k.tm.grid;
+  TC c = make_tc(k, 0.0);  // rtol 0: never converges during the timing
+  hipEvent_t e0, e1;
+  hipError_t e;
+  if ((e = hipEventCreate(&e0)) != hipSuccess) return e;
+  if ((e = hipEventCreate(&e1)) != hipSuccess) return e;
+  hipLaunchKernelGGL(tz_fill, dim3(G), dim3(kBlock), 0, st, c, ws->b, 1.0);
+  auto run = [&](auto V) {
+    using T = std::remove_pointer_t<decltype(V.r)>;
+    hipLaunchKernelGGL(tz_init_a<T>, dim3(G), dim3(kBlock), 0, st, c, (const double*)ws->b, V.r,
+                       V.rh, V.p, V.x);
+    hipLaunchKernelGGL(tz_spmv_transposed<T>, dim3(G), dim3(kBlock), 0, st, c, V.val,
+                       (const T*)V.rh, V.w);
+    hipLaunchKernelGGL(tz_init_b<T>, dim3(G), dim3(kBlock), 0, st, c, (const T*)V.r,
+                       (const T*)V.w, (const T*)V.p, V.ph, V.minv);
+    (void)hipEventRecord(e0, st);
+    for (int it = 0; it < iters; ++it) {
+      hipLaunchKernelGGL(tz_spmv_v<T>, dim3(G), dim3(kBlock), 0, st, c, V.val, (const T*)V.ph,
+                         (const T*)V.r, V.minv, V.v, V.s, V.sh);
+      hipLaunchKernelGGL(tz_spmv_t<T>, dim3(G), dim3(kBlock), 0, st, c, V.val, (const T*)V.sh,
+                         (const T*)V.s, V.t);
+      hipLaunchKernelGGL(tz_update_x<T>, dim3(G), dim3(kBlock), 0, st, c, V.x, V.r,
+                         (const T*)V.rh, (const T*)V.ph, (const T*)V.sh, (const T*)V.s,
+                         (const T*)V.t);
+      hipLaunchKernelGGL(tz_update_p<T>, dim3(G), dim3(kBlock), 0, st, c, (const T*)V.r, V.p,
+                         V.ph, (const T*)V.v, (const T*)V.w, V.minv);
+    }
+    (void)hipEventRecord(e1, st);
+  };
+  if (mixed) {
+    TV<float> V{ws->r32, ws->rh32, ws->p32, ws->v32, ws->ph32, ws->s32, ws->sh32, ws->t32,
+                ws->x32, ws->w32, k.val32, k.minv32};
+    run(V);
+  } else {
+    TV<double> V{ws->r, ws->rh, ws->p, ws->v, ws->ph, ws->s, ws->sh, ws->t, ws->x, ws->w, k.val,
+                 k.minv};
+    run(V);
+  }
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  e = hipEventElapsedTime(ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return e;
+}
+
+void tiled_release(Workspace* ws) {
+  GraphCache* gc = static_cast<GraphCache*>(ws->graphs);
+  if (!gc) return;
+  for (auto& g : gc->g)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  delete gc;
+  ws->graphs = nullptr;
+}
+
+}  // namespace tf

@@ -24,6 +24,7 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_profile",
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
+           "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs",
            "tf_nr_destroy", "tf_last_error"]
 
 
@@ -33,7 +34,8 @@ class StepStats(ctypes.Structure):
                 ("krylov_iterations", ctypes.c_int32 * MAX_NR),
                 ("du_norms", ctypes.c_double * MAX_NR),
                 ("ms_element", ctypes.c_double), ("ms_assembly", ctypes.c_double),
-                ("ms_krylov", ctypes.c_double), ("ms_total", ctypes.c_double)]
+                ("ms_krylov", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("fp64_fallbacks", ctypes.c_int32)]
 
     def as_dict(self):
         n = self.nr_iterations
@@ -42,11 +44,12 @@ class StepStats(ctypes.Structure):
                     krylov_iterations=list(self.krylov_iterations[:n]),
                     du_norms=list(self.du_norms[:n]), ms_element=self.ms_element,
                     ms_assembly=self.ms_assembly, ms_krylov=self.ms_krylov,
-                    ms_total=self.ms_total)
+                    ms_total=self.ms_total, fp64_fallbacks=self.fp64_fallbacks)
 
 
 class KernelTimes(ctypes.Structure):
     _fields_ = [("spmv_ms", ctypes.c_double * 2), ("spmv_launches", ctypes.c_int64 * 2),
+                ("spmv_t_ms", ctypes.c_double * 2), ("spmv_t_launches", ctypes.c_int64 * 2),
                 ("element_ms", ctypes.c_double), ("assembly_ms", ctypes.c_double),
                 ("assemblies", ctypes.c_int64)]
 
@@ -98,6 +101,10 @@ def load():
     L.tf_nr_advance.argtypes = [_h, _d, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _d,
                                 ctypes.POINTER(StepStats)]
+    L.tf_batch_create.argtypes = [ctypes.c_int32, _i, _i, _i, _d, ctypes.POINTER(_h)]
+    L.tf_nr_autotune.argtypes = [_h, ctypes.c_int32, _i, _d]
+    L.tf_nr_set_tile_size.argtypes = [_h, ctypes.c_int32]
+    L.tf_nr_set_graphs.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_destroy.argtypes = [_h]
     L.tf_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
@@ -121,6 +128,8 @@ def _check(code):
 
 class NewtonRaphsonSolver:
     """One tissue on one device (a tf_handle)."""
+
+    n_tissues = 1
 
     def __init__(self, n_nodes, conn, refpos):
         L = load()
@@ -186,14 +195,19 @@ class NewtonRaphsonSolver:
              allow_not_converged=False):
         u_n = _f64(u_n).reshape(-1)
         out = np.empty_like(u_n)
-        st = StepStats()
+        st = (StepStats * self.n_tissues)()
         code = load().tf_nr_step(self._h, _p(u_n, _d), float(dt), float(tol), int(maxit),
-                                 PREC[precision], float(krylov_rtol), _p(out, _d),
-                                 ctypes.byref(st))
-        if code == TF_ERR_NOT_CONVERGED and allow_not_converged:
-            return out, st.as_dict()
-        _check(code)
-        return out, st.as_dict()
+                                 PREC[precision], float(krylov_rtol), _p(out, _d), st)
+        if not (code == TF_ERR_NOT_CONVERGED and allow_not_converged):
+            _check(code)
+        return out, self._stats(st, 0)
+
+    def _stats(self, arr, step):
+        """Stats of one step: a dict, or one dict per tissue for a batch."""
+        nt = self.n_tissues
+        if nt == 1:
+            return arr[step].as_dict()
+        return [arr[step * nt + t].as_dict() for t in range(nt)]
 
     # ---- pieces
     def assemble(self, uk, un, dt):
@@ -254,13 +268,28 @@ class NewtonRaphsonSolver:
         """n_steps device-resident steps; returns (positions or None, per-step stats)."""
         u0 = None if u0 is None else _f64(u0).reshape(-1)
         out = np.empty(3 * self.n_nodes) if want_positions else None
-        st = (StepStats * max(n_steps, 1))()
+        st = (StepStats * (max(n_steps, 1) * self.n_tissues))()
         code = load().tf_nr_advance(self._h, _p(u0, _d), int(n_steps), float(dt), float(tol),
                                     int(maxit), PREC[precision], float(krylov_rtol), _p(out, _d),
                                     st)
         if code not in (TF_OK, TF_ERR_NOT_CONVERGED):
             _check(code)
-        return out, [st[i].as_dict() for i in range(n_steps)]
+        return out, [self._stats(st, i) for i in range(n_steps)]
+
+    # ---- tiles of the fused Krylov solver
+    def autotune(self, precision="mixed"):
+        """Times the candidate tile sizes on the assembled operator, keeps the
+        fastest; returns (slices per tile, ms per solver iteration)."""
+        spt = np.zeros(1, np.int32); ms = np.zeros(1)
+        _check(load().tf_nr_autotune(self._h, PREC[precision], _p(spt, _i), _p(ms, _d)))
+        return int(spt[0]), float(ms[0])
+
+    def set_tile_size(self, slices_per_tile):
+        _check(load().tf_nr_set_tile_size(self._h, int(slices_per_tile)))
+
+    def set_graphs(self, enable=True):
+        """Krylov iteration chunks replayed through captured hipGraphs (default on)."""
+        _check(load().tf_nr_set_graphs(self._h, 1 if enable else 0))
 
     # ---- adjoint (implicit-function theorem per step)
     def adj_forward(self, u0, inv_Fg_steps, dt, elem_placement, elem_type, spans, tol=1e-8,
@@ -295,6 +324,9 @@ class NewtonRaphsonSolver:
         _check(load().tf_nr_kernel_times(self._h, ctypes.byref(t)))
         return dict(spmv_ms_fp64=t.spmv_ms[0], spmv_ms_fp32=t.spmv_ms[1],
                     spmv_launches_fp64=t.spmv_launches[0], spmv_launches_fp32=t.spmv_launches[1],
+                    spmv_t_ms_fp64=t.spmv_t_ms[0], spmv_t_ms_fp32=t.spmv_t_ms[1],
+                    spmv_t_launches_fp64=t.spmv_t_launches[0],
+                    spmv_t_launches_fp32=t.spmv_t_launches[1],
                     element_ms=t.element_ms, assembly_ms=t.assembly_ms, assemblies=t.assemblies)
 
     def pattern_info(self):
@@ -309,3 +341,36 @@ class NewtonRaphsonSolver:
         _check(load().tf_nr_element_outputs(self._h, _p(ge, _d), _p(gv, _d), _p(Ke, _d),
                                             _p(Kv, _d)))
         return ge, gv, Ke, Kv
+
+
+class TissueBatch(NewtonRaphsonSolver):
+    """Many independent tissues in one handle (tf_batch_create).
+
+    meshes: sequence of (n_nodes, conn, refpos) with node ids local to each
+    tissue.  Per-node and per-prism arrays of every call are the tissues'
+    arrays concatenated in order; step/advance return one stats dict per
+    tissue.  No reference counterpart: the reference runs one tissue per
+    process."""
+
+    def __init__(self, meshes):
+        L = load()
+        nn = np.array([int(m[0]) for m in meshes], np.int32)
+        conns = [np.ascontiguousarray(m[1], dtype=np.int32).reshape(-1, 6) for m in meshes]
+        ne = np.array([c.shape[0] for c in conns], np.int32)
+        self.n_tissues = len(meshes)
+        self.node_offsets = np.concatenate([[0], np.cumsum(nn)]).astype(np.int64)
+        self.prism_offsets = np.concatenate([[0], np.cumsum(ne)]).astype(np.int64)
+        self.n_nodes = int(self.node_offsets[-1])
+        self.n_prisms = int(self.prism_offsets[-1])
+        self.conn = np.ascontiguousarray(np.concatenate(conns))
+        refpos = np.ascontiguousarray(
+            np.concatenate([_f64(m[2]).reshape(-1, 18) for m in meshes]))
+        h = _h()
+        _check(L.tf_batch_create(self.n_tissues, _p(nn, _i), _p(ne, _i), _p(self.conn, _i),
+                                 _p(refpos, _d), ctypes.byref(h)))
+        self._h = h
+
+    def split_nodes(self, a):
+        """Splits a concatenated per-node array (n_nodes x k) into tissues."""
+        a = np.asarray(a).reshape(self.n_nodes, -1)
+        return [a[self.node_offsets[t]:self.node_offsets[t + 1]] for t in range(self.n_tissues)]

@@ -179,3 +179,53 @@ def growth_increment(rate_per_hour_xy, dt):
     inc[:, 1, 1] = g
     inc[:, 2, 2] = 1.0
     return inc
+
+
+def ellipse_for_prisms(target_prisms, n_layers, aspect=0.7, side=1.0, layer_thickness=2.5):
+    """Ellipse mesh (semi-axes a and aspect*a, lattice side `side`) sized so
+    that n_layers prism layers hold about target_prisms prisms: an equilateral
+    lattice of side s has about pi a b / (sqrt(3)/4 s^2) triangles."""
+    n_tri = target_prisms / n_layers
+    a = np.sqrt(n_tri * np.sqrt(3.0) / 4.0 * side * side / (np.pi * aspect))
+    return ellipse(a, aspect * a, side, layer_thickness * n_layers, n_layers)
+
+
+def batch_ellipses(n=64, prism_range=(2000, 40000), layer_range=(2, 6), E_range=(100.0, 2000.0),
+                   seed=4):
+    """Independent elliptical tissues for a batched run: prism counts
+    log-uniform in prism_range, layer counts uniform in layer_range, aspect
+    ratio uniform in [0.6, 0.9], one Young's modulus per tissue uniform in
+    E_range, nu = 0.3, internal viscosity 10 Pa s, external drag 16000
+    (apical) / 10 (basal), in-plane growth from a per-tissue seeded Gaussian
+    random field (mean 0.02/h, sd 0.01/h, correlation length 15 um).
+    Returns one dict per tissue (mesh, per-prism E/nu/eta, per-node ext_visc,
+    growth rate per prism in 1/h, surface flags, seeds)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(prism_range[0]), np.log(prism_range[1])
+    out = []
+    for t in range(n):
+        target = float(np.exp(rng.uniform(lo, hi)))
+        layers = int(rng.integers(layer_range[0], layer_range[1] + 1))
+        aspect = float(rng.uniform(0.6, 0.9))
+        E = float(rng.uniform(*E_range))
+        m = ellipse_for_prisms(target, layers, aspect)
+        ne, nn = len(m["conn"]), len(m["nodes"])
+        ext = np.zeros((nn, 3))
+        ext[m["node_placement"] == 1] = 16000.0
+        ext[m["node_placement"] == 0] = 10.0
+        cent = m["nodes"][m["conn"]].mean(1)[:, :2]
+        gseed = seed * 1000 + t
+        rate = gaussian_random_field(cent, 0.02, 0.01, 15.0, gseed)
+        place, spans = element_placement(m["conn"], m["node_placement"])
+        etype = element_type(m["conn"], m["node_type"])
+        out.append(dict(mesh=m, E=np.full(ne, E), nu=np.full(ne, 0.3), eta=np.full(ne, 10.0),
+                        ext_visc=ext, rate=rate, place=place, spans=spans, etype=etype,
+                        layers=layers, growth_seed=gseed, target_prisms=target))
+    return out
+
+
+def rates_xyz(rate_per_hour):
+    """Per-prism growth rates (x, y, z) in 1/s for in-plane isotropic growth."""
+    r = np.zeros((len(rate_per_hour), 3))
+    r[:, 0] = r[:, 1] = np.asarray(rate_per_hour) / 3600.0
+    return r
