@@ -35,51 +35,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from tissueorigami_amd import nr, tissue  # noqa: E402
+from tissueorigami_amd.workloads import CONFIGS, make_workload  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-
-CONFIGS = {
-    # (kind, geometry, layers, thickness, dt, E_apical, E_other, nu, eta, growth)
-    "c1": dict(kind="rectangle", lx=10.0, ly=5.0, side=1.0, layers=2, thickness=2.0, dt=10.0,
-               E_ap=1000.0, E_other=1000.0, nu=0.3, eta=0.0, grf=False,
-               desc="rectangle 10x5 um, side 1 um, 2 layers"),
-    "c2": dict(kind="ellipse", a=100.0, b=70.0, side=2.0, layers=4, thickness=20.0, dt=60.0,
-               E_ap=1000.0, E_other=100.0, nu=0.3, eta=10.0, grf=True,
-               desc="ellipse 100x70 um, side 2 um, 4 layers over 20 um"),
-    "c3": dict(kind="ellipse", a=200.0, b=140.0, side=1.0, layers=8, thickness=30.0, dt=60.0,
-               E_ap=1000.0, E_other=100.0, nu=0.3, eta=10.0, grf=True,
-               desc="ellipse 200x140 um, side 1 um, 8 layers over 30 um"),
-}
-
-
-def make_workload(name, seed=2024):
-    c = CONFIGS[name]
-    if c["kind"] == "rectangle":
-        m = tissue.rectangle(c["lx"], c["ly"], c["side"], c["thickness"], c["layers"])
-    else:
-        m = tissue.ellipse(c["a"], c["b"], c["side"], c["thickness"], c["layers"])
-    ne, nn = len(m["conn"]), len(m["nodes"])
-    top = m["layer"] == m["n_layers"] - 1
-    E = np.where(top, c["E_ap"], c["E_other"])
-    nu = np.full(ne, c["nu"])
-    eta = np.full(ne, c["eta"])
-    ext = np.zeros((nn, 3))
-    ext[m["node_placement"] == 1] = 16000.0
-    ext[m["node_placement"] == 0] = 10.0
-    if c["grf"]:
-        cent = m["nodes"][m["conn"]].mean(1)[:, :2]
-        rate = tissue.gaussian_random_field(cent, 0.02, 0.01, 15.0, seed)
-    else:
-        rate = np.where(m["layer"] == 0, 0.01, 0.0)
-    place, spans = tissue.element_placement(m["conn"], m["node_placement"])
-    etype = tissue.element_type(m["conn"], m["node_type"])
-    inc = tissue.growth_increment(rate, c["dt"])
-    rates = np.zeros((ne, 3))
-    rates[:, 0] = rate / 3600.0
-    rates[:, 1] = rate / 3600.0
-    return dict(mesh=m, E=E, nu=nu, eta=eta, ext=ext, inc=inc, rates=rates, dt=c["dt"],
-                place=place, spans=spans, etype=etype, cfg=c, seed=seed)
-
 
 def invert_batch(F):
     return np.linalg.inv(F).reshape(-1, 9)
@@ -127,16 +85,95 @@ def spmv_t_bytes(n_blocks, n_nodes, s):
     return n_blocks * (9 * s + 4) + n_nodes * s * (3 + 3 + 3)
 
 
+def bicgstab_iteration_bytes(n_blocks, n_nodes, s):
+    """Algorithmic bytes of one BiCGStab iteration of the tiled solver: the
+    two fused SpMVs, tz_update_x (reads x, ph, sh, s, t, rh; writes x, r) and
+    tz_update_p (reads r, p, v, w and the block-Jacobi inverse; writes p, ph)."""
+    return (spmv_v_bytes(n_blocks, n_nodes, s) + spmv_t_bytes(n_blocks, n_nodes, s)
+            + n_nodes * s * (8 * 3) + n_nodes * s * (6 * 3 + 9))
+
+
+def assembly_bytes(n_prisms, n_blocks, n_nodes, s, visc):
+    """Algorithmic bytes of the element kernel and the slot-map assembly of one
+    NR iteration (tangent precision s bytes): element inputs (connectivity,
+    reference shape, InvFg, Lame/viscosity, positions at two time levels),
+    18x18 tangents and 18-forces written, then each block gathered once with
+    its source index, the operator (9 values per block) and the block-Jacobi
+    inverse written, node forces read and the residual written."""
+    nk = 2 if visc else 1
+    elem = n_prisms * (24 + 144 + 72 + 24 + 2 * 144 + nk * 324 * s + 2 * 144)
+    asm = n_prisms * 36 * (4 + nk * 9 * s) + n_blocks * 9 * s + n_nodes * 9 * (s + 8)
+    forces = n_prisms * 2 * 144 + n_nodes * (3 * 8 * 4)
+    return elem + asm + forces
+
+
 def profile_step(solver, w, u, Fg, precision, krylov_rtol):
     """One extra, untimed step with per-launch event timing of the SpMVs
-    (direct launches, no graphs); returns the kernel times."""
+    (direct launches, no graphs); returns the kernel times and the step's
+    BiCGStab iterations (the launches that did work: a chunk can end with
+    launches that return at once after convergence)."""
     solver.growth_tensor(Fg)
     solver.profile(True)
-    solver.advance(1, w["dt"], u0=u, precision=precision, krylov_rtol=krylov_rtol,
-                   want_positions=False)
+    _, st = solver.advance(1, w["dt"], u0=u, precision=precision, krylov_rtol=krylov_rtol,
+                           want_positions=False)
     kt = solver.kernel_times()
     solver.profile(False)
+    kt["krylov_iterations"] = st[0]["krylov_iterations_total"]
     return kt
+
+
+def stream_copy_ceiling(mib=1024, reps=10):
+    """Device copy bandwidth on this GPU (GB/s, bytes read + written): the
+    best of the library's plain 16-byte-vector copy kernels over `mib` MiB."""
+    return nr.copy_bandwidth(mib, reps)
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "spmv_traffic_c3.json")
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    k = d.get("kernels", {}).get(kernel)
+    if not k or d.get("workload") != workload:
+        return None
+    return k.get("hbm_bytes_per_launch")
+
+
+def run_adjoint(krylov_rtol=1e-10):
+    """C5: gradient of the apical shape loss after 20 steps with respect to
+    the 64 growth-bump coefficients; cost of the backward sweep against the
+    forward solve (device time and wall time)."""
+    from tissueorigami_amd.workloads import c5_config, growth_steps
+    c = c5_config()
+    m = c["mesh"]
+    s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+    s.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
+    inv, dinv = growth_steps(c["phi"] @ c["coef"], c["dt"], c["n_steps"])
+    u0 = m["nodes"].reshape(-1)
+    # warm-up pair (first launches, graph capture)
+    s.adj_forward(u0, inv, c["dt"], c["place"], c["etype"], c["spans"], tol=1e-8, maxit=20,
+                  krylov_rtol=krylov_rtol)
+    s.adj_backward(dinv, np.ones(3 * len(m["nodes"])), krylov_rtol=krylov_rtol)
+    t0 = time.perf_counter()
+    u, stf = s.adj_forward(u0, inv, c["dt"], c["place"], c["etype"], c["spans"], tol=1e-8,
+                           maxit=20, krylov_rtol=krylov_rtol)
+    t1 = time.perf_counter()
+    x = u.reshape(-1, 3)
+    dL = np.zeros_like(x)
+    dL[c["apical"]] = 2 * (x[c["apical"]] - c["target"])
+    dL_dr, stb = s.adj_backward(dinv, dL.reshape(-1), krylov_rtol=krylov_rtol)
+    t2 = time.perf_counter()
+    s.close()
+    return dict(workload="c5: c1 mesh, 64 Gaussian growth bumps, 20 steps of 10 s",
+                forward_wall_s=t1 - t0, backward_wall_s=t2 - t1,
+                forward_device_ms=stf["ms_total"], backward_device_ms=stb["ms_total"],
+                cost_per_forward_wall=(t2 - t1) / (t1 - t0),
+                cost_per_forward_device=stb["ms_total"] / stf["ms_total"],
+                forward_nr_iterations=stf["nr_iterations"],
+                backward_krylov_iterations=stb["krylov_iterations_total"])
 
 
 def cpu_baseline(w, u, Fg, krylov_its_per_nr, budget_s=25.0):
@@ -160,8 +197,17 @@ def cpu_baseline(w, u, Fg, krylov_its_per_nr, budget_s=25.0):
     cap = int(max(2, min(400, (budget_s - probe) / per_it)))
     du, k, ta, ts = P.nr_iteration_sparse(u, u, w["dt"], 1e-30, cap)
     t_nr = ta + ts / max(k, 1) * krylov_its_per_nr
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return dict(value=1.0 / t_nr, unit="NR iterations/s", cores=oracle.num_threads(),
-                kind="port",
+                kind="port", cpu_model=cpu_model, nproc=os.cpu_count(),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
                 sample=(f"one NR iteration of the same workload on the host: BSR assembly "
                         f"{ta:.2f} s + {k} BiCGStab iterations {ts:.2f} s, extrapolated to "
                         f"{krylov_its_per_nr:.0f} BiCGStab iterations per NR iteration (measured on "
@@ -236,8 +282,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--mixed-rtol", type=float, default=1e-5,
-                    help="relative residual of each FP32 linear solve (the NR loop refines)")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every solver kernel directly (no hipGraph replay)")
+    ap.add_argument("--mixed-rtol", type=float, default=1e-3,
+                    help="relative residual of each FP32 linear solve (inexact Newton forcing "
+                         "term; the NR loop with the FP64 residual refines)")
     ap.add_argument("--fp64-rtol", type=float, default=1e-10)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -256,6 +306,8 @@ def main():
     solver.set_growth_rates(w["rates"])
     solver.set_surface_flags(w["place"], w["etype"], w["spans"])
     setup_s = time.perf_counter() - t0
+    if args.no_graphs:
+        solver.set_graphs(False)
     n_blocks, n_slots = solver.pattern_info()
     print(f"[bench] setup {setup_s:.2f} s, {n_blocks} blocks, {n_slots} SELL slots",
           file=sys.stderr, flush=True)
@@ -265,6 +317,9 @@ def main():
     u_w, Fg_w, _, wall_w = run_steps(solver, w, u0, Fg0, args.warmup, "mixed",
                                      krylov_rtol=args.mixed_rtol)
     print(f"[bench] warmup {args.warmup} steps {wall_w:.2f} s", file=sys.stderr, flush=True)
+    spt, ms_it = solver.autotune("mixed")
+    print(f"[bench] autotuned tile: {spt} slices ({ms_it:.3f} ms per FP32 solver iteration)",
+          file=sys.stderr, flush=True)
     u_m, _, tm, wall_m = run_steps(solver, w, u_w, Fg_w, args.steps, "mixed",
                                    krylov_rtol=args.mixed_rtol)
     print(f"[bench] mixed: {tm['nr_iterations']} NR its, {tm['krylov_iterations']} Krylov its, "
@@ -276,6 +331,7 @@ def main():
     res["value"] = tm["nr_iterations"] / wall_m
     res["ms_per_step"] = 1000.0 * wall_m / args.steps
     fp64 = None
+    td = None
     if not args.no_fp64:
         u_d, _, td, wall_d = run_steps(solver, w, u_w, Fg_w, args.steps, "fp64",
                                        krylov_rtol=args.fp64_rtol)
@@ -300,34 +356,61 @@ def main():
                          dof=3 * nn, dt_s=w["dt"], nr_tol=1e-8, krylov_rtol_fp64=args.fp64_rtol,
                          krylov_rtol_mixed=args.mixed_rtol, precond="node-block Jacobi",
                          krylov="BiCGStab (reduced synchronisation)",
-                         growth_seed=w["seed"], setup_s=setup_s)
+                         growth_seed=w["seed"], setup_s=setup_s, slices_per_tile=spt)
     # roofline of the dominant kernel: the FP32 SpMV v = J ph of the mixed
     # solve (tz_spmv_v<float>), timed with HIP events on the solver's stream
     # in one extra profiled step from the warm state
     kt_m = profile_step(solver, w, u_w, Fg_w, "mixed", args.mixed_rtol)
     res["kernel_times_mixed_step"] = kt_m
+    workload = res["config"]["workload"]
     if kt_m["spmv_launches_fp32"] > 0:
-        avg_s = kt_m["spmv_ms_fp32"] / kt_m["spmv_launches_fp32"] / 1000.0
+        avg_s = kt_m["spmv_ms_fp32"] / kt_m["krylov_iterations"] / 1000.0
         b = spmv_v_bytes(n_blocks, nn, 4)
         ach = b / avg_s / 1e9
         res["roofline"] = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                               frac=ach / HBM_PEAK_GBS, traffic=None, kernel="tz_spmv_v<float>",
-                               algorithmic_bytes=b, avg_launch_ms=avg_s * 1e3,
+                               frac=ach / HBM_PEAK_GBS,
+                               traffic=measured_traffic("tz_spmv_v<float>", workload),
+                               kernel="tz_spmv_v<float>", algorithmic_bytes=b,
+                               avg_launch_ms=avg_s * 1e3,
+                               launches_with_work=kt_m["krylov_iterations"],
                                launches=kt_m["spmv_launches_fp32"])
-        if kt_m["spmv_t_launches_fp32"] > 0:
-            avg_t = kt_m["spmv_t_ms_fp32"] / kt_m["spmv_t_launches_fp32"] / 1000.0
-            bt = spmv_t_bytes(n_blocks, nn, 4)
-            res["roofline_spmv_t_fp32"] = dict(kernel="tz_spmv_t<float>", achieved=bt / avg_t / 1e9,
-                                               frac=bt / avg_t / 1e9 / HBM_PEAK_GBS,
-                                               avg_launch_ms=avg_t * 1e3, algorithmic_bytes=bt)
+        avg_t = kt_m["spmv_t_ms_fp32"] / kt_m["krylov_iterations"] / 1000.0
+        bt = spmv_t_bytes(n_blocks, nn, 4)
+        res["roofline_spmv_t_fp32"] = dict(kernel="tz_spmv_t<float>", achieved=bt / avg_t / 1e9,
+                                           frac=bt / avg_t / 1e9 / HBM_PEAK_GBS,
+                                           avg_launch_ms=avg_t * 1e3, algorithmic_bytes=bt,
+                                           traffic=measured_traffic("tz_spmv_t<float>", workload))
     if fp64:
         kt_d = profile_step(solver, w, u_w, Fg_w, "fp64", args.fp64_rtol)
         if kt_d["spmv_launches_fp64"] > 0:
-            avg_s = kt_d["spmv_ms_fp64"] / kt_d["spmv_launches_fp64"] / 1000.0
+            avg_s = kt_d["spmv_ms_fp64"] / kt_d["krylov_iterations"] / 1000.0
             b = spmv_v_bytes(n_blocks, nn, 8)
             res["roofline_fp64_spmv"] = dict(kernel="tz_spmv_v<double>", achieved=b / avg_s / 1e9,
                                              frac=b / avg_s / 1e9 / HBM_PEAK_GBS,
                                              avg_launch_ms=avg_s * 1e3, algorithmic_bytes=b)
+    # whole NR iteration: element kernel + assembly + k BiCGStab iterations
+    visc = bool(np.any(w["eta"] != 0))
+    for key, t, sz in (("mixed", tm, 4), ("fp64", td if fp64 else None, 8)):
+        if t is None:
+            continue
+        k_per = t["krylov_iterations"] / t["nr_iterations"]
+        bnr = (assembly_bytes(ne, n_blocks, nn, sz, visc)
+               + k_per * bicgstab_iteration_bytes(n_blocks, nn, sz))
+        secs = (t["ms_element"] + t["ms_assembly"] + t["ms_krylov"]) / 1e3 / t["nr_iterations"]
+        res.setdefault("roofline_nr_iteration", {})[key] = dict(
+            algorithmic_bytes=bnr, krylov_iterations=k_per, device_s=secs,
+            achieved=bnr / secs / 1e9, frac=bnr / secs / 1e9 / HBM_PEAK_GBS)
+    ceiling = stream_copy_ceiling()
+    res["stream_copy_gbs"] = ceiling
+    if "roofline" in res:
+        res["roofline"]["frac_of_stream_copy"] = res["roofline"]["achieved"] / ceiling
+    print(f"[bench] STREAM copy ceiling {ceiling:.0f} GB/s", file=sys.stderr, flush=True)
+    if not args.no_adjoint:
+        adj = run_adjoint()
+        res["adjoint"] = adj
+        res["adjoint_cost_per_forward"] = adj["cost_per_forward_device"]
+        print(f"[bench] adjoint: backward/forward {adj['cost_per_forward_device']:.2f} (device), "
+              f"{adj['cost_per_forward_wall']:.2f} (wall)", file=sys.stderr, flush=True)
     if not args.no_batch and args.config == "c3":
         res["batch"] = run_batch(args.steps, args.warmup, args.mixed_rtol)
     if not args.no_cpu_baseline:

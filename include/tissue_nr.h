@@ -252,6 +252,13 @@ typedef struct tf_kernel_times {
 tf_status tf_nr_profile(tf_handle* h, int32_t enable);
 tf_status tf_nr_kernel_times(tf_handle* h, tf_kernel_times* out);
 
+/* Device copy bandwidth: for each of six plain 16-byte-vector copy kernels
+ * (tf_copy_kernel: 1, 2 or 4 vectors per thread, ordinary or non-temporal
+ * accesses) `reps` launches over `bytes` bytes timed with HIP events; the
+ * best rate in GB/s counting bytes read + written.  The STREAM-copy ceiling
+ * the kernels' HBM fractions are set against.  No reference counterpart. */
+tf_status tf_copy_bandwidth(int64_t bytes, int32_t reps, double* gb_per_s);
+
 /* Replays of the Krylov iteration chunks through captured hipGraphs
  * (default on; profiling launches directly).  No reference counterpart. */
 tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable);

@@ -1,29 +1,8 @@
 """Synthetic test problems shared by the oracle and GPU tests."""
 import numpy as np
 
-from tissueorigami_amd import tissue
-
-
-def c1_config(seed=0):
-    """Plain-FP64 small rectangle: 10 x 5 um, side 1 um, 2 prism layers of 1 um
-    (198 nodes, 200 prisms), E = 1000 Pa, nu = 0.3, no internal viscosity,
-    in-plane growth 0.01/h on the basal layer, dt = 10 s.  External drag on
-    the apical and basal surfaces keeps the system non-singular, so no DOF is
-    fixed."""
-    m = tissue.rectangle(10.0, 5.0, 1.0, 2.0, 2)
-    ne = len(m["conn"])
-    nn = len(m["nodes"])
-    E = np.full(ne, 1000.0)
-    nu = np.full(ne, 0.3)
-    eta = np.zeros(ne)
-    ext = np.zeros((nn, 3))
-    ext[m["node_placement"] == 1] = 16000.0
-    ext[m["node_placement"] == 0] = 10.0
-    rate = np.where(m["layer"] == 0, 0.01, 0.0)
-    place, spans = tissue.element_placement(m["conn"], m["node_placement"])
-    etype = tissue.element_type(m["conn"], m["node_type"])
-    return dict(mesh=m, E=E, nu=nu, eta=eta, ext_visc=ext, rate=rate, dt=10.0, n_steps=50,
-                place=place, spans=spans, etype=etype)
+from tissueorigami_amd import tissue, workloads
+from tissueorigami_amd.workloads import c1_config, c5_config  # noqa: F401
 
 
 def perturbed_problem(seed=0, visc=True, fixed=False, lx=4.0, ly=3.0, layers=2):
@@ -58,42 +37,6 @@ def growth_invFg(orc, Fg):
     return orc.inv3_many(Fg.reshape(-1, 9))
 
 
-def c5_config(seed=5):
-    """Adjoint configuration: the C1 mesh with an in-plane growth rate (1/h)
-    given by 64 Gaussian bumps (sigma 12 um) on a seeded jittered 8 x 8 grid,
-    coefficients ~ N(0.02, 0.005^2); 20 steps of 10 s; loss = sum over apical
-    nodes of |x - target|^2 with a seeded target surface."""
-    c = c1_config()
-    m = c["mesh"]
-    rng = np.random.default_rng(seed)
-    gx, gy = np.meshgrid((np.arange(8) + 0.5) * 10.0 / 8, (np.arange(8) + 0.5) * 5.0 / 8)
-    centres = np.stack([gx.ravel(), gy.ravel()], 1) + rng.uniform(-0.3, 0.3, (64, 2))
-    coef = rng.normal(0.02, 0.005, 64)
-    cent = m["nodes"][m["conn"]].mean(1)[:, :2]
-    d2 = ((cent[:, None, :] - centres[None, :, :]) ** 2).sum(-1)
-    phi = np.exp(-d2 / (2 * 12.0 ** 2))          # n_prisms x 64
-    apical = np.where(m["node_placement"] == 1)[0]
-    target = m["nodes"][apical] + rng.normal(0, 0.05, (len(apical), 3))
-    c.update(phi=phi, coef=coef, apical=apical, target=target, n_steps=20, dt=10.0)
-    return c
-
-
 def growth_steps(orc, rate_per_hour, dt, n_steps):
-    """InvFg of every step for a constant in-plane rate, accumulated as
-    ShapeBase::growShapeByFg does (F_g <- dF_g F_g, LU inverse), and its
-    derivative with respect to the rate in 1/s: d/dr diag(g^-n, g^-n, 1) with
-    g = exp(r dt)."""
-    from tissueorigami_amd import tissue
-    ne = len(rate_per_hour)
-    inc = tissue.growth_increment(rate_per_hour, dt)
-    Fg = np.tile(np.eye(3), (ne, 1, 1))
-    inv = np.empty((n_steps, ne, 9))
-    dinv = np.zeros((n_steps, ne, 9))
-    r = np.asarray(rate_per_hour) / 3600.0
-    for s in range(n_steps):
-        Fg = inc @ Fg
-        inv[s] = orc.inv3_many(Fg)
-        d = -(s + 1) * dt * np.exp(-(s + 1) * r * dt)
-        dinv[s, :, 0] = d
-        dinv[s, :, 4] = d
-    return inv, dinv
+    """growth_steps of the package with the oracle's 3x3 inverse."""
+    return workloads.growth_steps(rate_per_hour, dt, n_steps, inv3=orc.inv3_many)

@@ -378,4 +378,50 @@ hipError_t launch_inverse(int32_t n_prisms, const double* Fg, double* invFg, hip
   return hipGetLastError();
 }
 
+namespace {
+// Plain 16-byte-vector copies: the bandwidth reference (STREAM copy).  A
+// workgroup copies U x 256 consecutive vectors, each thread issuing its U
+// loads before its U stores, with ordinary or non-temporal accesses.
+typedef float v4f __attribute__((ext_vector_type(4)));
+template <int U, bool NT>
+__global__ __launch_bounds__(kBlock) void tf_copy_kernel(const v4f* __restrict__ src,
+                                                         v4f* __restrict__ dst, int64_t n4) {
+  const int64_t base = (int64_t)blockIdx.x * (kBlock * U) + threadIdx.x;
+  if (base + (int64_t)(U - 1) * kBlock < n4) {
+    v4f v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      v[u] = NT ? __builtin_nontemporal_load(src + base + u * kBlock) : src[base + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (NT) __builtin_nontemporal_store(v[u], dst + base + u * kBlock);
+      else dst[base + u * kBlock] = v[u];
+    }
+  } else {
+    for (int u = 0; u < U; ++u)
+      if (base + u * kBlock < n4) dst[base + u * kBlock] = src[base + u * kBlock];
+  }
+}
+
+template <int U, bool NT>
+hipError_t launch_copy_t(const void* src, void* dst, int64_t n4, hipStream_t st) {
+  const int64_t grid = (n4 + kBlock * U - 1) / (kBlock * U);
+  if (grid > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((tf_copy_kernel<U, NT>), dim3((unsigned)grid), dim3(kBlock), 0, st,
+                     (const v4f*)src, (v4f*)dst, n4);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_copy(const void* src, void* dst, int64_t n4, int variant, hipStream_t st) {
+  switch (variant) {
+    case 0: return launch_copy_t<1, false>(src, dst, n4, st);
+    case 1: return launch_copy_t<2, false>(src, dst, n4, st);
+    case 2: return launch_copy_t<4, false>(src, dst, n4, st);
+    case 3: return launch_copy_t<1, true>(src, dst, n4, st);
+    case 4: return launch_copy_t<2, true>(src, dst, n4, st);
+    default: return launch_copy_t<4, true>(src, dst, n4, st);
+  }
+}
+
 }  // namespace tf

@@ -207,6 +207,10 @@ hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double r
 // x_k += dx and per-tissue |dx|_2 into norms (host).
 hipError_t tiled_norm_update(const KrylovArgs& k, const double* dx, double* xk, double* norms);
 hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int iters, float* ms);
+// Plain copy of n4 16-byte vectors (bandwidth reference); kCopyVariants
+// shapes (vectors per thread, ordinary or non-temporal accesses).
+constexpr int kCopyVariants = 6;
+hipError_t launch_copy(const void* src, void* dst, int64_t n4, int variant, hipStream_t st);
 // Releases the captured graphs of a workspace.
 void tiled_release(Workspace* ws);
 // dst = src on the tissues whose tactive entry is 1 (others untouched).

@@ -1427,6 +1427,49 @@ tf_status tf_nr_kernel_times(tf_handle* h, tf_kernel_times* out) {
   return TF_OK;
 }
 
+tf_status tf_copy_bandwidth(int64_t bytes, int32_t reps, double* gb_per_s) {
+  if (bytes < (1 << 20) || reps < 1 || !gb_per_s) {
+    set_error("tf_copy_bandwidth: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  int dev = 0;
+  tf_status s = check_device(&dev);
+  if (s != TF_OK) return s;
+  const int64_t n4 = bytes / 16;
+  void *src = nullptr, *dst = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&src, n4 * 16);
+  if (e == hipSuccess) e = hipMalloc(&dst, n4 * 16);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipMemsetAsync(src, 0, n4 * 16, st);
+  double best = 0.0;
+  for (int v = 0; v < kCopyVariants && e == hipSuccess; ++v) {
+    float ms = 0.f;
+    e = launch_copy(src, dst, n4, v, st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy(src, dst, n4, v, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && ms > 0.f)
+      best = std::max(best, 2.0 * (double)(n4 * 16) * reps / (ms * 1e-3) / 1e9);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (src) (void)hipFree(src);
+  if (dst) (void)hipFree(dst);
+  if (e != hipSuccess) {
+    set_error(std::string("tf_copy_bandwidth: ") + hipGetErrorString(e));
+    return TF_ERR_HIP;
+  }
+  *gb_per_s = best;
+  return TF_OK;
+}
+
 tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable) {
   if (!h) {
     set_error("tf_nr_set_graphs: invalid handle");

@@ -126,26 +126,65 @@ __device__ inline void minv_apply(const T* m, const T* in, T* out, int i) {
   out[3 * i + 2] = M[6] * a + M[7] * b + M[8] * d;
 }
 
+// One SELL-64 row (lane) of a slice: a = sum over the row's blocks of
+// J_block x_col.  U consecutive block steps are loaded before any is
+// accumulated, so U column-index loads, 3U gathers and 9U value loads are in
+// flight together; the accumulation order (block by block, in k order) is
+// that of the plain loop, so the result does not depend on U.
+#ifndef TF_SPMV_UNROLL32
+#define TF_SPMV_UNROLL32 4
+#endif
+#ifndef TF_SPMV_UNROLL64
+#define TF_SPMV_UNROLL64 2
+#endif
 template <typename T>
 __device__ inline void row_spmv(int sl, int lane, const TC& c, const T* __restrict__ val,
                                 const T* __restrict__ x, T& a0, T& a1, T& a2) {
+  constexpr int U = sizeof(T) == 4 ? TF_SPMV_UNROLL32 : TF_SPMV_UNROLL64;
   const int64_t base = c.slice_base[sl];
   const int width = (int)((c.slice_base[sl + 1] - base) >> 6);
   a0 = 0; a1 = 0; a2 = 0;
   const T* v0 = val + base + lane;
   const int32_t* cp = c.col + base + lane;
   const int64_t ns = c.n_slots;
-  for (int k = 0; k < width; ++k) {
+  int k = 0;
+  for (; k + U <= width; k += U) {
+    int cc[U];
+    T xv[U][3], m[U][9];
+#pragma unroll
+    for (int u = 0; u < U; ++u) cc[u] = __builtin_nontemporal_load(cp + (int64_t)(k + u) * 64);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T* vv = v0 + (int64_t)(k + u) * 64;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) m[u][q] = __builtin_nontemporal_load(vv + q * ns);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      xv[u][0] = x[3 * cc[u]];
+      xv[u][1] = x[3 * cc[u] + 1];
+      xv[u][2] = x[3 * cc[u] + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a0 += m[u][0] * xv[u][0] + m[u][1] * xv[u][1] + m[u][2] * xv[u][2];
+      a1 += m[u][3] * xv[u][0] + m[u][4] * xv[u][1] + m[u][5] * xv[u][2];
+      a2 += m[u][6] * xv[u][0] + m[u][7] * xv[u][1] + m[u][8] * xv[u][2];
+    }
+  }
+  for (; k < width; ++k) {
     const int64_t o = (int64_t)k * 64;
     const int cc = __builtin_nontemporal_load(cp + o);
-    const T x0 = x[3 * cc], x1 = x[3 * cc + 1], x2 = x[3 * cc + 2];
     const T* vv = v0 + o;
-    a0 += __builtin_nontemporal_load(vv) * x0 + __builtin_nontemporal_load(vv + ns) * x1 +
-          __builtin_nontemporal_load(vv + 2 * ns) * x2;
-    a1 += __builtin_nontemporal_load(vv + 3 * ns) * x0 + __builtin_nontemporal_load(vv + 4 * ns) * x1 +
-          __builtin_nontemporal_load(vv + 5 * ns) * x2;
-    a2 += __builtin_nontemporal_load(vv + 6 * ns) * x0 + __builtin_nontemporal_load(vv + 7 * ns) * x1 +
-          __builtin_nontemporal_load(vv + 8 * ns) * x2;
+    const T m0 = __builtin_nontemporal_load(vv), m1 = __builtin_nontemporal_load(vv + ns),
+            m2 = __builtin_nontemporal_load(vv + 2 * ns), m3 = __builtin_nontemporal_load(vv + 3 * ns),
+            m4 = __builtin_nontemporal_load(vv + 4 * ns), m5 = __builtin_nontemporal_load(vv + 5 * ns),
+            m6 = __builtin_nontemporal_load(vv + 6 * ns), m7 = __builtin_nontemporal_load(vv + 7 * ns),
+            m8 = __builtin_nontemporal_load(vv + 8 * ns);
+    const T x0 = x[3 * cc], x1 = x[3 * cc + 1], x2 = x[3 * cc + 2];
+    a0 += m0 * x0 + m1 * x1 + m2 * x2;
+    a1 += m3 * x0 + m4 * x1 + m5 * x2;
+    a2 += m6 * x0 + m7 * x1 + m8 * x2;
   }
 }
 

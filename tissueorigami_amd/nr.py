@@ -25,6 +25,7 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
            "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs",
+           "tf_copy_bandwidth",
            "tf_nr_destroy", "tf_last_error"]
 
 
@@ -105,6 +106,7 @@ def load():
     L.tf_nr_autotune.argtypes = [_h, ctypes.c_int32, _i, _d]
     L.tf_nr_set_tile_size.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_set_graphs.argtypes = [_h, ctypes.c_int32]
+    L.tf_copy_bandwidth.argtypes = [ctypes.c_int64, ctypes.c_int32, _d]
     L.tf_nr_destroy.argtypes = [_h]
     L.tf_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
@@ -124,6 +126,13 @@ def _f64(a):
 def _check(code):
     if code != TF_OK:
         raise TFError(code, load().tf_last_error().decode())
+
+
+def copy_bandwidth(mib=1024, reps=10):
+    """GB/s (read + write) of the library's plain copy kernel over mib MiB."""
+    out = np.zeros(1)
+    _check(load().tf_copy_bandwidth(int(mib) << 20, int(reps), _p(out, _d)))
+    return float(out[0])
 
 
 class NewtonRaphsonSolver:
