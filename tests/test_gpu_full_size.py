@@ -1,0 +1,96 @@
+"""Parity at the benchmark's sizes, through properties that do not depend on
+size: C2 (51 k prisms) against the oracle's sparse step, C3 (1.62 M prisms)
+through agreement of the two precisions, the residual at the converged
+positions and bitwise reproducibility.  Every test here needs an MI355X."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def nr():
+    from tissueorigami_amd import nr as _nr
+    _nr.load()
+    return _nr
+
+
+def _solver(nr, w):
+    m = w["mesh"]
+    s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+    s.set_material(w["E"], w["nu"], w["eta"], w["ext"])
+    return s
+
+
+def test_c2_step_matches_oracle_sparse_step(orc, nr):
+    """One C2 time step (ellipse 100 x 70 um, 4 layers, viscous, random growth
+    field): FP64 device step against the oracle's BSR + BiCGStab step, both
+    with a 1e-12 linear solve; positions within 1e-9 relative, the same NR
+    iteration count and the same |du| sequence (relative 1e-6 where |du| is
+    above the linear-solve noise)."""
+    from tissueorigami_amd.workloads import make_workload
+    w = make_workload("c2")
+    m = w["mesh"]
+    nn = len(m["nodes"])
+    inv = orc.inv3_many(w["inc"])
+    u0 = m["nodes"].reshape(-1).copy()
+    s = _solver(nr, w)
+    s.set_growth(inv)
+    A = s.update_visc_surface(u0, w["place"], w["etype"], w["spans"])
+    A_o = orc.visc_surfaces(nn, m["conn"], u0, w["place"], w["etype"], w["spans"])
+    assert np.abs(A - A_o).max() <= 1e-12 * np.abs(A_o).max()
+    ug, st = s.step(u0, w["dt"], 1e-8, 20, "fp64", 1e-12)
+    lam, mu = orc.lame(w["E"], w["nu"])
+    P = orc.Problem(m["conn"], m["refpos"], inv, lam, mu, w["eta"], w["ext"], A_o)
+    uo, du, kit, conv = P.nr_step_sparse(u0, w["dt"], 1e-8, 20, 1e-12, 50000)
+    assert st["converged"] and conv
+    assert st["nr_iterations"] == len(du)
+    big = du > 1e-6
+    assert np.allclose(np.array(st["du_norms"])[big], du[big], rtol=1e-6)
+    assert np.abs(ug - uo).max() <= 1e-9 * np.abs(uo).max()
+    s.close()
+
+
+@pytest.fixture(scope="module")
+def c3(nr):
+    from tissueorigami_amd.workloads import make_workload
+    w = make_workload("c3")
+    s = _solver(nr, w)
+    s.set_growth_rates(w["rates"])
+    s.set_surface_flags(w["place"], w["etype"], w["spans"])
+    yield w, s
+    s.close()
+
+
+def _c3_step(w, s, precision, rtol):
+    ne = len(w["mesh"]["conn"])
+    s.growth_tensor(np.tile(np.eye(3).reshape(9), (ne, 1)))
+    u, st = s.advance(1, w["dt"], u0=w["mesh"]["nodes"].reshape(-1), precision=precision,
+                      krylov_rtol=rtol)
+    return u, st[0]
+
+
+def test_c3_mixed_equals_fp64_and_is_reproducible(c3):
+    """1.62 M prisms: the mixed step (FP32 tangents and Krylov, inexact
+    Newton, FP64 residual) lands within 1e-8 relative of the FP64 step, and
+    repeating each gives the same bits."""
+    w, s = c3
+    um, stm = _c3_step(w, s, "mixed", 1e-3)
+    ud, std = _c3_step(w, s, "fp64", 1e-10)
+    assert stm["converged"] and std["converged"]
+    assert np.abs(um - ud).max() <= 1e-8 * np.abs(ud).max()
+    um2, stm2 = _c3_step(w, s, "mixed", 1e-3)
+    assert np.array_equal(um, um2)
+    assert stm["krylov_iterations"] == stm2["krylov_iterations"]
+
+
+def test_c3_converged_positions_are_a_root(c3):
+    """The FP64 residual assembled at the converged positions of the mixed
+    step is below 1e-6 of the residual at the start of the step."""
+    w, s = c3
+    u0 = w["mesh"]["nodes"].reshape(-1)
+    um, st = _c3_step(w, s, "mixed", 1e-3)
+    # the growth tensor and the surfaces of that step are the handle's state now
+    g0, _ = s.assemble(u0, u0, w["dt"])
+    g1, _ = s.assemble(um, u0, w["dt"])
+    assert np.linalg.norm(g1) <= 1e-6 * np.linalg.norm(g0)

@@ -1,0 +1,86 @@
+"""Readers and writers of the reference's save-file formats
+(tissueorigami_amd/io.py).  The writers are checked byte for byte against the
+reference's own run07007 regression outputs through the SHA-256 recorded when
+the fixture was extracted; the readers through round trips."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import replay
+from tissueorigami_amd import io
+
+
+@pytest.fixture(scope="module")
+def d():
+    return replay.load()
+
+
+def _sha(path):
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+def test_frame_writer_reproduces_reference_save_frame(d, tmp_path):
+    p = tmp_path / "Save_Frame"
+    io.write_frames(p, d["times"], d["frames"], d["node_info"], d["conn"], d["refpos"])
+    assert _sha(p) == str(d["sha256_Save_Frame"])
+    fr = io.read_frames(p)
+    assert np.array_equal(fr["positions"], d["frames"])
+    assert np.array_equal(fr["times"], d["times"])
+    assert all(np.array_equal(c, d["conn"]) for c in fr["conn"])
+
+
+def test_binary_writers_reproduce_reference_files(d, tmp_path):
+    nf, ne = d["Fg"].shape[:2]
+    nn = d["forces"].shape[1]
+    io.write_growth(tmp_path / "g", d["Fg"])
+    io.write_growth_rate(tmp_path / "r", d["growth_rate"])
+    io.write_physical_prop(tmp_path / "pp", d["E"], d["eta"], d["zremodel"], d["ext_visc"])
+    io.write_forces(tmp_path / "f", d["forces"])
+    io.write_tension_compression(tmp_path / "tc", d["strain"])
+    io.write_growth_redistribution(tmp_path / "gr", d["redistribution"])
+    for k in range(int(d["n_packing_frames"])):
+        io.write_packing(tmp_path / "pk", np.zeros((0, 2)), np.zeros((0, 3)), np.zeros((0, 3)),
+                         mode="a")
+    for k in range(int(d["n_binding_frames"])):
+        io.write_node_binding(tmp_path / "nb", np.zeros((0, 2)), mode="a")
+    io.write_specific_types(tmp_path / "st", d["actin"], d["ecm"])
+    for name, f in (("Save_Growth", "g"), ("Save_GrowthRate", "r"),
+                    ("Save_PhysicalProp", "pp"), ("Save_Force", "f"),
+                    ("Save_TensionCompression", "tc"), ("Save_GrowthRedistribution", "gr"),
+                    ("Save_Packing", "pk"), ("Save_NodeBinding", "nb"),
+                    ("Save_SpecificElementAndNodeTypes", "st")):
+        assert _sha(tmp_path / f) == str(d["sha256_" + name]), name
+    assert np.array_equal(io.read_growth(tmp_path / "g", ne).reshape(nf, ne, 9), d["Fg"])
+    pp = io.read_physical_prop(tmp_path / "pp", ne, nn)
+    assert np.array_equal(pp["ext_visc"], d["ext_visc"]) and np.array_equal(pp["E"], d["E"])
+
+
+def test_round_trips(tmp_path):
+    rng = np.random.default_rng(3)
+    from tissueorigami_amd import tissue
+    m = tissue.rectangle(4.0, 3.0, 1.0, 2.0, 2)
+    info = np.stack([m["node_placement"], m["node_type"], m["at_circumference"]], 1)
+    io.write_mesh(tmp_path / "t.mesh", m["nodes"], info, m["conn"], m["refpos"])
+    nodes, ninfo, conn, refpos = io.read_mesh(tmp_path / "t.mesh")
+    assert np.array_equal(nodes, m["nodes"]) and np.array_equal(ninfo, info)
+    assert np.array_equal(conn, m["conn"]) and np.array_equal(refpos, m["refpos"])
+    pairs = rng.integers(0, 100, (5, 2))
+    io.write_node_binding(tmp_path / "nb", pairs)
+    io.write_node_binding(tmp_path / "nb", pairs[:2], mode="a")
+    nb = io.read_node_binding(tmp_path / "nb")
+    assert np.array_equal(nb[0], pairs) and np.array_equal(nb[1], pairs[:2])
+    f0, f1 = rng.normal(size=(5, 3)), rng.normal(size=(5, 3))
+    io.write_packing(tmp_path / "pk", pairs, f0, f1)
+    (pp, ff), = io.read_packing(tmp_path / "pk")
+    assert np.array_equal(pp, pairs) and np.array_equal(ff, np.concatenate([f0, f1], 1))
+    io.write_specific_types(tmp_path / "st", [1, 2], [3], [[4, 1]], [[5, 2], [6, 2]])
+    st = io.read_specific_types(tmp_path / "st")
+    assert st["actin"].tolist() == [1, 2] and st["ecm"].tolist() == [3]
+    assert st["node_bands"].tolist() == [[5, 2], [6, 2]]
+
+
+def test_du_log_reader_matches_fixture(d):
+    du = d["du_log"]
+    assert du.shape[0] >= 10
+    assert np.all(du[:, 0] > 0)
