@@ -49,7 +49,7 @@ def run_steps(solver, w, u0, Fg0, n_steps, precision, krylov_rtol=1e-10):
     returns the final state and accumulated statistics."""
     solver.growth_tensor(Fg0)
     tot = dict(nr_iterations=0, krylov_iterations=0, ms_element=0.0, ms_assembly=0.0,
-               ms_krylov=0.0, converged=True, per_step=[], fp64_fallbacks=0)
+               ms_krylov=0.0, converged=True, per_step=[], fp64_fallbacks=0, cg_fallbacks=0)
     # positions and growth state are uploaded before the timed region
     solver.advance(0, w["dt"], u0=u0, want_positions=False)
     t0 = time.perf_counter()
@@ -63,6 +63,7 @@ def run_steps(solver, w, u0, Fg0, n_steps, precision, krylov_rtol=1e-10):
             tot[k] += st[k]
         tot["converged"] &= st["converged"]
         tot["fp64_fallbacks"] += st["fp64_fallbacks"]
+        tot["cg_fallbacks"] += st["cg_fallbacks"]
         tot["per_step"].append(dict(nr=st["nr_iterations"], krylov=st["krylov_iterations"],
                                     du=[float("%.3e" % d) for d in st["du_norms"]]))
         print(f"[bench]   {precision} step: NR {st['nr_iterations']} krylov {st['krylov_iterations']} "
@@ -105,6 +106,19 @@ def assembly_bytes(n_prisms, n_blocks, n_nodes, s, visc):
     asm = n_prisms * 36 * (4 + nk * 9 * s) + n_blocks * 9 * s + n_nodes * 9 * (s + 8)
     forces = n_prisms * 2 * 144 + n_nodes * (3 * 8 * 4)
     return elem + asm + forces
+
+
+def cg_spmv_bytes(n_blocks, n_nodes, s):
+    """tzc_spmv (q = J p with p = z + beta p_old formed on the fly, dot (p, q)):
+    block values and columns; per node z and p_old gathered once each, p and q
+    written."""
+    return n_blocks * (9 * s + 4) + n_nodes * s * (3 + 3 + 3 + 3)
+
+
+def cg_iteration_bytes(n_blocks, n_nodes, s):
+    """One CG iteration: tzc_spmv and tzc_update (reads x, p, r, q and the
+    block-Jacobi inverse; writes x, r, z)."""
+    return cg_spmv_bytes(n_blocks, n_nodes, s) + n_nodes * s * (6 + 3 + 6 + 3 + 9 + 3)
 
 
 def profile_step(solver, w, u, Fg, precision, krylov_rtol):
@@ -214,7 +228,7 @@ def cpu_baseline(w, u, Fg, krylov_its_per_nr, budget_s=25.0):
                         f"the GPU run); Pardiso substituted by the same block-Jacobi BiCGStab"))
 
 
-def run_batch(steps, warmup, mixed_rtol, seed=4, n=64, sequential_sample=8):
+def run_batch(steps, warmup, mixed_rtol, mixed_krylov="cg", seed=4, n=64, sequential_sample=8):
     """The batched sweep: returns a dict for the JSON line."""
     ts = tissue.batch_ellipses(n, seed=seed)
     dt = 60.0
@@ -225,6 +239,7 @@ def run_batch(steps, warmup, mixed_rtol, seed=4, n=64, sequential_sample=8):
     b.set_material(cat("E"), cat("nu"), cat("eta"), cat("ext_visc"))
     b.set_growth_rates(np.concatenate([tissue.rates_xyz(c["rate"]) for c in ts]))
     b.set_surface_flags(cat("place"), cat("etype"), cat("spans"))
+    b.set_krylov("mixed", mixed_krylov)
     setup_s = time.perf_counter() - t0
     u0 = np.concatenate([c["mesh"]["nodes"].reshape(-1) for c in ts])
     b.advance(max(warmup, 1), dt, u0=u0, precision="mixed", krylov_rtol=mixed_rtol,
@@ -238,6 +253,7 @@ def run_batch(steps, warmup, mixed_rtol, seed=4, n=64, sequential_sample=8):
     nr_its = sum(st["nr_iterations"] for step in stats for st in step)
     conv = all(st["converged"] for step in stats for st in step)
     fb = sum(st["fp64_fallbacks"] for step in stats for st in step)
+    cgfb = sum(st["cg_fallbacks"] for step in stats for st in step)
     # launches are shared: the batch's wall time per step is that of its
     # slowest tissue; the rate counts every tissue's NR iterations
     b.close()
@@ -252,6 +268,7 @@ def run_batch(steps, warmup, mixed_rtol, seed=4, n=64, sequential_sample=8):
         s1.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
         s1.set_growth_rates(tissue.rates_xyz(c["rate"]))
         s1.set_surface_flags(c["place"], c["etype"], c["spans"])
+        s1.set_krylov("mixed", mixed_krylov)
         s1.advance(max(warmup, 1), dt, u0=m["nodes"].reshape(-1), precision="mixed",
                    krylov_rtol=mixed_rtol, want_positions=False)
         t1 = time.perf_counter()
@@ -265,7 +282,8 @@ def run_batch(steps, warmup, mixed_rtol, seed=4, n=64, sequential_sample=8):
                nodes=int(sum(len(c["mesh"]["nodes"]) for c in ts)), seed=seed, dt_s=dt,
                steps=steps, slices_per_tile=spt, nr_iterations=nr_its,
                nr_iterations_per_s=nr_its / wall, time_steps_per_s=steps / wall, wall_s=wall,
-               converged=conv, fp64_fallbacks=fb, setup_s=setup_s,
+               converged=conv, fp64_fallbacks=fb, cg_fallbacks=cgfb, krylov_mixed=mixed_krylov,
+               setup_s=setup_s,
                sequential_estimate_s=seq_est, batch_speedup_vs_sequential=seq_est / wall,
                sequential_sample=f"{sequential_sample} of the {n} tissues, scaled by prism count")
     print(f"[bench] batch: {steps} steps {wall:.2f} s, {nr_its} NR its; sequential estimate "
@@ -289,6 +307,8 @@ def main():
                     help="relative residual of each FP32 linear solve (inexact Newton forcing "
                          "term; the NR loop with the FP64 residual refines)")
     ap.add_argument("--fp64-rtol", type=float, default=1e-10)
+    ap.add_argument("--mixed-krylov", default="cg", choices=["cg", "bicgstab"],
+                    help="Krylov method of the FP32 inner solves (FP64 uses BiCGStab)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -308,6 +328,7 @@ def main():
     setup_s = time.perf_counter() - t0
     if args.no_graphs:
         solver.set_graphs(False)
+    solver.set_krylov("mixed", args.mixed_krylov)
     n_blocks, n_slots = solver.pattern_info()
     print(f"[bench] setup {setup_s:.2f} s, {n_blocks} blocks, {n_slots} SELL slots",
           file=sys.stderr, flush=True)
@@ -349,13 +370,13 @@ def main():
     res["mixed"] = dict(nr_iterations=tm["nr_iterations"], krylov_iterations=tm["krylov_iterations"],
                         wall_s=wall_m, ms_element=tm["ms_element"], ms_assembly=tm["ms_assembly"],
                         ms_krylov=tm["ms_krylov"], converged=tm["converged"],
-                        fp64_fallbacks=tm["fp64_fallbacks"],
+                        fp64_fallbacks=tm["fp64_fallbacks"], cg_fallbacks=tm["cg_fallbacks"],
                         time_steps_per_s=args.steps / wall_m, per_step=tm["per_step"])
     res["adjoint_cost_per_forward"] = None
     res["config"] = dict(workload=f"{args.config}: {w['cfg']['desc']}", prisms=ne, nodes=nn,
                          dof=3 * nn, dt_s=w["dt"], nr_tol=1e-8, krylov_rtol_fp64=args.fp64_rtol,
                          krylov_rtol_mixed=args.mixed_rtol, precond="node-block Jacobi",
-                         krylov="BiCGStab (reduced synchronisation)",
+                         krylov_mixed=args.mixed_krylov, krylov_fp64="bicgstab",
                          growth_seed=w["seed"], setup_s=setup_s, slices_per_tile=spt)
     # roofline of the dominant kernel: the FP32 SpMV v = J ph of the mixed
     # solve (tz_spmv_v<float>), timed with HIP events on the solver's stream
@@ -365,15 +386,17 @@ def main():
     workload = res["config"]["workload"]
     if kt_m["spmv_launches_fp32"] > 0:
         avg_s = kt_m["spmv_ms_fp32"] / kt_m["krylov_iterations"] / 1000.0
-        b = spmv_v_bytes(n_blocks, nn, 4)
+        if args.mixed_krylov == "cg":
+            kname, b = "tzc_spmv<float>", cg_spmv_bytes(n_blocks, nn, 4)
+        else:
+            kname, b = "tz_spmv_v<float>", spmv_v_bytes(n_blocks, nn, 4)
         ach = b / avg_s / 1e9
         res["roofline"] = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                               frac=ach / HBM_PEAK_GBS,
-                               traffic=measured_traffic("tz_spmv_v<float>", workload),
-                               kernel="tz_spmv_v<float>", algorithmic_bytes=b,
-                               avg_launch_ms=avg_s * 1e3,
+                               frac=ach / HBM_PEAK_GBS, traffic=measured_traffic(kname, workload),
+                               kernel=kname, algorithmic_bytes=b, avg_launch_ms=avg_s * 1e3,
                                launches_with_work=kt_m["krylov_iterations"],
                                launches=kt_m["spmv_launches_fp32"])
+    if kt_m["spmv_t_launches_fp32"] > 0:
         avg_t = kt_m["spmv_t_ms_fp32"] / kt_m["krylov_iterations"] / 1000.0
         bt = spmv_t_bytes(n_blocks, nn, 4)
         res["roofline_spmv_t_fp32"] = dict(kernel="tz_spmv_t<float>", achieved=bt / avg_t / 1e9,
@@ -390,12 +413,14 @@ def main():
                                              avg_launch_ms=avg_s * 1e3, algorithmic_bytes=b)
     # whole NR iteration: element kernel + assembly + k BiCGStab iterations
     visc = bool(np.any(w["eta"] != 0))
-    for key, t, sz in (("mixed", tm, 4), ("fp64", td if fp64 else None, 8)):
+    for key, t, sz, meth in (("mixed", tm, 4, args.mixed_krylov),
+                             ("fp64", td if fp64 else None, 8, "bicgstab")):
         if t is None:
             continue
         k_per = t["krylov_iterations"] / t["nr_iterations"]
-        bnr = (assembly_bytes(ne, n_blocks, nn, sz, visc)
-               + k_per * bicgstab_iteration_bytes(n_blocks, nn, sz))
+        it_bytes = (cg_iteration_bytes if meth == "cg" else bicgstab_iteration_bytes)(
+            n_blocks, nn, sz)
+        bnr = assembly_bytes(ne, n_blocks, nn, sz, visc) + k_per * it_bytes
         secs = (t["ms_element"] + t["ms_assembly"] + t["ms_krylov"]) / 1e3 / t["nr_iterations"]
         res.setdefault("roofline_nr_iteration", {})[key] = dict(
             algorithmic_bytes=bnr, krylov_iterations=k_per, device_s=secs,
@@ -412,7 +437,7 @@ def main():
         print(f"[bench] adjoint: backward/forward {adj['cost_per_forward_device']:.2f} (device), "
               f"{adj['cost_per_forward_wall']:.2f} (wall)", file=sys.stderr, flush=True)
     if not args.no_batch and args.config == "c3":
-        res["batch"] = run_batch(args.steps, args.warmup, args.mixed_rtol)
+        res["batch"] = run_batch(args.steps, args.warmup, args.mixed_rtol, args.mixed_krylov)
     if not args.no_cpu_baseline:
         k_per_nr = (fp64 or {}).get("krylov_iterations", tm["krylov_iterations"]) / max(
             1, (fp64 or {}).get("nr_iterations", tm["nr_iterations"]))

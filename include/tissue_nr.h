@@ -70,6 +70,9 @@ typedef struct tf_step_stats {
   int32_t fp64_fallbacks;                  /* mixed precision: NR iterations whose
                                               FP32 solve failed and was redone
                                               with the FP64 operator */
+  int32_t cg_fallbacks;                    /* NR iterations whose CG solve failed
+                                              (indefinite or stalled) and was
+                                              redone with BiCGStab */
 } tf_step_stats;
 
 typedef struct tf_handle tf_handle;
@@ -202,12 +205,23 @@ tf_status tf_batch_create(int32_t n_tissues, const int32_t* nodes_per_tissue,
 
 /* Tile size of the fused Krylov solver (consecutive 64-row slices per
  * workgroup; a tile never spans two tissues).  tf_nr_autotune times 10
- * solver iterations on the assembled operator of the given precision for
- * each candidate size and keeps the fastest; tf_nr_set_tile_size sets it.
+ * iterations of the precision's Krylov method (tf_nr_set_krylov) on the
+ * assembled operator of that precision for each candidate size and keeps the
+ * fastest; tf_nr_set_tile_size sets it.
  * No reference counterpart. */
 tf_status tf_nr_autotune(tf_handle* h, int32_t precision_mode, int32_t* slices_per_tile,
                          double* ms_per_iteration);
 tf_status tf_nr_set_tile_size(tf_handle* h, int32_t slices_per_tile);
+
+/* Krylov method of the linear solves of one precision mode (no reference
+ * counterpart; the reference factorises with Pardiso).  BiCGStab handles any
+ * Jacobian.  CG (one SpMV per iteration) suits Jacobians that are symmetric
+ * positive definite up to the small viscous asymmetry; a tissue whose CG
+ * solve meets non-positive curvature or stalls is solved again with BiCGStab
+ * in the same NR iteration (tf_step_stats.cg_fallbacks).  Defaults: FP64
+ * BiCGStab, mixed CG.  Bound systems always use BiCGStab. */
+enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1 };
+tf_status tf_nr_set_krylov(tf_handle* h, int32_t precision_mode, int32_t method);
 
 /* Adjoint of n_steps implicit steps with respect to one growth parameter r_e
  * per prism.  No counterpart in the reference (which has no derivatives); the

@@ -52,17 +52,20 @@ def _batch_physical(nr, orc, cs, dt, spt):
     return b, u0
 
 
-@pytest.mark.parametrize("precision,rtol,spt", [("fp64", 1e-10, 8), ("fp64", 1e-10, 2),
-                                                ("mixed", 1e-5, 8), ("mixed", 1e-10, 4)])
-def test_batch_step_is_bitwise_each_tissue_alone(orc, nr, precision, rtol, spt):
+@pytest.mark.parametrize("precision,rtol,spt,method", [
+    ("fp64", 1e-10, 8, "bicgstab"), ("fp64", 1e-10, 2, "bicgstab"), ("fp64", 1e-10, 4, "cg"),
+    ("mixed", 1e-3, 8, "cg"), ("mixed", 1e-5, 8, "bicgstab"), ("mixed", 1e-10, 4, "cg")])
+def test_batch_step_is_bitwise_each_tissue_alone(orc, nr, precision, rtol, spt, method):
     cs = _ellipses()
     dt = 60.0
     b, u0 = _batch_physical(nr, orc, cs, dt, spt)
+    b.set_krylov(precision, method)
     ub, stats = b.step(u0, dt, 1e-8, 20, precision, rtol)
     assert len(stats) == len(cs)
     parts = b.split_nodes(ub)
     for t, c in enumerate(cs):
         s = _alone_physical(nr, orc, c, dt, spt)
+        s.set_krylov(precision, method)
         ua, st = s.step(c["mesh"]["nodes"].reshape(-1), dt, 1e-8, 20, precision, rtol)
         assert st["converged"] and stats[t]["converged"]
         assert st["krylov_iterations"] == stats[t]["krylov_iterations"], t
@@ -104,21 +107,23 @@ def _batch(nr, ps, spt):
 
 
 def test_mixed_falls_back_to_fp64_per_tissue(nr):
-    """Where the FP32 solve of a tissue fails, that tissue alone is solved
-    with the FP64 operator: every tissue converges to its FP64 root, the batch
-    still equals each tissue alone, and the fallbacks are counted."""
+    """Where the CG or the FP32 solve of a tissue fails, that tissue alone is
+    solved again with BiCGStab, then with the FP64 operator: every tissue
+    converges to its FP64 root, the batch still equals each tissue alone, and
+    the fallbacks are counted."""
     ps = [problems.perturbed_problem(**h) for h in HARD]
     b = _batch(nr, ps, 8)
     un = np.concatenate([p["un"] for p in ps])
     ub, stats = b.step(un, 7.0, 1e-8, 20, "mixed", 1e-5)
     parts = b.split_nodes(ub)
-    assert sum(st["fp64_fallbacks"] for st in stats) > 0
+    assert sum(st["fp64_fallbacks"] + st["cg_fallbacks"] for st in stats) > 0
     for t, p in enumerate(ps):
         s = _alone(nr, p, 8)
         ua, st = s.step(p["un"], 7.0, 1e-8, 20, "mixed", 1e-5)
         u64, st64 = s.step(p["un"], 7.0, 1e-8, 20, "fp64", 1e-10)
         assert st["converged"] and stats[t]["converged"] and st64["converged"]
         assert st["fp64_fallbacks"] == stats[t]["fp64_fallbacks"]
+        assert st["cg_fallbacks"] == stats[t]["cg_fallbacks"]
         assert np.array_equal(parts[t].reshape(-1), ua), t
         assert np.abs(ua - u64).max() <= 1e-8 * np.abs(u64).max()
         s.close()
@@ -215,3 +220,22 @@ def test_graph_replays_equal_direct_launches(orc, nr, precision, rtol):
                                np.concatenate([c["etype"] for c in cs]),
                                np.concatenate([c["spans"] for c in cs]))
     ga.close(); da.close()
+
+
+def test_fp64_cg_matches_fp64_bicgstab(orc, nr):
+    """FP64 CG on the nearly symmetric Jacobian reaches the BiCGStab step's
+    positions (both linear solves to 1e-10) within 1e-9 relative, and the
+    dense oracle step."""
+    p = problems.perturbed_problem(seed=9, visc=True, fixed=True, lx=6.0, ly=5.0, layers=3)
+    s = _alone(nr, p, 8)
+    ub, stb = s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-10)
+    s.set_krylov("fp64", "cg")
+    uc, stc = s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-10)
+    uo, du, conv = orc.Problem(p["mesh"]["conn"], p["refpos"], p["invFg"],
+                               *orc.lame(p["E"], p["nu"]), p["eta"], p["ext_visc"], p["area"],
+                               p["fixed"]).nr_step_dense(p["un"], p["dt"], 1e-8, 20)
+    assert stb["converged"] and stc["converged"] and conv
+    scale = np.abs(uo).max()
+    assert np.abs(uc - ub).max() <= 1e-9 * scale
+    assert np.abs(uc - uo).max() <= 1e-9 * scale
+    s.close()

@@ -29,8 +29,11 @@
 namespace tf {
 namespace {
 
-constexpr int kEPB = 32;              // prisms per workgroup
-constexpr int kThreads = kEPB * 6;    // 192
+#ifndef TF_ELEM_PER_BLOCK
+#define TF_ELEM_PER_BLOCK 32
+#endif
+constexpr int kEPB = TF_ELEM_PER_BLOCK;  // prisms per workgroup
+constexpr int kThreads = kEPB * 6;
 
 struct ElemConst {
   double sfd[6][3][6];  // shape function derivatives per Gauss point

@@ -202,11 +202,19 @@ hipError_t tiled_bicgstab(const KrylovArgs& k, const double* b, double* x, doubl
 // inner_rtol (at most maxit iterations in all), and FP64 residual refinement
 // while rtol < inner_rtol.  A tissue whose FP32 solve breaks down, diverges
 // or reaches maxit ends with that state.
+// Krylov methods of the tiled solvers.
+enum KrylovMethod { KRYLOV_BICGSTAB = 0, KRYLOV_CG = 1 };
 hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double rtol,
-                       double inner_rtol, int maxit, int sync_every, TiledResult* res);
+                       double inner_rtol, int maxit, int sync_every, int method,
+                       TiledResult* res);
+// Preconditioned CG (FP64) on all active tissues; a non-positive curvature
+// or rho ends a tissue's solve as a breakdown.
+hipError_t tiled_cg(const KrylovArgs& k, const double* b, double* x, double rtol, int maxit,
+                    int sync_every, TiledResult* res);
 // x_k += dx and per-tissue |dx|_2 into norms (host).
 hipError_t tiled_norm_update(const KrylovArgs& k, const double* dx, double* xk, double* norms);
-hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int iters, float* ms);
+hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int method, int iters,
+                                 float* ms);
 // Plain copy of n4 16-byte vectors (bandwidth reference); kCopyVariants
 // shapes (vectors per thread, ordinary or non-temporal accesses).
 constexpr int kCopyVariants = 6;

@@ -37,6 +37,8 @@ struct tf_handle {
   int32_t *d_tile_slice = nullptr, *d_tile_tissue = nullptr, *d_tissue_tile = nullptr;
   int spt = 8;
   int autotuned_spt = 0;
+  // Krylov method per precision [FP64, mixed]
+  int krylov_method[2] = {KRYLOV_BICGSTAB, KRYLOV_CG};
   int device = 0;
   hipStream_t st = nullptr;
   DevPattern pat;
@@ -737,14 +739,17 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     tf_status s = assemble(h, dt, mixed);
     if (s != TF_OK) return s;
     KrylovArgs k = krylov_args(h);
+    const int method = h->n_pairs > 0 ? KRYLOV_BICGSTAB : h->krylov_method[mixed];
+    const double inner = std::max(rtol, kMixedInnerRtol);
     if (h->n_pairs > 0) {
       int kit = 0, state = 0;
       TF_CHECK(krylov_bicgstab(k, w.g, w.du, rtol, kmax, &kit, &state, 16));
       res.state[0] = state;
       res.iters[0] = kit;
     } else if (mixed) {
-      TF_CHECK(tiled_mixed(k, w.g, w.du, rtol, std::max(rtol, kMixedInnerRtol), kMixedCap, 16,
-                           &res));
+      TF_CHECK(tiled_mixed(k, w.g, w.du, rtol, inner, kMixedCap, 16, method, &res));
+    } else if (method == KRYLOV_CG) {
+      TF_CHECK(tiled_cg(k, w.g, w.du, rtol, kmax, 16, &res));
     } else {
       TF_CHECK(tiled_bicgstab(k, w.g, w.du, rtol, kmax, 16, &res));
     }
@@ -754,35 +759,55 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     TF_CHECK(hipEventElapsedTime(&t_el, h->ev[0], h->ev[1]));
     TF_CHECK(hipEventElapsedTime(&t_as, h->ev[1], h->ev[2]));
     TF_CHECK(hipEventElapsedTime(&t_kr, h->ev[2], h->ev[3]));
-    // A tissue whose FP32 solve broke down, diverged or stalled is solved
-    // again with the FP64 operator; the other tissues keep their increments.
-    bool any_fb = false;
-    if (mixed && h->n_pairs == 0)
+    // Tissues whose solve failed are solved again, each on its own, by the
+    // next method of the chain CG -> BiCGStab (same precision) -> BiCGStab
+    // with the FP64 operator; the other tissues keep their increments.
+    auto failed = [&](std::vector<int32_t>& f) {
+      bool any_f = false;
       for (int T = 0; T < nt; ++T) {
-        fb[T] = (act[T] && res.state[T] != K_CONVERGED) ? 1 : 0;
-        any_fb |= fb[T] != 0;
+        f[T] = (act[T] && res.state[T] != K_CONVERGED) ? 1 : 0;
+        any_f |= f[T] != 0;
       }
-    if (any_fb) {
-      TF_CHECK(hipMemcpyAsync(w.tactive, fb.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
+      return any_f;
+    };
+    auto retry = [&](const std::vector<int32_t>& f, int fp64_operator) -> tf_status {
+      TF_CHECK(hipEventRecord(h->ev[4], h->st));
+      TF_CHECK(hipMemcpyAsync(w.tactive, f.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
                               h->st));
-      s = assemble(h, dt, 0);
-      if (s != TF_OK) return s;
-      k = krylov_args(h);
-      TF_CHECK(tiled_bicgstab(k, w.g, w.tmp, rtol, kmax, 16, &fres));
-      TF_CHECK(tiled_copy_active(k, w.tmp, w.du));
+      if (fp64_operator) {
+        tf_status as = assemble(h, dt, 0);
+        if (as != TF_OK) return as;
+      }
+      KrylovArgs kr = krylov_args(h);
+      if (!fp64_operator && mixed)
+        TF_CHECK(tiled_mixed(kr, w.g, w.tmp, rtol, inner, kMixedCap, 16, KRYLOV_BICGSTAB, &fres));
+      else
+        TF_CHECK(tiled_bicgstab(kr, w.g, w.tmp, rtol, kmax, 16, &fres));
+      TF_CHECK(tiled_copy_active(kr, w.tmp, w.du));
       TF_CHECK(hipMemcpyAsync(w.tactive, act.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
                               h->st));
-      TF_CHECK(hipEventRecord(h->ev[3], h->st));
-      TF_CHECK(hipEventSynchronize(h->ev[3]));
+      TF_CHECK(hipEventRecord(h->ev[5], h->st));
+      TF_CHECK(hipEventSynchronize(h->ev[5]));
       float t_fb = 0;
-      TF_CHECK(hipEventElapsedTime(&t_fb, h->ev[0], h->ev[3]));
+      TF_CHECK(hipEventElapsedTime(&t_fb, h->ev[4], h->ev[5]));
       t_kr += t_fb;
       for (int T = 0; T < nt; ++T)
-        if (fb[T]) {
+        if (f[T]) {
           res.state[T] = fres.state[T];
           res.iters[T] += fres.iters[T];
-          S[T].fp64_fallbacks++;
         }
+      return TF_OK;
+    };
+    if (h->n_pairs == 0 && method == KRYLOV_CG && failed(fb)) {
+      tf_status rs = retry(fb, 0);
+      if (rs != TF_OK) return rs;
+      for (int T = 0; T < nt; ++T) S[T].cg_fallbacks += fb[T];
+    }
+    if (mixed && h->n_pairs == 0 && failed(fb)) {
+      tf_status rs = retry(fb, 1);
+      if (rs != TF_OK) return rs;
+      for (int T = 0; T < nt; ++T) S[T].fp64_fallbacks += fb[T];
+      k = krylov_args(h);
     }
     TF_CHECK(launch_bind_increment(h->pairs, h->n_pairs, w.du, h->st));
     TF_CHECK(tiled_norm_update(k, w.du, w.uk, norms.data()));
@@ -1366,8 +1391,9 @@ tf_status tf_nr_autotune(tf_handle* h, int32_t precision_mode, int32_t* slices_p
     if (s != TF_OK) return s;
     KrylovArgs k = krylov_args(h);
     float ms = 0.f;
-    TF_CHECK(tiled_time_iterations(k, mixed, 2, &ms));  // warm-up
-    TF_CHECK(tiled_time_iterations(k, mixed, 10, &ms));
+    const int method = h->krylov_method[mixed];
+    TF_CHECK(tiled_time_iterations(k, mixed, method, 2, &ms));  // warm-up
+    TF_CHECK(tiled_time_iterations(k, mixed, method, 10, &ms));
     const double per = ms / 10.0;
     if (per < best_ms) {
       best_ms = per;
@@ -1379,6 +1405,17 @@ tf_status tf_nr_autotune(tf_handle* h, int32_t precision_mode, int32_t* slices_p
   h->autotuned_spt = best;
   if (slices_per_tile) *slices_per_tile = best;
   if (ms_per_iteration) *ms_per_iteration = best_ms;
+  return TF_OK;
+}
+
+tf_status tf_nr_set_krylov(tf_handle* h, int32_t precision_mode, int32_t method) {
+  if (!h || (precision_mode != TF_PREC_FP64 && precision_mode != TF_PREC_MIXED) ||
+      (method != TF_KRYLOV_BICGSTAB && method != TF_KRYLOV_CG)) {
+    set_error("tf_nr_set_krylov: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  h->krylov_method[precision_mode == TF_PREC_MIXED ? 1 : 0] =
+      method == TF_KRYLOV_CG ? KRYLOV_CG : KRYLOV_BICGSTAB;
   return TF_OK;
 }
 

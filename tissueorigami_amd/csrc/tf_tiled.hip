@@ -137,9 +137,9 @@ __device__ inline void minv_apply(const T* m, const T* in, T* out, int i) {
 #ifndef TF_SPMV_UNROLL64
 #define TF_SPMV_UNROLL64 2
 #endif
-template <typename T>
-__device__ inline void row_spmv(int sl, int lane, const TC& c, const T* __restrict__ val,
-                                const T* __restrict__ x, T& a0, T& a1, T& a2) {
+template <typename T, typename Gather>
+__device__ inline void row_spmv_g(int sl, int lane, const TC& c, const T* __restrict__ val,
+                                  Gather gx, T& a0, T& a1, T& a2) {
   constexpr int U = sizeof(T) == 4 ? TF_SPMV_UNROLL32 : TF_SPMV_UNROLL64;
   const int64_t base = c.slice_base[sl];
   const int width = (int)((c.slice_base[sl + 1] - base) >> 6);
@@ -160,11 +160,7 @@ __device__ inline void row_spmv(int sl, int lane, const TC& c, const T* __restri
       for (int q = 0; q < 9; ++q) m[u][q] = __builtin_nontemporal_load(vv + q * ns);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      xv[u][0] = x[3 * cc[u]];
-      xv[u][1] = x[3 * cc[u] + 1];
-      xv[u][2] = x[3 * cc[u] + 2];
-    }
+    for (int u = 0; u < U; ++u) gx(cc[u], xv[u][0], xv[u][1], xv[u][2]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       a0 += m[u][0] * xv[u][0] + m[u][1] * xv[u][1] + m[u][2] * xv[u][2];
@@ -181,11 +177,25 @@ __device__ inline void row_spmv(int sl, int lane, const TC& c, const T* __restri
             m4 = __builtin_nontemporal_load(vv + 4 * ns), m5 = __builtin_nontemporal_load(vv + 5 * ns),
             m6 = __builtin_nontemporal_load(vv + 6 * ns), m7 = __builtin_nontemporal_load(vv + 7 * ns),
             m8 = __builtin_nontemporal_load(vv + 8 * ns);
-    const T x0 = x[3 * cc], x1 = x[3 * cc + 1], x2 = x[3 * cc + 2];
+    T x0, x1, x2;
+    gx(cc, x0, x1, x2);
     a0 += m0 * x0 + m1 * x1 + m2 * x2;
     a1 += m3 * x0 + m4 * x1 + m5 * x2;
     a2 += m6 * x0 + m7 * x1 + m8 * x2;
   }
+}
+
+// Plain gather of x.
+template <typename T>
+__device__ inline void row_spmv(int sl, int lane, const TC& c, const T* __restrict__ val,
+                                const T* __restrict__ x, T& a0, T& a1, T& a2) {
+  row_spmv_g<T>(sl, lane, c, val,
+                [x](int cc, T& x0, T& x1, T& x2) {
+                  x0 = x[3 * cc];
+                  x1 = x[3 * cc + 1];
+                  x2 = x[3 * cc + 2];
+                },
+                a0, a1, a2);
 }
 
 // r = rh = p = b, x = 0 (x = 0 also for tissues that are not solved).
@@ -386,6 +396,148 @@ __global__ __launch_bounds__(kBlock) void tz_update_p(TC c, const T* r, T* p, T*
   reduce_store(acc, 1, outs, t.L);
 }
 
+// ---------------------------------------------------------------------------
+// Preconditioned CG for nearly symmetric Jacobians, two launches per
+// iteration:
+//   tzc_spmv:   convergence test on |r|; beta = rho / rho_old;
+//               p = z + beta p_old formed on the fly for every gathered column
+//               and stored for the tile's own rows (into the other p buffer);
+//               q = J p; partial (p, q)
+//   tzc_update: alpha = rho / (p, q); x += alpha p; r -= alpha q; z = M^-1 r;
+//               partials (r, z), (r, r) for the next iteration
+// (r, z) and (r, r) alternate between partial slots 0/3 and 1/4 with the
+// iteration's parity, which the host passes, and (p, q) uses slot 2, so no
+// launch reads a slot its own workgroups write.  rho_old is written by the
+// update kernel only.  A non-positive (p, q) or rho (indefinite operator) is a
+// breakdown: the caller falls back to BiCGStab.
+enum { TC_RZ0 = 0, TC_RR0 = 1, TC_PQ = 2, TC_RZ1 = 3, TC_RR1 = 4 };
+
+template <typename T>
+__device__ inline T fma_t(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) return fmaf(a, b, c);
+  else return fma(a, b, c);
+}
+
+// r = b, x = 0, p_old = 0, z = M^-1 r; partials (r, z), (r, r); rho_old = 1.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tzc_init(TC c, const double* b, T* r, T* z, T* x,
+                                                   T* p, const T* minv) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  double acc[2] = {0.0, 0.0};
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      r[3 * i + d] = (T)b[3 * i + d];
+      x[3 * i + d] = 0;
+      p[3 * i + d] = 0;
+    }
+    minv_apply(minv, r, z, i);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      acc[0] += (double)r[3 * i + d] * (double)z[3 * i + d];
+      acc[1] += (double)r[3 * i + d] * (double)r[3 * i + d];
+    }
+  }
+  double* outs[2] = {TP(c, TC_RZ0), TP(c, TC_RR0)};
+  reduce_store(acc, 2, outs, t.L);
+  if (t.first && threadIdx.x == 0) {
+    c.scal[8 * t.T + TS_RHO] = 1.0;
+    c.state[2 * t.T] = c.active[t.T] ? K_RUN : K_CONVERGED;
+    c.state[2 * t.T + 1] = 0;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tzc_spmv(TC c, int par, const T* __restrict__ val,
+                                                   const T* __restrict__ z,
+                                                   const T* __restrict__ pold,
+                                                   T* __restrict__ pnew, T* __restrict__ q) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (c.state[2 * t.T] != K_RUN) return;
+  const double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
+  const double rr = tissue_sum(c, par ? TC_RR1 : TC_RR0, t.T);
+  const double rnorm = sqrt(rr);
+  const bool first_it = (c.state[2 * t.T + 1] == 0);
+  double* sc = c.scal + 8 * t.T;
+  const double bnorm = first_it ? rnorm : sc[TS_BNORM];
+  const double beta = rho / sc[TS_RHO];
+  int state = K_RUN;
+  if (!isfinite(rnorm) || rnorm > kDivergence * bnorm) state = K_BREAKDOWN;
+  else if (rnorm <= c.rtol * bnorm) state = K_CONVERGED;
+  else if (!(rho > 0.0) || !isfinite(beta)) state = K_BREAKDOWN;
+  if (t.first && threadIdx.x == 0) {
+    if (first_it) sc[TS_BNORM] = bnorm;
+    sc[TS_RNORM] = rnorm;
+    if (state != K_RUN) c.state[2 * t.T] = state;
+  }
+  if (state != K_RUN) return;
+  const T tb = (T)beta;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc[1] = {0.0};
+  for (int sl = t.s0 + wave; sl < t.s1; sl += kW) {
+    T a0, a1, a2;
+    row_spmv_g<T>(sl, lane, c, val,
+                  [z, pold, tb](int cc, T& x0, T& x1, T& x2) {
+                    x0 = fma_t(tb, pold[3 * cc], z[3 * cc]);
+                    x1 = fma_t(tb, pold[3 * cc + 1], z[3 * cc + 1]);
+                    x2 = fma_t(tb, pold[3 * cc + 2], z[3 * cc + 2]);
+                  },
+                  a0, a1, a2);
+    const int row = sl * 64 + lane;
+    if (row < c.n_nodes) {
+      const T p0 = fma_t(tb, pold[3 * row], z[3 * row]);
+      const T p1 = fma_t(tb, pold[3 * row + 1], z[3 * row + 1]);
+      const T p2 = fma_t(tb, pold[3 * row + 2], z[3 * row + 2]);
+      pnew[3 * row] = p0; pnew[3 * row + 1] = p1; pnew[3 * row + 2] = p2;
+      q[3 * row] = a0; q[3 * row + 1] = a1; q[3 * row + 2] = a2;
+      acc[0] += (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
+    }
+  }
+  double* outs[1] = {TP(c, TC_PQ)};
+  reduce_store(acc, 1, outs, t.L);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tzc_update(TC c, int par, T* x, T* r, T* z, const T* p,
+                                                     const T* q, const T* minv) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (c.state[2 * t.T] != K_RUN) return;
+  const double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
+  const double pq = tissue_sum(c, TC_PQ, t.T);
+  const double alpha = rho / pq;
+  if (!(pq > 0.0) || !isfinite(alpha)) {
+    if (t.first && threadIdx.x == 0) c.state[2 * t.T] = K_BREAKDOWN;
+    return;
+  }
+  const T ta = (T)alpha;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  double acc[2] = {0.0, 0.0};
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int k = 3 * i + d;
+      x[k] = fma_t(ta, p[k], x[k]);
+      r[k] = fma_t(-ta, q[k], r[k]);
+    }
+    minv_apply(minv, r, z, i);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      acc[0] += (double)r[3 * i + d] * (double)z[3 * i + d];
+      acc[1] += (double)r[3 * i + d] * (double)r[3 * i + d];
+    }
+  }
+  double* outs[2] = {TP(c, par ? TC_RZ0 : TC_RZ1), TP(c, par ? TC_RR0 : TC_RR1)};
+  reduce_store(acc, 2, outs, t.L);
+  if (t.first && threadIdx.x == 0) {
+    c.scal[8 * t.T + TS_RHO] = rho;
+    c.state[2 * t.T + 1] = c.state[2 * t.T + 1] + 1;
+  }
+}
+
 // |dx|^2 per tile (slot 0) and xk += dx.
 __global__ __launch_bounds__(kBlock) void tz_norm_update(TC c, const double* dx, double* xk) {
   TileCtx t;
@@ -565,30 +717,35 @@ void launch_chunk(const TC& c, const TV<T>& V, int G, hipStream_t st, int chunk,
 }
 
 // Captured chunks: a full chunk of iterations is recorded once into a
-// hipGraph per (precision, kernel arguments, chunk length) and replayed with
-// one hipGraphLaunch.  The kernels read every per-tissue scalar and flag
-// from device memory, so a replay is the same work as the direct launches.
-// The chunk holds kernel launches only (no memset or copy nodes).
-struct ChunkGraph {
-  hipGraphExec_t exec = nullptr;
+// hipGraph per key (solver, precision, kernel arguments, chunk length, parity
+// of the first iteration) and replayed with one hipGraphLaunch.  The kernels
+// read every per-tissue scalar and flag from device memory, so a replay is
+// the same work as the direct launches.  A chunk holds kernel launches only
+// (no memset or copy nodes).
+struct GraphKey {
+  int kind, tsize, chunk, parity;
   TC c;
   const void* ptrs[12];
-  int chunk = 0, tsize = 0;
+};
+struct ChunkGraph {
+  hipGraphExec_t exec = nullptr;
+  GraphKey key;
   uint64_t used = 0;
 };
-constexpr int kGraphSlots = 4;
+constexpr int kGraphSlots = 6;
 struct GraphCache {
   ChunkGraph g[kGraphSlots];
   uint64_t clock = 0;
 };
 
-template <typename T>
-hipError_t run_chunk(const KrylovArgs& k, const TC& c, const TV<T>& V, int chunk) {
+// Launches `launch(prof)` directly when graphs are off or profiling is on;
+// otherwise replays (capturing first if needed) the graph of `key`.
+template <typename F>
+hipError_t run_captured(const KrylovArgs& k, const GraphKey& key, F launch) {
   Workspace* ws = k.ws;
   hipStream_t st = k.L.stream;
-  const int G = k.tm.grid;
   if (!ws->graphs_on || (k.prof && k.prof->on)) {
-    launch_chunk<T>(c, V, G, st, chunk, k.prof);
+    launch(k.prof);
     return hipGetLastError();
   }
   GraphCache* gc = static_cast<GraphCache*>(ws->graphs);
@@ -596,11 +753,9 @@ hipError_t run_chunk(const KrylovArgs& k, const TC& c, const TV<T>& V, int chunk
     gc = new GraphCache();
     ws->graphs = gc;
   }
-  const void* ptrs[12] = {V.r, V.rh, V.p, V.v, V.ph, V.s, V.sh, V.t, V.x, V.w, V.val, V.minv};
   ChunkGraph* hit = nullptr;
   for (auto& g : gc->g)
-    if (g.exec && g.tsize == (int)sizeof(T) && g.chunk == chunk &&
-        std::memcmp(&g.c, &c, sizeof(TC)) == 0 && std::memcmp(g.ptrs, ptrs, sizeof(ptrs)) == 0) {
+    if (g.exec && std::memcmp(&g.key, &key, sizeof(GraphKey)) == 0) {
       hit = &g;
       break;
     }
@@ -612,7 +767,7 @@ hipError_t run_chunk(const KrylovArgs& k, const TC& c, const TV<T>& V, int chunk
     if (hit->exec) (void)hipGraphExecDestroy(hit->exec);
     hit->exec = nullptr;
     if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
-    launch_chunk<T>(c, V, G, st, chunk, nullptr);
+    launch(static_cast<Prof*>(nullptr));
     hipGraph_t graph = nullptr;
     hipError_t le = hipGetLastError();
     e = hipStreamEndCapture(st, &graph);
@@ -623,13 +778,110 @@ hipError_t run_chunk(const KrylovArgs& k, const TC& c, const TV<T>& V, int chunk
       hit->exec = nullptr;
       return e;
     }
-    hit->c = c;
-    std::memcpy(hit->ptrs, ptrs, sizeof(ptrs));
-    hit->chunk = chunk;
-    hit->tsize = (int)sizeof(T);
+    hit->key = key;
   }
   hit->used = ++gc->clock;
   return hipGraphLaunch(hit->exec, st);
+}
+
+GraphKey make_key(int kind, int tsize, int chunk, int parity, const TC& c, const void* const* ptrs,
+                  int n_ptrs) {
+  GraphKey key;
+  std::memset(&key, 0, sizeof(key));  // padding too: keys are compared with memcmp
+  key.kind = kind;
+  key.tsize = tsize;
+  key.chunk = chunk;
+  key.parity = parity;
+  key.c = c;
+  for (int i = 0; i < n_ptrs; ++i) key.ptrs[i] = ptrs[i];
+  return key;
+}
+
+template <typename T>
+hipError_t run_chunk(const KrylovArgs& k, const TC& c, const TV<T>& V, int chunk) {
+  const void* ptrs[12] = {V.r, V.rh, V.p, V.v, V.ph, V.s, V.sh, V.t, V.x, V.w, V.val, V.minv};
+  const GraphKey key = make_key(0, (int)sizeof(T), chunk, 0, c, ptrs, 12);
+  const int G = k.tm.grid;
+  hipStream_t st = k.L.stream;
+  return run_captured(k, key, [&](Prof* pf) { launch_chunk<T>(c, V, G, st, chunk, pf); });
+}
+
+// CG vectors: r, z = M^-1 r, x, the two p buffers, q = J p.
+template <typename T>
+struct CV {
+  T *r, *z, *x, *p0, *p1, *q;
+  const T* val;
+  const T* minv;
+};
+
+template <typename T>
+void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chunk, int it0,
+                     Prof* pf) {
+  for (int it = 0; it < chunk; ++it) {
+    if (pf && pf->on && pf->used + 1 > kProfPairs) {
+      (void)hipStreamSynchronize(st);
+      prof_harvest(pf);
+    }
+    const int par = (it0 + it) & 1;
+    const T* pold = par ? V.p1 : V.p0;
+    T* pnew = par ? V.p0 : V.p1;
+    const bool on = pf && pf->on && pf->used < kProfPairs;
+    if (on) {
+      pf->kind[pf->used] = sizeof(T) == 4 ? 1 : 0;
+      (void)hipEventRecord(pf->ev[2 * pf->used], st);
+    }
+    hipLaunchKernelGGL(tzc_spmv<T>, dim3(G), dim3(kBlock), 0, st, c, par, V.val, (const T*)V.z,
+                       pold, pnew, V.q);
+    if (on) {
+      (void)hipEventRecord(pf->ev[2 * pf->used + 1], st);
+      pf->used++;
+    }
+    hipLaunchKernelGGL(tzc_update<T>, dim3(G), dim3(kBlock), 0, st, c, par, V.x, V.r, V.z,
+                       (const T*)pnew, (const T*)V.q, V.minv);
+  }
+}
+
+template <typename T>
+hipError_t tiled_cg_impl(const KrylovArgs& k, const CV<T>& V, const double* b, double rtol,
+                         int maxit, int sync_every, TiledResult* res) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  const int G = k.tm.grid;
+  const int nt = k.tm.n_tissues;
+  TC c = make_tc(k, rtol);
+  hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, c, b, V.r, V.z, V.x, V.p0,
+                     V.minv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const void* ptrs[8] = {V.r, V.z, V.x, V.p0, V.p1, V.q, V.val, V.minv};
+  int done = 0;
+  while (true) {
+    const int chunk = std::min(sync_every, maxit - done);
+    const GraphKey key = make_key(1, (int)sizeof(T), chunk, done & 1, c, ptrs, 8);
+    const int it0 = done;
+    e = run_captured(k, key, [&](Prof* pf) { launch_cg_chunk<T>(c, V, G, st, chunk, it0, pf); });
+    if (e != hipSuccess) return e;
+    done += chunk;
+    if (done >= maxit) {  // one more convergence test on the last residuals
+      const int par = done & 1;
+      hipLaunchKernelGGL(tzc_spmv<T>, dim3(G), dim3(kBlock), 0, st, c, par, V.val,
+                         (const T*)V.z, (const T*)(par ? V.p1 : V.p0), par ? V.p0 : V.p1, V.q);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(ws->host_tstate, ws->tstate, 2 * sizeof(int32_t) * nt,
+                            hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    prof_harvest(k.prof);
+    bool running = false;
+    for (int q = 0; q < nt; ++q) running |= (ws->host_tstate[2 * q] == K_RUN);
+    if (!running || done >= maxit) break;
+  }
+  for (int q = 0; q < nt; ++q) {
+    res->state[q] = ws->host_tstate[2 * q];
+    res->iters[q] = ws->host_tstate[2 * q + 1];
+  }
+  return hipSuccess;
 }
 
 template <typename T>
@@ -682,8 +934,15 @@ hipError_t tiled_bicgstab(const KrylovArgs& k, const double* b, double* x, doubl
   return tiled_bicgstab_impl<double>(k, V, b, rtol, maxit, sync_every, res);
 }
 
+hipError_t tiled_cg(const KrylovArgs& k, const double* b, double* x, double rtol, int maxit,
+                    int sync_every, TiledResult* res) {
+  Workspace* ws = k.ws;
+  CV<double> V{ws->r, ws->s, x, ws->p, ws->ph, ws->v, k.val, k.minv};
+  return tiled_cg_impl<double>(k, V, b, rtol, maxit, sync_every, res);
+}
+
 hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double rtol,
-                       double inner_rtol, int maxit, int sync_every, TiledResult* res) {
+                       double inner_rtol, int maxit, int sync_every, int method, TiledResult* res) {
   Workspace* ws = k.ws;
   hipStream_t st = k.L.stream;
   const int G = k.tm.grid;
@@ -731,7 +990,13 @@ hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double r
     hipLaunchKernelGGL(tz_scale, dim3(G), dim3(kBlock), 0, st, c, rsrc, ws->tnorm, ws->b);
     int left = maxit;
     for (int T = 0; T < nt; ++T) left = std::min(left, maxit - res->iters[T]);
-    e = tiled_bicgstab_impl<float>(k, V, ws->b, inner_rtol, std::max(left, 1), sync_every, &inner);
+    if (method == KRYLOV_CG) {
+      CV<float> C{ws->r32, ws->s32, ws->x32, ws->p32, ws->ph32, ws->v32, k.val32, k.minv32};
+      e = tiled_cg_impl<float>(k, C, ws->b, inner_rtol, std::max(left, 1), sync_every, &inner);
+    } else {
+      e = tiled_bicgstab_impl<float>(k, V, ws->b, inner_rtol, std::max(left, 1), sync_every,
+                                     &inner);
+    }
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(tz_axpy, dim3(G), dim3(kBlock), 0, st, c, ws->x32, ws->tnorm, x);
     for (int T = 0; T < nt; ++T)
@@ -787,9 +1052,10 @@ hipError_t tiled_norm_update(const KrylovArgs& k, const double* dx, double* xk, 
   return hipSuccess;
 }
 
-// Times `iters` fused iterations (FP64 or FP32 operator) on the current
-// operator, for the tile-size autotuner.
-hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int iters, float* ms) {
+// Times `iters` iterations of the given method (FP64 or FP32 operator) on the
+// current operator, for the tile-size autotuner.
+hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int method, int iters,
+                                 float* ms) {
   Workspace* ws = k.ws;
   hipStream_t st = k.L.stream;
   const int G = k.tm.grid;
@@ -821,7 +1087,23 @@ hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int iters, floa
     }
     (void)hipEventRecord(e1, st);
   };
-  if (mixed) {
+  auto run_cg = [&](auto V) {
+    using T = std::remove_pointer_t<decltype(V.r)>;
+    hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, c, (const double*)ws->b, V.r,
+                       V.z, V.x, V.p0, V.minv);
+    (void)hipEventRecord(e0, st);
+    launch_cg_chunk<T>(c, V, G, st, iters, 0, nullptr);
+    (void)hipEventRecord(e1, st);
+  };
+  if (method == KRYLOV_CG) {
+    if (mixed) {
+      CV<float> V{ws->r32, ws->s32, ws->x32, ws->p32, ws->ph32, ws->v32, k.val32, k.minv32};
+      run_cg(V);
+    } else {
+      CV<double> V{ws->r, ws->s, ws->x, ws->p, ws->ph, ws->v, k.val, k.minv};
+      run_cg(V);
+    }
+  } else if (mixed) {
     TV<float> V{ws->r32, ws->rh32, ws->p32, ws->v32, ws->ph32, ws->s32, ws->sh32, ws->t32,
                 ws->x32, ws->w32, k.val32, k.minv32};
     run(V);

@@ -24,7 +24,7 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_profile",
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
-           "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs",
+           "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs", "tf_nr_set_krylov",
            "tf_copy_bandwidth",
            "tf_nr_destroy", "tf_last_error"]
 
@@ -36,7 +36,7 @@ class StepStats(ctypes.Structure):
                 ("du_norms", ctypes.c_double * MAX_NR),
                 ("ms_element", ctypes.c_double), ("ms_assembly", ctypes.c_double),
                 ("ms_krylov", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("fp64_fallbacks", ctypes.c_int32)]
+                ("fp64_fallbacks", ctypes.c_int32), ("cg_fallbacks", ctypes.c_int32)]
 
     def as_dict(self):
         n = self.nr_iterations
@@ -45,7 +45,8 @@ class StepStats(ctypes.Structure):
                     krylov_iterations=list(self.krylov_iterations[:n]),
                     du_norms=list(self.du_norms[:n]), ms_element=self.ms_element,
                     ms_assembly=self.ms_assembly, ms_krylov=self.ms_krylov,
-                    ms_total=self.ms_total, fp64_fallbacks=self.fp64_fallbacks)
+                    ms_total=self.ms_total, fp64_fallbacks=self.fp64_fallbacks,
+                    cg_fallbacks=self.cg_fallbacks)
 
 
 class KernelTimes(ctypes.Structure):
@@ -106,6 +107,7 @@ def load():
     L.tf_nr_autotune.argtypes = [_h, ctypes.c_int32, _i, _d]
     L.tf_nr_set_tile_size.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_set_graphs.argtypes = [_h, ctypes.c_int32]
+    L.tf_nr_set_krylov.argtypes = [_h, ctypes.c_int32, ctypes.c_int32]
     L.tf_copy_bandwidth.argtypes = [ctypes.c_int64, ctypes.c_int32, _d]
     L.tf_nr_destroy.argtypes = [_h]
     L.tf_last_error.restype = ctypes.c_char_p
@@ -295,6 +297,11 @@ class NewtonRaphsonSolver:
 
     def set_tile_size(self, slices_per_tile):
         _check(load().tf_nr_set_tile_size(self._h, int(slices_per_tile)))
+
+    def set_krylov(self, precision, method):
+        """Krylov method ("bicgstab" or "cg") of one precision mode."""
+        _check(load().tf_nr_set_krylov(self._h, PREC[precision],
+                                       {"bicgstab": 0, "cg": 1}[method]))
 
     def set_graphs(self, enable=True):
         """Krylov iteration chunks replayed through captured hipGraphs (default on)."""
