@@ -1131,6 +1131,28 @@ int orc_nr_step_sparse(int n_nodes, int n_el, const int32_t* conn, const double*
   return it;
 }
 
+// The sparse Jacobian J = K^T at (uk, un) and the residual gSum: block rows
+// rowptr (n_nodes + 1), block columns col, 9 values per block (row-major).
+// Called with col == nullptr it returns the number of blocks only.
+long orc_assemble_bsr(int n_nodes, int n_el, const int32_t* conn, const double* refpos,
+                      const double* invFg, const double* lambda, const double* mu,
+                      const double* eta, const double* ext_visc, const double* visc_surf,
+                      const uint8_t* fixed, const double* uk, const double* un, double dt,
+                      int32_t* rowptr, int32_t* col, double* val, double* g) {
+  Problem P{n_nodes, n_el, conn, refpos, invFg, lambda, mu, eta, ext_visc, visc_surf,
+            fixed, nullptr, 0};
+  Bsr M;
+  build_pattern(P, M);
+  if (!col) return (long)M.col.size();
+  std::vector<double> gs(3 * (size_t)n_nodes);
+  assemble_bsr(P, uk, un, dt, M, gs.data());
+  std::copy(M.rowptr.begin(), M.rowptr.end(), rowptr);
+  std::copy(M.col.begin(), M.col.end(), col);
+  std::copy(M.val.begin(), M.val.end(), val);
+  if (g) std::copy(gs.begin(), gs.end(), g);
+  return (long)M.col.size();
+}
+
 // One NR iteration of the sparse path with a fixed Krylov iteration budget;
 // the CPU-baseline timing unit.  Returns BiCGStab iterations run.
 int orc_nr_iteration_sparse(int n_nodes, int n_el, const int32_t* conn, const double* refpos,

@@ -53,6 +53,9 @@ def lib():
                                               _d, _d, _u8, _d, _d, ctypes.c_double,
                                               ctypes.c_double, ctypes.c_int, _d, _d, _d]
         L.orc_nr_iteration_sparse.restype = ctypes.c_int
+        L.orc_assemble_bsr.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d, _d, _d, _d,
+                                       _d, _u8, _d, _d, ctypes.c_double, _i, _i, _d, _d]
+        L.orc_assemble_bsr.restype = ctypes.c_long
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -168,6 +171,21 @@ class Problem:
         if it < 0:
             raise RuntimeError("oracle BiCGStab breakdown")
         return out, du[:it], kit[:it], bool(conv[0])
+
+    def assemble_bsr(self, uk, un, dt):
+        """Sparse J = K^T (block rows, block columns, 3x3 blocks) and gSum."""
+        n = 3 * self.n_nodes
+        uk = _f64(uk).reshape(n); un = _f64(un).reshape(n)
+        L = lib()
+        args = (self.n_nodes, self.n_el, _p(self.conn, _i), _p(self.refpos, _d),
+                _p(self.invFg, _d), _p(self.lam, _d), _p(self.mu, _d), _p(self.eta, _d),
+                _p(self.ext_visc, _d), _p(self.visc_surf, _d), _p(self.fixed, _u8),
+                _p(uk, _d), _p(un, _d), float(dt))
+        nb = L.orc_assemble_bsr(*args, None, None, None, None)
+        rowptr = np.empty(self.n_nodes + 1, np.int32); col = np.empty(nb, np.int32)
+        val = np.empty((nb, 3, 3)); g = np.empty(n)
+        L.orc_assemble_bsr(*args, _p(rowptr, _i), _p(col, _i), _p(val, _d), _p(g, _d))
+        return rowptr, col, val, g
 
     def nr_iteration_sparse(self, uk, un, dt, krylov_rtol, krylov_maxit):
         n = 3 * self.n_nodes

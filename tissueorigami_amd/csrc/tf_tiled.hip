@@ -24,6 +24,8 @@
 #include <cstring>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "tf_internal.h"
 
 namespace tf {
@@ -684,16 +686,16 @@ struct TV {
 // One chunk of `chunk` iterations (4 launches each), launched directly.
 template <typename T>
 void launch_chunk(const TC& c, const TV<T>& V, int G, hipStream_t st, int chunk, Prof* pf) {
+  // profiled launches carry their own start/stop events (recorded by the
+  // runtime at the kernel's start and end)
   auto timed = [&](int which, auto launch) {
     const bool on = pf && pf->on && pf->used < kProfPairs;
     if (on) {
       pf->kind[pf->used] = 2 * which + (sizeof(T) == 4 ? 1 : 0);
-      (void)hipEventRecord(pf->ev[2 * pf->used], st);
-    }
-    launch();
-    if (on) {
-      (void)hipEventRecord(pf->ev[2 * pf->used + 1], st);
+      launch(pf->ev[2 * pf->used], pf->ev[2 * pf->used + 1]);
       pf->used++;
+    } else {
+      launch(nullptr, nullptr);
     }
   };
   for (int it = 0; it < chunk; ++it) {
@@ -701,13 +703,13 @@ void launch_chunk(const TC& c, const TV<T>& V, int G, hipStream_t st, int chunk,
       (void)hipStreamSynchronize(st);
       prof_harvest(pf);
     }
-    timed(0, [&] {
-      hipLaunchKernelGGL(tz_spmv_v<T>, dim3(G), dim3(kBlock), 0, st, c, V.val, (const T*)V.ph,
-                         (const T*)V.r, V.minv, V.v, V.s, V.sh);
+    timed(0, [&](hipEvent_t e0, hipEvent_t e1) {
+      hipExtLaunchKernelGGL(tz_spmv_v<T>, dim3(G), dim3(kBlock), 0, st, e0, e1, 0, c, V.val,
+                            (const T*)V.ph, (const T*)V.r, V.minv, V.v, V.s, V.sh);
     });
-    timed(1, [&] {
-      hipLaunchKernelGGL(tz_spmv_t<T>, dim3(G), dim3(kBlock), 0, st, c, V.val, (const T*)V.sh,
-                         (const T*)V.s, V.t);
+    timed(1, [&](hipEvent_t e0, hipEvent_t e1) {
+      hipExtLaunchKernelGGL(tz_spmv_t<T>, dim3(G), dim3(kBlock), 0, st, e0, e1, 0, c, V.val,
+                            (const T*)V.sh, (const T*)V.s, V.t);
     });
     hipLaunchKernelGGL(tz_update_x<T>, dim3(G), dim3(kBlock), 0, st, c, V.x, V.r, (const T*)V.rh,
                        (const T*)V.ph, (const T*)V.sh, (const T*)V.s, (const T*)V.t);
@@ -826,16 +828,15 @@ void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chu
     const T* pold = par ? V.p1 : V.p0;
     T* pnew = par ? V.p0 : V.p1;
     const bool on = pf && pf->on && pf->used < kProfPairs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (on) {
       pf->kind[pf->used] = sizeof(T) == 4 ? 1 : 0;
-      (void)hipEventRecord(pf->ev[2 * pf->used], st);
-    }
-    hipLaunchKernelGGL(tzc_spmv<T>, dim3(G), dim3(kBlock), 0, st, c, par, V.val, (const T*)V.z,
-                       pold, pnew, V.q);
-    if (on) {
-      (void)hipEventRecord(pf->ev[2 * pf->used + 1], st);
+      ev0 = pf->ev[2 * pf->used];
+      ev1 = pf->ev[2 * pf->used + 1];
       pf->used++;
     }
+    hipExtLaunchKernelGGL(tzc_spmv<T>, dim3(G), dim3(kBlock), 0, st, ev0, ev1, 0, c, par, V.val,
+                          (const T*)V.z, pold, pnew, V.q);
     hipLaunchKernelGGL(tzc_update<T>, dim3(G), dim3(kBlock), 0, st, c, par, V.x, V.r, V.z,
                        (const T*)pnew, (const T*)V.q, V.minv);
   }
