@@ -136,6 +136,12 @@ tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int
 tf_status tf_nr_assemble(tf_handle* h, const double* uk, const double* un, double dt,
                          double* g_out, double* internal_out);
 
+/* tf_nr_assemble in a chosen precision (no reference counterpart): with
+ * TF_PREC_MIXED the element tangents and J are FP32 (forces and g stay FP64);
+ * tf_nr_element_outputs then returns the FP32 tangents. */
+tf_status tf_nr_assemble_ex(tf_handle* h, const double* uk, const double* un, double dt,
+                            int32_t precision_mode, double* g_out, double* internal_out);
+
 /* Solves J x = b with the Jacobian of the last tf_nr_assemble (J = K^T, the
  * transposed solve of solveWithPardiso, NewtonRaphsonSolver.cpp:716-723;
  * binding handled as N^T J N + I_slave, NewtonRaphsonSolver.cpp:128-193).
@@ -157,7 +163,8 @@ tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz);
 tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja, double* a,
                         int64_t* nnz);
 
-/* Per-prism outputs of the last tf_nr_assemble, for element-level parity:
+/* Per-prism outputs of the last assembly, for element-level parity (the
+ * tangents are the FP32 ones after a mixed tf_nr_assemble_ex):
  * elastic and internal viscous element forces (n_prisms x 18,
  * ElementalElasticSystemForces / ElementalInternalViscousSystemForces before
  * fixed-DOF masking) and the 18 x 18 tangents TriPointKe / TriPointKv in the

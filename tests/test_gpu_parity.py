@@ -52,6 +52,29 @@ def test_element_kernel_matches_oracle(orc, nr, visc):
     s.close()
 
 
+@pytest.mark.parametrize("visc", [False, True])
+def test_mixed_element_tangents_match_oracle(orc, nr, visc):
+    """The FP32 tangents of the mixed path (closed forms, FP32 arithmetic from
+    FP64 kinematics) equal the oracle's FP64 tangents to FP32 rounding, and
+    the forces stay FP64."""
+    p = problems.perturbed_problem(seed=1, visc=visc)
+    s = _gpu_solver(nr, p)
+    s.assemble(p["uk"], p["un"], p["dt"], precision="mixed")
+    ge, gv, Ke, Kv = s.element_outputs()
+    m = p["mesh"]
+    lam, mu = orc.lame(p["E"], p["nu"])
+    pos = p["uk"].reshape(-1, 3)[m["conn"]].reshape(-1, 18)
+    vel = ((p["uk"] - p["un"]) * (1.0 / p["dt"])).reshape(-1, 3)[m["conn"]].reshape(-1, 18)
+    oge, ogv, oKe, oKv = orc.elements(pos, p["refpos"], p["invFg"], lam, mu, p["eta"], vel,
+                                      p["dt"], 1)
+    for a, b in ((ge, oge), (gv, ogv)):
+        assert np.abs(a - b).max() <= 1e-13 * max(np.abs(b).max(), 1e-300)
+    for a, b in ((Ke, oKe), (Kv, oKv)):
+        scale = max(np.abs(b).max(), 1e-300)
+        assert np.abs(a - b).max() <= 2e-6 * scale
+    s.close()
+
+
 @pytest.mark.parametrize("visc,fixed", [(False, False), (True, False), (True, True)])
 def test_assembled_jacobian_and_residual(orc, nr, visc, fixed):
     p = problems.perturbed_problem(seed=2, visc=visc, fixed=fixed)

@@ -6,7 +6,7 @@
 // NewtonRaphsonSolver.cpp:350-360, ShapeBase.cpp:2935-2983), then adds the
 // external-viscosity diagonal (NewtonRaphsonSolver.cpp:399-423) and clears
 // fixed rows and columns (calcutateFixedK, NewtonRaphsonSolver.cpp:195-219).
-// Here one thread owns one 3x3 block of J = K^T (the operator of the
+// Here one thread owns one 3x3 block (slot) of J = K^T (the operator of the
 // transposed Pardiso solve) and gathers its contributions through a
 // precomputed list sorted by element id, so the sum of every entry is formed
 // in exactly the reference's order and the result is bitwise reproducible.
@@ -87,15 +87,18 @@ __device__ inline void gather_blocks(const KT* __restrict__ Kel, const int32_t* 
 template <bool VISC, bool MIXED>
 __global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
   const int64_t n_slots = a.pat.n_slots;
-  const int n_rows = a.pat.n_slices * kSlice;
-  // one thread per (padded) block row; its slots are base + k*64 + lane, so
-  // consecutive lanes touch consecutive slots (coalesced planar stores)
-  for (int R = blockIdx.x * kBlock + threadIdx.x; R < n_rows; R += gridDim.x * kBlock) {
-    const int sl = R / kSlice, lane = R % kSlice;
+  // one thread per slot: a workgroup takes a slice, its four wavefronts take
+  // every fourth column k of the slice, lane = row; consecutive lanes touch
+  // consecutive slots (coalesced planar stores) and all slots of a slice are
+  // gathered concurrently
+  constexpr int kCols = kBlock / kSlice;
+  const int lane = threadIdx.x % kSlice, kq = threadIdx.x / kSlice;
+  for (int sl = blockIdx.x; sl < a.pat.n_slices; sl += gridDim.x) {
+    const int R = sl * kSlice + lane;
     const int64_t sbase = a.pat.slice_base[sl];
     const int width = (int)((a.pat.slice_base[sl + 1] - sbase) / kSlice);
     const int row_len = (R < a.n_nodes) ? a.pat.row_len[R] : 0;
-    for (int kk = 0; kk < width; ++kk) {
+    for (int kk = kq; kk < width; kk += kCols) {
     const int64_t slot = sbase + (int64_t)kk * kSlice + lane;
     if (kk >= row_len) {  // padding slot: zero block
       if (MIXED) {
@@ -328,7 +331,7 @@ int grid_for(int64_t n) {
 }  // namespace
 
 hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st) {
-  const int g1 = grid_for((int64_t)a.pat.n_slices * kSlice);
+  const int g1 = (int)std::min<int64_t>(a.pat.n_slices, 65535 * 4);
   if (a.any_visc) {
     if (mixed) hipLaunchKernelGGL((assemble_kernel<true, true>), dim3(g1), dim3(kBlock), 0, st, a);
     else hipLaunchKernelGGL((assemble_kernel<true, false>), dim3(g1), dim3(kBlock), 0, st, a);

@@ -122,6 +122,74 @@ __device__ inline void matmul3(const double A[3][3], const double B[3][3], doubl
     }
 }
 
+// Mixed precision, phase 2: the FP32 tangent block (a, b) of one prism from
+// its FP64 Gauss-point data, in closed forms that need about a fifth of the
+// operations of the reference-order evaluation, with contraction allowed:
+//   elastic  K_ik = sum_gp w [cl Na_i Nb_k + cm Na_k Nb_i
+//                             + d_ik (cm Na.Nb + vol Na.sigma.Nb)]
+//   viscous  B_a^T B~_b = (Na.Nb I + Nb (x) Na) / 2 (the Voigt B of node a
+//            times that of node b with halved shear rows), so
+//            (B_a^T B~_b P)_ij = (Na.Nb P_ij + Nb_i (Na^T P)_j) / 2 and
+//            (S (Na Nb^T - Nb Na^T))_ij = (S Na)_i Nb_j - (S Nb)_i Na_j;
+//            Kv = sum_gp w detF detdXde [eta B_a^T B~_b P + S (...)].
+// Kv is returned in the same (untransposed) order as phase 2b's K.
+__device__ inline void mixed_blocks(const GPData* sd, const GPVisc* svd, int el, int ia, int ib,
+                                    bool visc, float w, float Ke[9], float Kv[9]) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int q = 0; q < 9; ++q) { Ke[q] = 0.f; Kv[q] = 0.f; }
+  const bool has_eta = visc && svd[el * 6].eta != 0;
+#pragma unroll 1
+  for (int g = 0; g < 6; ++g) {
+    const GPData& d = sd[el * 6 + g];
+    float Na[3], Nb[3], sg[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { Na[j] = (float)d.dN[ia][j]; Nb[j] = (float)d.dN[ib][j]; }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) sg[q] = (float)d.sig[q];
+    const float cl = (float)d.cl, cm = (float)d.cm;
+    const float vol = (float)(d.detF * d.detdXde);
+    const float dot = Na[0] * Nb[0] + Na[1] * Nb[1] + Na[2] * Nb[2];
+    float sNb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sNb[i] = sg[3 * i] * Nb[0] + sg[3 * i + 1] * Nb[1] + sg[3 * i + 2] * Nb[2];
+    const float geo = Na[0] * sNb[0] + Na[1] * sNb[1] + Na[2] * sNb[2];
+    const float diag = cm * dot + vol * geo;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float v = cl * Na[i] * Nb[k] + cm * Na[k] * Nb[i];
+        if (i == k) v += diag;
+        Ke[3 * i + k] += v * w;
+      }
+    if (has_eta) {
+      const GPVisc& dv = svd[el * 6 + g];
+      float P[9], S[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) { P[q] = (float)dv.P[q]; S[q] = (float)dv.sv[q]; }
+      float NaP[3], SNa[3], SNb[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) NaP[j] = Na[0] * P[j] + Na[1] * P[3 + j] + Na[2] * P[6 + j];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        SNa[i] = S[3 * i] * Na[0] + S[3 * i + 1] * Na[1] + S[3 * i + 2] * Na[2];
+        SNb[i] = S[3 * i] * Nb[0] + S[3 * i + 1] * Nb[1] + S[3 * i + 2] * Nb[2];
+      }
+      const float eta = (float)dv.eta;
+      const float f = vol * w;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const float k1 = 0.5f * (dot * P[3 * i + j] + Nb[i] * NaP[j]);
+          const float k2 = SNa[i] * Nb[j] - SNb[i] * Na[j];
+          Kv[3 * i + j] += (eta * k1 + k2) * f;
+        }
+    }
+  }
+}
+
 template <bool VISC, bool MIXED>
 __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementArgs a) {
   __shared__ GPData sd[kEPB * 6];
@@ -290,9 +358,31 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
   __syncthreads();
 
   const double w = c.w;
+  if (MIXED) {
+    // ---------------- phase 2 (mixed): FP32 elastic and viscous blocks -----
+    for (int task = tid; task < kEPB * 36; task += kThreads) {
+      const int el = task / 36;
+      const int ab = task % 36;
+      const int e = e0 + el;
+      if (e >= a.n_prisms) break;
+      const int ia = ab / 6, ib = ab % 6;
+      float Ke[9], Kv[9];
+      mixed_blocks(sd, svd, el, ia, ib, VISC, (float)w, Ke, Kv);
+      const size_t base = ((size_t)e * 36 + ab) * 9;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) a.Ke32[base + q] = Ke[q];
+      if (VISC) {
+        // reference storage: TriPointKv[3b+j][3a+i] -> block (b, a), entry (j, i)
+        const size_t bv = ((size_t)e * 36 + ib * 6 + ia) * 9;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) a.Kv32[bv + 3 * j + i] = Kv[3 * i + j];
+      }
+    }
+  } else {
   // ---------------- phase 2a: elastic 3x3 blocks ---------------------------
-  // (FP32 arithmetic from the FP64 kinematics in mixed precision)
-  using T = typename std::conditional<MIXED, float, double>::type;
+  using T = double;
   const T tw = (T)w;
   for (int task = tid; task < kEPB * 36; task += kThreads) {
     const int el = task / 36;
@@ -312,7 +402,7 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
 #pragma unroll
       for (int q = 0; q < 9; ++q) sg[q] = (T)d.sig[q];
       const T cl = (T)d.cl, cm = (T)d.cm;
-      const T vol = MIXED ? (T)(d.detF * d.detdXde) : (T)d.detF * (T)d.detdXde;
+      const T vol = (T)d.detF * (T)d.detdXde;
       T dot = (T)0, geo = (T)0;
 #pragma unroll
       for (int j = 0; j < 3; ++j) dot += Na[j] * Nb[j];
@@ -334,13 +424,8 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
         }
     }
     const size_t base = ((size_t)e * 36 + ab) * 9;
-    if (MIXED) {
 #pragma unroll
-      for (int q = 0; q < 9; ++q) a.Ke32[base + q] = (float)K[q];
-    } else {
-#pragma unroll
-      for (int q = 0; q < 9; ++q) a.Ke[base + q] = (double)K[q];
-    }
+    for (int q = 0; q < 9; ++q) a.Ke[base + q] = K[q];
   }
 
   // ---------------- phase 2b: viscous 3x3 blocks (transposed storage) -----
@@ -429,19 +514,14 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
       }
       // reference storage: TriPointKv[3b+j][3a+i] -> block (b, a), entry (j, i)
       const size_t base = ((size_t)e * 36 + ib * 6 + ia) * 9;
-      if (MIXED) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) a.Kv32[base + 3 * j + i] = (float)K[3 * i + j];
-      } else {
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) a.Kv[base + 3 * j + i] = (double)K[3 * i + j];
-      }
+        for (int j = 0; j < 3; ++j) a.Kv[base + 3 * j + i] = K[3 * i + j];
     }
   }
+
+  }  // FP64 phase 2
 
   // ---------------- phase 3: nodal forces ----------------------------------
   if (tid < kEPB * 6) {

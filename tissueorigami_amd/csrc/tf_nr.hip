@@ -658,18 +658,23 @@ tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u, const int32_t
 
 tf_status tf_nr_assemble(tf_handle* h, const double* uk, const double* un, double dt,
                          double* g_out, double* internal_out) {
+  return tf_nr_assemble_ex(h, uk, un, dt, TF_PREC_FP64, g_out, internal_out);
+}
+
+tf_status tf_nr_assemble_ex(tf_handle* h, const double* uk, const double* un, double dt,
+                            int32_t precision_mode, double* g_out, double* internal_out) {
   if (h && h->batch) {
     set_error("tf_nr_assemble works on single-tissue handles only");
     return TF_ERR_ARG;
   }
-  if (!h || !uk || !un) {
+  if (!h || !uk || !un || (precision_mode != TF_PREC_FP64 && precision_mode != TF_PREC_MIXED)) {
     set_error("tf_nr_assemble: invalid arguments");
     return TF_ERR_ARG;
   }
   const size_t n = 3 * (size_t)h->Nn;
   TF_CHECK(hipMemcpyAsync(h->ws.uk, uk, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
   TF_CHECK(hipMemcpyAsync(h->ws.un, un, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
-  tf_status s = assemble(h, dt, 0);
+  tf_status s = assemble(h, dt, precision_mode == TF_PREC_MIXED ? 1 : 0);
   if (s != TF_OK) return s;
   if (g_out) TF_CHECK(hipMemcpyAsync(g_out, h->ws.g, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
   if (internal_out)
@@ -1341,14 +1346,24 @@ tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke
     set_error("tf_nr_element_outputs: invalid handle");
     return TF_ERR_ARG;
   }
-  if (!h->assembled || h->last_mixed) {
-    set_error("tf_nr_element_outputs needs a preceding FP64 tf_nr_assemble");
+  if (!h->assembled) {
+    set_error("tf_nr_element_outputs needs a preceding assembly");
     return TF_ERR_ARG;
   }
   const size_t Ne = h->Ne;
   if (ge) TF_CHECK(hipMemcpy(ge, h->ws.elemFe, sizeof(double) * Ne * 18, hipMemcpyDeviceToHost));
   if (gv) TF_CHECK(hipMemcpy(gv, h->ws.elemFv, sizeof(double) * Ne * 18, hipMemcpyDeviceToHost));
   std::vector<double> blk(Ne * 324);
+  std::vector<float> blk32;
+  auto fetch = [&](const double* src64, const float* src32) -> hipError_t {
+    if (!h->last_mixed)
+      return hipMemcpy(blk.data(), src64, sizeof(double) * blk.size(), hipMemcpyDeviceToHost);
+    blk32.resize(blk.size());
+    hipError_t e = hipMemcpy(blk32.data(), src32, sizeof(float) * blk32.size(),
+                             hipMemcpyDeviceToHost);
+    for (size_t i = 0; i < blk.size(); ++i) blk[i] = blk32[i];
+    return e;
+  };
   auto unpack = [&](double* out) {
     for (size_t e = 0; e < Ne; ++e)
       for (int A = 0; A < 6; ++A)
@@ -1358,12 +1373,12 @@ tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke
               out[e * 324 + (3 * A + r) * 18 + 3 * B + c] = blk[((e * 36) + A * 6 + B) * 9 + 3 * r + c];
   };
   if (Ke) {
-    TF_CHECK(hipMemcpy(blk.data(), h->ws.elemKe, sizeof(double) * blk.size(), hipMemcpyDeviceToHost));
+    TF_CHECK(fetch(h->ws.elemKe, h->ws.elemKe32));
     unpack(Ke);
   }
   if (Kv) {
     if (h->any_visc) {
-      TF_CHECK(hipMemcpy(blk.data(), h->ws.elemKv, sizeof(double) * blk.size(), hipMemcpyDeviceToHost));
+      TF_CHECK(fetch(h->ws.elemKv, h->ws.elemKv32));
       unpack(Kv);
     } else {
       std::memset(Kv, 0, sizeof(double) * Ne * 324);

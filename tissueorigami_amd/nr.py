@@ -24,7 +24,7 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_profile",
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
-           "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs", "tf_nr_set_krylov",
+           "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs", "tf_nr_set_krylov", "tf_nr_assemble_ex",
            "tf_copy_bandwidth",
            "tf_nr_destroy", "tf_last_error"]
 
@@ -86,6 +86,7 @@ def load():
     L.tf_nr_step.argtypes = [_h, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
                              ctypes.c_int32, ctypes.c_double, _d, ctypes.POINTER(StepStats)]
     L.tf_nr_assemble.argtypes = [_h, _d, _d, ctypes.c_double, _d, _d]
+    L.tf_nr_assemble_ex.argtypes = [_h, _d, _d, ctypes.c_double, ctypes.c_int32, _d, _d]
     L.tf_nr_solve.argtypes = [_h, _d, _d, ctypes.c_double, ctypes.c_int32, _i]
     L.tf_csr_pattern.argtypes = [_h, _i, _i, _i64]
     L.tf_nr_get_csr.argtypes = [_h, ctypes.c_double, _i, _i, _d, _i64]
@@ -221,11 +222,11 @@ class NewtonRaphsonSolver:
         return [arr[step * nt + t].as_dict() for t in range(nt)]
 
     # ---- pieces
-    def assemble(self, uk, un, dt):
+    def assemble(self, uk, un, dt, precision="fp64"):
         uk = _f64(uk).reshape(-1); un = _f64(un).reshape(-1)
         g = np.empty_like(uk); gi = np.empty_like(uk)
-        _check(load().tf_nr_assemble(self._h, _p(uk, _d), _p(un, _d), float(dt), _p(g, _d),
-                                     _p(gi, _d)))
+        _check(load().tf_nr_assemble_ex(self._h, _p(uk, _d), _p(un, _d), float(dt),
+                                        PREC[precision], _p(g, _d), _p(gi, _d)))
         return g, gi
 
     def solve(self, b, rtol=1e-10, maxit=20000):
