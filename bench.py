@@ -291,6 +291,40 @@ def run_batch(steps, warmup, mixed_rtol, mixed_krylov="cg", seed=4, n=64, sequen
     return out
 
 
+def run_bound(w, solver_steps, mixed_rtol, mixed_krylov):
+    """C3 with the reference's circumference binding (x and y of every
+    circumference node bound to the basal node of its column): mixed steps
+    on the explicit bound operator N^T J N + I_slave."""
+    m = w["mesh"]
+    nn, ne, n2 = len(m["nodes"]), len(m["conn"]), m["n2d"]
+    circ = np.where(m["at_circumference"][:n2] == 1)[0]
+    pairs = np.array([[3 * (L * n2 + k) + d, 3 * k + d] for k in circ
+                      for L in range(1, m["n_layers"] + 1) for d in (0, 1)], np.int32)
+    t0 = time.perf_counter()
+    s = nr.NewtonRaphsonSolver(nn, m["conn"], m["refpos"])
+    s.set_material(w["E"], w["nu"], w["eta"], w["ext"], None, pairs)
+    s.set_growth_rates(w["rates"])
+    s.set_surface_flags(w["place"], w["etype"], w["spans"])
+    s.set_krylov("mixed", mixed_krylov)
+    setup_s = time.perf_counter() - t0
+    _, n_slots_bound = s.pattern_info()
+    u0 = m["nodes"].reshape(-1).copy()
+    s.advance(1, w["dt"], u0=u0, precision="mixed", krylov_rtol=mixed_rtol, want_positions=False)
+    t0 = time.perf_counter()
+    u, stats = s.advance(solver_steps, w["dt"], None, 1e-8, 20, "mixed", mixed_rtol)
+    wall = time.perf_counter() - t0
+    d = u - u0
+    s.close()
+    nr_its = sum(st["nr_iterations"] for st in stats)
+    return dict(workload="c3 with circumference binding (x, y of circumference nodes to the "
+                         "basal node of their column)", pairs=int(len(pairs)),
+                steps=solver_steps, nr_iterations=nr_its, nr_iterations_per_s=nr_its / wall,
+                time_steps_per_s=solver_steps / wall,
+                krylov_iterations=int(sum(st["krylov_iterations_total"] for st in stats)),
+                converged=all(st["converged"] for st in stats), setup_s=setup_s,
+                max_slave_master_gap=float(np.abs(d[pairs[:, 0]] - d[pairs[:, 1]]).max()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +335,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-binding", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every solver kernel directly (no hipGraph replay)")
     ap.add_argument("--mixed-rtol", type=float, default=1e-3,
@@ -436,6 +471,10 @@ def main():
         res["adjoint_cost_per_forward"] = adj["cost_per_forward_device"]
         print(f"[bench] adjoint: backward/forward {adj['cost_per_forward_device']:.2f} (device), "
               f"{adj['cost_per_forward_wall']:.2f} (wall)", file=sys.stderr, flush=True)
+    if not args.no_binding and args.config == "c3":
+        res["bound"] = run_bound(w, args.steps, args.mixed_rtol, args.mixed_krylov)
+        print(f"[bench] bound: {res['bound']['nr_iterations_per_s']:.2f} NR its/s, "
+              f"{res['bound']['time_steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
     if not args.no_batch and args.config == "c3":
         res["batch"] = run_batch(args.steps, args.warmup, args.mixed_rtol, args.mixed_krylov)
     if not args.no_cpu_baseline:

@@ -87,7 +87,11 @@ tf_status tf_nr_create(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,
 /* Per-prism Young's modulus E, Poisson ratio nu and internal viscosity eta;
  * per-node external viscosity (n_nodes x 3); fixed DOF mask (n_nodes x 3,
  * Node::FixedPos, may be NULL); slave/master DOF pairs (n_pairs x 2,
- * NewtonRaphsonSolver::slaveMasterList, may be NULL).  Lame parameters follow
+ * NewtonRaphsonSolver::slaveMasterList, may be NULL).  With pairs, the solves
+ * use the bound operator N^T J N + I_slave of calculateBoundKWithSlavesMasterDoF
+ * (NewtonRaphsonSolver.cpp:128-183) stored explicitly on its own pattern, built
+ * here when the pair list changes (a DOF is the slave of one master at most and
+ * no master is a slave; TF_ERR_ARG otherwise).  Lame parameters follow
  * Prism::setElasticProperties (Prism.cpp:464-515).  Replaces the physical
  * property state read by ShapeBase (ShapeBase.h:129-143) and Node (Node.h:21-58). */
 tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu,
@@ -226,7 +230,7 @@ tf_status tf_nr_set_tile_size(tf_handle* h, int32_t slices_per_tile);
  * positive definite up to the small viscous asymmetry; a tissue whose CG
  * solve meets non-positive curvature or stalls is solved again with BiCGStab
  * in the same NR iteration (tf_step_stats.cg_fallbacks).  Defaults: FP64
- * BiCGStab, mixed CG.  Bound systems always use BiCGStab. */
+ * BiCGStab, mixed CG. */
 enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1 };
 tf_status tf_nr_set_krylov(tf_handle* h, int32_t precision_mode, int32_t method);
 

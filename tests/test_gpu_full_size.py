@@ -81,6 +81,40 @@ def test_c2_jacobian_equals_oracle_bsr(orc, nr):
     s.close()
 
 
+def test_c2_circumference_binding_matches_oracle_sparse(orc, nr):
+    """C2 with the reference's circumference binding (x and y of every
+    circumference node bound to the basal node of its column, as
+    Simulation::bindCircumferenceXY builds it): the bound FP64 step (explicit
+    N^T J N + I on the tiled solver) against the oracle's sparse bound step,
+    and the mixed step within 1e-8 of it."""
+    from tissueorigami_amd.workloads import make_workload
+    w = make_workload("c2")
+    m = w["mesh"]
+    nn, n2 = len(m["nodes"]), m["n2d"]
+    circ = np.where(m["at_circumference"][:n2] == 1)[0]
+    pairs = np.array([[3 * (L * n2 + k) + d, 3 * k + d] for k in circ
+                      for L in range(1, m["n_layers"] + 1) for d in (0, 1)], np.int32)
+    inv = orc.inv3_many(w["inc"])
+    u0 = m["nodes"].reshape(-1).copy()
+    A = orc.visc_surfaces(nn, m["conn"], u0, w["place"], w["etype"], w["spans"])
+    s = nr.NewtonRaphsonSolver(nn, m["conn"], m["refpos"])
+    s.set_material(w["E"], w["nu"], w["eta"], w["ext"], None, pairs)
+    s.set_growth(inv)
+    s.set_visc_surface(A)
+    ug, st = s.step(u0, w["dt"], 1e-8, 20, "fp64", 1e-12)
+    um, stm = s.step(u0, w["dt"], 1e-8, 20, "mixed", 1e-3)
+    lam, mu = orc.lame(w["E"], w["nu"])
+    P = orc.Problem(m["conn"], m["refpos"], inv, lam, mu, w["eta"], w["ext"], A, None, pairs)
+    uo, du, kit, conv = P.nr_step_sparse(u0, w["dt"], 1e-8, 20, 1e-12, 50000)
+    assert st["converged"] and stm["converged"] and conv
+    scale = np.abs(uo).max()
+    assert np.abs(ug - uo).max() <= 1e-9 * scale
+    assert np.abs(um - uo).max() <= 1e-8 * scale
+    d = ug - u0
+    assert np.abs(d[pairs[:, 0]] - d[pairs[:, 1]]).max() <= 1e-12
+    s.close()
+
+
 @pytest.fixture(scope="module")
 def c3(nr):
     from tissueorigami_amd.workloads import make_workload

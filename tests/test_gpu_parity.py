@@ -155,6 +155,29 @@ def test_binding_step_matches_oracle(orc, nr):
     s.close()
 
 
+@pytest.mark.parametrize("precision,method,rtol", [("fp64", "bicgstab", 1e-12),
+                                                   ("mixed", "cg", 1e-3),
+                                                   ("mixed", "bicgstab", 1e-5)])
+def test_bound_system_on_tiled_solvers_matches_oracle(orc, nr, precision, method, rtol):
+    """Binding as the explicit operator N^T J N + I_slave: the tiled solvers
+    (FP64, and mixed precision with CG or BiCGStab) reach the oracle's dense
+    bound step, and slaves move with their masters."""
+    p = problems.perturbed_problem(seed=5, visc=True)
+    nn = len(p["mesh"]["nodes"])
+    pairs = np.array([[3 * 7 + 0, 3 * 2 + 0], [3 * 7 + 1, 3 * 2 + 1], [3 * 11 + 2, 3 * 3 + 2],
+                      [3 * (nn - 2) + 0, 3 * 4 + 0], [3 * 9 + 0, 3 * 2 + 0]], np.int32)
+    s = _gpu_solver(nr, p, pairs)
+    s.set_krylov(precision, method)
+    u_gpu, st = s.step(p["un"], p["dt"], 1e-8, 20, precision, rtol)
+    u_orc, du, conv = _oracle_problem(orc, p, pairs).nr_step_dense(p["un"], p["dt"], 1e-8, 20)
+    assert st["converged"] and conv
+    tol = 1e-9 if precision == "fp64" else 1e-8
+    assert np.abs(u_gpu - u_orc).max() <= tol * np.abs(u_orc).max()
+    for sl, ms in pairs:
+        assert u_gpu[sl] - p["un"][sl] == pytest.approx(u_gpu[ms] - p["un"][ms], abs=1e-12)
+    s.close()
+
+
 def test_c1_rectangle_50_steps_match_oracle(orc, nr):
     """Plain FP64 small rectangle, 50 steps of 10 s against the oracle's
     dense-LU (Pardiso stand-in) path: node positions within 1e-9 relative."""

@@ -345,6 +345,73 @@ hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st) {
   return hipGetLastError();
 }
 
+namespace {
+// A_b slot by slot (one thread per slot, as assemble_kernel), and the
+// block-Jacobi inverse of its diagonal blocks.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bound_gather_kernel(BoundOp b, int32_t n_nodes,
+                                                              int64_t nsJ, const T* __restrict__ valJ,
+                                                              T* __restrict__ valB,
+                                                              T* __restrict__ minvB) {
+  constexpr int kCols = kBlock / kSlice;
+  const int64_t nsB = b.pat.n_slots;
+  const int lane = threadIdx.x % kSlice, kq = threadIdx.x / kSlice;
+  for (int sl = blockIdx.x; sl < b.pat.n_slices; sl += gridDim.x) {
+    const int R = sl * kSlice + lane;
+    const int64_t sbase = b.pat.slice_base[sl];
+    const int width = (int)((b.pat.slice_base[sl + 1] - sbase) / kSlice);
+    const int row_len = (R < n_nodes) ? b.pat.row_len[R] : 0;
+    for (int kk = kq; kk < width; kk += kCols) {
+      const int64_t slot = sbase + (int64_t)kk * kSlice + lane;
+      double J[9];
+      if (kk >= row_len) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) valB[(size_t)q * nsB + slot] = (T)0;
+        continue;
+      }
+      const int bm = b.blockmap[slot];
+      if (bm >= 0) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) J[q] = (double)valJ[(size_t)q * nsJ + bm];
+      } else {
+        const int ex = -bm - 1;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          double acc = 0.0;
+          for (int t = b.ex_ptr[9 * ex + q]; t < b.ex_ptr[9 * ex + q + 1]; ++t)
+            acc = acc + (double)valJ[b.ex_src[t]];
+          if (b.ex_one[9 * ex + q]) acc = acc + 1.0;
+          J[q] = acc;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 9; ++q) valB[(size_t)q * nsB + slot] = (T)J[q];
+      if (b.pat.col[slot] == R) {
+        double M[9], X[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) M[q] = (double)(T)J[q];
+        inv3_dev(M, X);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) minvB[9 * R + q] = (T)X[q];
+      }
+    }
+  }
+}
+}  // namespace
+
+hipError_t launch_bound_gather(const BoundOp& b, int32_t n_nodes, int64_t n_slots_J,
+                               const double* valJ, const float* valJ32, int mixed,
+                               hipStream_t st) {
+  const int g1 = (int)std::min<int64_t>(b.pat.n_slices, 65535 * 4);
+  if (mixed)
+    hipLaunchKernelGGL(bound_gather_kernel<float>, dim3(g1), dim3(kBlock), 0, st, b, n_nodes,
+                       n_slots_J, valJ32, b.val32, b.minv32);
+  else
+    hipLaunchKernelGGL(bound_gather_kernel<double>, dim3(g1), dim3(kBlock), 0, st, b, n_nodes,
+                       n_slots_J, valJ, b.val, b.minv);
+  return hipGetLastError();
+}
+
 hipError_t launch_bind_forces(const int32_t* pairs, int n_pairs, double* g, hipStream_t st) {
   if (n_pairs == 0) return hipSuccess;
   hipLaunchKernelGGL(bind_forces_kernel, dim3(1), dim3(64), 0, st, pairs, n_pairs, g);

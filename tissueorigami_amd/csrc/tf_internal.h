@@ -33,6 +33,33 @@ struct DevPattern {
   int32_t* node_src = nullptr;    // n_prisms * 6: e*6 + a
 };
 
+// The operator of a bound system, A_b = N^T J N + I_slave (slave/master
+// binding, NewtonRaphsonSolver::calculateBoundKWithSlavesMasterDoF,
+// NewtonRaphsonSolver.cpp:128-183), stored explicitly on its own SELL
+// pattern.  Slot s of A_b copies J slot blockmap[s] when blockmap[s] >= 0;
+// otherwise exception ex = -blockmap[s] - 1 lists, per entry q (0..8), the J
+// values to sum (planar indices q' * n_slots_J + slot_J, in ascending DOF
+// order) in ex_src[ex_ptr[9 ex + q] .. ex_ptr[9 ex + q + 1]), plus 1 where
+// ex_one[9 ex + q] (a slave's diagonal).
+struct BoundOp {
+  bool ready = false;
+  DevPattern pat;  // slice_base, col, diag_slot, row_len, tslot, n_slices, n_slots
+  int32_t* blockmap = nullptr;
+  int32_t* ex_ptr = nullptr;
+  int32_t* ex_src = nullptr;
+  uint8_t* ex_one = nullptr;
+  int32_t n_ex = 0;
+  double* val = nullptr;   // 9 * n_slots planar
+  float* val32 = nullptr;
+  double* minv = nullptr;  // block-Jacobi inverse of A_b
+  float* minv32 = nullptr;
+};
+// A_b from the assembled J (FP64, or FP32 with mixed) and its block-Jacobi
+// inverse.
+hipError_t launch_bound_gather(const BoundOp& b, int32_t n_nodes, int64_t n_slots_J,
+                               const double* valJ, const float* valJ32, int mixed,
+                               hipStream_t st);
+
 // Tiles of SELL slices for the fused solver: tile t covers slices
 // [tile_slice[t], tile_slice[t+1]) of tissue tile_tissue[t]; tissue T owns
 // tiles [tissue_tile[T], tissue_tile[T+1]).  grid = n_tiles rounded up to a

@@ -56,6 +56,10 @@ struct tf_handle {
   int32_t* pairs = nullptr;
   int32_t n_pairs = 0;
   std::vector<int32_t> h_pairs;
+  // explicit bound operator for slave/master binding, and its buffers
+  BoundOp bound;
+  std::vector<void*> bound_allocs;
+  std::vector<int32_t> bound_pairs;  // pairs the bound operator was built for
   bool any_visc = false, have_material = false, have_growth = false, have_area = false;
   bool assembled = false;
   int last_mixed = 0;
@@ -410,6 +414,194 @@ tf_status download_nodes(tf_handle* h, double* user, const double* src) {
   return TF_OK;
 }
 
+void free_bound(tf_handle* h) {
+  for (void* p : h->bound_allocs) (void)hipFree(p);
+  h->bound_allocs.clear();
+  h->bound = BoundOp();
+}
+
+template <typename T>
+hipError_t balloc(tf_handle* h, T** p, size_t count, const T* init) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return e;
+  h->bound_allocs.push_back(q);
+  *p = (T*)q;
+  if (init && count) e = hipMemcpy(q, init, sizeof(T) * count, hipMemcpyHostToDevice);
+  return e;
+}
+
+// Pattern, slot map and exception lists of A_b = N^T J N + I_slave for the
+// handle's slave/master pairs (one tissue).  A master is never a slave and a
+// DOF is the slave of one master at most, as the reference's binding lists
+// guarantee (Simulation.cpp:1081-1113).
+tf_status build_bound(tf_handle* h) {
+  free_bound(h);
+  const int Nn = h->Nn, n = 3 * Nn;
+  const int np = h->n_pairs;
+  std::vector<int32_t> M(n);
+  std::iota(M.begin(), M.end(), 0);
+  for (int p = 0; p < np; ++p) {
+    const int sd = h->h_pairs[2 * p], md = h->h_pairs[2 * p + 1];
+    if (M[sd] != sd) {
+      set_error("slave DOF " + std::to_string(sd) + " is bound twice");
+      return TF_ERR_ARG;
+    }
+    M[sd] = md;
+  }
+  for (int p = 0; p < np; ++p)
+    if (M[h->h_pairs[2 * p + 1]] != h->h_pairs[2 * p + 1]) {
+      set_error("a master DOF is itself a slave");
+      return TF_ERR_ARG;
+    }
+  // M^-1(i): i itself (when not a slave) and its slaves, ascending
+  std::vector<std::vector<int32_t>> Minv(n);
+  std::vector<uint8_t> node_bound(Nn, 0);  // node has a slave or master DOF
+  for (int d = 0; d < n; ++d) {
+    if (M[d] == d) Minv[d].push_back(d);
+  }
+  for (int d = 0; d < n; ++d)
+    if (M[d] != d) {
+      Minv[M[d]].push_back(d);
+      node_bound[d / 3] = 1;
+      node_bound[M[d] / 3] = 1;
+    }
+  for (auto& v : Minv) std::sort(v.begin(), v.end());
+  const std::vector<int32_t>& rp = h->h_rowptr;
+  const std::vector<int32_t>& nb = h->h_nbr;
+  auto slotJ = [&](int R, int C) -> int64_t {
+    const int32_t* b = nb.data() + rp[R];
+    const int32_t* e = nb.data() + rp[R + 1];
+    const int32_t* q = std::lower_bound(b, e, C);
+    if (q == e || *q != C) return -1;
+    return h->h_slice_base[R / kSlice] + (int64_t)(q - b) * kSlice + (R % kSlice);
+  };
+  // block pattern of A_b
+  std::vector<int32_t> brp(Nn + 1, 0), bnb;
+  bnb.reserve(nb.size() + 64);
+  std::vector<int32_t> cand;
+  for (int R = 0; R < Nn; ++R) {
+    cand.clear();
+    cand.push_back(R);
+    if (!node_bound[R]) {
+      for (int k = rp[R]; k < rp[R + 1]; ++k)
+        for (int q = 0; q < 3; ++q) cand.push_back(M[3 * nb[k] + q] / 3);
+    } else {
+      for (int pp = 0; pp < 3; ++pp)
+        for (int d : Minv[3 * R + pp])
+          for (int k = rp[d / 3]; k < rp[d / 3 + 1]; ++k)
+            for (int q = 0; q < 3; ++q) cand.push_back(M[3 * nb[k] + q] / 3);
+    }
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    bnb.insert(bnb.end(), cand.begin(), cand.end());
+    brp[R + 1] = (int32_t)bnb.size();
+  }
+  // SELL-64 of A_b (same slices as J)
+  const int n_slices = (Nn + kSlice - 1) / kSlice;
+  std::vector<int32_t> sb(n_slices + 1, 0), rl(Nn, 0), diag(Nn, -1);
+  int64_t total = 0;
+  for (int sl = 0; sl < n_slices; ++sl) {
+    int w = 0;
+    for (int R = sl * kSlice; R < std::min(Nn, (sl + 1) * kSlice); ++R) {
+      rl[R] = brp[R + 1] - brp[R];
+      w = std::max(w, rl[R]);
+    }
+    sb[sl] = (int32_t)total;
+    total += (int64_t)w * kSlice;
+    if (total > INT32_MAX) {
+      set_error("bound pattern too large for 32-bit slot indices");
+      return TF_ERR_ARG;
+    }
+  }
+  sb[n_slices] = (int32_t)total;
+  std::vector<int32_t> col(total, 0), tsl(total, 0), bmap(total, 0);
+  auto slotB = [&](int R, int C) -> int64_t {
+    const int32_t* b = bnb.data() + brp[R];
+    const int32_t* e = bnb.data() + brp[R + 1];
+    const int32_t* q = std::lower_bound(b, e, C);
+    return sb[R / kSlice] + (int64_t)(q - b) * kSlice + (R % kSlice);
+  };
+  std::vector<int32_t> ex_ptr(1, 0), ex_src;
+  std::vector<uint8_t> ex_one;
+  int n_ex = 0;
+  const int64_t nsJ = h->pat.n_slots;
+  for (int sl = 0; sl < n_slices; ++sl) {
+    const int w = (sb[sl + 1] - sb[sl]) / kSlice;
+    for (int lane = 0; lane < kSlice; ++lane) {
+      const int R = sl * kSlice + lane;
+      for (int k = 0; k < w; ++k) {
+        const int64_t slot = sb[sl] + (int64_t)k * kSlice + lane;
+        if (R >= Nn || k >= rl[R]) {  // padding
+          col[slot] = std::min(R, Nn - 1);
+          tsl[slot] = (int32_t)slot;
+          bmap[slot] = 0;
+          continue;
+        }
+        const int C = bnb[brp[R] + k];
+        col[slot] = C;
+        if (C == R) diag[R] = (int32_t)slot;
+        {
+          const int32_t* b = bnb.data() + brp[C];
+          const int32_t* e = bnb.data() + brp[C + 1];
+          const int32_t* q = std::lower_bound(b, e, R);
+          if (q == e || *q != R) {
+            set_error("internal: bound pattern is not symmetric");
+            return TF_ERR_ARG;
+          }
+        }
+        tsl[slot] = (int32_t)slotB(C, R);
+        if (!node_bound[R] && !node_bound[C]) {
+          bmap[slot] = (int32_t)slotJ(R, C);
+          continue;
+        }
+        // exception block: per entry, the J values of M^-1(i) x M^-1(j)
+        for (int pp = 0; pp < 3; ++pp)
+          for (int qq = 0; qq < 3; ++qq) {
+            const int i = 3 * R + pp, j = 3 * C + qq;
+            uint8_t one = 0;
+            if (M[i] != i || M[j] != j) {
+              one = (i == j) ? 1 : 0;  // slave row / column: identity only
+            } else {
+              for (int d : Minv[i]) {
+                const int Rp = d / 3, pr = d % 3;
+                for (int kk = rp[Rp]; kk < rp[Rp + 1]; ++kk)
+                  for (int q2 = 0; q2 < 3; ++q2) {
+                    const int e = 3 * nb[kk] + q2;
+                    if (M[e] != j) continue;
+                    ex_src.push_back((int32_t)((3 * pr + q2) * nsJ + slotJ(Rp, nb[kk])));
+                  }
+              }
+            }
+            ex_one.push_back(one);
+            ex_ptr.push_back((int32_t)ex_src.size());
+          }
+        bmap[slot] = -1 - n_ex;
+        ++n_ex;
+      }
+    }
+  }
+  BoundOp& B = h->bound;
+  B.pat.n_slices = n_slices;
+  B.pat.n_slots = total;
+  TF_CHECK(balloc(h, &B.pat.slice_base, sb.size(), sb.data()));
+  TF_CHECK(balloc(h, &B.pat.col, col.size(), col.data()));
+  TF_CHECK(balloc(h, &B.pat.diag_slot, diag.size(), diag.data()));
+  TF_CHECK(balloc(h, &B.pat.row_len, rl.size(), rl.data()));
+  TF_CHECK(balloc(h, &B.pat.tslot, tsl.size(), tsl.data()));
+  TF_CHECK(balloc(h, &B.blockmap, bmap.size(), bmap.data()));
+  TF_CHECK(balloc(h, &B.ex_ptr, ex_ptr.size(), ex_ptr.data()));
+  TF_CHECK(balloc(h, &B.ex_src, ex_src.size(), ex_src.empty() ? nullptr : ex_src.data()));
+  TF_CHECK(balloc(h, &B.ex_one, ex_one.size(), ex_one.empty() ? nullptr : ex_one.data()));
+  B.n_ex = n_ex;
+  TF_CHECK(balloc<double>(h, &B.val, (size_t)total * 9, nullptr));
+  TF_CHECK(balloc<float>(h, &B.val32, (size_t)total * 9, nullptr));
+  TF_CHECK(balloc<double>(h, &B.minv, (size_t)Nn * 9, nullptr));
+  TF_CHECK(balloc<float>(h, &B.minv32, (size_t)Nn * 9, nullptr));
+  B.ready = true;
+  return TF_OK;
+}
+
 // Common tail of tf_nr_create and tf_batch_create: stream, pattern, slot map,
 // device buffers, tiles.  conn holds internal node ids.  Frees h on failure.
 tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos, tf_handle** out) {
@@ -467,6 +659,15 @@ KrylovArgs krylov_args(tf_handle* h) {
   k.minv32 = h->ws.minv32;
   k.pairs = h->pairs;
   k.n_pairs = h->n_pairs;
+  if (h->bound.ready) {
+    // bound system: the explicit A_b, solved like any unbound operator
+    k.pat = h->bound.pat;
+    k.val = h->bound.val;
+    k.val32 = h->bound.val32;
+    k.minv = h->bound.minv;
+    k.minv32 = h->bound.minv32;
+    k.n_pairs = 0;
+  }
   k.ws = &h->ws;
   k.L = h->L;
   k.prof = &h->prof;
@@ -509,6 +710,8 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
   aa.g = w.g; aa.internal = w.internal;
   TF_CHECK(launch_assemble(aa, mixed, h->st));
   TF_CHECK(launch_bind_forces(h->pairs, h->n_pairs, w.g, h->st));
+  if (h->bound.ready)
+    TF_CHECK(launch_bound_gather(h->bound, h->Nn, h->pat.n_slots, w.val, w.val32, mixed, h->st));
   TF_CHECK(hipEventRecord(h->ev[2], h->st));
   h->assembled = true;
   h->last_mixed = mixed;
@@ -589,8 +792,16 @@ tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, co
   }
   if (n_pairs > 0)
     TF_CHECK(hipMemcpy(h->pairs, slave_master, sizeof(int32_t) * 2 * n_pairs, hipMemcpyHostToDevice));
+  const bool pairs_changed = n_pairs != h->n_pairs || !h->bound.ready;
   h->n_pairs = n_pairs;
   h->any_visc = any_visc;
+  if (n_pairs == 0) {
+    free_bound(h);
+  } else if (pairs_changed || h->bound_pairs != h->h_pairs) {
+    tf_status bs = build_bound(h);
+    if (bs != TF_OK) return bs;
+    h->bound_pairs = h->h_pairs;
+  }
   if (any_visc) {
     tf_status s = ensure_visc(h);
     if (s != TF_OK) return s;
@@ -744,9 +955,10 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     tf_status s = assemble(h, dt, mixed);
     if (s != TF_OK) return s;
     KrylovArgs k = krylov_args(h);
-    const int method = h->n_pairs > 0 ? KRYLOV_BICGSTAB : h->krylov_method[mixed];
+    const bool classical = h->n_pairs > 0 && !h->bound.ready;
+    const int method = classical ? KRYLOV_BICGSTAB : h->krylov_method[mixed];
     const double inner = std::max(rtol, kMixedInnerRtol);
-    if (h->n_pairs > 0) {
+    if (classical) {
       int kit = 0, state = 0;
       TF_CHECK(krylov_bicgstab(k, w.g, w.du, rtol, kmax, &kit, &state, 16));
       res.state[0] = state;
@@ -803,12 +1015,12 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
         }
       return TF_OK;
     };
-    if (h->n_pairs == 0 && method == KRYLOV_CG && failed(fb)) {
+    if (!classical && method == KRYLOV_CG && failed(fb)) {
       tf_status rs = retry(fb, 0);
       if (rs != TF_OK) return rs;
       for (int T = 0; T < nt; ++T) S[T].cg_fallbacks += fb[T];
     }
-    if (mixed && h->n_pairs == 0 && failed(fb)) {
+    if (mixed && !classical && failed(fb)) {
       tf_status rs = retry(fb, 1);
       if (rs != TF_OK) return rs;
       for (int T = 0; T < nt; ++T) S[T].fp64_fallbacks += fb[T];
@@ -898,7 +1110,7 @@ tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int
   std::vector<tf_step_stats> local(h->n_tissues);
   tf_step_stats* S = stats ? stats : local.data();
   int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
-  if (mixed && h->n_pairs > 0) mixed = 0;  // bound systems use the FP64 operator
+  if (mixed && h->n_pairs > 0 && !h->bound.ready) mixed = 0;
   const size_t n = 3 * (size_t)h->Nn;
   Workspace& w = h->ws;
   tf_status up = upload_nodes(h, w.un, u_n);
@@ -988,7 +1200,7 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
   }
   h->have_positions = true;
   int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
-  if (mixed && h->n_pairs > 0) mixed = 0;
+  if (mixed && h->n_pairs > 0 && !h->bound.ready) mixed = 0;
   const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
   tf_status worst = TF_OK;
   std::vector<tf_step_stats> S(h->n_tissues);
@@ -1547,6 +1759,7 @@ tf_status tf_nr_destroy(tf_handle* h) {
   if (!h) return TF_OK;
   if (h->st) (void)hipStreamSynchronize(h->st);
   tiled_release(&h->ws);
+  free_bound(h);
   for (auto& e : h->prof.ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
