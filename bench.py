@@ -347,7 +347,9 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(a tissue would shard by node partition with a halo exchange; see DESIGN.md)",
+              "(a tissue would shard by a 2D partition of its triangle mesh, every part keeping "
+              "all its layers, with one halo exchange per SpMV; batched tissues whole, by "
+              "bin packing)",
               file=sys.stderr)
         sys.exit(2)
     w = make_workload(args.config)

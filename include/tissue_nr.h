@@ -263,9 +263,10 @@ tf_status tf_adj_backward(tf_handle* h, const double* inv_Fg_steps, const double
  * tf_nr_profile(h, 1) resets the counters and brackets every SpMV launch
  * with HIP events on the handle's stream; tf_nr_kernel_times reads them. */
 typedef struct tf_kernel_times {
-  double spmv_ms[2];          /* summed durations of the SpMV v = J ph, fused with
-                                 s = r - alpha v and sh = M^-1 s: [0] FP64, [1] FP32
-                                 (the binding path's plain SpMV counts here too) */
+  double spmv_ms[2];          /* summed durations of the first SpMV of each Krylov
+                                 iteration: BiCGStab's v = J ph (fused with
+                                 s = r - alpha v, sh = M^-1 s) or CG's q = J p;
+                                 [0] FP64, [1] FP32 */
   int64_t spmv_launches[2];
   double spmv_t_ms[2];        /* the SpMV t = J sh fused with (t, s), (t, t) */
   int64_t spmv_t_launches[2];

@@ -97,11 +97,6 @@ struct Workspace {
          *sh = nullptr, *t = nullptr, *x = nullptr, *b = nullptr, *tmp = nullptr, *w = nullptr;
   float *r32 = nullptr, *rh32 = nullptr, *p32 = nullptr, *v32 = nullptr, *ph32 = nullptr,
         *s32 = nullptr, *sh32 = nullptr, *t32 = nullptr, *x32 = nullptr, *w32 = nullptr;
-  double* partials = nullptr;  // kParts * 4
-  double* scal = nullptr;      // Krylov scalars
-  int32_t* flag = nullptr;     // [0] state, [1] iterations
-  double* host_pinned = nullptr;
-  int32_t* host_flag = nullptr;
   // per-tissue state of the tiled solver
   int32_t* tstate = nullptr;   // 2 per tissue
   double* tscal = nullptr;     // 8 per tissue
@@ -116,17 +111,14 @@ struct Workspace {
   int graphs_on = 1;
 };
 
-constexpr int kParts = 2048;  // max workgroups of a reduction grid
 constexpr int kBlock = 256;
 
-enum Scal { S_RHO = 0, S_RHO_NEW, S_ALPHA, S_OMEGA, S_BNORM, S_RNORM, S_COUNT };
 enum KState { K_RUN = 0, K_CONVERGED = 1, K_BREAKDOWN = 2 };
 // A Krylov residual above kDivergence |b| ends the solve as a breakdown.
 constexpr double kDivergence = 1e8;
 
 struct Launch {
   hipStream_t stream;
-  int grid;  // workgroups of vector kernels (multiple of 8, <= kParts)
 };
 
 // ---- per-prism kernel (tf_element.hip) ----

@@ -280,11 +280,6 @@ tf_status alloc_workspace(tf_handle* h) {
   double** vecs[] = {&w.un, &w.uk, &w.g, &w.du, &w.internal, &w.r, &w.rh, &w.p, &w.v,
                      &w.ph, &w.s, &w.sh, &w.t, &w.x, &w.b, &w.tmp, &w.w};
   for (double** v : vecs) TF_CHECK(dalloc(h, v, n));
-  TF_CHECK(dalloc(h, &w.partials, (size_t)kParts * 5));
-  TF_CHECK(dalloc(h, &w.scal, 16));
-  TF_CHECK(dalloc(h, &w.flag, 4));
-  TF_CHECK(hipHostMalloc((void**)&w.host_pinned, 16 * sizeof(double), hipHostMallocDefault));
-  TF_CHECK(hipHostMalloc((void**)&w.host_flag, 4 * sizeof(int32_t), hipHostMallocDefault));
   TF_CHECK(hipMemset(w.elemFv, 0, sizeof(double) * (size_t)h->Ne * 18));
   return TF_OK;
 }
@@ -639,10 +634,7 @@ tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos
     set_error("upload failed");
     return fail(TF_ERR_HIP);
   }
-  int grid = (int)std::min<int64_t>(kParts, ((int64_t)Nn + kBlock - 1) / kBlock);
-  grid = std::max(8, (grid + 7) / 8 * 8);
   h->L.stream = h->st;
-  h->L.grid = grid;
   if ((s = alloc_tissue_state(h)) != TF_OK) return fail(s);
   if ((s = build_tiles(h, 8)) != TF_OK) return fail(s);
   *out = h;
@@ -955,15 +947,9 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     tf_status s = assemble(h, dt, mixed);
     if (s != TF_OK) return s;
     KrylovArgs k = krylov_args(h);
-    const bool classical = h->n_pairs > 0 && !h->bound.ready;
-    const int method = classical ? KRYLOV_BICGSTAB : h->krylov_method[mixed];
+    const int method = h->krylov_method[mixed];
     const double inner = std::max(rtol, kMixedInnerRtol);
-    if (classical) {
-      int kit = 0, state = 0;
-      TF_CHECK(krylov_bicgstab(k, w.g, w.du, rtol, kmax, &kit, &state, 16));
-      res.state[0] = state;
-      res.iters[0] = kit;
-    } else if (mixed) {
+    if (mixed) {
       TF_CHECK(tiled_mixed(k, w.g, w.du, rtol, inner, kMixedCap, 16, method, &res));
     } else if (method == KRYLOV_CG) {
       TF_CHECK(tiled_cg(k, w.g, w.du, rtol, kmax, 16, &res));
@@ -1015,12 +1001,12 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
         }
       return TF_OK;
     };
-    if (!classical && method == KRYLOV_CG && failed(fb)) {
+    if (method == KRYLOV_CG && failed(fb)) {
       tf_status rs = retry(fb, 0);
       if (rs != TF_OK) return rs;
       for (int T = 0; T < nt; ++T) S[T].cg_fallbacks += fb[T];
     }
-    if (mixed && !classical && failed(fb)) {
+    if (mixed && failed(fb)) {
       tf_status rs = retry(fb, 1);
       if (rs != TF_OK) return rs;
       for (int T = 0; T < nt; ++T) S[T].fp64_fallbacks += fb[T];
@@ -1109,8 +1095,7 @@ tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int
   }
   std::vector<tf_step_stats> local(h->n_tissues);
   tf_step_stats* S = stats ? stats : local.data();
-  int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
-  if (mixed && h->n_pairs > 0 && !h->bound.ready) mixed = 0;
+  const int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
   const size_t n = 3 * (size_t)h->Nn;
   Workspace& w = h->ws;
   tf_status up = upload_nodes(h, w.un, u_n);
@@ -1199,8 +1184,7 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
     if (up != TF_OK) return up;
   }
   h->have_positions = true;
-  int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
-  if (mixed && h->n_pairs > 0 && !h->bound.ready) mixed = 0;
+  const int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
   const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
   tf_status worst = TF_OK;
   std::vector<tf_step_stats> S(h->n_tissues);
@@ -1770,8 +1754,6 @@ tf_status tf_nr_destroy(tf_handle* h) {
   if (h->ws.tpart) (void)hipFree(h->ws.tpart);
   if (h->ws.host_tstate) (void)hipHostFree(h->ws.host_tstate);
   if (h->ws.host_tnorm) (void)hipHostFree(h->ws.host_tnorm);
-  if (h->ws.host_pinned) (void)hipHostFree(h->ws.host_pinned);
-  if (h->ws.host_flag) (void)hipHostFree(h->ws.host_flag);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->st) (void)hipStreamDestroy(h->st);
