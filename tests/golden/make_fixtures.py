@@ -3,12 +3,14 @@
 Reads (as data only) the Pardiso regression outputs and inputs shipped with the
 reference repository:
 
-  tests/sim_pardiso_reg/run07007/expected_out/{Save_Frame, Save_Growth,
-      Save_PhysicalProp, Save_Force, Save_NodeBinding,
-      Save_SpecificElementAndNodeTypes, tmp}
-  tests/sim_no_pardiso_reg/run07007/{smallRectangle.mesh, modelinput07007}
+  tests/sim_pardiso_reg/run0700x/expected_out/{Save_Frame, Save_Growth,
+      Save_GrowthRate, Save_PhysicalProp, Save_Force, Save_NodeBinding,
+      Save_GrowthRedistribution, Save_Packing, Save_TensionCompression,
+      Save_SpecificElementAndNodeTypes, tmp}   (the last two where present)
+  tests/sim_no_pardiso_reg/run0700x/{<mesh>, modelinput0700x}
 
-and writes tests/golden/run07007.npz with the readers of tissueorigami_amd/io.py
+for x = 7 (smallRectangle.mesh) and x = 8 (smallWingDisc.mesh), and writes
+tests/golden/run0700x.npz with the readers of tissueorigami_amd/io.py
 (the formats of the writers in TissueFolding/SourceCode/Simulation.cpp:8430-8638
 and 1674-1737).  The SHA-256 of every file read is stored beside the data, so
 that the writers of tissueorigami_amd/io.py can be checked byte for byte
@@ -40,10 +42,12 @@ def sha256(path):
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
 
 
-def make_run07007():
-    exp = os.path.join(REF, "tests/sim_pardiso_reg/run07007/expected_out")
-    inp = os.path.join(REF, "tests/sim_no_pardiso_reg/run07007")
-    nodes0, node_info0, conn, refpos = io.read_mesh(os.path.join(inp, "smallRectangle.mesh"))
+def make_run(run, mesh_name):
+    """run07007 (rectangle, no symmetry, no binding) and run07008 (half wing
+    disc, symmetric in y, circumference x/y bound to the basal node)."""
+    exp = os.path.join(REF, f"tests/sim_pardiso_reg/{run}/expected_out")
+    inp = os.path.join(REF, f"tests/sim_no_pardiso_reg/{run}")
+    nodes0, node_info0, conn, refpos = io.read_mesh(os.path.join(inp, mesh_name))
     fr = io.read_frames(os.path.join(exp, "Save_Frame"))
     times, frames, node_info = fr["times"], fr["positions"], fr["node_info"]
     for C in fr["conn"]:
@@ -53,37 +57,53 @@ def make_run07007():
     rate = io.read_growth_rate(os.path.join(exp, "Save_GrowthRate"), n_e)
     pp = io.read_physical_prop(os.path.join(exp, "Save_PhysicalProp"), n_e, n_n)
     forces = io.read_forces(os.path.join(exp, "Save_Force"), n_n)
-    strain = io.read_tension_compression(os.path.join(exp, "Save_TensionCompression"), n_e)
     redist = io.read_growth_redistribution(os.path.join(exp, "Save_GrowthRedistribution"), n_e)
     packing = io.read_packing(os.path.join(exp, "Save_Packing"))
     assert all(len(p) == 0 for p, _ in packing)
     types = io.read_specific_types(os.path.join(exp, "Save_SpecificElementAndNodeTypes"))
     binding = io.read_node_binding(os.path.join(exp, "Save_NodeBinding"))
-    assert all(len(b) == 0 for b in binding)
-    du_log = io.read_du_log(os.path.join(exp, "tmp"))
-    nmax = max(len(s) for s in du_log)
-    du = np.full((len(du_log), nmax), np.nan)
-    for k, s in enumerate(du_log):
-        du[k, :len(s)] = s
-    mi = os.path.join(inp, "modelinput07007")
+    assert all(np.array_equal(b, binding[0]) for b in binding), "binding changes between frames"
+    extra, names = {}, ["Save_Frame", "Save_Growth", "Save_GrowthRate", "Save_PhysicalProp",
+                        "Save_Force", "Save_GrowthRedistribution", "Save_Packing",
+                        "Save_NodeBinding", "Save_SpecificElementAndNodeTypes"]
+    tc = os.path.join(exp, "Save_TensionCompression")
+    if os.path.exists(tc):
+        extra["strain"] = io.read_tension_compression(tc, n_e)
+        names.append("Save_TensionCompression")
+    tmp = os.path.join(exp, "tmp")
+    if os.path.exists(tmp):
+        du_log = io.read_du_log(tmp)
+        nmax = max(len(s) for s in du_log)
+        du = np.full((len(du_log), nmax), np.nan)
+        for k, s in enumerate(du_log):
+            du[k, :len(s)] = s
+        extra["du_log"] = du
+        for line in open(tmp):
+            if "dorsalTipIndex:" in line:      # Simulation.cpp:8227
+                w = line.split()
+                extra["printed_tips"] = np.array([int(w[w.index("dorsalTipIndex:") + 1]),
+                                                  int(w[w.index("ventralTipIndex:") + 1])])
+                break
+    mi = os.path.join(inp, "modelinput" + run[3:])
     nu = float(modelinput_value(mi, "PoissonsRatio:"))
     dt = float(modelinput_value(mi, "TimeStep(sec):"))
-    hashes = {f"sha256_{name}": sha256(os.path.join(exp, name)) for name in (
-        "Save_Frame", "Save_Growth", "Save_GrowthRate", "Save_PhysicalProp", "Save_Force",
-        "Save_TensionCompression", "Save_GrowthRedistribution", "Save_Packing",
-        "Save_NodeBinding", "Save_SpecificElementAndNodeTypes")}
-    out = os.path.join(HERE, "run07007.npz")
+    sym_y = int(modelinput_value(mi, "symmetricInY(bool):"))
+    bind_circ = int(modelinput_value(mi, "bindCircumferenceXYToBasal(bool):"))
+    hashes = {f"sha256_{name}": sha256(os.path.join(exp, name)) for name in names}
+    out = os.path.join(HERE, run + ".npz")
     np.savez_compressed(
         out, times=times, frames=frames, node_info=node_info, mesh_nodes=nodes0,
         mesh_node_info=node_info0, conn=conn, refpos=refpos, Fg=Fg, growth_rate=rate,
         E=pp["E"], eta=pp["eta"], zremodel=pp["z_remodelling"], ext_visc=pp["ext_visc"],
-        forces=forces, strain=strain, redistribution=redist, n_packing_frames=len(packing),
-        actin=types["actin"], ecm=types["ecm"], n_binding_frames=len(binding), du_log=du,
-        nu=nu, dt=dt, **hashes)
+        forces=forces, redistribution=redist, n_packing_frames=len(packing),
+        actin=types["actin"], ecm=types["ecm"], binding=binding[0],
+        n_binding_frames=len(binding), nu=nu, dt=dt, symmetric_y=sym_y,
+        bind_circumference=bind_circ, **extra, **hashes)
     print("wrote", out, "frames", n_f, "nodes", n_n, "elements", n_e)
 
 
 if __name__ == "__main__":
     if not os.path.isdir(REF):
         sys.exit("reference repository not present; fixtures are committed already")
-    make_run07007()
+    make_run("run07007", "smallRectangle.mesh")
+    make_run("run07008", "smallWingDisc.mesh")

@@ -211,24 +211,59 @@ def test_c1_rectangle_50_steps_match_oracle(orc, nr):
     s.close()
 
 
-@pytest.mark.parametrize("k", range(10))
-def test_run07007_replay_on_gpu(orc, nr, k):
-    d = replay.load()
+REPLAY = [("run07007", k) for k in range(10)] + [("run07008", k) for k in range(-1, 10)]
+# propagated Save_Frame / Save_Force rounding, as in test_oracle_fixtures.py
+POS_TOL = {"run07007": 5e-9, "run07008": 2e-7}
+FORCE_TOL = {"run07007": 2e-5, "run07008": 5e-3}
+
+
+@pytest.mark.parametrize("run,k", REPLAY)
+def test_pardiso_run_replay_on_gpu(orc, nr, run, k):
+    """Each step of the reference's Pardiso runs run07007 and run07008 (y fixed
+    on the symmetry line, circumference x/y bound to the basal node) replayed
+    in FP64 on the device: positions, |du| log and Save_Force."""
+    d = replay.load(run)
     s_in = replay.step_inputs(d, k, orc)
     nn = d["frames"].shape[1]
     s = nr.NewtonRaphsonSolver(nn, s_in["conn"], s_in["refpos"])
-    s.set_material(s_in["E"], s_in["nu"], s_in["eta"], s_in["ext_visc"])
+    s.set_material(s_in["E"], s_in["nu"], s_in["eta"], s_in["ext_visc"], s_in["fixed"],
+                   s_in["pairs"])
     s.set_growth(s_in["invFg"])
     s.set_visc_surface(s_in["visc_surf"])
     u, st = s.step(s_in["un"], s_in["dt"], 1e-8, 20, "fp64", 1e-13)
     assert st["converged"]
-    assert np.abs(u - s_in["expected"]).max() < 5e-9
+    assert np.abs(u - s_in["expected"]).max() < POS_TOL[run]
     du = np.array(st["du_norms"])
     assert len(du) == len(s_in["du_log"])
     big = s_in["du_log"] > 1e-10
     assert np.allclose(du[big], s_in["du_log"][big], rtol=1e-5, atol=0)
-    g, gi = s.assemble(s_in["expected"], s_in["un"], s_in["dt"])
-    assert np.abs(gi - s_in["forces"]).max() < 2e-5
+    if k >= 0:
+        g, gi = s.assemble(s_in["expected"], s_in["un"], s_in["dt"])
+        assert np.abs(gi - s_in["forces"]).max() < FORCE_TOL[run]
+    s.close()
+
+
+@pytest.mark.parametrize("k", [-1, 4, 9])
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_run07008_mixed_precision_replay(orc, nr, k, method):
+    """run07008 in mixed precision (FP32 tangents and bound Krylov operator,
+    forcing term 1e-3): the step lands on the reference's frame as the FP64
+    step does."""
+    d = replay.load("run07008")
+    s_in = replay.step_inputs(d, k, orc)
+    s = nr.NewtonRaphsonSolver(d["frames"].shape[1], s_in["conn"], s_in["refpos"])
+    s.set_material(s_in["E"], s_in["nu"], s_in["eta"], s_in["ext_visc"], s_in["fixed"],
+                   s_in["pairs"])
+    s.set_growth(s_in["invFg"])
+    s.set_visc_surface(s_in["visc_surf"])
+    s.set_krylov("mixed", method)
+    u, st = s.step(s_in["un"], s_in["dt"], 1e-8, 20, "mixed", 1e-3)
+    assert st["converged"]
+    assert np.abs(u - s_in["expected"]).max() < POS_TOL["run07008"]
+    dd = u - s_in["un"]
+    p = s_in["pairs"]
+    assert np.abs(dd[p[:, 0]] - dd[p[:, 1]]).max() <= 1e-12
+    assert np.all(u[s_in["fixed"] == 1] == s_in["un"][s_in["fixed"] == 1])
     s.close()
 
 

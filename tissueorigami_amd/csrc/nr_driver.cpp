@@ -9,6 +9,9 @@
 // Input (whitespace separated): n_nodes n_prisms dt tol maxit
 //   n_nodes lines:  x y z ext_visc_x ext_visc_y ext_visc_z viscosity_surface fx fy fz
 //   n_prisms lines: 6 node ids, 18 reference coordinates, E nu eta, 9 InvFg entries
+//   optionally: n_pairs, then n_pairs lines "slave_dof master_dof"
+//     (NewtonRaphsonSolver::slaveMasterList, as Simulation::bindCircumferenceXY
+//     fills it)
 // Output: one line per node with the converged x y z (%.17g), then a line
 //   "iterations <k>" and one "du <norm>" line per NR iteration.
 #include <cstdio>
@@ -65,11 +68,26 @@ int main(int argc, char** argv) {
     std::cerr << "bad input body\n";
     return 2;
   }
+  std::vector<std::vector<int>> slaveMasterList;
+  int n_pairs = 0;
+  if (in >> n_pairs) {
+    for (int p = 0; p < n_pairs; ++p) {
+      int sl = -1, ms = -1;
+      in >> sl >> ms;
+      slaveMasterList.push_back({sl, ms});
+    }
+    if (!in) {
+      std::cerr << "bad pair list\n";
+      return 2;
+    }
+  }
   std::vector<double> norms;
   try {
     tissue_nr::NewtonRaphsonSolver NRSolver(3, nn);
     NRSolver.threshold = tol;
     NRSolver.krylovTolerance = 1e-13;
+    NRSolver.slaveMasterList = slaveMasterList;
+    NRSolver.boundNodesWithSlaveMasterDefinition = !slaveMasterList.empty();
     if (mode == "step") {
       tf_step_stats st = NRSolver.updateStepNR(Nodes, Elements, dt, maxit);
       for (int k = 0; k < st.nr_iterations; ++k) norms.push_back(st.du_norms[k]);
