@@ -6,7 +6,7 @@ reference repository:
   tests/sim_pardiso_reg/run0700x/expected_out/{Save_Frame, Save_Growth,
       Save_GrowthRate, Save_PhysicalProp, Save_Force, Save_NodeBinding,
       Save_GrowthRedistribution, Save_Packing, Save_TensionCompression,
-      Save_SpecificElementAndNodeTypes, tmp}   (the last two where present)
+      Save_SpecificElementAndNodeTypes, tmp, MeshFromEndPoint.mesh}
   tests/sim_no_pardiso_reg/run0700x/{<mesh>, modelinput0700x}
 
 for x = 7 (smallRectangle.mesh) and x = 8 (smallWingDisc.mesh), and writes
@@ -89,6 +89,11 @@ def make_run(run, mesh_name):
     dt = float(modelinput_value(mi, "TimeStep(sec):"))
     sym_y = int(modelinput_value(mi, "symmetricInY(bool):"))
     bind_circ = int(modelinput_value(mi, "bindCircumferenceXYToBasal(bool):"))
+    end = os.path.join(exp, "MeshFromEndPoint.mesh")
+    end_nodes, end_info, end_conn, _ = io.read_mesh(end)
+    assert np.array_equal(end_conn, conn)
+    extra["end_nodes"], extra["end_node_info"] = end_nodes, end_info
+    names.append("MeshFromEndPoint.mesh")
     hashes = {f"sha256_{name}": sha256(os.path.join(exp, name)) for name in names}
     out = os.path.join(HERE, run + ".npz")
     np.savez_compressed(

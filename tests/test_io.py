@@ -58,6 +58,17 @@ def test_binary_writers_reproduce_reference_files(d, tmp_path):
     assert np.array_equal(pp["ext_visc"], d["ext_visc"]) and np.array_equal(pp["E"], d["E"])
 
 
+def test_end_mesh_writer_reproduces_reference_file(d, tmp_path):
+    """MeshFromEndPoint.mesh (6 significant digits) written again from the
+    values read out of it, and read back as a .mesh input."""
+    p = tmp_path / "MeshFromEndPoint.mesh"
+    io.write_end_mesh(p, d["end_nodes"], d["end_node_info"], d["conn"])
+    assert _sha(p) == str(d["sha256_MeshFromEndPoint.mesh"])
+    nodes, info, conn, refpos = io.read_mesh(p)
+    assert np.array_equal(nodes, d["end_nodes"]) and np.array_equal(conn, d["conn"])
+    assert np.array_equal(refpos.reshape(-1, 6, 3), nodes[conn])
+
+
 def test_round_trips(tmp_path):
     rng = np.random.default_rng(3)
     from tissueorigami_amd import tissue

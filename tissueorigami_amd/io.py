@@ -7,6 +7,8 @@ files written here load in its tools (VisualiseTissueFolding, the
   .mesh input          Simulation::initiateNodesFromMeshInput /
                        initiateElementsFromMeshInput / initiatePrismFromMeshInput
                        (Simulation.cpp:2286-2321, 2431-2447)
+  MeshFromEndPoint.mesh  the final state as a new .mesh input
+                       (Simulation::writeRelaxedMeshFromCurrentState, Simulation.cpp:1833-1879)
   Save_Frame (text)    Simulation::writeSaveFileStepHeader / writeNodes /
                        writeElements / writeSaveFileStepFooter (Simulation.cpp:8445-8519)
   Save_Growth          F_g, 9 doubles per element per frame (writeGrowth, 8572-8587)
@@ -66,6 +68,26 @@ def write_mesh(path, nodes, node_info, conn, refpos):
         for c, r in zip(np.asarray(conn, int), np.asarray(refpos, float).reshape(-1, 18)):
             f.write("1\t" + "     ".join(str(v) for v in c) + "     " +
                     "   ".join("%.17g" % v for v in r) + "    \n")
+
+
+def write_end_mesh(path, positions, node_info, conn):
+    """MeshFromEndPoint.mesh as Simulation::writeRelaxedMeshFromCurrentState
+    writes it (Simulation.cpp:1833-1879): the current positions become both the
+    node positions and each prism's reference positions; every value at the
+    stream's default precision (%g, 6 digits) followed by " \t"."""
+    P = np.asarray(positions, float).reshape(-1, 3)
+    info = np.asarray(node_info, int).reshape(-1, 3)
+    C = np.asarray(conn, int).reshape(-1, 6)
+    out = [f"{len(P)}\n"]
+    for x, i in zip(P, info):
+        out.append(" \t".join(["%g" % x[0], "%g" % x[1], "%g" % x[2],
+                               str(i[0]), str(i[1]), str(i[2])]) + "\n")
+    out.append(f"{len(C)}\n")
+    for c in C:
+        out.append("1 \t" + "".join(f"{v} \t" for v in c) +
+                   "".join("%g \t" % P[v][j] for v in c for j in range(3)) + "\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
 
 
 # ---------------------------------------------------------------- Save_Frame
