@@ -91,7 +91,9 @@ tf_status tf_nr_create(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,
  * use the bound operator N^T J N + I_slave of calculateBoundKWithSlavesMasterDoF
  * (NewtonRaphsonSolver.cpp:128-183) stored explicitly on its own pattern, built
  * here when the pair list changes (a DOF is the slave of one master at most and
- * no master is a slave; TF_ERR_ARG otherwise).  Lame parameters follow
+ * no master is a slave; TF_ERR_ARG otherwise).  On a batch handle the pairs
+ * use the concatenated DOF numbering and both DOF of a pair must belong to
+ * one tissue.  Lame parameters follow
  * Prism::setElasticProperties (Prism.cpp:464-515).  Replaces the physical
  * property state read by ShapeBase (ShapeBase.h:129-143) and Node (Node.h:21-58). */
 tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu,
@@ -208,8 +210,9 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
  * batch; every tissue keeps its own NR convergence test and its own Krylov
  * scalars, so its result equals the tissue stepped alone.  Step statistics
  * are reported per tissue: tf_nr_step fills n_tissues entries, tf_nr_advance
- * n_steps x n_tissues.  Binding, tf_nr_assemble/solve/get_csr/csr_pattern and
- * the adjoint take single-tissue handles only. */
+ * n_steps x n_tissues.  Binding pairs (tf_nr_set_material) bind DOF within
+ * a tissue.  tf_nr_assemble/solve/get_csr/csr_pattern and the adjoint take
+ * single-tissue handles only. */
 tf_status tf_batch_create(int32_t n_tissues, const int32_t* nodes_per_tissue,
                           const int32_t* prisms_per_tissue, const int32_t* conn,
                           const double* ref_pos, tf_handle** out);

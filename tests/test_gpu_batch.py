@@ -239,3 +239,57 @@ def test_fp64_cg_matches_fp64_bicgstab(orc, nr):
     assert np.abs(uc - ub).max() <= 1e-9 * scale
     assert np.abs(uc - uo).max() <= 1e-9 * scale
     s.close()
+
+
+def _circumference_pairs(m):
+    """x and y of every circumference column bound to its basal node, as
+    Simulation::bindCircumferenceXY builds them (node ids local to the mesh)."""
+    n2 = m["n2d"]
+    circ = np.where(m["at_circumference"][:n2] == 1)[0]
+    return np.array([[3 * (L * n2 + k) + d, 3 * k + d] for k in circ
+                     for L in range(1, m["n_layers"] + 1) for d in (0, 1)], np.int32)
+
+
+@pytest.mark.parametrize("precision,rtol,method", [
+    ("fp64", 1e-10, "bicgstab"), ("mixed", 1e-3, "cg"), ("mixed", 1e-5, "bicgstab")])
+def test_batch_binding_is_bitwise_each_tissue_alone(orc, nr, precision, rtol, method):
+    """Circumference binding in every tissue of a batch (the pairs of tissue t
+    numbered in the concatenated node order): each tissue's step equals the
+    same bound tissue stepped alone, bit for bit, and its slaves move with
+    their masters."""
+    cs = _ellipses(4, seed=9)
+    dt = 60.0
+    b, u0 = _batch_physical(nr, orc, cs, dt, 8)
+    offs = np.concatenate([[0], np.cumsum([len(c["mesh"]["nodes"]) for c in cs])])
+    pairs = [_circumference_pairs(c["mesh"]) for c in cs]
+    cat = lambda k: np.concatenate([c[k] for c in cs])
+    b.set_material(cat("E"), cat("nu"), cat("eta"), cat("ext_visc"), None,
+                   np.concatenate([p + 3 * offs[t] for t, p in enumerate(pairs)]))
+    b.set_krylov(precision, method)
+    ub, stats = b.step(u0, dt, 1e-8, 20, precision, rtol)
+    parts = b.split_nodes(ub)
+    for t, c in enumerate(cs):
+        s = _alone_physical(nr, orc, c, dt, 8)
+        s.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"], None, pairs[t])
+        s.set_krylov(precision, method)
+        u_start = c["mesh"]["nodes"].reshape(-1)
+        ua, st = s.step(u_start, dt, 1e-8, 20, precision, rtol)
+        assert st["converged"] and stats[t]["converged"]
+        assert st["krylov_iterations"] == stats[t]["krylov_iterations"], t
+        assert st["du_norms"] == stats[t]["du_norms"], t
+        assert np.array_equal(parts[t].reshape(-1), ua), t
+        d = ua - u_start
+        assert np.abs(d[pairs[t][:, 0]] - d[pairs[t][:, 1]]).max() <= 1e-12
+        s.close()
+    b.close()
+
+
+def test_batch_rejects_a_pair_across_tissues(orc, nr):
+    cs = _ellipses(2, seed=9)
+    b, u0 = _batch_physical(nr, orc, cs, 60.0, 8)
+    n0 = len(cs[0]["mesh"]["nodes"])
+    cat = lambda k: np.concatenate([c[k] for c in cs])
+    with pytest.raises(nr.TFError, match="binds two tissues"):
+        b.set_material(cat("E"), cat("nu"), cat("eta"), cat("ext_visc"), None,
+                       np.array([[3 * n0, 0]], np.int32))
+    b.close()

@@ -212,22 +212,30 @@ __global__ __launch_bounds__(kBlock) void node_force_kernel(AssembleArgs a) {
 // Slave/master binding of the residual, pair by pair in list order
 // (NewtonRaphsonSolver.cpp:148-158).  One thread: the list is short and the
 // reference order is sequential.
-__global__ void bind_forces_kernel(const int32_t* pairs, int n_pairs, double* g) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int p = 0; p < n_pairs; ++p) {
-    const int s = pairs[2 * p], m = pairs[2 * p + 1];
-    const double slaveForce = g[s];
-    double masterForce = g[m];
-    masterForce += slaveForce;
+// One thread per master: its slaves' forces are added to it one by one in
+// the order of the pair list, as the reference's serial loop adds them, and
+// cleared.  A slave is never a master and belongs to one master, so no two
+// threads touch the same entry.
+__global__ void bind_forces_kernel(const int32_t* ptr, const int32_t* master,
+                                   const int32_t* slave, int n_masters, double* g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_masters) return;
+  const int m = master[i];
+  double masterForce = g[m];
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const int s = slave[k];
+    masterForce += g[s];
     g[s] = 0.0;
-    g[m] = masterForce;
   }
+  g[m] = masterForce;
 }
 
-// equateSlaveDisplacementsToMasters (NewtonRaphsonSolver.cpp:185-193).
+// equateSlaveDisplacementsToMasters (NewtonRaphsonSolver.cpp:185-193), one
+// thread per pair (masters are never slaves, so no entry is read after it is
+// written).
 __global__ void bind_increment_kernel(const int32_t* pairs, int n_pairs, double* du) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int p = 0; p < n_pairs; ++p) du[pairs[2 * p]] = du[pairs[2 * p + 1]];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n_pairs) du[pairs[2 * p]] = du[pairs[2 * p + 1]];
 }
 
 // Exposed-surface area of one triangle of a prism (Prism::calculateApicalArea /
@@ -412,15 +420,17 @@ hipError_t launch_bound_gather(const BoundOp& b, int32_t n_nodes, int64_t n_slot
   return hipGetLastError();
 }
 
-hipError_t launch_bind_forces(const int32_t* pairs, int n_pairs, double* g, hipStream_t st) {
-  if (n_pairs == 0) return hipSuccess;
-  hipLaunchKernelGGL(bind_forces_kernel, dim3(1), dim3(64), 0, st, pairs, n_pairs, g);
+hipError_t launch_bind_forces(const BoundOp& b, double* g, hipStream_t st) {
+  if (b.n_masters == 0) return hipSuccess;
+  hipLaunchKernelGGL(bind_forces_kernel, dim3(grid_for(b.n_masters)), dim3(kBlock), 0, st,
+                     b.bm_ptr, b.bm_master, b.bm_slave, b.n_masters, g);
   return hipGetLastError();
 }
 
 hipError_t launch_bind_increment(const int32_t* pairs, int n_pairs, double* du, hipStream_t st) {
   if (n_pairs == 0) return hipSuccess;
-  hipLaunchKernelGGL(bind_increment_kernel, dim3(1), dim3(64), 0, st, pairs, n_pairs, du);
+  hipLaunchKernelGGL(bind_increment_kernel, dim3(grid_for(n_pairs)), dim3(kBlock), 0, st, pairs,
+                     n_pairs, du);
   return hipGetLastError();
 }
 

@@ -53,6 +53,12 @@ struct BoundOp {
   float* val32 = nullptr;
   double* minv = nullptr;  // block-Jacobi inverse of A_b
   float* minv32 = nullptr;
+  // the pairs grouped by master: master bm_master[i] has the slaves
+  // bm_slave[bm_ptr[i] .. bm_ptr[i+1]) in the order of the pair list
+  int32_t* bm_ptr = nullptr;
+  int32_t* bm_master = nullptr;
+  int32_t* bm_slave = nullptr;
+  int32_t n_masters = 0;
 };
 // A_b from the assembled J (FP64, or FP32 with mixed) and its block-Jacobi
 // inverse.
@@ -168,7 +174,7 @@ struct AssembleArgs {
   double* internal;
 };
 hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st);
-hipError_t launch_bind_forces(const int32_t* pairs, int n_pairs, double* g, hipStream_t st);
+hipError_t launch_bind_forces(const BoundOp& b, double* g, hipStream_t st);
 hipError_t launch_bind_increment(const int32_t* pairs, int n_pairs, double* du, hipStream_t st);
 hipError_t launch_visc_surface(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,
                                const double* u, const int32_t* node_ptr,

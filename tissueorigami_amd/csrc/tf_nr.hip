@@ -427,9 +427,10 @@ hipError_t balloc(tf_handle* h, T** p, size_t count, const T* init) {
 }
 
 // Pattern, slot map and exception lists of A_b = N^T J N + I_slave for the
-// handle's slave/master pairs (one tissue).  A master is never a slave and a
-// DOF is the slave of one master at most, as the reference's binding lists
-// guarantee (Simulation.cpp:1081-1113).
+// handle's slave/master pairs (internal numbering; in a batch both DOF of a
+// pair lie in one tissue, so A_b keeps the tissues' rows apart and keeps J's
+// slices).  A master is never a slave and a DOF is the slave of one master at
+// most, as the reference's binding lists guarantee (Simulation.cpp:1081-1113).
 tf_status build_bound(tf_handle* h) {
   free_bound(h);
   const int Nn = h->Nn, n = 3 * Nn;
@@ -593,6 +594,26 @@ tf_status build_bound(tf_handle* h) {
   TF_CHECK(balloc<float>(h, &B.val32, (size_t)total * 9, nullptr));
   TF_CHECK(balloc<double>(h, &B.minv, (size_t)Nn * 9, nullptr));
   TF_CHECK(balloc<float>(h, &B.minv32, (size_t)Nn * 9, nullptr));
+  // the pairs grouped by master, masters in order of first appearance, each
+  // master's slaves in pair order
+  std::vector<int32_t> first(n, -1), masters, cnt;
+  for (int p = 0; p < np; ++p) {
+    const int md = h->h_pairs[2 * p + 1];
+    if (first[md] < 0) {
+      first[md] = (int32_t)masters.size();
+      masters.push_back(md);
+      cnt.push_back(0);
+    }
+    ++cnt[first[md]];
+  }
+  std::vector<int32_t> bptr(masters.size() + 1, 0), bsl(np);
+  for (size_t i = 0; i < masters.size(); ++i) bptr[i + 1] = bptr[i] + cnt[i];
+  std::vector<int32_t> fill(bptr.begin(), bptr.end() - 1);
+  for (int p = 0; p < np; ++p) bsl[fill[first[h->h_pairs[2 * p + 1]]]++] = h->h_pairs[2 * p];
+  TF_CHECK(balloc(h, &B.bm_ptr, bptr.size(), bptr.data()));
+  TF_CHECK(balloc(h, &B.bm_master, masters.size(), masters.data()));
+  TF_CHECK(balloc(h, &B.bm_slave, bsl.size(), bsl.data()));
+  B.n_masters = (int32_t)masters.size();
   B.ready = true;
   return TF_OK;
 }
@@ -701,7 +722,7 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
   aa.minv = w.minv; aa.minv32 = w.minv32;
   aa.g = w.g; aa.internal = w.internal;
   TF_CHECK(launch_assemble(aa, mixed, h->st));
-  TF_CHECK(launch_bind_forces(h->pairs, h->n_pairs, w.g, h->st));
+  if (h->n_pairs > 0) TF_CHECK(launch_bind_forces(h->bound, w.g, h->st));
   if (h->bound.ready)
     TF_CHECK(launch_bound_gather(h->bound, h->Nn, h->pat.n_slots, w.val, w.val32, mixed, h->st));
   TF_CHECK(hipEventRecord(h->ev[2], h->st));
@@ -754,23 +775,33 @@ tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, co
     mu[e] = E[e] / 2.0 / (1 + v);
     if (eta[e] != 0) any_visc = true;
   }
-  if (h->batch && n_pairs > 0) {
-    set_error("slave/master binding is not supported for a batch of tissues");
-    return TF_ERR_ARG;
-  }
   std::vector<uint8_t> fu((size_t)h->n_user * 3, 0);
   if (fixed)
     for (size_t i = 0; i < fu.size(); ++i) fu[i] = fixed[i] ? 1 : 0;
   std::vector<uint8_t> flags = to_internal<uint8_t>(h, fu.data(), 3, (uint8_t)1);
   std::vector<double> ev = to_internal<double>(h, ext_visc, 3, 0.0);
-  h->h_pairs.assign(slave_master, slave_master + 2 * (size_t)n_pairs);
+  // pairs in the caller's DOF numbering (tissues concatenated) -> the
+  // internal padded numbering; both DOF of a pair must be in one tissue
+  h->h_pairs.resize(2 * (size_t)n_pairs);
+  const int nU = h->n_user;
   for (int p = 0; p < n_pairs; ++p) {
     const int s = slave_master[2 * p], m = slave_master[2 * p + 1];
-    if (s < 0 || m < 0 || s >= 3 * Nn || m >= 3 * Nn || s == m) {
+    if (s < 0 || m < 0 || s >= 3 * nU || m >= 3 * nU || s == m) {
       set_error("slave/master pair out of range");
       return TF_ERR_ARG;
     }
-    flags[s] |= 2;
+    const int ts = (int)(std::upper_bound(h->user_off.begin(), h->user_off.end(), s / 3) -
+                         h->user_off.begin()) - 1;
+    const int tm = (int)(std::upper_bound(h->user_off.begin(), h->user_off.end(), m / 3) -
+                         h->user_off.begin()) - 1;
+    if (ts != tm) {
+      set_error("slave/master pair " + std::to_string(p) + " binds two tissues");
+      return TF_ERR_ARG;
+    }
+    const int shift = 3 * (h->int_off[ts] - h->user_off[ts]);
+    h->h_pairs[2 * p] = s + shift;
+    h->h_pairs[2 * p + 1] = m + shift;
+    flags[s + shift] |= 2;
   }
   TF_CHECK(hipMemcpy(h->lam, lam.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(h->mu, mu.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
@@ -783,7 +814,8 @@ tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, co
     h->pairs = p;
   }
   if (n_pairs > 0)
-    TF_CHECK(hipMemcpy(h->pairs, slave_master, sizeof(int32_t) * 2 * n_pairs, hipMemcpyHostToDevice));
+    TF_CHECK(hipMemcpy(h->pairs, h->h_pairs.data(), sizeof(int32_t) * 2 * n_pairs,
+                       hipMemcpyHostToDevice));
   const bool pairs_changed = n_pairs != h->n_pairs || !h->bound.ready;
   h->n_pairs = n_pairs;
   h->any_visc = any_visc;
