@@ -94,14 +94,16 @@ def bicgstab_iteration_bytes(n_blocks, n_nodes, s):
             + n_nodes * s * (8 * 3) + n_nodes * s * (6 * 3 + 9))
 
 
-def assembly_bytes(n_prisms, n_blocks, n_nodes, s, visc):
+def assembly_bytes(n_prisms, n_blocks, n_nodes, s, visc, combined=False):
     """Algorithmic bytes of the element kernel and the slot-map assembly of one
     NR iteration (tangent precision s bytes): element inputs (connectivity,
     reference shape, InvFg, Lame/viscosity, positions at two time levels),
     18x18 tangents and 18-forces written, then each block gathered once with
     its source index, the operator (9 values per block) and the block-Jacobi
-    inverse written, node forces read and the residual written."""
-    nk = 2 if visc else 1
+    inverse written, node forces read and the residual written.  With viscosity
+    the FP64 path writes and gathers an elastic and a viscous tangent; the
+    mixed path (combined) one FP32 tangent holding both."""
+    nk = 2 if visc and not combined else 1
     elem = n_prisms * (24 + 144 + 72 + 24 + 2 * 144 + nk * 324 * s + 2 * 144)
     asm = n_prisms * 36 * (4 + nk * 9 * s) + n_blocks * 9 * s + n_nodes * 9 * (s + 8)
     forces = n_prisms * 2 * 144 + n_nodes * (3 * 8 * 4)
@@ -457,7 +459,7 @@ def main():
         k_per = t["krylov_iterations"] / t["nr_iterations"]
         it_bytes = (cg_iteration_bytes if meth == "cg" else bicgstab_iteration_bytes)(
             n_blocks, nn, sz)
-        bnr = assembly_bytes(ne, n_blocks, nn, sz, visc) + k_per * it_bytes
+        bnr = assembly_bytes(ne, n_blocks, nn, sz, visc, combined=key == "mixed") + k_per * it_bytes
         secs = (t["ms_element"] + t["ms_assembly"] + t["ms_krylov"]) / 1e3 / t["nr_iterations"]
         res.setdefault("roofline_nr_iteration", {})[key] = dict(
             algorithmic_bytes=bnr, krylov_iterations=k_per, device_s=secs,

@@ -164,13 +164,16 @@ tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz);
 /* The assembled K of the last tf_nr_assemble in the reference orientation,
  * as constructiaForPardiso builds it: entries with |K_ij| > threshold (the
  * reference uses 1e-13), row-major, ascending columns, 0-based.  Binding
- * pairs are applied as calculateBoundKWithSlavesMasterDoF does.  Pass
+ * pairs are applied as calculateBoundKWithSlavesMasterDoF does.  After a
+ * mixed assembly the values are those of the FP32 operator.  Pass
  * ja == NULL to query *nnz first. */
 tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja, double* a,
                         int64_t* nnz);
 
 /* Per-prism outputs of the last assembly, for element-level parity (the
- * tangents are the FP32 ones after a mixed tf_nr_assemble_ex):
+ * tangents are the FP32 ones after a mixed tf_nr_assemble_ex, recomputed from
+ * that call's inputs: the mixed assembly itself keeps only their per-block
+ * sums, so the FP32 tangents need tf_nr_assemble_ex as the last assembly):
  * elastic and internal viscous element forces (n_prisms x 18,
  * ElementalElasticSystemForces / ElementalInternalViscousSystemForces before
  * fixed-DOF masking) and the 18 x 18 tangents TriPointKe / TriPointKv in the

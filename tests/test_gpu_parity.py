@@ -104,6 +104,27 @@ def test_assembled_jacobian_and_residual(orc, nr, visc, fixed):
     s.close()
 
 
+@pytest.mark.parametrize("visc,fixed", [(False, False), (True, False), (True, True)])
+def test_mixed_operator_matches_oracle(orc, nr, visc, fixed):
+    """The FP32 operator of the mixed assembly equals the oracle's FP64 K to
+    FP32 rounding.  The element kernel sums each prism's elastic block and the
+    viscous block that the reference's transposed storage places at the same
+    K block; a misplaced viscous block would be off by its asymmetry (5e-5 of
+    the largest entry here).  The residual stays FP64."""
+    p = problems.perturbed_problem(seed=2, visc=visc, fixed=fixed)
+    s = _gpu_solver(nr, p)
+    g, gi = s.assemble(p["uk"], p["un"], p["dt"], precision="mixed")
+    K, og, ogi = _oracle_problem(orc, p).assemble_dense(p["uk"], p["un"], p["dt"])
+    n = K.shape[0]
+    ia, ja, a = s.get_csr(threshold=-1.0)
+    G = np.zeros((n, n))
+    G[np.repeat(np.arange(n), np.diff(ia)), ja] = a
+    scale = np.abs(K).max()
+    assert np.abs(G - K).max() <= 4e-6 * scale
+    assert np.abs(g - og).max() <= 1e-13 * np.abs(og).max()
+    s.close()
+
+
 def test_assembly_is_bitwise_deterministic(nr):
     p = problems.perturbed_problem(seed=3, visc=True)
     s = _gpu_solver(nr, p)

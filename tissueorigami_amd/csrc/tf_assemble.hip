@@ -340,7 +340,7 @@ int grid_for(int64_t n) {
 
 hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st) {
   const int g1 = (int)std::min<int64_t>(a.pat.n_slices, 65535 * 4);
-  if (a.any_visc) {
+  if (a.any_visc && !(mixed && a.combined)) {
     if (mixed) hipLaunchKernelGGL((assemble_kernel<true, true>), dim3(g1), dim3(kBlock), 0, st, a);
     else hipLaunchKernelGGL((assemble_kernel<true, false>), dim3(g1), dim3(kBlock), 0, st, a);
   } else {

@@ -190,8 +190,76 @@ __device__ inline void mixed_blocks(const GPData* sd, const GPVisc* svd, int el,
   }
 }
 
-template <bool VISC, bool MIXED>
+// K block (a, b) of the mixed-precision operator in one pass: the elastic
+// block (a, b) plus the viscous contribution that the reference's transposed
+// storage of TriPointKv places at block (a, b), which is the viscous block
+// (b, a) of mixed_blocks transposed.  With Na = dN_a, Nb = dN_b that
+// contribution at entry (j, i) is
+//   (eta/2 (dot P_ij + Na_i (Nb^T P)_j) + (S Nb)_i Na_j - (S Na)_i Nb_j) f.
+// The assembly then gathers one block per element instead of two.
+__device__ inline void mixed_sum_block(const GPData* sd, const GPVisc* svd, int el, int ia, int ib,
+                                       bool visc, float w, float K[9]) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int q = 0; q < 9; ++q) K[q] = 0.f;
+  const bool has_eta = visc && svd[el * 6].eta != 0;
+#pragma unroll 1
+  for (int g = 0; g < 6; ++g) {
+    const GPData& d = sd[el * 6 + g];
+    float Na[3], Nb[3], sg[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { Na[j] = (float)d.dN[ia][j]; Nb[j] = (float)d.dN[ib][j]; }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) sg[q] = (float)d.sig[q];
+    const float cl = (float)d.cl, cm = (float)d.cm;
+    const float vol = (float)(d.detF * d.detdXde);
+    const float dot = Na[0] * Nb[0] + Na[1] * Nb[1] + Na[2] * Nb[2];
+    float sNb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) sNb[i] = sg[3 * i] * Nb[0] + sg[3 * i + 1] * Nb[1] + sg[3 * i + 2] * Nb[2];
+    const float geo = Na[0] * sNb[0] + Na[1] * sNb[1] + Na[2] * sNb[2];
+    const float diag = cm * dot + vol * geo;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float v = cl * Na[i] * Nb[k] + cm * Na[k] * Nb[i];
+        if (i == k) v += diag;
+        K[3 * i + k] += v * w;
+      }
+    if (has_eta) {
+      const GPVisc& dv = svd[el * 6 + g];
+      float P[9], S[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) { P[q] = (float)dv.P[q]; S[q] = (float)dv.sv[q]; }
+      float NbP[3], SNa[3], SNb[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) NbP[j] = Nb[0] * P[j] + Nb[1] * P[3 + j] + Nb[2] * P[6 + j];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        SNa[i] = S[3 * i] * Na[0] + S[3 * i + 1] * Na[1] + S[3 * i + 2] * Na[2];
+        SNb[i] = S[3 * i] * Nb[0] + S[3 * i + 1] * Nb[1] + S[3 * i + 2] * Nb[2];
+      }
+      const float eta = (float)dv.eta;
+      const float f = vol * w;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const float k1 = 0.5f * (dot * P[3 * i + j] + Na[i] * NbP[j]);
+          const float k2 = SNb[i] * Na[j] - SNa[i] * Nb[j];
+          K[3 * j + i] += (eta * k1 + k2) * f;
+        }
+    }
+  }
+}
+
+// MODE 0: FP64 tangents (oracle operation order); 1: FP32 elastic and
+// viscous tangents stored apart (Ke32, Kv32, for element-level parity);
+// 2: FP32 blocks of both summed per K block into Ke32 (mixed-precision NR).
+template <bool VISC, int MODE>
 __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementArgs a) {
+  constexpr bool MIXED = MODE != 0;
   __shared__ GPData sd[kEPB * 6];
   __shared__ GPVisc svd[VISC ? kEPB * 6 : 1];
   const int tid = threadIdx.x;
@@ -366,9 +434,16 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
       const int e = e0 + el;
       if (e >= a.n_prisms) break;
       const int ia = ab / 6, ib = ab % 6;
+      const size_t base = ((size_t)e * 36 + ab) * 9;
+      if (MODE == 2) {
+        float K[9];
+        mixed_sum_block(sd, svd, el, ia, ib, VISC, (float)w, K);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) a.Ke32[base + q] = K[q];
+        continue;
+      }
       float Ke[9], Kv[9];
       mixed_blocks(sd, svd, el, ia, ib, VISC, (float)w, Ke, Kv);
-      const size_t base = ((size_t)e * 36 + ab) * 9;
 #pragma unroll
       for (int q = 0; q < 9; ++q) a.Ke32[base + q] = Ke[q];
       if (VISC) {
@@ -592,20 +667,21 @@ ElemConst make_const() {
 
 }  // namespace
 
-hipError_t launch_element(const ElementArgs& a, int mixed, hipStream_t st) {
+hipError_t launch_element(const ElementArgs& a, int mode, hipStream_t st) {
   static const ElemConst c = make_const();
   if (a.n_prisms == 0) return hipSuccess;
   const int grid = (a.n_prisms + kEPB - 1) / kEPB;
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, st, c, a);
+  };
   if (a.any_visc) {
-    if (mixed)
-      hipLaunchKernelGGL((element_kernel<true, true>), dim3(grid), dim3(kThreads), 0, st, c, a);
-    else
-      hipLaunchKernelGGL((element_kernel<true, false>), dim3(grid), dim3(kThreads), 0, st, c, a);
+    if (mode == 2) go(element_kernel<true, 2>);
+    else if (mode == 1) go(element_kernel<true, 1>);
+    else go(element_kernel<true, 0>);
   } else {
-    if (mixed)
-      hipLaunchKernelGGL((element_kernel<false, true>), dim3(grid), dim3(kThreads), 0, st, c, a);
-    else
-      hipLaunchKernelGGL((element_kernel<false, false>), dim3(grid), dim3(kThreads), 0, st, c, a);
+    if (mode == 2) go(element_kernel<false, 2>);
+    else if (mode == 1) go(element_kernel<false, 1>);
+    else go(element_kernel<false, 0>);
   }
   return hipGetLastError();
 }

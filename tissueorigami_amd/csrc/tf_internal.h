@@ -147,7 +147,9 @@ struct ElementArgs {
   double* Fe;
   double* Fv;
 };
-hipError_t launch_element(const ElementArgs& a, int mixed, hipStream_t st);
+// mode 0: FP64 tangents; 1: FP32 elastic and viscous tangents apart;
+// 2: FP32 elastic + viscous summed per K block into Ke32
+hipError_t launch_element(const ElementArgs& a, int mode, hipStream_t st);
 
 // ---- assembly (tf_assemble.hip) ----
 struct AssembleArgs {
@@ -160,6 +162,7 @@ struct AssembleArgs {
   const double* Fe;
   const double* Fv;
   int any_visc;
+  int combined;  // mixed: Ke32 already holds elastic + viscous (element mode 2)
   const double* ext_visc;
   const double* area;
   const uint8_t* dof_flags;  // bit0 fixed, bit1 slave

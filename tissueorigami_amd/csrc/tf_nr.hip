@@ -63,6 +63,10 @@ struct tf_handle {
   bool any_visc = false, have_material = false, have_growth = false, have_area = false;
   bool assembled = false;
   int last_mixed = 0;
+  // ws.uk / ws.un still hold the inputs of the last assembly (tf_nr_assemble_ex),
+  // and last_dt its time step: tf_nr_element_outputs recomputes mixed tangents from them
+  bool assembly_inputs_kept = false;
+  double last_dt = 0.0;
   Workspace ws;
   Launch L;
   Prof prof;
@@ -707,7 +711,8 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
                  h->any_visc ? 1 : 0, w.elemKe, w.elemKv, w.elemKe32, w.elemKv32, w.elemFe,
                  w.elemFv};
   TF_CHECK(hipEventRecord(h->ev[0], h->st));
-  TF_CHECK(launch_element(ea, mixed, h->st));
+  // mixed: FP32 elastic + viscous blocks summed per K block (one gather each)
+  TF_CHECK(launch_element(ea, mixed ? 2 : 0, h->st));
   TF_CHECK(hipEventRecord(h->ev[1], h->st));
   AssembleArgs aa;
   aa.n_nodes = h->Nn;
@@ -715,6 +720,7 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
   aa.Ke = w.elemKe; aa.Kv = w.elemKv; aa.Ke32 = w.elemKe32; aa.Kv32 = w.elemKv32;
   aa.Fe = w.elemFe; aa.Fv = w.elemFv;
   aa.any_visc = h->any_visc ? 1 : 0;
+  aa.combined = mixed ? 1 : 0;
   aa.ext_visc = h->extv; aa.area = h->area; aa.dof_flags = h->dof_flags;
   aa.uk = w.uk; aa.un = w.un; aa.dt = dt;
   aa.val = mixed ? nullptr : w.val;
@@ -728,6 +734,8 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
   TF_CHECK(hipEventRecord(h->ev[2], h->st));
   h->assembled = true;
   h->last_mixed = mixed;
+  h->last_dt = dt;
+  h->assembly_inputs_kept = false;
   return TF_OK;
 }
 
@@ -911,6 +919,7 @@ tf_status tf_nr_assemble_ex(tf_handle* h, const double* uk, const double* un, do
   TF_CHECK(hipMemcpyAsync(h->ws.un, un, sizeof(double) * n, hipMemcpyHostToDevice, h->st));
   tf_status s = assemble(h, dt, precision_mode == TF_PREC_MIXED ? 1 : 0);
   if (s != TF_OK) return s;
+  h->assembly_inputs_kept = true;
   if (g_out) TF_CHECK(hipMemcpyAsync(g_out, h->ws.g, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
   if (internal_out)
     TF_CHECK(hipMemcpyAsync(internal_out, h->ws.internal, sizeof(double) * n, hipMemcpyDeviceToHost, h->st));
@@ -1478,14 +1487,21 @@ tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja
     set_error("tf_nr_get_csr: invalid arguments");
     return TF_ERR_ARG;
   }
-  if (!h->assembled || h->last_mixed) {
-    set_error("tf_nr_get_csr needs a preceding FP64 tf_nr_assemble");
+  if (!h->assembled) {
+    set_error("tf_nr_get_csr needs a preceding tf_nr_assemble");
     return TF_ERR_ARG;
   }
   const int Nn = h->Nn;
   const int64_t S = h->pat.n_slots;
   std::vector<double> val((size_t)S * 9);
-  TF_CHECK(hipMemcpy(val.data(), h->ws.val, sizeof(double) * val.size(), hipMemcpyDeviceToHost));
+  if (h->last_mixed) {  // the FP32 operator of the mixed assembly
+    std::vector<float> v32(val.size());
+    TF_CHECK(hipMemcpy(v32.data(), h->ws.val32, sizeof(float) * v32.size(),
+                       hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < val.size(); ++i) val[i] = v32[i];
+  } else {
+    TF_CHECK(hipMemcpy(val.data(), h->ws.val, sizeof(double) * val.size(), hipMemcpyDeviceToHost));
+  }
   auto slot_of = [&](int R, int C) -> int64_t {
     const int32_t* b = h->h_nbr.data() + h->h_rowptr[R];
     const int32_t* e = h->h_nbr.data() + h->h_rowptr[R + 1];
@@ -1579,6 +1595,20 @@ tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke
     return TF_ERR_ARG;
   }
   const size_t Ne = h->Ne;
+  if (h->last_mixed && (Ke || Kv)) {
+    // the mixed assembly keeps only the summed FP32 blocks: the separate FP32
+    // tangents are recomputed from the assembly's inputs
+    if (!h->assembly_inputs_kept) {
+      set_error("tf_nr_element_outputs: mixed tangents need a preceding tf_nr_assemble_ex");
+      return TF_ERR_ARG;
+    }
+    Workspace& w = h->ws;
+    ElementArgs ea{h->Ne, h->conn, h->refpos, h->invFg, h->lam, h->mu, h->eta, w.uk, w.un,
+                   h->last_dt, h->any_visc ? 1 : 0, w.elemKe, w.elemKv, w.elemKe32, w.elemKv32,
+                   w.elemFe, w.elemFv};
+    TF_CHECK(launch_element(ea, 1, h->st));
+    TF_CHECK(hipStreamSynchronize(h->st));
+  }
   if (ge) TF_CHECK(hipMemcpy(ge, h->ws.elemFe, sizeof(double) * Ne * 18, hipMemcpyDeviceToHost));
   if (gv) TF_CHECK(hipMemcpy(gv, h->ws.elemFv, sizeof(double) * Ne * 18, hipMemcpyDeviceToHost));
   std::vector<double> blk(Ne * 324);
