@@ -295,6 +295,20 @@ tf_status tf_copy_bandwidth(int64_t bytes, int32_t reps, double* gb_per_s);
  * (default on; profiling launches directly).  No reference counterpart. */
 tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable);
 
+/* Two-level preconditioner of the mixed-precision CG solve (single-tissue
+ * handles; no reference counterpart).  Block Jacobi is completed by a coarse
+ * correction P A_c^-1 P^T r, where P holds the rigid-body modes (three
+ * translations, three rotations) of node aggregates: the nodes whose
+ * reference (x, y) lies in one square cell of side cell_size, through all
+ * layers.  A_c = P^T J P is formed on the device and inverted (FP64) at every
+ * mixed assembly.  cell_size < 0: automatic (about 400 aggregates over the
+ * tissue's (x, y) extent; the default); 0: off (block Jacobi only).  The
+ * converged positions do not depend on it, only the Krylov iteration count.
+ * tf_nr_coarse_info reports the aggregates of the last build and whether the
+ * last mixed solve used the correction. */
+tf_status tf_nr_set_coarse(tf_handle* h, double cell_size);
+tf_status tf_nr_coarse_info(tf_handle* h, int32_t* aggregates, int32_t* active);
+
 /* Size of the Jacobian pattern: stored 3x3 blocks (n_blocks, the structural
  * nonzeros / 9) and padded SELL-64 slots.  No reference counterpart. */
 tf_status tf_nr_pattern_info(tf_handle* h, int64_t* n_blocks, int64_t* n_slots);

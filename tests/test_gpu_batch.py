@@ -1,7 +1,8 @@
 """Batched tissues (tf_batch_create): one launch per solver stage for many
 independent meshes of uneven size.  Each tissue keeps its own Krylov scalars,
 convergence flag and NR test, so a tissue stepped in a batch must give the
-result of the same tissue stepped alone with the same tile size, bit for bit.
+result of the same tissue stepped alone with the same tile size, bit for bit
+(alone with block Jacobi too: batches have no coarse correction).
 Every test here needs an MI355X."""
 import numpy as np
 import pytest
@@ -33,6 +34,7 @@ def _growth_inputs(orc, c, dt):
 def _alone_physical(nr, orc, c, dt, spt):
     m = c["mesh"]
     s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+    s.set_coarse(0)  # batches solve with block Jacobi only
     s.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
     s.set_growth(_growth_inputs(orc, c, dt))
     s.update_visc_surface(m["nodes"].reshape(-1), c["place"], c["etype"], c["spans"])
@@ -86,6 +88,7 @@ HARD = [dict(seed=11, lx=4.0, ly=3.0, layers=2, visc=True, fixed=False),
 def _alone(nr, p, spt):
     m = p["mesh"]
     s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], p["refpos"])
+    s.set_coarse(0)
     s.set_material(p["E"], p["nu"], p["eta"], p["ext_visc"], p["fixed"])
     s.set_growth(p["invFg"])
     s.set_visc_surface(p["area"])
@@ -159,6 +162,7 @@ def test_batch_advance_is_bitwise_each_tissue_alone(nr):
     for t, c in enumerate(cfgs):
         m = c["m"]
         a = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+        a.set_coarse(0)
         a.set_material(c["E"], c["nu"], c["eta"], c["ext"])
         a.set_growth_rates(c["rates"])
         a.set_surface_flags(c["place"], c["etype"], c["spans"])

@@ -14,7 +14,10 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtissue_nr.so")
 DRIVER = os.path.join(HERE, "nr_driver")
-SOURCES = ["tf_element.hip", "tf_assemble.hip", "tf_krylov.hip", "tf_tiled.hip", "tf_adjoint.hip", "tf_nr.hip"]
+SOURCES = ["tf_element.hip", "tf_assemble.hip", "tf_krylov.hip", "tf_tiled.hip", "tf_coarse.hip",
+           "tf_adjoint.hip", "tf_nr.hip"]
+# rocSOLVER (getrf / getri of the coarse matrix) and rocBLAS under it
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -46,7 +49,7 @@ def build(verbose=False):
     if failed:
         raise RuntimeError("hipcc failed for " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    subprocess.check_call([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs])
+    subprocess.check_call([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs, *LIBS])
     os.replace(tmp, LIB)
     # C++ driver over the drop-in surface of include/tissue_nr.hpp
     subprocess.check_call(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),

@@ -59,6 +59,55 @@ struct BoundOp {
   int32_t* bm_master = nullptr;
   int32_t* bm_slave = nullptr;
   int32_t n_masters = 0;
+  // host copy of the pattern (for the coarse space's block lists)
+  std::vector<int32_t> h_slice_base, h_col, h_row_len;
+};
+
+// Two-level preconditioner of the FP32 CG solve (tf_coarse.hip):
+//   M^-1 r = D^-1 r + P A_c^-1 P^T r,   A_c = P^T A P,
+// with D the block diagonal of the operator A, and P the rigid-body modes
+// (three translations, three rotations about the centroid) of node
+// aggregates: the nodes whose reference (x, y) falls in one square cell,
+// through all layers.  Fixed and slave DOF have zero rows in P.  A_c is
+// formed on the device from the FP32 operator at every assembly; its FP64
+// inverse comes from Newton-Schulz steps X <- X (2I - A_c X) (rocBLAS GEMMs)
+// started from a reference inverse (rocSOLVER getrf / getri, refactorised
+// only when the steps do not converge); it is applied as a dense FP32
+// product.
+struct CoarseOp {
+  bool built = false;    // aggregates and block lists match the operator pattern
+  bool for_bound = false;
+  bool active = false;   // the factorisation of the current operator succeeded
+  int32_t n_agg = 0, n_c = 0, n_cpairs = 0, n_gemv = 0, agg_total = 0;
+  int32_t* node_agg = nullptr;  // per node
+  float* node_d = nullptr;      // per node: (X - centroid) / radius, 3 floats
+  int32_t* agg_ptr = nullptr;   // nodes of each aggregate, ascending
+  int32_t* agg_nodes = nullptr;
+  int32_t* cp_ptr = nullptr;    // operator slots of each aggregate pair, ascending
+  int32_t* cp_slot = nullptr;
+  int32_t* cp_row = nullptr;    // row (node) of each listed slot
+  int32_t* cp_IK = nullptr;     // the pair's aggregates (I, K)
+  float* list_d = nullptr;      // node offsets in aggregate-list order (3 per entry)
+  uint8_t* list_mask = nullptr; // fixed / slave bits of the listed node (bit p: DOF p)
+  double* pairblk = nullptr;    // 36 per pair: (P_I^T A P_K), row-major 6x6
+  double* Ac = nullptr;         // n_c x n_c column-major
+  double* Xref = nullptr;       // reference inverse (column-major, rocSOLVER)
+  double* X = nullptr;          // inverse of the current A_c, and two work matrices
+  double* X2 = nullptr;
+  double* T = nullptr;
+  unsigned long long* resid = nullptr;  // max |I - A_c X| (bits of a non-negative double)
+  bool have_inv = false;        // Xref holds an inverse
+  int32_t refactors = 0;        // rocSOLVER factorisations so far
+  int32_t* ipiv = nullptr;
+  int32_t* info = nullptr;      // 2: getrf, getri
+  int32_t ld32 = 0;             // row stride of inv32 (n_c rounded up to 64)
+  float* inv32 = nullptr;       // A_c^-1, row-major FP32
+  float* w = nullptr;           // P^T r (ld32 entries, zero past n_c)
+  float* y = nullptr;           // A_c^-1 P^T r
+  double* cpart = nullptr;      // per workgroup of the product: partial (w, y)
+  std::vector<void*> allocs;
+  void* blas = nullptr;         // rocblas_handle
+  int32_t gen = 0;              // incremented by every build (captured graphs key on it)
 };
 // A_b from the assembled J (FP64, or FP32 with mixed) and its block-Jacobi
 // inverse.
@@ -217,6 +266,8 @@ struct KrylovArgs {
   Launch L;
   Prof* prof;
   TileMap tm;
+  const CoarseOp* coarse = nullptr;  // two-level FP32 CG preconditioner when active
+  const uint8_t* dof_flags = nullptr;
 };
 // Solves J x = b (FP64).  Returns the iteration count in *iters, the Krylov
 // state (K_CONVERGED/K_BREAKDOWN/K_RUN=cap reached) in *state.
@@ -282,6 +333,24 @@ hipError_t launch_adjoint_element(const AdjointLaunch& l, hipStream_t st);
 hipError_t launch_transpose_operator(int64_t n_slots, const int32_t* tslot, const double* val,
                                      double* valT, int32_t n_nodes, const double* minv,
                                      double* minvT, hipStream_t st);
+
+// ---- two-level preconditioner (tf_coarse.hip) ----
+// Aggregates and block lists for an operator pattern (host arrays of the
+// pattern: slice_base, col, row_len) and the node reference coordinates X
+// (3 per node); cell is the aggregate cell size (same units as X).
+hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double cell,
+                        const std::vector<int32_t>& slice_base, const std::vector<int32_t>& col,
+                        const std::vector<int32_t>& row_len);
+void coarse_release(CoarseOp& c);
+// A_c of the FP32 operator (col / val32 of the pattern the lists were built
+// for), its FP64 inverse and the FP32 copy; c.active reports success.
+hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, int64_t n_slots,
+                         const uint8_t* dof_flags, hipStream_t st);
+// z += P A_c^-1 P^T r (FP32, one tissue), with the partial (w, y) per product
+// workgroup in c.cpart; nothing when the tissue's state (state[0]) is not
+// running.
+void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, float* z,
+                  int32_t n_nodes, const uint8_t* dof_flags, hipStream_t st);
 
 void set_error(const std::string& s);
 

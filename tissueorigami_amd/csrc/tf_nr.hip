@@ -59,6 +59,11 @@ struct tf_handle {
   // explicit bound operator for slave/master binding, and its buffers
   BoundOp bound;
   std::vector<void*> bound_allocs;
+  // two-level preconditioner of the FP32 CG solve (single-tissue handles):
+  // cell size of the aggregates (< 0: automatic, about 400 aggregates; 0: off)
+  CoarseOp coarse;
+  double coarse_cell = -1.0;
+  std::vector<double> h_X;  // node reference coordinates (first prism holding the node)
   std::vector<int32_t> bound_pairs;  // pairs the bound operator was built for
   bool any_visc = false, have_material = false, have_growth = false, have_area = false;
   bool assembled = false;
@@ -417,6 +422,8 @@ void free_bound(tf_handle* h) {
   for (void* p : h->bound_allocs) (void)hipFree(p);
   h->bound_allocs.clear();
   h->bound = BoundOp();
+  h->coarse.built = false;  // its block lists follow the operator's pattern
+  h->coarse.active = false;
 }
 
 template <typename T>
@@ -584,6 +591,9 @@ tf_status build_bound(tf_handle* h) {
   BoundOp& B = h->bound;
   B.pat.n_slices = n_slices;
   B.pat.n_slots = total;
+  B.h_slice_base = sb;
+  B.h_col = col;
+  B.h_row_len = rl;
   TF_CHECK(balloc(h, &B.pat.slice_base, sb.size(), sb.data()));
   TF_CHECK(balloc(h, &B.pat.col, col.size(), col.data()));
   TF_CHECK(balloc(h, &B.pat.diag_slot, diag.size(), diag.data()));
@@ -660,6 +670,17 @@ tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos
     return fail(TF_ERR_HIP);
   }
   h->L.stream = h->st;
+  h->h_X.assign(3 * (size_t)Nn, 0.0);
+  {
+    std::vector<uint8_t> seen(Nn, 0);
+    for (size_t e = 0; e < Ne; ++e)
+      for (int a = 0; a < 6; ++a) {
+        const int nd = conn[6 * e + a];
+        if (nd < 0 || nd >= (int)Nn || seen[nd]) continue;
+        seen[nd] = 1;
+        for (int k = 0; k < 3; ++k) h->h_X[3 * (size_t)nd + k] = ref_pos[18 * e + 3 * a + k];
+      }
+  }
   if ((s = alloc_tissue_state(h)) != TF_OK) return fail(s);
   if ((s = build_tiles(h, 8)) != TF_OK) return fail(s);
   *out = h;
@@ -689,7 +710,50 @@ KrylovArgs krylov_args(tf_handle* h) {
   k.L = h->L;
   k.prof = &h->prof;
   k.tm = h->tm;
+  k.dof_flags = h->dof_flags;
+  k.coarse = h->coarse.active ? &h->coarse : nullptr;
   return k;
+}
+
+// The coarse space of the FP32 CG preconditioner for the operator just
+// assembled (mixed precision, one tissue): aggregates and block lists when
+// the operator's pattern changed, then A_c and its inverse.
+tf_status coarse_prepare(tf_handle* h) {
+  CoarseOp& c = h->coarse;
+  c.active = false;
+  if (h->batch || h->coarse_cell == 0.0) return TF_OK;
+  const bool bound = h->bound.ready;
+  if (!c.built || c.for_bound != bound) {
+    double cell = h->coarse_cell;
+    if (cell < 0.0) {  // automatic: about 400 aggregates over the (x, y) bounding box
+      double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+      for (int i = 0; i < h->Nn; ++i) {
+        x0 = std::min(x0, h->h_X[3 * i]); x1 = std::max(x1, h->h_X[3 * i]);
+        y0 = std::min(y0, h->h_X[3 * i + 1]); y1 = std::max(y1, h->h_X[3 * i + 1]);
+      }
+      cell = std::sqrt(std::max((x1 - x0) * (y1 - y0), 1e-30) / 400.0);
+    }
+    TF_CHECK(hipStreamSynchronize(h->st));
+    tiled_release(&h->ws);  // captured chunks hold the previous coarse buffers
+    const BoundOp& B = h->bound;
+    hipError_t e = bound ? coarse_build(c, h->Nn, h->h_X.data(), cell, B.h_slice_base, B.h_col,
+                                        B.h_row_len)
+                         : coarse_build(c, h->Nn, h->h_X.data(), cell, h->h_slice_base, h->h_col,
+                                        h->h_row_len);
+    if (e == hipErrorInvalidValue) {
+      set_error("coarse space: cell size gives too many aggregates");
+      return TF_ERR_ARG;
+    }
+    TF_CHECK(e);
+    c.for_bound = bound;
+    ++c.gen;
+  }
+  if (bound)
+    TF_CHECK(coarse_factor(c, h->bound.pat.col, h->bound.val32, h->bound.pat.n_slots,
+                           h->dof_flags, h->st));
+  else
+    TF_CHECK(coarse_factor(c, h->pat.col, h->ws.val32, h->pat.n_slots, h->dof_flags, h->st));
+  return TF_OK;
 }
 
 // Element kernel + slot-map assembly + residual and binding of the residual.
@@ -987,8 +1051,14 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
                             h->st));
     tf_status s = assemble(h, dt, mixed);
     if (s != TF_OK) return s;
-    KrylovArgs k = krylov_args(h);
     const int method = h->krylov_method[mixed];
+    if (mixed && method == KRYLOV_CG) {
+      s = coarse_prepare(h);
+      if (s != TF_OK) return s;
+    } else {
+      h->coarse.active = false;
+    }
+    KrylovArgs k = krylov_args(h);
     const double inner = std::max(rtol, kMixedInnerRtol);
     if (mixed) {
       TF_CHECK(tiled_mixed(k, w.g, w.du, rtol, inner, kMixedCap, 16, method, &res));
@@ -1780,6 +1850,28 @@ tf_status tf_copy_bandwidth(int64_t bytes, int32_t reps, double* gb_per_s) {
   return TF_OK;
 }
 
+tf_status tf_nr_set_coarse(tf_handle* h, double cell_size) {
+  if (!h || std::isnan(cell_size)) {
+    set_error("tf_nr_set_coarse: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  h->coarse_cell = cell_size < 0.0 ? -1.0 : cell_size;
+  tiled_release(&h->ws);
+  coarse_release(h->coarse);
+  return TF_OK;
+}
+
+tf_status tf_nr_coarse_info(tf_handle* h, int32_t* aggregates, int32_t* active) {
+  if (!h) {
+    set_error("tf_nr_coarse_info: invalid handle");
+    return TF_ERR_ARG;
+  }
+  if (aggregates) *aggregates = h->coarse.built ? h->coarse.n_agg : 0;
+  if (active) *active = h->coarse.active ? 1 : 0;
+  return TF_OK;
+}
+
 tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable) {
   if (!h) {
     set_error("tf_nr_set_graphs: invalid handle");
@@ -1806,6 +1898,7 @@ tf_status tf_nr_destroy(tf_handle* h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   tiled_release(&h->ws);
   free_bound(h);
+  coarse_release(h->coarse);
   for (auto& e : h->prof.ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);

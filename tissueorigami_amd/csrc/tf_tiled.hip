@@ -95,6 +95,10 @@ struct TC {
   int stride;
   const int32_t* active; // per tissue: 0 = skip the solve (x = 0)
   double rtol;
+  // two-level CG (one tissue): per-workgroup partial (w, y) of the coarse
+  // product, which completes (r, z) = (r, D^-1 r) + (P^T r, A_c^-1 P^T r)
+  const double* cpart;
+  int n_cpart;
 };
 
 enum { TS_RHO = 0, TS_ALPHA, TS_OMEGA, TS_BNORM, TS_RNORM };
@@ -459,7 +463,8 @@ __global__ __launch_bounds__(kBlock) void tzc_spmv(TC c, int par, const T* __res
   TileCtx t;
   if (!tile_ctx(c, t)) return;
   if (c.state[2 * t.T] != K_RUN) return;
-  const double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
+  double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
+  if (c.n_cpart) rho += ordered_sum(c.cpart, 0, c.n_cpart);
   const double rr = tissue_sum(c, par ? TC_RR1 : TC_RR0, t.T);
   const double rnorm = sqrt(rr);
   const bool first_it = (c.state[2 * t.T + 1] == 0);
@@ -508,7 +513,8 @@ __global__ __launch_bounds__(kBlock) void tzc_update(TC c, int par, T* x, T* r, 
   TileCtx t;
   if (!tile_ctx(c, t)) return;
   if (c.state[2 * t.T] != K_RUN) return;
-  const double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
+  double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
+  if (c.n_cpart) rho += ordered_sum(c.cpart, 0, c.n_cpart);
   const double pq = tissue_sum(c, TC_PQ, t.T);
   const double alpha = rho / pq;
   if (!(pq > 0.0) || !isfinite(alpha)) {
@@ -814,7 +820,18 @@ struct CV {
   T *r, *z, *x, *p0, *p1, *q;
   const T* val;
   const T* minv;
+  const CoarseOp* co = nullptr;  // FP32 only: z += P A_c^-1 P^T r after every z = D^-1 r
+  const uint8_t* flags = nullptr;
+  int32_t n_nodes = 0;
 };
+
+// The coarse correction of z (FP32 CG with an active coarse space).
+template <typename T>
+void coarse_step(const TC& c, const CV<T>& V, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (V.co) coarse_apply(*V.co, c.state, V.r, V.z, V.n_nodes, V.flags, st);
+  }
+}
 
 template <typename T>
 void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chunk, int it0,
@@ -839,6 +856,7 @@ void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chu
                           (const T*)V.z, pold, pnew, V.q);
     hipLaunchKernelGGL(tzc_update<T>, dim3(G), dim3(kBlock), 0, st, c, par, V.x, V.r, V.z,
                        (const T*)pnew, (const T*)V.q, V.minv);
+    coarse_step<T>(c, V, st);
   }
 }
 
@@ -850,15 +868,23 @@ hipError_t tiled_cg_impl(const KrylovArgs& k, const CV<T>& V, const double* b, d
   const int G = k.tm.grid;
   const int nt = k.tm.n_tissues;
   TC c = make_tc(k, rtol);
+  if (V.co) {
+    c.cpart = V.co->cpart;
+    c.n_cpart = V.co->n_gemv;
+  }
   hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, c, b, V.r, V.z, V.x, V.p0,
                      V.minv);
+  coarse_step<T>(c, V, st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const void* ptrs[8] = {V.r, V.z, V.x, V.p0, V.p1, V.q, V.val, V.minv};
+  const void* ptrs[12] = {V.r, V.z, V.x, V.p0, V.p1, V.q, V.val, V.minv,
+                          V.co ? V.co->inv32 : nullptr, V.co ? V.co->node_d : nullptr,
+                          V.co ? V.co->agg_nodes : nullptr,
+                          V.co ? (const void*)(intptr_t)V.co->gen : nullptr};
   int done = 0;
   while (true) {
     const int chunk = std::min(sync_every, maxit - done);
-    const GraphKey key = make_key(1, (int)sizeof(T), chunk, done & 1, c, ptrs, 8);
+    const GraphKey key = make_key(1, (int)sizeof(T), chunk, done & 1, c, ptrs, 12);
     const int it0 = done;
     e = run_captured(k, key, [&](Prof* pf) { launch_cg_chunk<T>(c, V, G, st, chunk, it0, pf); });
     if (e != hipSuccess) return e;
@@ -993,6 +1019,11 @@ hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double r
     for (int T = 0; T < nt; ++T) left = std::min(left, maxit - res->iters[T]);
     if (method == KRYLOV_CG) {
       CV<float> C{ws->r32, ws->s32, ws->x32, ws->p32, ws->ph32, ws->v32, k.val32, k.minv32};
+      if (k.coarse && k.coarse->active) {
+        C.co = k.coarse;
+        C.flags = k.dof_flags;
+        C.n_nodes = k.n_nodes;
+      }
       e = tiled_cg_impl<float>(k, C, ws->b, inner_rtol, std::max(left, 1), sync_every, &inner);
     } else {
       e = tiled_bicgstab_impl<float>(k, V, ws->b, inner_rtol, std::max(left, 1), sync_every,
@@ -1090,15 +1121,26 @@ hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int method, int
   };
   auto run_cg = [&](auto V) {
     using T = std::remove_pointer_t<decltype(V.r)>;
-    hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, c, (const double*)ws->b, V.r,
+    TC cc = c;
+    if (V.co) {
+      cc.cpart = V.co->cpart;
+      cc.n_cpart = V.co->n_gemv;
+    }
+    hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, cc, (const double*)ws->b, V.r,
                        V.z, V.x, V.p0, V.minv);
+    coarse_step<T>(cc, V, st);
     (void)hipEventRecord(e0, st);
-    launch_cg_chunk<T>(c, V, G, st, iters, 0, nullptr);
+    launch_cg_chunk<T>(cc, V, G, st, iters, 0, nullptr);
     (void)hipEventRecord(e1, st);
   };
   if (method == KRYLOV_CG) {
     if (mixed) {
       CV<float> V{ws->r32, ws->s32, ws->x32, ws->p32, ws->ph32, ws->v32, k.val32, k.minv32};
+      if (k.coarse && k.coarse->active) {
+        V.co = k.coarse;
+        V.flags = k.dof_flags;
+        V.n_nodes = k.n_nodes;
+      }
       run_cg(V);
     } else {
       CV<double> V{ws->r, ws->s, ws->x, ws->p, ws->ph, ws->v, k.val, k.minv};

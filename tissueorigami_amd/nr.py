@@ -25,7 +25,7 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
            "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs", "tf_nr_set_krylov", "tf_nr_assemble_ex",
-           "tf_copy_bandwidth",
+           "tf_copy_bandwidth", "tf_nr_set_coarse", "tf_nr_coarse_info",
            "tf_nr_destroy", "tf_last_error"]
 
 
@@ -110,6 +110,8 @@ def load():
     L.tf_nr_set_graphs.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_set_krylov.argtypes = [_h, ctypes.c_int32, ctypes.c_int32]
     L.tf_copy_bandwidth.argtypes = [ctypes.c_int64, ctypes.c_int32, _d]
+    L.tf_nr_set_coarse.argtypes = [_h, ctypes.c_double]
+    L.tf_nr_coarse_info.argtypes = [_h, _i, _i]
     L.tf_nr_destroy.argtypes = [_h]
     L.tf_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
@@ -303,6 +305,17 @@ class NewtonRaphsonSolver:
         """Krylov method ("bicgstab" or "cg") of one precision mode."""
         _check(load().tf_nr_set_krylov(self._h, PREC[precision],
                                        {"bicgstab": 0, "cg": 1}[method]))
+
+    def set_coarse(self, cell_size=-1.0):
+        """Two-level preconditioner of the mixed CG solve: aggregate cell size
+        (negative: automatic, the default; 0: block Jacobi only)."""
+        _check(load().tf_nr_set_coarse(self._h, float(cell_size)))
+
+    def coarse_info(self):
+        """(aggregates of the last coarse build, whether the last mixed solve used it)."""
+        a = np.zeros(1, np.int32); on = np.zeros(1, np.int32)
+        _check(load().tf_nr_coarse_info(self._h, _p(a, _i), _p(on, _i)))
+        return int(a[0]), bool(on[0])
 
     def set_graphs(self, enable=True):
         """Krylov iteration chunks replayed through captured hipGraphs (default on)."""
