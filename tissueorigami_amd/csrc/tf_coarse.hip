@@ -41,6 +41,7 @@ namespace {
 constexpr int kMinAggNodes = 16;   // smaller cells are merged into the nearest larger one
 constexpr int kMaxAggregates = 8192;
 constexpr int kGemvRows = kBlock / 64;  // rows of A_c^-1 per product workgroup
+constexpr int kSplit = 4;               // workgroups per aggregate in the restriction
 
 template <typename T>
 hipError_t upload(CoarseOp& c, T** p, const std::vector<T>& v, size_t count = 0) {
@@ -156,60 +157,68 @@ __global__ __launch_bounds__(kBlock) void coarse_diag_kernel(int32_t n_c, double
   if (r < n_c && dense[(size_t)r * n_c + r] == 0.0) dense[(size_t)r * n_c + r] = 1.0;
 }
 
-// FP32 row-major copy (row stride ld) of the column-major FP64 inverse.
+// FP32 row-major copy (row stride ld) of the column-major FP64 inverse,
+// through 32 x 32 LDS tiles (grid: tiles of rows x tiles of columns).
 __global__ __launch_bounds__(kBlock) void coarse_convert_kernel(int32_t n_c, int32_t ld,
                                                                 const double* X, float* inv32) {
-  const int64_t n = (int64_t)n_c * n_c;
-  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * kBlock) {
-    const int64_t r = t / n_c, cc = t % n_c;
-    inv32[r * ld + cc] = (float)X[cc * n_c + r];
+  __shared__ float tile[32][33];
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + tx, cc = c0 + k;  // read down a column of X
+    tile[k][tx] = (r < n_c && cc < n_c) ? (float)X[(size_t)cc * n_c + r] : 0.f;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, cc = c0 + tx;  // write along a row of inv32
+    if (r < n_c && cc < n_c) inv32[(size_t)r * ld + cc] = tile[tx][k];
   }
 }
 
 // max |I - T| over all entries (a non-negative double's bits order like the
-// double, so an integer maximum is exact and order-free).
+// double, so an integer maximum is exact and order-free); grid: row blocks x
+// columns.
 __global__ __launch_bounds__(kBlock) void coarse_resid_kernel(int32_t n_c, const double* T,
                                                               unsigned long long* out) {
-  const int64_t n = (int64_t)n_c * n_c;
+  const int r = blockIdx.x * kBlock + threadIdx.x, cc = blockIdx.y;
   double m = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * kBlock) {
-    const int64_t r = t % n_c, cc = t / n_c;
-    const double v = fabs((r == cc ? 1.0 : 0.0) - T[t]);
-    m = v > m || v != v ? v : m;  // NaN wins
+  if (r < n_c) {
+    const double v = fabs((r == cc ? 1.0 : 0.0) - T[(size_t)cc * n_c + r]);
+    m = v == v ? v : 1e300;  // a NaN counts as a huge residual
   }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_xor(m, off, 64);
-    m = o > m || o != o ? o : m;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0.0)
+    atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
-// T <- 2I - T.
+// T <- 2I - T; grid: row blocks x columns.
 __global__ __launch_bounds__(kBlock) void coarse_twoI_kernel(int32_t n_c, double* T) {
-  const int64_t n = (int64_t)n_c * n_c;
-  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * kBlock) {
-    const int64_t r = t % n_c, cc = t / n_c;
-    T[t] = (r == cc ? 2.0 : 0.0) - T[t];
+  const int r = blockIdx.x * kBlock + threadIdx.x, cc = blockIdx.y;
+  if (r < n_c) {
+    double& t = T[(size_t)cc * n_c + r];
+    t = (r == cc ? 2.0 : 0.0) - t;
   }
 }
 
 __global__ void coarse_zero_u64(unsigned long long* p) { *p = 0ull; }
 
-// w = P^T r, one workgroup per aggregate; the offsets and masks are stored
-// in list order, only r is gathered.
+// w = P^T r as kSplit partial vectors (w_s of stride ld, summed in order by
+// the product kernel): kSplit workgroups per aggregate, each over a fixed
+// part of the aggregate's node list; the offsets and masks are stored in list
+// order, only r is gathered.
 __global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
-    int32_t n_agg, const int32_t* __restrict__ agg_ptr, const int32_t* __restrict__ agg_nodes,
-    const float* __restrict__ list_d, const uint8_t* __restrict__ list_mask,
-    const float* __restrict__ r, const int32_t* __restrict__ state, float* __restrict__ w) {
+    int32_t n_agg, int32_t ld, const int32_t* __restrict__ agg_ptr,
+    const int32_t* __restrict__ agg_nodes, const float* __restrict__ list_d,
+    const uint8_t* __restrict__ list_mask, const float* __restrict__ r,
+    const int32_t* __restrict__ state, float* __restrict__ w) {
   __shared__ double red[6][kBlock / 64];
-  const int a = blockIdx.x;
+  const int a = blockIdx.x / kSplit, part = blockIdx.x % kSplit;
   if (a >= n_agg || state[0] != K_RUN) return;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const int beg = agg_ptr[a], end = agg_ptr[a + 1];
+  const int a0 = agg_ptr[a], len = agg_ptr[a + 1] - a0;
+  const int beg = a0 + (int)((int64_t)len * part / kSplit);
+  const int end = a0 + (int)((int64_t)len * (part + 1) / kSplit);
   for (int t = beg + threadIdx.x; t < end; t += kBlock) {
     const int i = agg_nodes[t];
     const uint8_t mk = list_mask[t];
@@ -228,7 +237,7 @@ __global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
   block_sum<6>(acc, red);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) w[6 * a + q] = (float)acc[q];
+    for (int q = 0; q < 6; ++q) w[(size_t)part * ld + 6 * a + q] = (float)acc[q];
 }
 
 // Fixed / slave bits of every listed node (list order), from the DOF flags.
@@ -264,16 +273,21 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(int32_t n_c, int32_
   double dot = 0.0;
   if (row < n_c) {
     const v4f* a = (const v4f*)(inv32 + (size_t)row * ld);
-    const v4f* x = (const v4f*)w;
     float s = 0.f;
     for (int j = lane; j < ld / 4; j += 64) {
-      const v4f u = a[j], v = x[j];
+      v4f v = ((const v4f*)w)[j];
+#pragma unroll
+      for (int p = 1; p < kSplit; ++p) v += ((const v4f*)(w + (size_t)p * ld))[j];
+      const v4f u = a[j];
       s += u.x * v.x + u.y * v.y + u.z * v.z + u.w * v.w;
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) y[row] = s;
-    dot = (double)w[row] * (double)s;
+    float wr = w[row];
+#pragma unroll
+    for (int p = 1; p < kSplit; ++p) wr += w[(size_t)p * ld + row];
+    dot = (double)wr * (double)s;
   }
   if (lane == 0) wy[wave] = dot;
   __syncthreads();
@@ -481,7 +495,7 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double ce
   if ((e = upload(c, &c.ipiv, std::vector<int32_t>(), nc)) != hipSuccess) return e;
   if ((e = upload(c, &c.info, std::vector<int32_t>(), 2)) != hipSuccess) return e;
   if ((e = upload(c, &c.inv32, std::vector<float>(nc * ld, 0.f))) != hipSuccess) return e;
-  if ((e = upload(c, &c.w, std::vector<float>(ld, 0.f))) != hipSuccess) return e;
+  if ((e = upload(c, &c.w, std::vector<float>(kSplit * ld, 0.f))) != hipSuccess) return e;
   if ((e = upload(c, &c.y, std::vector<float>(nc, 0.f))) != hipSuccess) return e;
   if ((e = upload(c, &c.cpart, std::vector<double>((size_t)c.n_gemv, 0.0))) != hipSuccess) return e;
   c.built = true;
@@ -501,8 +515,8 @@ hipError_t residual(CoarseOp& c, rocblas_handle hb, hipStream_t st, double* out)
                     nc, c.X, nc, &zero, c.T, nc) != rocblas_status_success)
     return hipErrorUnknown;
   hipLaunchKernelGGL(coarse_zero_u64, dim3(1), dim3(1), 0, st, c.resid);
-  hipLaunchKernelGGL(coarse_resid_kernel, dim3(grid_nodes(n2)), dim3(kBlock), 0, st, nc, c.T,
-                     c.resid);
+  hipLaunchKernelGGL(coarse_resid_kernel, dim3((nc + kBlock - 1) / kBlock, nc), dim3(kBlock), 0,
+                     st, nc, c.T, c.resid);
   unsigned long long bits = 0;
   hipError_t e;
   if ((e = hipMemcpyAsync(&bits, c.resid, sizeof(bits), hipMemcpyDeviceToHost, st)) != hipSuccess)
@@ -584,7 +598,8 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, in
       if ((e = residual(c, hb, st, &r)) != hipSuccess) return e;  // T = A_c X
       if (r < 1e-5) { ok = true; break; }
       if (!(r < 0.25) || it == 6) break;  // too far from X_ref (or NaN)
-      hipLaunchKernelGGL(coarse_twoI_kernel, dim3(grid_nodes(n2)), dim3(kBlock), 0, st, nc, c.T);
+      hipLaunchKernelGGL(coarse_twoI_kernel, dim3((nc + kBlock - 1) / kBlock, nc), dim3(kBlock), 0,
+                         st, nc, c.T);
       if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, c.X,
                         nc, c.T, nc, &zero, c.X2, nc) != rocblas_status_success)
         return hipErrorUnknown;
@@ -592,8 +607,8 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, in
     }
   }
   if (!ok) return hipSuccess;
-  hipLaunchKernelGGL(coarse_convert_kernel, dim3(grid_nodes(n2)), dim3(kBlock), 0, st, nc,
-                     c.ld32, c.X, c.inv32);
+  hipLaunchKernelGGL(coarse_convert_kernel, dim3((nc + 31) / 32, (nc + 31) / 32), dim3(kBlock), 0,
+                     st, nc, c.ld32, c.X, c.inv32);
   hipLaunchKernelGGL(coarse_mask_kernel, dim3(grid_nodes(c.agg_total)), dim3(kBlock), 0, st,
                      c.agg_total, c.agg_nodes, dof_flags, c.list_mask);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -603,8 +618,9 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, in
 
 void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, float* z,
                   int32_t n_nodes, const uint8_t* dof_flags, hipStream_t st) {
-  hipLaunchKernelGGL(coarse_restrict_kernel, dim3(c.n_agg), dim3(kBlock), 0, st, c.n_agg,
-                     c.agg_ptr, c.agg_nodes, c.list_d, c.list_mask, r, state, c.w);
+  hipLaunchKernelGGL(coarse_restrict_kernel, dim3(c.n_agg * kSplit), dim3(kBlock), 0, st,
+                     c.n_agg, c.ld32, c.agg_ptr, c.agg_nodes, c.list_d, c.list_mask, r, state,
+                     c.w);
   hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.n_c, c.ld32,
                      c.inv32, c.w, state, c.y, c.cpart);
   hipLaunchKernelGGL(coarse_prolong_kernel, dim3(grid_nodes(n_nodes)), dim3(kBlock), 0, st,

@@ -102,7 +102,7 @@ struct CoarseOp {
   int32_t* info = nullptr;      // 2: getrf, getri
   int32_t ld32 = 0;             // row stride of inv32 (n_c rounded up to 64)
   float* inv32 = nullptr;       // A_c^-1, row-major FP32
-  float* w = nullptr;           // P^T r (ld32 entries, zero past n_c)
+  float* w = nullptr;           // P^T r as partial vectors of ld32 entries (zero past n_c)
   float* y = nullptr;           // A_c^-1 P^T r
   double* cpart = nullptr;      // per workgroup of the product: partial (w, y)
   std::vector<void*> allocs;
