@@ -175,21 +175,27 @@ __global__ __launch_bounds__(kBlock) void coarse_convert_kernel(int32_t n_c, int
   }
 }
 
-// max |I - T| over all entries (a non-negative double's bits order like the
-// double, so an integer maximum is exact and order-free); grid: row blocks x
-// columns.
+// max |I - T| of every column (one workgroup per column; a NaN counts as a
+// huge residual); the host takes the maximum over the columns.
 __global__ __launch_bounds__(kBlock) void coarse_resid_kernel(int32_t n_c, const double* T,
-                                                              unsigned long long* out) {
-  const int r = blockIdx.x * kBlock + threadIdx.x, cc = blockIdx.y;
+                                                              double* colmax) {
+  __shared__ double red[kBlock / 64];
+  const int cc = blockIdx.x;
   double m = 0.0;
-  if (r < n_c) {
+  for (int r = threadIdx.x; r < n_c; r += kBlock) {
     const double v = fabs((r == cc ? 1.0 : 0.0) - T[(size_t)cc * n_c + r]);
-    m = v == v ? v : 1e300;  // a NaN counts as a huge residual
+    m = fmax(m, v == v ? v : 1e300);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-  if ((threadIdx.x & 63) == 0 && m > 0.0)
-    atomicMax(out, (unsigned long long)__double_as_longlong(m));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+#pragma unroll
+    for (int k = 1; k < kBlock / 64; ++k) t = fmax(t, red[k]);
+    colmax[cc] = t;
+  }
 }
 
 // T <- 2I - T; grid: row blocks x columns.
@@ -201,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void coarse_twoI_kernel(int32_t n_c, double
   }
 }
 
-__global__ void coarse_zero_u64(unsigned long long* p) { *p = 0ull; }
+
 
 // w = P^T r as kSplit partial vectors (w_s of stride ld, summed in order by
 // the product kernel): kSplit workgroups per aggregate, each over a fixed
@@ -491,7 +497,7 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double ce
   if ((e = upload(c, &c.Xref, std::vector<double>(), nc * nc)) != hipSuccess) return e;
   if ((e = upload(c, &c.X2, std::vector<double>(), nc * nc)) != hipSuccess) return e;
   if ((e = upload(c, &c.T, std::vector<double>(), nc * nc)) != hipSuccess) return e;
-  if ((e = upload(c, &c.resid, std::vector<unsigned long long>(1, 0ull))) != hipSuccess) return e;
+  if ((e = upload(c, &c.colmax, std::vector<double>(nc, 0.0))) != hipSuccess) return e;
   if ((e = upload(c, &c.ipiv, std::vector<int32_t>(), nc)) != hipSuccess) return e;
   if ((e = upload(c, &c.info, std::vector<int32_t>(), 2)) != hipSuccess) return e;
   if ((e = upload(c, &c.inv32, std::vector<float>(nc * ld, 0.f))) != hipSuccess) return e;
@@ -514,16 +520,15 @@ hipError_t residual(CoarseOp& c, rocblas_handle hb, hipStream_t st, double* out)
   if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, c.Ac,
                     nc, c.X, nc, &zero, c.T, nc) != rocblas_status_success)
     return hipErrorUnknown;
-  hipLaunchKernelGGL(coarse_zero_u64, dim3(1), dim3(1), 0, st, c.resid);
-  hipLaunchKernelGGL(coarse_resid_kernel, dim3((nc + kBlock - 1) / kBlock, nc), dim3(kBlock), 0,
-                     st, nc, c.T, c.resid);
-  unsigned long long bits = 0;
+  hipLaunchKernelGGL(coarse_resid_kernel, dim3(nc), dim3(kBlock), 0, st, nc, c.T, c.colmax);
+  std::vector<double> cm(nc);
   hipError_t e;
-  if ((e = hipMemcpyAsync(&bits, c.resid, sizeof(bits), hipMemcpyDeviceToHost, st)) != hipSuccess)
+  if ((e = hipMemcpyAsync(cm.data(), c.colmax, sizeof(double) * nc, hipMemcpyDeviceToHost,
+                          st)) != hipSuccess)
     return e;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-  double r;
-  std::memcpy(&r, &bits, sizeof(r));
+  double r = 0.0;
+  for (double v : cm) r = std::max(r, v);
   *out = r;
   return hipSuccess;
 }

@@ -95,7 +95,7 @@ struct CoarseOp {
   double* X = nullptr;          // inverse of the current A_c, and two work matrices
   double* X2 = nullptr;
   double* T = nullptr;
-  unsigned long long* resid = nullptr;  // max |I - A_c X| (bits of a non-negative double)
+  double* colmax = nullptr;     // per column: max |I - A_c X|
   bool have_inv = false;        // Xref holds an inverse
   int32_t refactors = 0;        // rocSOLVER factorisations so far
   int32_t* ipiv = nullptr;
