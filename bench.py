@@ -117,10 +117,22 @@ def cg_spmv_bytes(n_blocks, n_nodes, s):
     return n_blocks * (9 * s + 4) + n_nodes * s * (3 + 3 + 3 + 3)
 
 
-def cg_iteration_bytes(n_blocks, n_nodes, s):
+def cg_iteration_bytes(n_blocks, n_nodes, s, n_coarse=0):
     """One CG iteration: tzc_spmv and tzc_update (reads x, p, r, q and the
-    block-Jacobi inverse; writes x, r, z)."""
-    return cg_spmv_bytes(n_blocks, n_nodes, s) + n_nodes * s * (6 + 3 + 6 + 3 + 9 + 3)
+    block-Jacobi inverse; writes x, r, z), and with a coarse space of n_coarse
+    DOF the correction z += P A_c^-1 P^T r: restriction (list index, offsets,
+    mask, r gathered), the dense FP32 inverse read once, prolongation
+    (aggregate, offsets, flags, z read and written)."""
+    b = cg_spmv_bytes(n_blocks, n_nodes, s) + n_nodes * s * (6 + 3 + 6 + 3 + 9 + 3)
+    if n_coarse:
+        b += n_nodes * (4 + 12 + 1 + 12) + 4 * n_coarse * n_coarse + n_nodes * (4 + 12 + 3 + 24)
+    return b
+
+
+def coarse_setup_bytes(n_blocks, n_coarse, gemms):
+    """Per assembly: A_c from the FP32 operator (values, slot and row lists)
+    and `gemms` dense FP64 products of the Newton-Schulz refinement."""
+    return n_blocks * (36 + 4 + 4) + gemms * 3 * 8 * n_coarse * n_coarse
 
 
 def profile_step(solver, w, u, Fg, precision, krylov_rtol):
@@ -346,6 +358,9 @@ def main():
     ap.add_argument("--fp64-rtol", type=float, default=1e-10)
     ap.add_argument("--mixed-krylov", default="cg", choices=["cg", "bicgstab"],
                     help="Krylov method of the FP32 inner solves (FP64 uses BiCGStab)")
+    ap.add_argument("--coarse-cell", type=float, default=-1.0,
+                    help="aggregate cell size of the two-level CG preconditioner (um; negative: "
+                         "automatic, about 400 aggregates; 0: block Jacobi only)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -368,6 +383,7 @@ def main():
     if args.no_graphs:
         solver.set_graphs(False)
     solver.set_krylov("mixed", args.mixed_krylov)
+    solver.set_coarse(args.coarse_cell)
     n_blocks, n_slots = solver.pattern_info()
     print(f"[bench] setup {setup_s:.2f} s, {n_blocks} blocks, {n_slots} SELL slots",
           file=sys.stderr, flush=True)
@@ -384,6 +400,7 @@ def main():
                                    krylov_rtol=args.mixed_rtol)
     print(f"[bench] mixed: {tm['nr_iterations']} NR its, {tm['krylov_iterations']} Krylov its, "
           f"{wall_m:.2f} s", file=sys.stderr, flush=True)
+    n_agg, coarse_on = solver.coarse_info()  # of the mixed steps (an FP64 step turns it off)
     res = dict(metric="Newton steps/s at 1.6M prisms, mixed vs FP64 @1e-8 rel err; adjoint cost/fwd",
                unit="NR iterations/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32/f64 mixed",
@@ -410,11 +427,16 @@ def main():
                         wall_s=wall_m, ms_element=tm["ms_element"], ms_assembly=tm["ms_assembly"],
                         ms_krylov=tm["ms_krylov"], converged=tm["converged"],
                         fp64_fallbacks=tm["fp64_fallbacks"], cg_fallbacks=tm["cg_fallbacks"],
-                        time_steps_per_s=args.steps / wall_m, per_step=tm["per_step"])
+                        time_steps_per_s=args.steps / wall_m, per_step=tm["per_step"],
+                        coarse=dict(aggregates=n_agg, coarse_dof=6 * n_agg, active=coarse_on))
     res["adjoint_cost_per_forward"] = None
     res["config"] = dict(workload=f"{args.config}: {w['cfg']['desc']}", prisms=ne, nodes=nn,
                          dof=3 * nn, dt_s=w["dt"], nr_tol=1e-8, krylov_rtol_fp64=args.fp64_rtol,
-                         krylov_rtol_mixed=args.mixed_rtol, precond="node-block Jacobi",
+                         krylov_rtol_mixed=args.mixed_rtol,
+                         precond_fp64="node-block Jacobi",
+                         precond_mixed=("node-block Jacobi + coarse correction (rigid-body modes "
+                                        f"of {n_agg} column aggregates)" if coarse_on
+                                        else "node-block Jacobi"),
                          krylov_mixed=args.mixed_krylov, krylov_fp64="bicgstab",
                          growth_seed=w["seed"], setup_s=setup_s, slices_per_tile=spt)
     # roofline of the dominant kernel: the FP32 SpMV v = J ph of the mixed
@@ -450,16 +472,22 @@ def main():
             res["roofline_fp64_spmv"] = dict(kernel="tz_spmv_v<double>", achieved=b / avg_s / 1e9,
                                              frac=b / avg_s / 1e9 / HBM_PEAK_GBS,
                                              avg_launch_ms=avg_s * 1e3, algorithmic_bytes=b)
-    # whole NR iteration: element kernel + assembly + k BiCGStab iterations
+    # whole NR iteration: element kernel + assembly (+ coarse setup) + k Krylov iterations
     visc = bool(np.any(w["eta"] != 0))
     for key, t, sz, meth in (("mixed", tm, 4, args.mixed_krylov),
                              ("fp64", td if fp64 else None, 8, "bicgstab")):
         if t is None:
             continue
         k_per = t["krylov_iterations"] / t["nr_iterations"]
-        it_bytes = (cg_iteration_bytes if meth == "cg" else bicgstab_iteration_bytes)(
-            n_blocks, nn, sz)
+        nco = 6 * n_agg if (key == "mixed" and coarse_on and meth == "cg") else 0
+        if meth == "cg":
+            it_bytes = cg_iteration_bytes(n_blocks, nn, sz, nco)
+        else:
+            it_bytes = bicgstab_iteration_bytes(n_blocks, nn, sz)
         bnr = assembly_bytes(ne, n_blocks, nn, sz, visc, combined=key == "mixed") + k_per * it_bytes
+        if nco:
+            # two Newton-Schulz steps (4 GEMMs) and a residual product per assembly
+            bnr += coarse_setup_bytes(n_blocks, nco, 5)
         secs = (t["ms_element"] + t["ms_assembly"] + t["ms_krylov"]) / 1e3 / t["nr_iterations"]
         res.setdefault("roofline_nr_iteration", {})[key] = dict(
             algorithmic_bytes=bnr, krylov_iterations=k_per, device_s=secs,
