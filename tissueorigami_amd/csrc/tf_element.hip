@@ -16,6 +16,8 @@
 //            stresses and leaves grad N, sigma and the scalar factors in LDS;
 //   phase 2: one thread per (prism, 3x3 block) sums its block over the Gauss
 //            points and writes 9 contiguous values (coalesced across lanes);
+//            for the summed FP32 operator of the mixed-precision step, one
+//            thread per (prism, node a) forms the six blocks (a, 0..5);
 //   phase 3: one thread per (prism, node) sums the nodal forces.
 // Every floating-point operation is written in the order of the CPU oracle's
 // restatement and compiled without contraction, so the FP64 outputs agree
@@ -33,7 +35,8 @@ namespace {
 #define TF_ELEM_PER_BLOCK 32
 #endif
 constexpr int kEPB = TF_ELEM_PER_BLOCK;  // prisms per workgroup
-constexpr int kThreads = kEPB * 6;
+constexpr int kGP = kEPB * 6;             // (prism, Gauss point) threads of phases 1 and 3
+constexpr int kThreads = (kGP + 63) / 64 * 64;  // whole wavefronts
 
 struct ElemConst {
   double sfd[6][3][6];  // shape function derivatives per Gauss point
@@ -190,66 +193,85 @@ __device__ inline void mixed_blocks(const GPData* sd, const GPVisc* svd, int el,
   }
 }
 
-// K block (a, b) of the mixed-precision operator in one pass: the elastic
-// block (a, b) plus the viscous contribution that the reference's transposed
-// storage of TriPointKv places at block (a, b), which is the viscous block
-// (b, a) of mixed_blocks transposed.  With Na = dN_a, Nb = dN_b that
-// contribution at entry (j, i) is
-//   (eta/2 (dot P_ij + Na_i (Nb^T P)_j) + (S Nb)_i Na_j - (S Na)_i Nb_j) f.
-// The assembly then gathers one block per element instead of two.
-__device__ inline void mixed_sum_block(const GPData* sd, const GPVisc* svd, int el, int ia, int ib,
-                                       bool visc, float w, float K[9]) {
-#pragma clang fp contract(fast)
+// The six K blocks (a, 0..5) of one prism for the mixed-precision operator,
+// in one pass: each block is the elastic block (a, b) plus the viscous
+// contribution that the reference's transposed storage of TriPointKv places
+// at block (a, b), the viscous block (b, a) of mixed_blocks transposed.  Per
+// Gauss point, with Na = dN_a, Nb = dN_b, f = vol w, S' = f S and
+// P' = (f eta / 2) P, entry (r, c) of the sum is
+//   Na_r U_c + Nb_r V_c + Na_c W_r + (Na.Nb) P'_cr + d_rc (cm w Na.Nb + vol w Na.sigma.Nb)
+// with U = cl w Nb + S' Nb, V = cm w Na - S' Na, W = P'^T Nb; V and sigma^T Na
+// depend on a alone and are formed once per point.  The assembly then
+// gathers one block per element instead of two.
+__device__ inline void mixed_row_blocks(const GPData* sd, const GPVisc* svd, int el, int ia,
+                                        bool visc, float w, float K[6][9]) {
+  // every fused multiply-add is explicit (contraction stays off), so each
+  // instantiation of the kernel rounds alike and a prism's blocks do not
+  // depend on the other prisms of the launch
 #pragma unroll
-  for (int q = 0; q < 9; ++q) K[q] = 0.f;
+  for (int b = 0; b < 6; ++b)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) K[b][q] = 0.f;
   const bool has_eta = visc && svd[el * 6].eta != 0;
+  auto dot3 = [](const float* x, int sx, const float* y) {
+    return fmaf(x[2 * sx], y[2], fmaf(x[sx], y[1], x[0] * y[0]));
+  };
 #pragma unroll 1
   for (int g = 0; g < 6; ++g) {
     const GPData& d = sd[el * 6 + g];
-    float Na[3], Nb[3], sg[9];
+    float Na[3], sg[9];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) { Na[j] = (float)d.dN[ia][j]; Nb[j] = (float)d.dN[ib][j]; }
+    for (int j = 0; j < 3; ++j) Na[j] = (float)d.dN[ia][j];
 #pragma unroll
     for (int q = 0; q < 9; ++q) sg[q] = (float)d.sig[q];
-    const float cl = (float)d.cl, cm = (float)d.cm;
     const float vol = (float)(d.detF * d.detdXde);
-    const float dot = Na[0] * Nb[0] + Na[1] * Nb[1] + Na[2] * Nb[2];
-    float sNb[3];
+    const float clw = (float)d.cl * w, cmw = (float)d.cm * w, volw = vol * w;
+    float sTNa[3], V[3], Sp[9], Pp[9];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) sNb[i] = sg[3 * i] * Nb[0] + sg[3 * i + 1] * Nb[1] + sg[3 * i + 2] * Nb[2];
-    const float geo = Na[0] * sNb[0] + Na[1] * sNb[1] + Na[2] * sNb[2];
-    const float diag = cm * dot + vol * geo;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        float v = cl * Na[i] * Nb[k] + cm * Na[k] * Nb[i];
-        if (i == k) v += diag;
-        K[3 * i + k] += v * w;
-      }
+    for (int j = 0; j < 3; ++j) {
+      sTNa[j] = dot3(sg + j, 3, Na);  // (sigma^T Na)_j
+      V[j] = cmw * Na[j];
+    }
     if (has_eta) {
       const GPVisc& dv = svd[el * 6 + g];
-      float P[9], S[9];
+      const float fe2 = 0.5f * volw * (float)dv.eta;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) { P[q] = (float)dv.P[q]; S[q] = (float)dv.sv[q]; }
-      float NbP[3], SNa[3], SNb[3];
+      for (int q = 0; q < 9; ++q) { Sp[q] = volw * (float)dv.sv[q]; Pp[q] = fe2 * (float)dv.P[q]; }
 #pragma unroll
-      for (int j = 0; j < 3; ++j) NbP[j] = Nb[0] * P[j] + Nb[1] * P[3 + j] + Nb[2] * P[6 + j];
+      for (int c = 0; c < 3; ++c) V[c] = V[c] - dot3(Sp + 3 * c, 1, Na);
+    }
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        SNa[i] = S[3 * i] * Na[0] + S[3 * i + 1] * Na[1] + S[3 * i + 2] * Na[2];
-        SNb[i] = S[3 * i] * Nb[0] + S[3 * i + 1] * Nb[1] + S[3 * i + 2] * Nb[2];
-      }
-      const float eta = (float)dv.eta;
-      const float f = vol * w;
+    for (int b = 0; b < 6; ++b) {
+      float Nb[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Nb[j] = (float)d.dN[b][j];
+      const float dot = dot3(Na, 1, Nb);
+      const float diag = fmaf(volw, dot3(sTNa, 1, Nb), cmw * dot);
+      float U[3];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          const float k1 = 0.5f * (dot * P[3 * i + j] + Na[i] * NbP[j]);
-          const float k2 = SNb[i] * Na[j] - SNa[i] * Nb[j];
-          K[3 * j + i] += (eta * k1 + k2) * f;
+      for (int c = 0; c < 3; ++c) U[c] = clw * Nb[c];
+      if (has_eta) {
+        float W[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          U[c] = U[c] + dot3(Sp + 3 * c, 1, Nb);
+          W[c] = dot3(Pp + c, 3, Nb);  // (P'^T Nb)_c
         }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            K[b][3 * r + c] = fmaf(Na[r], U[c], fmaf(Nb[r], V[c],
+                              fmaf(Na[c], W[r], fmaf(dot, Pp[3 * c + r], K[b][3 * r + c]))));
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            K[b][3 * r + c] = fmaf(Na[r], U[c], fmaf(Nb[r], V[c], K[b][3 * r + c]));
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) K[b][4 * r] = K[b][4 * r] + diag;
     }
   }
 }
@@ -266,7 +288,7 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
   const int e0 = blockIdx.x * kEPB;
 
   // ---------------- phase 1: one thread per (prism, Gauss point) ----------
-  {
+  if (tid < kGP) {
     const int el = tid / 6;
     const int g = tid % 6;
     const int e = e0 + el;
@@ -426,7 +448,21 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
   __syncthreads();
 
   const double w = c.w;
-  if (MIXED) {
+  if (MODE == 2) {
+    // ---------------- phase 2 (mixed, summed): one thread per (prism, a) ----
+    for (int task = tid; task < kGP; task += kThreads) {
+      const int el = task / 6, ia = task % 6;
+      const int e = e0 + el;
+      if (e >= a.n_prisms) break;
+      float K[6][9];
+      mixed_row_blocks(sd, svd, el, ia, VISC, (float)w, K);
+      float* out = a.Ke32 + ((size_t)e * 36 + ia * 6) * 9;
+#pragma unroll
+      for (int b = 0; b < 6; ++b)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) out[9 * b + q] = K[b][q];
+    }
+  } else if (MIXED) {
     // ---------------- phase 2 (mixed): FP32 elastic and viscous blocks -----
     for (int task = tid; task < kEPB * 36; task += kThreads) {
       const int el = task / 36;
@@ -435,13 +471,6 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
       if (e >= a.n_prisms) break;
       const int ia = ab / 6, ib = ab % 6;
       const size_t base = ((size_t)e * 36 + ab) * 9;
-      if (MODE == 2) {
-        float K[9];
-        mixed_sum_block(sd, svd, el, ia, ib, VISC, (float)w, K);
-#pragma unroll
-        for (int q = 0; q < 9; ++q) a.Ke32[base + q] = K[q];
-        continue;
-      }
       float Ke[9], Kv[9];
       mixed_blocks(sd, svd, el, ia, ib, VISC, (float)w, Ke, Kv);
 #pragma unroll
@@ -599,7 +628,7 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
   }  // FP64 phase 2
 
   // ---------------- phase 3: nodal forces ----------------------------------
-  if (tid < kEPB * 6) {
+  if (tid < kGP) {
     const int el = tid / 6;
     const int k = tid % 6;
     const int e = e0 + el;
