@@ -476,6 +476,58 @@ void element_eval(const double pos[6][3], const double ref[6][3], const double i
   }
 }
 
+// Node-node packing state of one step (Simulation::pacingNodeCouples0/1,
+// Node::mass, Simulation::packingThreshold), set by orc_set_packing.  While
+// set, assemble_dense adds the packing forces and their Jacobian blocks.
+struct PackState {
+  std::vector<int32_t> pairs;  // id0, id1 per couple, in detection order
+  std::vector<double> mass;    // per node
+  double threshold = 0.0;
+  bool on = false;
+};
+PackState g_pack;
+
+// One couple of Simulation::calculatePackingForcesImplicit3D
+// (Simulation.cpp:7338-7400) and calculatePackingJacobian3D
+// (Simulation.cpp:7444-7490): force F on id0 (-F on id1) and the symmetric
+// 3x3 derivative dF0/dx0 = m [f/d (I - n n^T) + f' n n^T], in the
+// reference's operation order (multiplier 5000, sigmoid saturation 5,
+// shift 0.6 threshold, zero-distance guard 1e-3; detectPacingNodes,
+// Simulation.cpp:8018-8020).
+void packing_couple(const double* p0, const double* p1, double m0, double m1, double thr,
+                    double F[3], double D[9]) {
+  const double zeroThreshold = 1E-3;
+  const double multiplier = 5000.0, sigmoidSaturation = 5.0;
+  const double distanceShiftScale = 0.6 * thr;
+  const double dx = p0[0] - p1[0];
+  const double dy = p0[1] - p1[1];
+  const double dz = p0[2] - p1[2];
+  double d = std::pow((dx * dx + dy * dy + dz * dz), 0.5);
+  const double averageMass = 0.5 * (m0 + m1);
+  double normal[3] = {1.0, 0.0, 0.0};
+  if (d > zeroThreshold) {
+    normal[0] = dx / d;
+    normal[1] = dy / d;
+    normal[2] = dz / d;
+  } else {
+    d = zeroThreshold;
+  }
+  const double shifted_d = d - distanceShiftScale;
+  const double sigmoid = 1 / (1 + std::exp(sigmoidSaturation / thr * 2.0 * shifted_d));
+  const double Fmagnitude = multiplier * averageMass * sigmoid;
+  for (int i = 0; i < 3; ++i) F[i] = Fmagnitude * normal[i];
+  const double dSdXi = (-sigmoidSaturation / thr * 2.0) * sigmoid * (1 - sigmoid);
+  const double scale = multiplier * averageMass;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double nn = normal[i] * normal[j];       // gsl_blas_dgemm(n, n^T)
+      double v = (i == j ? 1.0 : 0.0) - nn;          // I - n n^T
+      v = v * (sigmoid / d);                         // gsl_matrix_scale
+      v = v + nn * dSdXi;                            // + n n^T dS/dd
+      D[3 * i + j] = v * scale;                      // * multiplier * mass
+    }
+}
+
 struct Problem {
   int n_nodes, n_el;
   const int32_t* conn;
@@ -567,9 +619,39 @@ void assemble_dense(const Problem& P, const double* uk, const double* un, double
       d = spdt * P.ext_visc[3 * i + j] + d;
     }
   }
+  // packing between node couples (Simulation.cpp:6680-6681, 6698): forces
+  // accumulated couple by couple into PackingForces, Jacobian blocks added
+  // after the external-viscosity diagonal and before the fixed-DOF pass
+  std::vector<double> pf(n, 0.0);
+  if (g_pack.on) {
+    const double thr = g_pack.threshold;
+    for (size_t c = 0; c + 1 < g_pack.pairs.size(); c += 2) {
+      const int id0 = g_pack.pairs[c], id1 = g_pack.pairs[c + 1];
+      double F[3], D[9];
+      packing_couple(uk + 3 * id0, uk + 3 * id1, g_pack.mass[id0], g_pack.mass[id1], thr, F, D);
+      for (int i = 0; i < 3; ++i) {
+        pf[3 * id0 + i] += F[i];
+        pf[3 * id1 + i] -= F[i];
+      }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          const double v = D[3 * i + j];
+          double& a = K[(size_t)(3 * id0 + i) * n + (3 * id0 + j)];
+          a = -v + a;
+          double& b = K[(size_t)(3 * id0 + i) * n + (3 * id1 + j)];
+          b = v + b;
+          double& c2 = K[(size_t)(3 * id1 + i) * n + (3 * id1 + j)];
+          c2 = -v + c2;
+          double& d2 = K[(size_t)(3 * id1 + i) * n + (3 * id0 + j)];
+          d2 = v + d2;
+        }
+    }
+  }
   for (int i = 0; i < n; ++i) {
     gsum[i] = ge[i] + gvi[i] + gve[i];
-    gsum[i] = gsum[i] + 0.0;  // addExernalForces with gExt = 0
+    // addPackingForces puts PackingForces + 0 into gExt (Simulation.cpp:8130-8151),
+    // addExernalForces adds gExt (NewtonRaphsonSolver.cpp:447-457)
+    gsum[i] = gsum[i] + (pf[i] + 0.0);
   }
   if (ge_out)
     for (int i = 0; i < n; ++i) ge_out[i] = ge[i] + gvi[i];
@@ -1188,6 +1270,227 @@ int orc_nr_iteration_sparse(int n_nodes, int n_el, const int32_t* conn, const do
 #endif
   if (du_out) std::memcpy(du_out, du.data(), sizeof(double) * n);
   return kit;
+}
+
+
+// ---------------------------------------------------------------------------
+// Node-node packing: the pre-step detection of Simulation::runOneStep
+// (Simulation.cpp:6065-6076) and the per-iteration couple terms.
+
+// Sets (pairs != NULL) or clears the packing couples that assemble_dense and
+// the dense NR step apply.  mass: Node::mass per node; threshold:
+// Simulation::packingThreshold.
+void orc_set_packing(const int32_t* pairs, int n_pairs, const double* mass, int n_nodes,
+                     double threshold) {
+  g_pack = PackState();
+  if (!pairs) return;
+  g_pack.pairs.assign(pairs, pairs + 2 * n_pairs);
+  g_pack.mass.assign(mass, mass + n_nodes);
+  g_pack.threshold = threshold;
+  g_pack.on = true;
+}
+
+void orc_packing_couple(const double* p0, const double* p1, double m0, double m1, double thr,
+                        double* F, double* D) {
+  packing_couple(p0, p1, m0, m1, thr, F, D);
+}
+
+// Prism::calculateReferenceVolume (Prism.cpp:888-922) through
+// ShapeBase::calculateVolumeForInputShapeStructure (ShapeBase.cpp:3364-3398):
+// eight triangles (two bases, each rectangle split in two), each with the
+// element centre as apex of a tetrahedron.
+void orc_reference_volumes(int n_el, const double* refpos, double* vol) {
+  static const int faces[8][3] = {{0, 1, 2}, {3, 4, 5}, {0, 1, 3}, {1, 3, 4},
+                                  {1, 2, 4}, {2, 4, 5}, {0, 2, 3}, {2, 3, 5}};
+  for (int e = 0; e < n_el; ++e) {
+    const double* P = refpos + 18 * e;
+    double centre[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 3; ++j) centre[j] += P[3 * i + j];
+    for (int j = 0; j < 3; ++j) centre[j] /= 6;
+    double total = 0;
+    for (int f = 0; f < 8; ++f) {
+      const double* a = P + 3 * faces[f][0];
+      const double* b = P + 3 * faces[f][1];
+      const double* c = P + 3 * faces[f][2];
+      double v1[3], v2[3], vm[3], base[3];
+      for (int j = 0; j < 3; ++j) {
+        v1[j] = b[j] - a[j];
+        v2[j] = c[j] - a[j];
+        vm[j] = centre[j] - a[j];
+      }
+      base[0] = v1[1] * v2[2] - v1[2] * v2[1];
+      base[1] = v1[2] * v2[0] - v1[0] * v2[2];
+      base[2] = v1[0] * v2[1] - v1[1] * v2[0];
+      const double nb = std::pow(base[0] * base[0] + base[1] * base[1] + base[2] * base[2], 0.5);
+      const double area = nb / 2;
+      double height = (vm[0] * base[0] + vm[1] * base[1] + vm[2] * base[2]) / nb;
+      if (height < 0) height *= (-1.0);
+      total += 1.0 / 3.0 * (height * area);
+    }
+    vol[e] = total;
+  }
+}
+
+// Simulation::updateNodeMasses (Simulation.cpp:7587-7596) with
+// ShapeBase::assignVolumesToNodes (ShapeBase.cpp:3749-3753): every element adds
+// VolumePerNode = det(F_g) V_ref / 6 (ShapeBase.cpp:1497-1499) to its nodes,
+// in element order.
+void orc_node_masses(int n_nodes, int n_el, const int32_t* conn, const double* vol,
+                     const double* Fg, double* mass) {
+  for (int i = 0; i < n_nodes; ++i) mass[i] = 0;
+  for (int e = 0; e < n_el; ++e) {
+    double M[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) M[i][j] = Fg[9 * e + 3 * i + j];
+    const double grown = det3(M) * vol[e];
+    const double per_node = grown / 6;
+    for (int a = 0; a < 6; ++a) mass[conn[6 * e + a]] += per_node;
+  }
+}
+
+// Simulation::calculateBoundingBox (Simulation.cpp:8388-8425) over the nodes
+// (no explicit ECM), with its `else if`: a node that lowers the minimum of a
+// coordinate is not tested against the maximum.  bb = {min x, y, z, max x, y, z}.
+void orc_bounding_box(int n_nodes, const double* u, int symmetric_y, double* bb) {
+  double lo[3] = {100000.0, 100000.0, 100000.0}, hi[3] = {-100000.0, -100000.0, -100000.0};
+  for (int k = 0; k < n_nodes; ++k)
+    for (int i = 0; i < 3; ++i) {
+      const double v = u[3 * k + i];
+      if (v < lo[i]) lo[i] = v;
+      else if (v > hi[i]) hi[i] = v;
+    }
+  if (symmetric_y) lo[1] = (-1.0) * hi[1];
+  for (int i = 0; i < 3; ++i) { bb[i] = lo[i]; bb[3 + i] = hi[i]; }
+}
+
+// Thresholds of Simulation::detectPacingNodes (Simulation.cpp:8002-8020) from
+// Simulation::getAverageSideLength (Simulation.cpp:3197-3260), for a tissue of
+// columnar elements (tissueType 0, no peripodial membrane):
+//  - every non-ablated columnar element adds its apical side-length average
+//    (Prism::getApicalSideLengthAverage, Prism.cpp:1329-1360) and, if basal or
+//    spanning the whole tissue, its basal one (1362-1393) to the column mean;
+//  - the element's last value goes to the 10 x 5 grid cell of its relative
+//    position in the bounding box (ShapeBase::calculateRelativePosInBoundingBox,
+//    ShapeBase.cpp:2091-2095, centre by getCentre, 247-257: cell
+//    (floor(rx), floor(ry / 2)), clamped);
+//  - packingThreshold = 0.4 * column mean; grid = 0.4 * cell means.
+// grid50 receives the squared detection thresholds (packingDetectionThresholdGridSq).
+void orc_packing_thresholds(int n_nodes, int n_el, const int32_t* conn, const double* u,
+                            const int32_t* elem_placement, const int32_t* elem_type,
+                            const uint8_t* spans, int symmetric_y, double* grid50,
+                            double* packing_threshold, double* bb) {
+  orc_bounding_box(n_nodes, u, symmetric_y, bb);
+  const double size[2] = {bb[3] - bb[0], bb[4] - bb[1]};
+  double sum[10][5] = {}, cnt[10][5] = {};
+  double dsumCol = 0.0;
+  int colCounter = 0;
+  auto side_avg = [&](int e, const int pr[3][2]) {
+    double dsum = 0.0;
+    int counter = 0;
+    for (int i = 0; i < 3; ++i) {
+      const int a = conn[6 * e + pr[i][0]], b = conn[6 * e + pr[i][1]];
+      const double dx = u[3 * a] - u[3 * b];
+      const double dy = u[3 * a + 1] - u[3 * b + 1];
+      const double dz = u[3 * a + 2] - u[3 * b + 2];
+      const double dsum2 = (dx * dx + dy * dy + dz * dz);
+      if (dsum2 > 0) {
+        dsum += std::pow((dx * dx + dy * dy + dz * dz), 0.5);
+        counter++;
+      }
+    }
+    if (counter > 0) dsum /= counter;
+    return dsum;
+  };
+  static const int apical[3][2] = {{3, 4}, {3, 5}, {4, 5}};
+  static const int basal[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int e = 0; e < n_el; ++e) {
+    if (elem_type[e] != 0) continue;
+    double currValue = side_avg(e, apical);
+    if (currValue > 0) { dsumCol += currValue; colCounter++; }
+    if (elem_placement[e] == 0 || spans[e]) {
+      currValue = side_avg(e, basal);
+      if (currValue > 0) { dsumCol += currValue; colCounter++; }
+    }
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < 6; ++a)
+      for (int j = 0; j < 3; ++j) c[j] += u[3 * conn[6 * e + a] + j];
+    c[0] /= 6; c[1] /= 6; c[2] /= 6;
+    const double rx = (c[0] - bb[0]) / size[0];
+    const double ry = (c[1] - bb[1]) / size[1];
+    int relX = (int)std::floor(rx);
+    int relY = (int)std::floor(ry / 2.0);
+    if (relX < 0) relX = 0;
+    if (relX > 9) relX = 9;
+    if (relY < 0) relY = 0;
+    if (relY > 4) relY = 4;
+    sum[relX][relY] += currValue;
+    cnt[relX][relY]++;
+  }
+  const double colAverageSideLength = dsumCol / (double)colCounter;
+  for (int i = 0; i < 10; ++i)
+    for (int j = 0; j < 5; ++j) {
+      double g = sum[i][j];
+      if (cnt[i][j] > 0) g /= cnt[i][j];
+      g = 0.4 * g;
+      grid50[5 * i + j] = g * g;
+    }
+  *packing_threshold = 0.4 * colAverageSideLength;
+}
+
+// The couple search of Simulation::detectPacingNodes (Simulation.cpp:8035-8078):
+// nodes that can pack (Node::checkIfNodeHasPacking, Node.cpp:128-146: mass
+// non-zero, no lateral owner (node_excluded), apical or basal) are tested
+// against every later node that can pack, of the same placement, that is not
+// an immediate neighbour (shares no element, ShapeBase.cpp:3260-3280); a couple
+// is kept when its squared distance is below the master's (lower id's) grid
+// threshold, indexed by the master's position in the bounding box.  The
+// curved-region release (thereIsEmergentEllipseMarking) and collapsed nodes
+// are not modelled.  Returns the number of couples (pairs filled up to cap).
+int orc_detect_packing(int n_nodes, int n_el, const int32_t* conn, const double* u,
+                       const int32_t* node_placement, const uint8_t* node_excluded,
+                       const double* mass, const double* grid50, const double* bb, int32_t* pairs,
+                       int cap) {
+  std::vector<std::vector<int32_t>> neig(n_nodes);
+  for (int e = 0; e < n_el; ++e)
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b)
+        if (a != b) neig[conn[6 * e + a]].push_back(conn[6 * e + b]);
+  for (auto& v : neig) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  auto can_pack = [&](int k) {
+    if (mass[k] == 0) return false;
+    if (node_excluded && node_excluded[k]) return false;
+    return node_placement[k] == 0 || node_placement[k] == 1;
+  };
+  const double size[2] = {bb[3] - bb[0], bb[4] - bb[1]};
+  int count = 0;
+  for (int k = 0; k < n_nodes; ++k) {
+    if (!can_pack(k)) continue;
+    const double* pos = u + 3 * k;
+    int gx = (int)std::floor((pos[0] - bb[0]) / size[0]);
+    int gy = (int)std::floor((pos[1] - bb[1]) / size[1]);
+    gx = std::min(std::max(gx, 0), 9);  // in range for positions inside the box
+    gy = std::min(std::max(gy, 0), 4);
+    const double t2 = grid50[5 * gx + gy];
+    for (int o = k + 1; o < n_nodes; ++o) {
+      if (!can_pack(o)) continue;
+      if (node_placement[k] != node_placement[o]) continue;
+      if (std::binary_search(neig[k].begin(), neig[k].end(), o)) continue;
+      const double* ps = u + 3 * o;
+      const double dx = pos[0] - ps[0];
+      const double dy = pos[1] - ps[1];
+      const double dz = pos[2] - ps[2];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 < t2) {
+        if (count < cap) { pairs[2 * count] = k; pairs[2 * count + 1] = o; }
+        count++;
+      }
+    }
+  }
+  return count;
 }
 
 }  // extern "C"

@@ -57,6 +57,17 @@ def lib():
                                        _d, _u8, _d, _d, ctypes.c_double, _i, _i, _d, _d]
         L.orc_assemble_bsr.restype = ctypes.c_long
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_packing.argtypes = [_i, ctypes.c_int, _d, ctypes.c_int, ctypes.c_double]
+        L.orc_packing_couple.argtypes = [_d, _d, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, _d, _d]
+        L.orc_reference_volumes.argtypes = [ctypes.c_int, _d, _d]
+        L.orc_node_masses.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d]
+        L.orc_bounding_box.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d]
+        L.orc_packing_thresholds.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _i, _i, _u8,
+                                             ctypes.c_int, _d, _d, _d]
+        L.orc_detect_packing.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _i, _u8, _d, _d, _d,
+                                         _i, ctypes.c_int]
+        L.orc_detect_packing.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -227,3 +238,77 @@ def visc_surfaces(n_nodes, conn, u, elem_placement, elem_type, spans):
     lib().orc_visc_surfaces(n_nodes, conn.shape[0], _p(conn, _i), _p(u, _d), _p(ep, _i),
                             _p(et, _i), _p(sp, _u8), _p(A, _d))
     return A
+
+
+# ---- node-node packing (Simulation.cpp:3197-3260, 7307-7572, 7997-8152) ----
+
+class packing:
+    """Context in which the oracle's dense assembly and NR step add the packing
+    couples `pairs` (n x 2, detection order) with node masses and the packing
+    threshold (Simulation::packingThreshold)."""
+
+    def __init__(self, pairs, mass, threshold):
+        self.pairs = _i32(pairs).reshape(-1, 2)
+        self.mass = _f64(mass)
+        self.threshold = float(threshold)
+
+    def __enter__(self):
+        lib().orc_set_packing(_p(self.pairs, _i), self.pairs.shape[0], _p(self.mass, _d),
+                              self.mass.size, self.threshold)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_packing(None, 0, None, 0, 0.0)
+        return False
+
+
+def packing_couple(p0, p1, m0, m1, threshold):
+    """Force on the first node of a couple and its 3x3 derivative dF0/dx0."""
+    p0 = _f64(p0); p1 = _f64(p1); F = np.empty(3); D = np.empty(9)
+    lib().orc_packing_couple(_p(p0, _d), _p(p1, _d), float(m0), float(m1), float(threshold),
+                             _p(F, _d), _p(D, _d))
+    return F, D.reshape(3, 3)
+
+
+def reference_volumes(refpos):
+    refpos = _f64(refpos).reshape(-1, 18); V = np.empty(refpos.shape[0])
+    lib().orc_reference_volumes(refpos.shape[0], _p(refpos, _d), _p(V, _d))
+    return V
+
+
+def node_masses(n_nodes, conn, vol, Fg):
+    conn = _i32(conn); vol = _f64(vol); Fg = _f64(Fg).reshape(-1, 9)
+    m = np.empty(n_nodes)
+    lib().orc_node_masses(n_nodes, vol.size, _p(conn, _i), _p(vol, _d), _p(Fg, _d), _p(m, _d))
+    return m
+
+
+def bounding_box(u, symmetric_y=False):
+    u = _f64(u).reshape(-1); bb = np.empty(6)
+    lib().orc_bounding_box(u.size // 3, _p(u, _d), int(symmetric_y), _p(bb, _d))
+    return bb
+
+
+def packing_thresholds(n_nodes, conn, u, elem_placement, elem_type, spans, symmetric_y=False):
+    """(squared detection thresholds on the 10 x 5 grid, packing threshold, bounding box)."""
+    conn = _i32(conn); u = _f64(u).reshape(-1)
+    ep = _i32(elem_placement); et = _i32(elem_type)
+    sp = np.ascontiguousarray(spans, dtype=np.uint8)
+    grid = np.empty(50); thr = np.zeros(1); bb = np.empty(6)
+    lib().orc_packing_thresholds(n_nodes, conn.size // 6, _p(conn, _i), _p(u, _d), _p(ep, _i),
+                                 _p(et, _i), _p(sp, _u8), int(symmetric_y), _p(grid, _d),
+                                 _p(thr, _d), _p(bb, _d))
+    return grid.reshape(10, 5), float(thr[0]), bb
+
+
+def detect_packing(n_nodes, conn, u, node_placement, mass, grid_sq, bb, node_excluded=None):
+    """Couples (n x 2, int32) in the reference's detection order."""
+    conn = _i32(conn); u = _f64(u).reshape(-1); npl = _i32(node_placement)
+    ex = None if node_excluded is None else np.ascontiguousarray(node_excluded, dtype=np.uint8)
+    mass = _f64(mass); grid = _f64(grid_sq).reshape(50); bb = _f64(bb)
+    args = (n_nodes, conn.size // 6, _p(conn, _i), _p(u, _d), _p(npl, _i), _p(ex, _u8),
+            _p(mass, _d), _p(grid, _d), _p(bb, _d))
+    n = lib().orc_detect_packing(*args, None, 0)
+    pairs = np.zeros((max(n, 1), 2), np.int32)
+    lib().orc_detect_packing(*args, _p(pairs, _i), n)
+    return pairs[:n]
