@@ -203,6 +203,46 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
                         int32_t maxit, int32_t precision_mode, double krylov_rtol, double* u_out,
                         tf_step_stats* stats);
 
+/* Node-node packing (volume exclusion between surface nodes), the contact
+ * terms Simulation::updateStepNR adds inside every NR iteration
+ * (Simulation.cpp:6680-6698).  Single-tissue handles without slave/master
+ * binding; not supported by the adjoint.
+ * tf_nr_set_packing: turns packing on (node_placement != NULL) or off.
+ *   node_placement: Node::tissuePlacement per node (0 basal, 1 apical; other
+ *   values never pack); node_excluded (may be NULL): nodes that never pack
+ *   (Node::hasLateralElementOwner, Node.cpp:128-146); symmetric_y: the
+ *   bounding box is mirrored in y (Simulation::symmetricY,
+ *   Simulation.cpp:8412-8414); capacity: couples per node the Jacobian's
+ *   contact columns hold at first (1..32; widened by detection when needed).
+ *   The Jacobian pattern is laid out again with the contact columns.
+ * tf_nr_set_node_mass: Node::mass per node (Simulation::updateNodeMasses,
+ *   Simulation.cpp:7587-7596).  tf_nr_update_node_mass computes it on the
+ *   device from F_g (n_prisms x 9, or NULL for the device-resident F_g of
+ *   tf_nr_advance) and the prisms' reference volumes
+ *   (Prism::calculateReferenceVolume, Prism.cpp:888-922); mass_out may be NULL.
+ * tf_nr_detect_packing: Simulation::detectPacingNodes (Simulation.cpp:7997-8078)
+ *   at positions u (NULL: the device-resident positions of tf_nr_advance):
+ *   the thresholds of getAverageSideLength (Simulation.cpp:3197-3260) and the
+ *   bounding box (Simulation.cpp:8388-8425) from the element placement and
+ *   type of tf_nr_set_surface_flags, and the node couples (without the
+ *   emergent-ellipse release of curved regions).  The couples hold for every
+ *   following tf_nr_step / tf_nr_assemble until the next detection;
+ *   tf_nr_advance detects at the start of every step, after the masses.
+ * tf_nr_get_packing: the couples of the last detection in the reference's
+ *   order (ascending master, then partner) and, per couple, the packing
+ *   forces of its two nodes in the last residual (6 doubles, as
+ *   Simulation::writePacking, Simulation.cpp:8601-8620); pass couples and
+ *   forces NULL to query the count.  thresholds (may be NULL, 57 doubles):
+ *   packingThreshold, the 50 squared detection thresholds of the 10 x 5 grid
+ *   (row-major), the bounding box (min x, y, z, max x, y, z). */
+tf_status tf_nr_set_packing(tf_handle* h, const int32_t* node_placement,
+                            const uint8_t* node_excluded, int32_t symmetric_y, int32_t capacity);
+tf_status tf_nr_set_node_mass(tf_handle* h, const double* mass);
+tf_status tf_nr_update_node_mass(tf_handle* h, const double* Fg, double* mass_out);
+tf_status tf_nr_detect_packing(tf_handle* h, const double* u, int32_t* n_couples);
+tf_status tf_nr_get_packing(tf_handle* h, int32_t* n_couples, int32_t* couples, double* forces,
+                            double* thresholds);
+
 /* Batch of independent tissues in one handle (no counterpart in the
  * reference, which simulates one tissue per process).  The tissues are
  * given concatenated: nodes_per_tissue / prisms_per_tissue, conn with node

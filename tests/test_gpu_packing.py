@@ -1,0 +1,228 @@
+"""Node-node packing on the device (Simulation::detectPacingNodes and the
+couple terms of Simulation::updateStepNR, Simulation.cpp:6680-6698, 7307-7572,
+7997-8078) against the CPU oracle, through the C ABI."""
+import numpy as np
+import pytest
+
+from tissueorigami_amd import tissue
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def nr():
+    from tissueorigami_amd import nr as _nr
+    _nr.load()
+    return _nr
+
+
+def slab_problem(orc, kind="same", gap=0.35, shift=(0.1, 0.05), seed=3, visc=True, fixed=True,
+                 clamp=False, E=(500.0, 2000.0)):
+    """Two prism slabs with facing apical surfaces, random materials, growth and
+    drag, positions perturbed from the reference shape.  clamp fixes the basal
+    (outer) surfaces of both slabs: the couples' repulsion then compresses and
+    shears the slabs instead of pushing them apart, whose lateral motion the
+    couples' negative tangential stiffness would otherwise destabilise."""
+    rng = np.random.default_rng(seed)
+    A = tissue.rectangle(4.0, 3.0, 1.0, 2.0, 2)
+    B = tissue.rectangle(4.0, 3.0, 0.25, 1.0, 1) if kind == "fine" else None
+    m = tissue.facing_slabs(A, gap, shift, upper=B)
+    nn, ne = len(m["nodes"]), len(m["conn"])
+    X = m["nodes"] + rng.normal(0, 0.003, size=(nn, 3))
+    refpos = X[m["conn"]].reshape(ne, 18)
+    Fg = np.eye(3)[None] + 0.02 * rng.normal(size=(ne, 3, 3))
+    place, spans = tissue.element_placement(m["conn"], m["node_placement"])
+    etype = tissue.element_type(m["conn"], m["node_type"])
+    un = X.reshape(-1).copy()
+    uk = un + rng.normal(0, 0.002, size=un.shape)
+    fx = None
+    if fixed:
+        fx = np.zeros((nn, 3), np.uint8)
+        fx[0, :] = 1
+        fx[m["n_lower_nodes"], :] = 1
+        apical = np.nonzero(m["node_placement"][:m["n_lower_nodes"]] == 1)[0]
+        fx[apical[0], 2] = 1  # a fixed DOF inside a couple
+        fx[apical[3], 0] = 1
+        if clamp:
+            fx[m["node_placement"] == 0, :] = 1
+    return dict(mesh=m, refpos=refpos, Fg=Fg, invFg=orc.inv3_many(Fg.reshape(-1, 9)),
+                E=rng.uniform(E[0], E[1], ne), nu=np.full(ne, 0.3),
+                eta=rng.uniform(5, 20, ne) if visc else np.zeros(ne),
+                ext_visc=rng.uniform(5, 50, (nn, 3)), place=place, spans=spans, etype=etype,
+                un=un, uk=uk, fixed=fx, dt=20.0,
+                area=orc.visc_surfaces(nn, m["conn"], un, place, etype, spans))
+
+
+def oracle_packing(orc, p, u):
+    m = p["mesh"]
+    nn = len(m["nodes"])
+    V = orc.reference_volumes(p["refpos"])
+    mass = orc.node_masses(nn, m["conn"], V, p["Fg"])
+    grid, thr, bb = orc.packing_thresholds(nn, m["conn"], u, p["place"], p["etype"], p["spans"])
+    pairs = orc.detect_packing(nn, m["conn"], u, m["node_placement"], mass, grid, bb)
+    return pairs, mass, grid, thr, bb
+
+
+def gpu_solver(nr, p, capacity=2):
+    m = p["mesh"]
+    s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], p["refpos"])
+    s.set_material(p["E"], p["nu"], p["eta"], p["ext_visc"], p["fixed"])
+    s.set_growth(p["invFg"])
+    s.set_visc_surface(p["area"])
+    s.set_surface_flags(p["place"], p["etype"], p["spans"])
+    s.set_packing(m["node_placement"], capacity=capacity)
+    return s
+
+
+def oracle_problem(orc, p):
+    lam, mu = orc.lame(p["E"], p["nu"])
+    return orc.Problem(p["mesh"]["conn"], p["refpos"], p["invFg"], lam, mu, p["eta"],
+                       p["ext_visc"], p["area"], p["fixed"])
+
+
+CASES = [("same", 0.3, (0.1, 0.05), 2), ("same", 0.3, (-0.1, -0.05), 2),
+         ("fine", 0.05, (0.125, 0.02), 1)]
+
+
+@pytest.mark.parametrize("kind,gap,shift,cap", CASES)
+def test_detection_matches_oracle(orc, nr, kind, gap, shift, cap):
+    p = slab_problem(orc, kind, gap, shift)
+    print(f"packing detection case {kind} gap {gap} shift {shift} capacity {cap}", flush=True)
+    pairs, mass, grid, thr, bb = oracle_packing(orc, p, p["un"])
+    assert len(pairs) > 0
+    s = gpu_solver(nr, p, cap)
+    dmass = s.update_node_mass(p["Fg"].reshape(-1, 9))
+    assert np.abs(dmass - mass).max() <= 1e-14 * np.abs(mass).max()
+    n = s.detect_packing(p["un"])
+    dpairs, forces, dthr, dgrid, dbb = s.packing()
+    assert n == len(pairs)
+    assert np.array_equal(dpairs, pairs)
+    assert np.array_equal(dbb, bb)
+    assert abs(dthr - thr) <= 1e-14 * thr
+    assert np.abs(dgrid - grid).max() <= 1e-14 * np.abs(grid).max()
+    s.close()
+
+
+@pytest.mark.parametrize("kind", ["same", "fine"])
+def test_operator_and_residual_with_packing_match_oracle(orc, nr, kind):
+    gap, shift = (0.3, (0.1, 0.05)) if kind == "same" else (0.05, (0.125, 0.02))
+    p = slab_problem(orc, kind, gap, shift)
+    pairs, mass, grid, thr, bb = oracle_packing(orc, p, p["un"])
+    s = gpu_solver(nr, p, 1)
+    s.set_node_mass(mass)
+    assert s.detect_packing(p["un"]) == len(pairs)
+    g, _ = s.assemble(p["uk"], p["un"], p["dt"])
+    ia, ja, a = s.get_csr(1e-13)
+    P = oracle_problem(orc, p)
+    K0, g0, _ = P.assemble_dense(p["uk"], p["un"], p["dt"])
+    with orc.packing(pairs, mass, thr):
+        K, og, _ = P.assemble_dense(p["uk"], p["un"], p["dt"])
+    assert np.abs(g - og).max() <= 1e-13 * np.abs(og).max()
+    oia, oja, oa = orc.csr_threshold(K)
+    assert np.array_equal(ia, oia) and np.array_equal(ja, oja)
+    assert np.abs(a - oa).max() <= 1e-13 * np.abs(oa).max()
+    # the couples couple the slabs: the pattern holds blocks the elements do not
+    assert len(ja) > len(orc.csr_threshold(K0)[1])
+    # per-couple forces of the residual: PackingForces of both nodes
+    _, forces, _, _, _ = s.packing()
+    pf = np.zeros_like(g)
+    pf_nodes = pf.reshape(-1, 3)
+    for (i, j), f in zip(pairs, forces):
+        pf_nodes[i] = f[:3]
+        pf_nodes[j] = f[3:]
+    free = (p["fixed"].reshape(-1) == 0)
+    diff = (og - g0)[free]
+    assert np.abs(pf[free] - diff).max() <= 1e-12 * np.abs(diff).max()
+    # the packing forces are there and sum to zero over the nodes (third law)
+    assert np.abs(forces).max() > 1.0
+    tot = pf.reshape(-1, 3).sum(0)
+    assert np.abs(tot).max() <= 1e-12 * np.abs(forces).max()
+    s.close()
+
+
+def test_packing_step_matches_oracle_dense_step(orc, nr):
+    p = slab_problem(orc, "same", 0.42, (0.1, 0.05), seed=8, clamp=True, E=(2000.0, 8000.0))
+    pairs, mass, grid, thr, bb = oracle_packing(orc, p, p["un"])
+    assert len(pairs) == 20
+    s = gpu_solver(nr, p)
+    s.set_node_mass(mass)
+    s.detect_packing(p["un"])
+    u, st = s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-12)
+    P = oracle_problem(orc, p)
+    with orc.packing(pairs, mass, thr):
+        ref, du, conv = P.nr_step_dense(p["un"], p["dt"], 1e-8, 20)
+    assert st["converged"] and conv
+    assert st["nr_iterations"] == len(du)
+    assert np.abs(u - ref).max() <= 1e-9 * np.abs(ref).max()
+    # the couples moved the slabs: the step differs from one without packing
+    with orc.packing(np.zeros((0, 2), np.int32), mass, thr):
+        free_ref, _, _ = P.nr_step_dense(p["un"], p["dt"], 1e-8, 20)
+    assert np.abs(free_ref - ref).max() > 1e-3
+    um, stm = s.step(p["un"], p["dt"], 1e-8, 20, "mixed", 1e-3)
+    assert stm["converged"]
+    assert np.abs(um - u).max() <= 1e-8 * np.abs(u).max()
+    s.close()
+
+
+def test_advance_with_packing_matches_per_step_path(orc, nr):
+    """tf_nr_advance (growth, surfaces, masses and couple detection on the
+    device every step) reproduces the host-driven per-step sequence."""
+    p = slab_problem(orc, "same", 0.42, (0.1, 0.05), seed=4, clamp=True, E=(2000.0, 8000.0))
+    m = p["mesh"]
+    nn, ne = len(m["nodes"]), len(m["conn"])
+    rng = np.random.default_rng(9)
+    rates = np.zeros((ne, 3))
+    rates[:, :2] = rng.uniform(0.5, 1.5, (ne, 1)) * 0.05 / 3600.0
+    a = gpu_solver(nr, p)
+    a.set_growth_rates(rates)
+    ua, stats = a.advance(4, p["dt"], u0=p["un"], krylov_rtol=1e-12)
+    n_dev = a.detect_packing()
+    b = gpu_solver(nr, p)
+    Fg = np.tile(np.eye(3), (ne, 1, 1))
+    inc = np.zeros((ne, 3, 3))
+    for i in range(3):
+        inc[:, i, i] = np.exp(rates[:, i] * p["dt"])
+    ub = p["un"].copy()
+    counts = []
+    for k in range(4):
+        Fg = inc @ Fg
+        b.set_growth(orc.inv3_many(Fg.reshape(-1, 9)))
+        b.update_visc_surface(ub, p["place"], p["etype"], p["spans"])
+        b.update_node_mass(Fg.reshape(-1, 9))
+        counts.append(b.detect_packing(ub))
+        ub, st = b.step(ub, p["dt"], 1e-8, 20, "fp64", 1e-12)
+        assert st["nr_iterations"] == stats[k]["nr_iterations"]
+    assert max(counts) > 0
+    assert np.abs(ua - ub).max() <= 1e-12 * np.abs(ub).max()
+    assert n_dev == b.detect_packing(ub)
+    a.close(); b.close()
+
+
+def test_packing_off_gives_the_plain_step(orc, nr):
+    p = slab_problem(orc, "same", 0.42, (0.1, 0.05), seed=5, clamp=True, E=(2000.0, 8000.0))
+    pairs, mass, grid, thr, bb = oracle_packing(orc, p, p["un"])
+    s = gpu_solver(nr, p)
+    s.set_node_mass(mass)
+    s.detect_packing(p["un"])
+    s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-12)
+    s.set_packing(None)
+    u_off, _ = s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-12)
+    m = p["mesh"]
+    q = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], p["refpos"])
+    q.set_material(p["E"], p["nu"], p["eta"], p["ext_visc"], p["fixed"])
+    q.set_growth(p["invFg"])
+    q.set_visc_surface(p["area"])
+    u_plain, _ = q.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-12)
+    # the unused contact columns hold exact zeros
+    assert np.abs(u_off - u_plain).max() <= 1e-13 * np.abs(u_plain).max()
+    s.close(); q.close()
+
+
+def test_packing_rejects_binding_and_the_adjoint(orc, nr):
+    p = slab_problem(orc, "same", 0.3, (0.1, 0.05))
+    s = gpu_solver(nr, p)
+    with pytest.raises(nr.TFError):
+        s.set_material(p["E"], p["nu"], p["eta"], p["ext_visc"], None, np.array([[3, 0]]))
+    with pytest.raises(nr.TFError):  # no detection yet
+        s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-12)
+    s.close()

@@ -100,14 +100,41 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
     const int row_len = (R < a.n_nodes) ? a.pat.row_len[R] : 0;
     for (int kk = kq; kk < width; kk += kCols) {
     const int64_t slot = sbase + (int64_t)kk * kSlice + lane;
-    if (kk >= row_len) {  // padding slot: zero block
+    if (kk >= row_len) {
+      // padding slot (zero block), or the contact slot of a packing couple
+      double J[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) J[q] = 0.0;
+      if (a.pk_cnt && R < a.n_nodes) {
+        const int e = kk - a.pat.slice_ew[sl];
+        if (e >= 0 && e < min(a.pk_cnt[R], a.pk_cap)) {
+          const int C = a.pk_inc[(size_t)R * a.pk_cap + e];
+          const int lo = min(R, C), hi = max(R, C);
+          double F[3], D[9];
+          pack_couple(a.uk + 3 * lo, a.uk + 3 * hi, a.pk_mass[lo], a.pk_mass[hi], a.pk_par[0],
+                      F, D);
+          // K(id0, id1) and K(id1, id0) receive dF0/dx0 on a zero entry
+          // (Simulation.cpp:7557-7567); J(R, C) = K(C, R)^T
+#pragma unroll
+          for (int p = 0; p < 3; ++p)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) J[3 * p + q] = D[3 * q + p] + 0.0;
+          if (a.dof_flags) {
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+              for (int q = 0; q < 3; ++q)
+                if ((a.dof_flags[3 * R + p] | a.dof_flags[3 * C + q]) & 1) J[3 * p + q] = 0.0;
+          }
+        }
+      }
       if (MIXED) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) a.val32[(size_t)q * n_slots + slot] = 0.0f;
+        for (int q = 0; q < 9; ++q) a.val32[(size_t)q * n_slots + slot] = (float)J[q];
       }
       if (a.val) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) a.val[(size_t)q * n_slots + slot] = 0.0;
+        for (int q = 0; q < 9; ++q) a.val[(size_t)q * n_slots + slot] = J[q];
       }
       continue;
     }
@@ -129,6 +156,22 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
         const double spdt = a.area[R] / a.dt;
 #pragma unroll
         for (int j = 0; j < 3; ++j) J[4 * j] = spdt * a.ext_visc[3 * R + j] + J[4 * j];
+        if (a.pk_cnt) {
+          // -dF/dx of every couple of R, in couple order (ascending partner:
+          // the couples where R is id1 come first), Simulation.cpp:7557-7567
+          const int nc = min(a.pk_cnt[R], a.pk_cap);
+          for (int e = 0; e < nc; ++e) {
+            const int o = a.pk_inc[(size_t)R * a.pk_cap + e];
+            const int lo = min(R, o), hi = max(R, o);
+            double F[3], D[9];
+            pack_couple(a.uk + 3 * lo, a.uk + 3 * hi, a.pk_mass[lo], a.pk_mass[hi], a.pk_par[0],
+                        F, D);
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+              for (int q = 0; q < 3; ++q) J[3 * p + q] = -D[3 * q + p] + J[3 * p + q];
+          }
+        }
       }
       if (a.dof_flags) {
 #pragma unroll
@@ -172,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
   }
 }
 
-// Node residual: gSum = ge + gvInternal + gvExternal (+ gExt = 0), fixed DOF
+// Node residual: gSum = ge + gvInternal + gvExternal + gExt (packing), fixed DOF
 // zeroed (NewtonRaphsonSolver.cpp:362-397, 441-457, 209); element forces
 // are summed in element order (ShapeBase.cpp:2362-2376).  internal receives
 // SystemForces (elastic + viscous per element, non-fixed DOF only).
@@ -195,13 +238,33 @@ __global__ __launch_bounds__(kBlock) void node_force_kernel(AssembleArgs a) {
         if (!fx[i]) sf[i] = sf[i] + (a.Fe[3 * (size_t)s + i] + a.Fv[3 * (size_t)s + i]);
       }
     }
+    // PackingForces of R, couple by couple (Simulation.cpp:7396-7401)
+    double fp[3] = {0.0, 0.0, 0.0};
+    if (a.pk_cnt) {
+      const int nc = min(a.pk_cnt[R], a.pk_cap);
+      for (int e = 0; e < nc; ++e) {
+        const int o = a.pk_inc[(size_t)R * a.pk_cap + e];
+        const int lo = min(R, o), hi = max(R, o);
+        double F[3], D[9];
+        pack_couple(a.uk + 3 * lo, a.uk + 3 * hi, a.pk_mass[lo], a.pk_mass[hi], a.pk_par[0], F,
+                    D);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) fp[i] = (R == lo) ? fp[i] + F[i] : fp[i] - F[i];
+      }
+      if (a.pk_force) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) a.pk_force[3 * R + i] = fp[i];
+      }
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const double sv = a.area[R] * a.ext_visc[3 * R + i];
       const double disp = (a.uk[3 * R + i] - a.un[3 * R + i]) * rdt;
       const double gve = -(sv * disp);
       double g = ge[i] + gv[i] + gve;
-      g = g + 0.0;
+      // gExt = PackingForces + 0 (Simulation.cpp:8130-8151), added by
+      // NewtonRaphsonSolver::addExernalForces
+      g = g + (fp[i] + 0.0);
       if (fx[i]) g = 0.0;
       a.g[3 * R + i] = g;
       if (a.internal) a.internal[3 * R + i] = sf[i];

@@ -31,6 +31,13 @@ struct DevPattern {
   int32_t* contrib_src = nullptr; // n_prisms * 36: e*36 + A*6 + B (K block A,B)
   int32_t* node_ptr = nullptr;    // n_nodes + 1
   int32_t* node_src = nullptr;    // n_prisms * 6: e*6 + a
+  // contact slots of node-node packing (null when the handle does not pack):
+  // slice s stores its rows' element blocks in columns [0, slice_ew[s]) and
+  // slice_cap[s] contact columns after them; the e-th couple of row R (in
+  // ascending partner order) sits in column slice_ew[s] + e, unused contact
+  // slots point at column R with zero values
+  int32_t* slice_ew = nullptr;    // n_slices
+  int32_t* slice_cap = nullptr;   // n_slices
 };
 
 // The operator of a bound system, A_b = N^T J N + I_slave (slave/master
@@ -224,8 +231,107 @@ struct AssembleArgs {
   float* minv32;
   double* g;
   double* internal;
+  // node-node packing (pk_cnt == nullptr: none): couples per node, partners
+  // (pk_cap per node, ascending), node masses, parameters (pk_par[0]: the
+  // packing threshold), and the per-node packing force of the residual
+  const int32_t* pk_cnt = nullptr;
+  const int32_t* pk_inc = nullptr;
+  int pk_cap = 0;
+  const double* pk_mass = nullptr;
+  const double* pk_par = nullptr;
+  double* pk_force = nullptr;
 };
 hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st);
+
+// ---- node-node packing (tf_packing.hip) ----
+// One couple id0 < id1 of Simulation::calculatePackingForcesImplicit3D
+// (Simulation.cpp:7338-7400) and calculatePackingJacobian3D
+// (Simulation.cpp:7444-7490): the force F on id0 (-F on id1) and the
+// symmetric D = dF0/dx0 = m [f/d (I - n n^T) + f' n n^T], with the constants
+// of Simulation::detectPacingNodes (multiplier 5000, sigmoid saturation 5,
+// Simulation.cpp:8018-8020), the sigmoid centred at 0.6 threshold and the
+// distance clamped to 1e-3, in the reference's operation order.
+__device__ inline void pack_couple(const double* p0, const double* p1, double m0, double m1,
+                                   double thr, double F[3], double D[9]) {
+#pragma clang fp contract(off)
+  const double multiplier = 5000.0, sat = 5.0;
+  const double shift = 0.6 * thr;
+  const double dx = p0[0] - p1[0];
+  const double dy = p0[1] - p1[1];
+  const double dz = p0[2] - p1[2];
+  double d = sqrt(dx * dx + dy * dy + dz * dz);
+  const double avg_mass = 0.5 * (m0 + m1);
+  double n[3] = {1.0, 0.0, 0.0};
+  if (d > 1e-3) {
+    n[0] = dx / d;
+    n[1] = dy / d;
+    n[2] = dz / d;
+  } else {
+    d = 1e-3;
+  }
+  const double shifted = d - shift;
+  const double sig = 1 / (1 + exp(sat / thr * 2.0 * shifted));
+  const double fmag = multiplier * avg_mass * sig;
+  F[0] = fmag * n[0];
+  F[1] = fmag * n[1];
+  F[2] = fmag * n[2];
+  const double dsig = (-sat / thr * 2.0) * sig * (1 - sig);
+  const double scale = multiplier * avg_mass;
+  const double sd = sig / d;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double nn = n[i] * n[j];
+      double v = (i == j ? 1.0 : 0.0) - nn;
+      v = v * sd;
+      v = v + nn * dsig;
+      D[3 * i + j] = v * scale;
+    }
+}
+
+// At most this many couples per node (detection reports more as an error).
+constexpr int kPackLocal = 32;
+
+struct PackDev {
+  bool on = false;
+  int32_t cap = 0;             // contact slots per row of the slices holding packable nodes
+  int symmetric_y = 0;
+  int32_t n_cand = 0;          // nodes that may pack
+  int32_t hash_size = 0;       // power of two
+  int32_t* place = nullptr;    // per node: 0 basal, 1 apical, -1 never packs
+  double* vol = nullptr;       // per prism: reference volume
+  double* mass = nullptr;      // per node (Node::mass)
+  int32_t* cnt = nullptr;      // per node: couples of the last detection
+  int32_t* inc = nullptr;      // per node: cap partners, ascending
+  double* par = nullptr;       // [0] packing threshold, [1] largest detection threshold,
+                               // [2..51] squared grid thresholds, [52..57] bounding box
+  double* force = nullptr;     // per node: packing force of the last residual
+  int32_t* head = nullptr;     // hash buckets
+  int32_t* next = nullptr;     // per node: next node of its bucket
+  int32_t* cell = nullptr;     // per prism: threshold-grid cell (-1: not columnar)
+  double* eval = nullptr;      // per prism: its grid value, apical and basal contributions
+  int32_t* ecnt = nullptr;     // per prism: bit 0 apical and bit 1 basal value counted
+  double* part = nullptr;      // per chunk partial sums of the thresholds
+  int32_t* flag = nullptr;     // [0] largest couple count of the last detection
+  int32_t n_chunks = 0;
+  bool have_mass = false, detected = false;
+  int32_t n_pairs = 0;         // couples of the last detection
+  int32_t max_count = 0;
+};
+// Reference volumes (Prism::calculateReferenceVolume, Prism.cpp:888-922).
+hipError_t launch_ref_volume(int32_t n_prisms, const double* refpos, double* vol, hipStream_t st);
+// Node::mass from F_g (Simulation::updateNodeMasses, Simulation.cpp:7587-7596).
+hipError_t launch_node_mass(int32_t n_nodes, const int32_t* node_ptr, const int32_t* node_src,
+                            const double* Fg, const double* vol, double* mass, hipStream_t st);
+// Thresholds and couples at positions u (Simulation::detectPacingNodes).
+hipError_t launch_packing_detect(PackDev& p, int32_t n_nodes, int32_t n_prisms,
+                                 const int32_t* conn, const double* u, const int32_t* elem_place,
+                                 const int32_t* elem_type, const uint8_t* spans,
+                                 const DevPattern& pat, hipStream_t st);
+// Column and transpose-slot entries of the contact slots for the last detection.
+hipError_t launch_contact_slots(const PackDev& p, int32_t n_nodes, const DevPattern& pat,
+                                hipStream_t st);
 hipError_t launch_bind_forces(const BoundOp& b, double* g, hipStream_t st);
 hipError_t launch_bind_increment(const int32_t* pairs, int n_pairs, double* du, hipStream_t st);
 hipError_t launch_visc_surface(int32_t n_nodes, int32_t n_prisms, const int32_t* conn,

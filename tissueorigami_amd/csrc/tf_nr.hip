@@ -43,6 +43,9 @@ struct tf_handle {
   hipStream_t st = nullptr;
   DevPattern pat;
   std::vector<int32_t> h_slice_base, h_col, h_row_len, h_rowptr, h_nbr;
+  // contact capacity and element width per slice (node-node packing)
+  std::vector<int32_t> h_slice_cap, h_slice_ew;
+  PackDev pack;
   // inputs
   int32_t* conn = nullptr;
   double* refpos = nullptr;
@@ -117,6 +120,18 @@ hipError_t dalloc(tf_handle* h, T** p, size_t count) {
   return e;
 }
 
+// Frees an array allocated by dalloc and clears the pointer.
+template <typename T>
+void dfree(tf_handle* h, T** p) {
+  if (!*p) return;
+  auto it = std::find(h->allocs.begin(), h->allocs.end(), (void*)*p);
+  if (it != h->allocs.end()) h->allocs.erase(it);
+  (void)hipFree(*p);
+  *p = nullptr;
+}
+
+tf_status layout_pattern(tf_handle* h, const int32_t* conn);
+
 tf_status check_device(int* dev) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
@@ -171,8 +186,25 @@ tf_status build_pattern(tf_handle* h, const int32_t* conn) {
     nbr.insert(nbr.end(), cand.begin(), cand.end());
     rowptr[R + 1] = (int32_t)nbr.size();
   }
-  // SELL-64
+  DevPattern& P0 = h->pat;
+  TF_CHECK(dalloc(h, &P0.node_ptr, Nn + 1));
+  TF_CHECK(dalloc(h, &P0.node_src, (size_t)Ne * 6));
+  TF_CHECK(hipMemcpy(P0.node_ptr, node_ptr.data(), sizeof(int32_t) * (Nn + 1), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(P0.node_src, node_src.data(), sizeof(int32_t) * node_src.size(), hipMemcpyHostToDevice));
+  return layout_pattern(h, conn);
+}
+
+// SELL-64 layout of the block pattern, transpose slot map and element slot
+// map.  Slices with a contact capacity (h_slice_cap, node-node packing) get
+// that many extra columns after their element columns.  Replaces the
+// slot-indexed device arrays of an earlier layout.
+tf_status layout_pattern(tf_handle* h, const int32_t* conn) {
+  const int Nn = h->Nn, Ne = h->Ne;
+  const std::vector<int32_t>& rowptr = h->h_rowptr;
+  const std::vector<int32_t>& nbr = h->h_nbr;
   const int n_slices = (Nn + kSlice - 1) / kSlice;
+  const bool contact = !h->h_slice_cap.empty();
+  std::vector<int32_t> ew(n_slices, 0);
   std::vector<int32_t>& sb = h->h_slice_base;
   sb.assign(n_slices + 1, 0);
   std::vector<int32_t>& rl = h->h_row_len;
@@ -184,6 +216,8 @@ tf_status build_pattern(tf_handle* h, const int32_t* conn) {
       rl[R] = rowptr[R + 1] - rowptr[R];
       w = std::max(w, rl[R]);
     }
+    ew[s] = w;
+    if (contact) w += h->h_slice_cap[s];
     sb[s] = (int32_t)total;
     total += (int64_t)w * kSlice;
     if (total > INT32_MAX) {
@@ -253,8 +287,11 @@ tf_status build_pattern(tf_handle* h, const int32_t* conn) {
     for (int e = 0; e < Ne; ++e)
       for (int ab = 0; ab < 36; ++ab) csrc[pos[slot_cache[(size_t)e * 36 + ab]]++] = e * 36 + ab;
   }
-  // upload
+  // upload (replacing the arrays of an earlier layout)
   DevPattern& P = h->pat;
+  int32_t** old_arrays[] = {&P.slice_base, &P.col, &P.diag_slot, &P.row_len, &P.tslot,
+                            &P.contrib_ptr, &P.contrib_src, &P.slice_ew, &P.slice_cap};
+  for (int32_t** a : old_arrays) dfree(h, a);
   P.n_slices = n_slices;
   P.n_slots = total;
   TF_CHECK(dalloc(h, &P.slice_base, n_slices + 1));
@@ -264,8 +301,14 @@ tf_status build_pattern(tf_handle* h, const int32_t* conn) {
   TF_CHECK(dalloc(h, &P.tslot, total));
   TF_CHECK(dalloc(h, &P.contrib_ptr, total + 1));
   TF_CHECK(dalloc(h, &P.contrib_src, (size_t)Ne * 36));
-  TF_CHECK(dalloc(h, &P.node_ptr, Nn + 1));
-  TF_CHECK(dalloc(h, &P.node_src, (size_t)Ne * 6));
+  if (contact) {
+    TF_CHECK(dalloc(h, &P.slice_ew, n_slices));
+    TF_CHECK(dalloc(h, &P.slice_cap, n_slices));
+    TF_CHECK(hipMemcpy(P.slice_ew, ew.data(), sizeof(int32_t) * n_slices, hipMemcpyHostToDevice));
+    TF_CHECK(hipMemcpy(P.slice_cap, h->h_slice_cap.data(), sizeof(int32_t) * n_slices,
+                       hipMemcpyHostToDevice));
+  }
+  h->h_slice_ew = ew;
   TF_CHECK(hipMemcpy(P.slice_base, sb.data(), sizeof(int32_t) * (n_slices + 1), hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.col, col.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.diag_slot, diag.data(), sizeof(int32_t) * Nn, hipMemcpyHostToDevice));
@@ -273,8 +316,16 @@ tf_status build_pattern(tf_handle* h, const int32_t* conn) {
   TF_CHECK(hipMemcpy(P.tslot, tslot.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.contrib_ptr, cptr.data(), sizeof(int32_t) * (total + 1), hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(P.contrib_src, csrc.data(), sizeof(int32_t) * csrc.size(), hipMemcpyHostToDevice));
-  TF_CHECK(hipMemcpy(P.node_ptr, node_ptr.data(), sizeof(int32_t) * (Nn + 1), hipMemcpyHostToDevice));
-  TF_CHECK(hipMemcpy(P.node_src, node_src.data(), sizeof(int32_t) * node_src.size(), hipMemcpyHostToDevice));
+  // operator values follow the slot count
+  Workspace& w = h->ws;
+  if (w.val) {
+    dfree(h, &w.val);
+    TF_CHECK(dalloc(h, &w.val, (size_t)total * 9));
+  }
+  if (w.val32) {
+    dfree(h, &w.val32);
+    TF_CHECK(dalloc(h, &w.val32, (size_t)total * 9));
+  }
   return TF_OK;
 }
 
@@ -756,6 +807,69 @@ tf_status coarse_prepare(tf_handle* h) {
   return TF_OK;
 }
 
+// Lays the pattern out again with `cap` contact columns in every slice that
+// holds a node that may pack (node-node packing).
+tf_status packing_relayout(tf_handle* h, int cap) {
+  PackDev& p = h->pack;
+  const int n_slices = (h->Nn + kSlice - 1) / kSlice;
+  std::vector<int32_t> place(h->Nn);
+  TF_CHECK(hipMemcpy(place.data(), p.place, sizeof(int32_t) * h->Nn, hipMemcpyDeviceToHost));
+  std::vector<int32_t> caps(n_slices, 0);
+  for (int k = 0; k < h->Nn; ++k)
+    if (place[k] >= 0) caps[k / kSlice] = cap;
+  std::vector<int32_t> conn((size_t)h->Ne * 6);
+  TF_CHECK(hipStreamSynchronize(h->st));
+  TF_CHECK(hipMemcpy(conn.data(), h->conn, sizeof(int32_t) * conn.size(), hipMemcpyDeviceToHost));
+  tiled_release(&h->ws);  // captured chunks hold the previous pattern
+  h->coarse.built = false;
+  h->coarse.active = false;
+  h->h_slice_cap = caps;
+  tf_status st = layout_pattern(h, conn.data());
+  if (st != TF_OK) return st;
+  dfree(h, &p.inc);
+  TF_CHECK(dalloc(h, &p.inc, (size_t)h->Nn * std::max(cap, 1)));
+  p.cap = cap;
+  p.detected = false;
+  h->assembled = false;
+  return TF_OK;
+}
+
+// Thresholds and couples at device positions u (Simulation::detectPacingNodes,
+// Simulation.cpp:7997-8078), widening the contact columns when a node has
+// more couples than they hold, then the contact slots of the pattern.
+tf_status packing_detect(tf_handle* h, const double* u) {
+  PackDev& p = h->pack;
+  if (!h->have_flags) {
+    set_error("node-node packing needs tf_nr_set_surface_flags (element placement and type)");
+    return TF_ERR_ARG;
+  }
+  if (!p.have_mass) {
+    set_error("node-node packing needs node masses (tf_nr_set_node_mass)");
+    return TF_ERR_ARG;
+  }
+  int32_t flag[2] = {0, 0};
+  for (int pass = 0; pass < 2; ++pass) {
+    TF_CHECK(launch_packing_detect(p, h->Nn, h->Ne, h->conn, u, h->surf_place, h->surf_type,
+                                   h->surf_spans, h->pat, h->st));
+    TF_CHECK(hipMemcpyAsync(flag, p.flag, sizeof(flag), hipMemcpyDeviceToHost, h->st));
+    TF_CHECK(hipStreamSynchronize(h->st));
+    if (flag[0] > kPackLocal) {
+      set_error("node-node packing: a node has " + std::to_string(flag[0]) +
+                " couples, more than " + std::to_string(kPackLocal));
+      return TF_ERR_ARG;
+    }
+    if (flag[0] <= p.cap) break;
+    tf_status st = packing_relayout(h, std::min(kPackLocal, std::max(flag[0], 2 * p.cap)));
+    if (st != TF_OK) return st;
+  }
+  TF_CHECK(launch_contact_slots(p, h->Nn, h->pat, h->st));
+  p.n_pairs = flag[1];
+  p.max_count = flag[0];
+  p.detected = true;
+  h->assembled = false;
+  return TF_OK;
+}
+
 // Element kernel + slot-map assembly + residual and binding of the residual.
 tf_status assemble(tf_handle* h, double dt, int mixed) {
   if (!h->have_material || !h->have_growth || !h->have_area) {
@@ -791,6 +905,15 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
   aa.val32 = w.val32;
   aa.minv = w.minv; aa.minv32 = w.minv32;
   aa.g = w.g; aa.internal = w.internal;
+  if (h->pack.on) {
+    PackDev& pk = h->pack;
+    if (!pk.detected) {
+      set_error("node-node packing is on: tf_nr_detect_packing must precede the step");
+      return TF_ERR_ARG;
+    }
+    aa.pk_cnt = pk.cnt; aa.pk_inc = pk.inc; aa.pk_cap = pk.cap;
+    aa.pk_mass = pk.mass; aa.pk_par = pk.par; aa.pk_force = pk.force;
+  }
   TF_CHECK(launch_assemble(aa, mixed, h->st));
   if (h->n_pairs > 0) TF_CHECK(launch_bind_forces(h->bound, w.g, h->st));
   if (h->bound.ready)
@@ -888,6 +1011,10 @@ tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, co
   if (n_pairs > 0)
     TF_CHECK(hipMemcpy(h->pairs, h->h_pairs.data(), sizeof(int32_t) * 2 * n_pairs,
                        hipMemcpyHostToDevice));
+  if (n_pairs > 0 && h->pack.on) {
+    set_error("slave/master binding together with node-node packing is not supported");
+    return TF_ERR_ARG;
+  }
   const bool pairs_changed = n_pairs != h->n_pairs || !h->bound.ready;
   h->n_pairs = n_pairs;
   h->any_visc = any_visc;
@@ -1306,6 +1433,15 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
     TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, w.uk, h->pat.node_ptr, h->pat.node_src,
                                  h->surf_place, h->surf_type, h->surf_spans, h->area, h->st));
     h->have_area = true;
+    if (h->pack.on) {
+      // node masses from the grown volumes, then the couples at the step's
+      // start positions (Simulation.cpp:6065-6076)
+      TF_CHECK(launch_node_mass(h->Nn, h->pat.node_ptr, h->pat.node_src, h->Fg, h->pack.vol,
+                                h->pack.mass, h->st));
+      h->pack.have_mass = true;
+      tf_status ps = packing_detect(h, w.uk);
+      if (ps != TF_OK) return ps;
+    }
     TF_CHECK(hipMemcpyAsync(w.un, w.uk, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
     tf_status r = nr_loop(h, dt, tol, maxit, mixed, rtol, S.data());
     if (stats) std::memcpy(stats + (size_t)s * h->n_tissues, S.data(), sizeof(tf_step_stats) * h->n_tissues);
@@ -1380,8 +1516,8 @@ tf_status tf_adj_forward(tf_handle* h, int32_t n_steps, const double* u0,
     set_error("tf_adj_forward: invalid arguments");
     return TF_ERR_ARG;
   }
-  if (h->n_pairs > 0 || h->batch) {
-    set_error("tf_adj_forward: binding and batches are not supported by the adjoint");
+  if (h->n_pairs > 0 || h->batch || h->pack.on) {
+    set_error("tf_adj_forward: binding, batches and packing are not supported by the adjoint");
     return TF_ERR_ARG;
   }
   const size_t n = 3 * (size_t)h->Nn, Ne = h->Ne;
@@ -1516,6 +1652,236 @@ tf_status tf_adj_backward(tf_handle* h, const double* inv_Fg_steps, const double
   return TF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Block columns of every row: the element neighbours and, after a packing
+// detection, the couples' partners, ascending.  jslot(C, R) is the J slot
+// of block (C, R).
+struct HostRows {
+  std::vector<int32_t> ptr, cols;
+  std::vector<int32_t> cnt, inc;
+  int cap = 0;
+};
+tf_status host_rows(tf_handle* h, HostRows& r) {
+  const int Nn = h->Nn;
+  const bool pk = h->pack.on && h->pack.detected;
+  if (pk) {
+    r.cap = h->pack.cap;
+    r.cnt.resize(Nn);
+    r.inc.resize((size_t)Nn * r.cap);
+    TF_CHECK(hipMemcpy(r.cnt.data(), h->pack.cnt, sizeof(int32_t) * Nn, hipMemcpyDeviceToHost));
+    TF_CHECK(hipMemcpy(r.inc.data(), h->pack.inc, sizeof(int32_t) * r.inc.size(),
+                       hipMemcpyDeviceToHost));
+  }
+  r.ptr.assign(Nn + 1, 0);
+  r.cols.clear();
+  for (int R = 0; R < Nn; ++R) {
+    const size_t b = r.cols.size();
+    r.cols.insert(r.cols.end(), h->h_nbr.begin() + h->h_rowptr[R],
+                  h->h_nbr.begin() + h->h_rowptr[R + 1]);
+    if (pk) {
+      for (int e = 0; e < std::min(r.cnt[R], r.cap); ++e)
+        r.cols.push_back(r.inc[(size_t)R * r.cap + e]);
+      std::sort(r.cols.begin() + b, r.cols.end());
+    }
+    r.ptr[R + 1] = (int32_t)r.cols.size();
+  }
+  return TF_OK;
+}
+int64_t jslot(const tf_handle* h, const HostRows& r, int C, int R) {
+  const int32_t* b = h->h_nbr.data() + h->h_rowptr[C];
+  const int32_t* e = h->h_nbr.data() + h->h_rowptr[C + 1];
+  const int32_t* p = std::lower_bound(b, e, R);
+  const int s = C / kSlice;
+  if (p != e && *p == R) return h->h_slice_base[s] + (int64_t)(p - b) * kSlice + (C % kSlice);
+  if (r.cnt.empty()) return -1;
+  for (int k = 0; k < std::min(r.cnt[C], r.cap); ++k)
+    if (r.inc[(size_t)C * r.cap + k] == R)
+      return h->h_slice_base[s] + (int64_t)(h->h_slice_ew[s] + k) * kSlice + (C % kSlice);
+  return -1;
+}
+}  // namespace
+
+extern "C" {
+
+tf_status tf_nr_set_packing(tf_handle* h, const int32_t* node_placement,
+                            const uint8_t* node_excluded, int32_t symmetric_y,
+                            int32_t capacity) {
+  if (!h) {
+    set_error("tf_nr_set_packing: invalid handle");
+    return TF_ERR_ARG;
+  }
+  PackDev& p = h->pack;
+  if (!node_placement) {  // off: every contact slot unused
+    if (p.on) {
+      TF_CHECK(hipMemsetAsync(p.cnt, 0, sizeof(int32_t) * h->Nn, h->st));
+      TF_CHECK(launch_contact_slots(p, h->Nn, h->pat, h->st));
+      TF_CHECK(hipStreamSynchronize(h->st));
+    }
+    p.on = false;
+    p.detected = false;
+    h->assembled = false;
+    return TF_OK;
+  }
+  if (h->batch || h->n_pairs > 0) {
+    set_error("tf_nr_set_packing: single-tissue handles without slave/master binding only");
+    return TF_ERR_ARG;
+  }
+  if (capacity < 1 || capacity > kPackLocal) {
+    set_error("tf_nr_set_packing: capacity must lie in [1, " + std::to_string(kPackLocal) + "]");
+    return TF_ERR_ARG;
+  }
+  const int Nn = h->Nn, Ne = h->Ne;
+  std::vector<int32_t> place(Nn);
+  int n_cand = 0;
+  for (int k = 0; k < Nn; ++k) {
+    const bool ok = (node_placement[k] == 0 || node_placement[k] == 1) &&
+                    !(node_excluded && node_excluded[k]);
+    place[k] = ok ? node_placement[k] : -1;
+    n_cand += ok ? 1 : 0;
+  }
+  if (!p.place) {
+    TF_CHECK(dalloc(h, &p.place, Nn));
+    TF_CHECK(dalloc(h, &p.vol, Ne));
+    TF_CHECK(dalloc(h, &p.mass, Nn));
+    TF_CHECK(dalloc(h, &p.cnt, Nn));
+    TF_CHECK(dalloc(h, &p.par, 64));
+    TF_CHECK(dalloc(h, &p.force, 3 * (size_t)Nn));
+    TF_CHECK(dalloc(h, &p.next, Nn));
+    TF_CHECK(dalloc(h, &p.cell, Ne));
+    TF_CHECK(dalloc(h, &p.eval, 3 * (size_t)Ne));
+    TF_CHECK(dalloc(h, &p.ecnt, Ne));
+    TF_CHECK(dalloc(h, &p.flag, 2));
+    p.n_chunks = (Ne + 2047) / 2048;
+    TF_CHECK(dalloc(h, &p.part, (size_t)p.n_chunks * 52 * 2));
+    int hs = 1024;
+    while (hs < 2 * Nn) hs *= 2;
+    p.hash_size = hs;
+    TF_CHECK(dalloc(h, &p.head, hs));
+    TF_CHECK(launch_ref_volume(Ne, h->refpos, p.vol, h->st));
+    TF_CHECK(hipMemsetAsync(p.cnt, 0, sizeof(int32_t) * Nn, h->st));
+    TF_CHECK(hipMemsetAsync(p.force, 0, sizeof(double) * 3 * Nn, h->st));
+  }
+  TF_CHECK(hipMemcpy(p.place, place.data(), sizeof(int32_t) * Nn, hipMemcpyHostToDevice));
+  p.n_cand = n_cand;
+  p.symmetric_y = symmetric_y ? 1 : 0;
+  p.on = true;
+  p.detected = false;
+  // contact columns in the slices of the nodes that may pack
+  const int n_slices = (Nn + kSlice - 1) / kSlice;
+  std::vector<int32_t> caps(n_slices, 0);
+  for (int k = 0; k < Nn; ++k)
+    if (place[k] >= 0) caps[k / kSlice] = capacity;
+  if (caps != h->h_slice_cap || !p.inc) {
+    tf_status st = packing_relayout(h, capacity);
+    if (st != TF_OK) return st;
+  }
+  return TF_OK;
+}
+
+tf_status tf_nr_set_node_mass(tf_handle* h, const double* mass) {
+  if (!h || !mass || !h->pack.on) {
+    set_error("tf_nr_set_node_mass: invalid arguments or packing off");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipMemcpy(h->pack.mass, mass, sizeof(double) * h->Nn, hipMemcpyHostToDevice));
+  h->pack.have_mass = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_update_node_mass(tf_handle* h, const double* Fg, double* mass_out) {
+  if (!h || !h->pack.on || (!Fg && !h->Fg)) {
+    set_error("tf_nr_update_node_mass: invalid arguments, packing off or no F_g");
+    return TF_ERR_ARG;
+  }
+  PackDev& p = h->pack;
+  const double* F = h->Fg;
+  if (Fg) {
+    double* d = nullptr;
+    TF_CHECK(hipMalloc(&d, sizeof(double) * 9 * (size_t)h->Ne));
+    hipError_t e = hipMemcpy(d, Fg, sizeof(double) * 9 * (size_t)h->Ne, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = launch_node_mass(h->Nn, h->pat.node_ptr, h->pat.node_src, d, p.vol, p.mass, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    (void)hipFree(d);
+    TF_CHECK(e);
+  } else {
+    TF_CHECK(launch_node_mass(h->Nn, h->pat.node_ptr, h->pat.node_src, F, p.vol, p.mass, h->st));
+  }
+  if (mass_out)
+    TF_CHECK(hipMemcpyAsync(mass_out, p.mass, sizeof(double) * h->Nn, hipMemcpyDeviceToHost,
+                            h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  p.have_mass = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_detect_packing(tf_handle* h, const double* u, int32_t* n_couples) {
+  if (!h || !h->pack.on || (!u && !h->have_positions)) {
+    set_error("tf_nr_detect_packing: invalid arguments, packing off or no positions");
+    return TF_ERR_ARG;
+  }
+  const double* src = h->ws.uk;
+  if (u) {
+    tf_status up = upload_nodes(h, h->ws.tmp, u);
+    if (up != TF_OK) return up;
+    src = h->ws.tmp;
+  }
+  tf_status st = packing_detect(h, src);
+  if (st != TF_OK) return st;
+  if (n_couples) *n_couples = h->pack.n_pairs;
+  return TF_OK;
+}
+
+tf_status tf_nr_get_packing(tf_handle* h, int32_t* n_couples, int32_t* couples, double* forces,
+                            double* thresholds) {
+  if (!h || !h->pack.on || !h->pack.detected || !n_couples) {
+    set_error("tf_nr_get_packing: invalid arguments or no detection");
+    return TF_ERR_ARG;
+  }
+  PackDev& p = h->pack;
+  const int Nn = h->Nn;
+  if (thresholds) {
+    double par[58];
+    TF_CHECK(hipMemcpy(par, p.par, sizeof(par), hipMemcpyDeviceToHost));
+    thresholds[0] = par[0];
+    for (int q = 0; q < 50; ++q) thresholds[1 + q] = par[2 + q];
+    for (int q = 0; q < 6; ++q) thresholds[51 + q] = par[52 + q];
+  }
+  if (!couples && !forces) {
+    *n_couples = p.n_pairs;
+    return TF_OK;
+  }
+  if (*n_couples < p.n_pairs) {
+    set_error("tf_nr_get_packing: output too small");
+    return TF_ERR_ARG;
+  }
+  std::vector<int32_t> cnt(Nn), inc((size_t)Nn * p.cap);
+  std::vector<double> f(3 * (size_t)Nn);
+  TF_CHECK(hipMemcpy(cnt.data(), p.cnt, sizeof(int32_t) * Nn, hipMemcpyDeviceToHost));
+  TF_CHECK(hipMemcpy(inc.data(), p.inc, sizeof(int32_t) * inc.size(), hipMemcpyDeviceToHost));
+  TF_CHECK(hipMemcpy(f.data(), p.force, sizeof(double) * f.size(), hipMemcpyDeviceToHost));
+  int c = 0;
+  for (int k = 0; k < Nn; ++k)
+    for (int e = 0; e < std::min(cnt[k], p.cap); ++e) {
+      const int o = inc[(size_t)k * p.cap + e];
+      if (o <= k) continue;
+      if (couples) {
+        couples[2 * c] = k;
+        couples[2 * c + 1] = o;
+      }
+      if (forces)
+        for (int i = 0; i < 3; ++i) {
+          forces[6 * c + i] = f[3 * (size_t)k + i];
+          forces[6 * c + 3 + i] = f[3 * (size_t)o + i];
+        }
+      ++c;
+    }
+  *n_couples = c;
+  return TF_OK;
+}
+
 tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz) {
   if (h && h->batch) {
     set_error("tf_csr_pattern works on single-tissue handles only");
@@ -1526,7 +1892,10 @@ tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz) {
     return TF_ERR_ARG;
   }
   const int Nn = h->Nn;
-  const int64_t total = 9 * (int64_t)h->h_nbr.size();
+  HostRows rows;
+  tf_status hs = host_rows(h, rows);
+  if (hs != TF_OK) return hs;
+  const int64_t total = 9 * (int64_t)rows.cols.size();
   if (!ja) {
     *nnz = total;
     return TF_OK;
@@ -1539,8 +1908,8 @@ tf_status tf_csr_pattern(tf_handle* h, int32_t* ia, int32_t* ja, int64_t* nnz) {
   for (int R = 0; R < Nn; ++R)
     for (int i = 0; i < 3; ++i) {
       if (ia) ia[3 * R + i] = (int32_t)c;
-      for (int k = h->h_rowptr[R]; k < h->h_rowptr[R + 1]; ++k)
-        for (int j = 0; j < 3; ++j) ja[c++] = 3 * h->h_nbr[k] + j;
+      for (int k = rows.ptr[R]; k < rows.ptr[R + 1]; ++k)
+        for (int j = 0; j < 3; ++j) ja[c++] = 3 * rows.cols[k] + j;
     }
   if (ia) ia[3 * Nn] = (int32_t)c;
   *nnz = c;
@@ -1572,12 +1941,9 @@ tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja
   } else {
     TF_CHECK(hipMemcpy(val.data(), h->ws.val, sizeof(double) * val.size(), hipMemcpyDeviceToHost));
   }
-  auto slot_of = [&](int R, int C) -> int64_t {
-    const int32_t* b = h->h_nbr.data() + h->h_rowptr[R];
-    const int32_t* e = h->h_nbr.data() + h->h_rowptr[R + 1];
-    const int32_t* p = std::lower_bound(b, e, C);
-    return h->h_slice_base[R / kSlice] + (int64_t)(p - b) * kSlice + (R % kSlice);
-  };
+  HostRows hr;
+  tf_status hs = host_rows(h, hr);
+  if (hs != TF_OK) return hs;
   // K(3R+i, 3C+j) = J(3C+j, 3R+i) = J block (C, R)[j][i]
   const int n = 3 * Nn;
   std::vector<std::map<int, double>> rows;
@@ -1589,9 +1955,9 @@ tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja
   for (int R = 0; R < Nn; ++R)
     for (int i = 0; i < 3; ++i) {
       tia[3 * R + i] = (int32_t)tja.size();
-      for (int k = h->h_rowptr[R]; k < h->h_rowptr[R + 1]; ++k) {
-        const int C = h->h_nbr[k];
-        const int64_t slot = slot_of(C, R);
+      for (int k = hr.ptr[R]; k < hr.ptr[R + 1]; ++k) {
+        const int C = hr.cols[k];
+        const int64_t slot = jslot(h, hr, C, R);
         for (int j = 0; j < 3; ++j) {
           const double v = val[(size_t)(3 * j + i) * S + slot];
           if (bind) {

@@ -26,6 +26,8 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
            "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs", "tf_nr_set_krylov", "tf_nr_assemble_ex",
            "tf_copy_bandwidth", "tf_nr_set_coarse", "tf_nr_coarse_info",
+           "tf_nr_set_packing", "tf_nr_set_node_mass", "tf_nr_update_node_mass",
+           "tf_nr_detect_packing", "tf_nr_get_packing",
            "tf_nr_destroy", "tf_last_error"]
 
 
@@ -112,6 +114,11 @@ def load():
     L.tf_copy_bandwidth.argtypes = [ctypes.c_int64, ctypes.c_int32, _d]
     L.tf_nr_set_coarse.argtypes = [_h, ctypes.c_double]
     L.tf_nr_coarse_info.argtypes = [_h, _i, _i]
+    L.tf_nr_set_packing.argtypes = [_h, _i, _u8, ctypes.c_int32, ctypes.c_int32]
+    L.tf_nr_set_node_mass.argtypes = [_h, _d]
+    L.tf_nr_update_node_mass.argtypes = [_h, _d, _d]
+    L.tf_nr_detect_packing.argtypes = [_h, _d, _i]
+    L.tf_nr_get_packing.argtypes = [_h, _i, _i, _d, _d]
     L.tf_nr_destroy.argtypes = [_h]
     L.tf_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
@@ -289,6 +296,50 @@ class NewtonRaphsonSolver:
         if code not in (TF_OK, TF_ERR_NOT_CONVERGED):
             _check(code)
         return out, [self._stats(st, i) for i in range(n_steps)]
+
+    # ---- node-node packing (Simulation::detectPacingNodes and the couple terms)
+    def set_packing(self, node_placement, node_excluded=None, symmetric_y=False, capacity=2):
+        """Turns packing on for the nodes' placements (0 basal, 1 apical), or
+        off with node_placement None."""
+        if node_placement is None:
+            _check(load().tf_nr_set_packing(self._h, None, None, 0, 0))
+            return
+        pl = np.ascontiguousarray(node_placement, dtype=np.int32).reshape(self.n_nodes)
+        ex = None if node_excluded is None else \
+            np.ascontiguousarray(node_excluded, dtype=np.uint8).reshape(self.n_nodes)
+        _check(load().tf_nr_set_packing(self._h, _p(pl, _i), _p(ex, _u8), int(bool(symmetric_y)),
+                                        int(capacity)))
+
+    def set_node_mass(self, mass):
+        m = _f64(mass).reshape(self.n_nodes)
+        _check(load().tf_nr_set_node_mass(self._h, _p(m, _d)))
+
+    def update_node_mass(self, Fg=None):
+        """Node masses from F_g (n_prisms x 9; None: the device-resident F_g)."""
+        F = None if Fg is None else _f64(Fg).reshape(self.n_prisms, 9)
+        out = np.empty(self.n_nodes)
+        _check(load().tf_nr_update_node_mass(self._h, _p(F, _d), _p(out, _d)))
+        return out
+
+    def detect_packing(self, u=None):
+        """Couples at positions u (None: the device-resident positions); returns their count."""
+        u = None if u is None else _f64(u).reshape(3 * self.n_nodes)
+        n = np.zeros(1, np.int32)
+        _check(load().tf_nr_detect_packing(self._h, _p(u, _d), _p(n, _i)))
+        return int(n[0])
+
+    def packing(self):
+        """(couples n x 2, forces n x 6 of the last residual, packing threshold,
+        squared detection thresholds 10 x 5, bounding box 6)."""
+        L = load()
+        n = np.zeros(1, np.int32)
+        thr = np.empty(57)
+        _check(L.tf_nr_get_packing(self._h, _p(n, _i), None, None, _p(thr, _d)))
+        pairs = np.zeros((max(int(n[0]), 1), 2), np.int32)
+        forces = np.zeros((max(int(n[0]), 1), 6))
+        _check(L.tf_nr_get_packing(self._h, _p(n, _i), _p(pairs, _i), _p(forces, _d), None))
+        k = int(n[0])
+        return pairs[:k], forces[:k], float(thr[0]), thr[1:51].reshape(10, 5), thr[51:57]
 
     # ---- tiles of the fused Krylov solver
     def autotune(self, precision="mixed"):

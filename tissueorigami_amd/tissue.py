@@ -231,31 +231,31 @@ def rates_xyz(rate_per_hour):
     return r
 
 
-def facing_slabs(m, gap, shift=(0.0, 0.0)):
-    """Two copies of an extruded mesh whose apical surfaces face each other.
+def facing_slabs(m, gap, shift=(0.0, 0.0), upper=None):
+    """Two extruded meshes whose apical surfaces face each other.
 
-    The second copy is the first mirrored in z about the plane gap / 2 above the
-    apical surface and shifted by `shift` in (x, y); its prisms are renumbered
-    (nodes 1 <-> 2 and 4 <-> 5) so that each basal triangle is again
-    counter-clockwise seen from its own apical side.  The copies share no
-    element, so their apical nodes are candidate packing couples
-    (Simulation::detectPacingNodes, Simulation.cpp:7997-8078) once they are
-    closer than the detection threshold."""
+    The upper mesh (a copy of m unless given) is mirrored in z so that its
+    apical surface lies `gap` above m's, and shifted by `shift` in (x, y); its
+    prisms are renumbered (nodes 1 <-> 2 and 4 <-> 5) so that each basal
+    triangle is again counter-clockwise seen from its own apical side.  The
+    two meshes share no element, so their apical nodes are candidate packing
+    couples (Simulation::detectPacingNodes, Simulation.cpp:7997-8078) once they
+    are closer than the detection threshold."""
+    u = m if upper is None else upper
     nodes = m["nodes"]
     nn = len(nodes)
     top = nodes[:, 2].max()
-    mirror = nodes.copy()
-    mirror[:, 2] = 2.0 * top + gap - nodes[:, 2]
+    mirror = u["nodes"].copy()
+    mirror[:, 2] = top + gap + (u["nodes"][:, 2].max() - u["nodes"][:, 2])
     mirror[:, 0] += shift[0]
     mirror[:, 1] += shift[1]
-    conn_b = m["conn"][:, [0, 2, 1, 3, 5, 4]] + nn
+    conn_b = u["conn"][:, [0, 2, 1, 3, 5, 4]] + nn
     out = dict(m)
     out["nodes"] = np.concatenate([nodes, mirror])
     out["conn"] = np.concatenate([m["conn"], conn_b]).astype(np.int32)
     out["refpos"] = out["nodes"][out["conn"]].reshape(-1, 18).copy()
-    out["node_placement"] = np.concatenate([m["node_placement"], m["node_placement"]])
-    out["node_type"] = np.concatenate([m["node_type"], m["node_type"]])
-    out["at_circumference"] = np.concatenate([m["at_circumference"], m["at_circumference"]])
-    out["layer"] = np.concatenate([m["layer"], m["layer"]])
+    for key in ("node_placement", "node_type", "at_circumference", "layer"):
+        out[key] = np.concatenate([m[key], u[key]])
+    out["n_lower_nodes"] = nn
     out["kind"] = m.get("kind", "mesh") + "-facing-slabs"
     return out
