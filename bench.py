@@ -339,6 +339,70 @@ def run_bound(w, solver_steps, mixed_rtol, mixed_krylov):
                 max_slave_master_gap=float(np.abs(d[pairs[:, 0]] - d[pairs[:, 1]]).max()))
 
 
+def run_packing(steps, warmup, mixed_rtol, mixed_krylov, name="c2", gap=0.6):
+    """Node-node packing at scale: two copies of a workload's ellipse facing
+    apical to apical (workloads.contact_workload); mixed steps with the couples
+    detected on the device at the start of every step, the FP64 steps from the
+    same state beside them, and the cost of one detection."""
+    from tissueorigami_amd.workloads import contact_workload
+    w = contact_workload(name, gap)
+    m = w["mesh"]
+    nn, ne = len(m["nodes"]), len(m["conn"])
+    t0 = time.perf_counter()
+    s = nr.NewtonRaphsonSolver(nn, m["conn"], m["refpos"])
+    s.set_material(w["E"], w["nu"], w["eta"], w["ext"], w["fixed"])
+    s.set_growth_rates(w["rates"])
+    s.set_surface_flags(w["place"], w["etype"], w["spans"])
+    s.set_krylov("mixed", mixed_krylov)
+    s.set_packing(m["node_placement"], capacity=2)
+    setup_s = time.perf_counter() - t0
+    blocks, slots = s.pattern_info()
+    u0 = m["nodes"].reshape(-1).copy()
+    Fg0 = np.tile(np.eye(3).reshape(9), (ne, 1))
+    out = dict(workload=w["desc"], prisms=ne, nodes=nn, steps=steps, setup_s=setup_s,
+               sell_slots=int(slots), blocks=int(blocks))
+    for prec, rtol in (("mixed", mixed_rtol), ("fp64", 1e-10)):
+        for _ in range(warmup):
+            s.growth_tensor(Fg0)
+            s.advance(1, w["dt"], u0=u0, precision=prec, krylov_rtol=rtol, want_positions=False)
+        s.growth_tensor(Fg0)
+        s.advance(0, w["dt"], u0=u0, want_positions=False)
+        couples, nr_its, kr_its, conv, fb = [], 0, 0, True, [0, 0, 0]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            _, st = s.advance(1, w["dt"], None, 1e-8, 20, prec, rtol, want_positions=False)
+            couples.append(s.packing_count())
+            nr_its += st[0]["nr_iterations"]
+            kr_its += st[0]["krylov_iterations_total"]
+            conv &= st[0]["converged"]
+            fb[0] += st[0]["cg_fallbacks"]
+            fb[1] += st[0]["fp64_fallbacks"]
+            fb[2] += st[0].get("gmres_fallbacks", 0)
+        wall = time.perf_counter() - t0
+        u, _ = s.advance(0, w["dt"])
+        out[prec] = dict(nr_iterations=nr_its, krylov_iterations=kr_its, wall_s=wall,
+                         nr_iterations_per_s=nr_its / wall, time_steps_per_s=steps / wall,
+                         couples_per_step=couples, converged=bool(conv), cg_fallbacks=fb[0],
+                         fp64_fallbacks=fb[1], gmres_fallbacks=fb[2])
+        out[prec + "_u"] = u
+        print(f"[bench] packing {prec}: couples {couples}, {nr_its} NR its, {wall:.2f} s",
+              file=sys.stderr, flush=True)
+    um, uf = out.pop("mixed_u"), out.pop("fp64_u")
+    out["mixed_rel_pos_error_vs_fp64"] = float(np.abs(um - uf).max() / np.abs(uf).max())
+    out["mixed_vs_fp64_speedup"] = (out["mixed"]["nr_iterations_per_s"] /
+                                    out["fp64"]["nr_iterations_per_s"])
+    # one detection at the step-start positions (device resident), wall time
+    s.detect_packing(u0)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        n = s.detect_packing()
+    out["detect_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    out["couples_at_start"] = s.detect_packing(u0)
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -350,6 +414,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-binding", action="store_true")
+    ap.add_argument("--no-packing", action="store_true")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every solver kernel directly (no hipGraph replay)")
     ap.add_argument("--mixed-rtol", type=float, default=1e-3,
@@ -507,6 +572,11 @@ def main():
         res["bound"] = run_bound(w, args.steps, args.mixed_rtol, args.mixed_krylov)
         print(f"[bench] bound: {res['bound']['nr_iterations_per_s']:.2f} NR its/s, "
               f"{res['bound']['time_steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+    if not args.no_packing and args.config == "c3":
+        res["packing"] = run_packing(args.steps, args.warmup, args.mixed_rtol, args.mixed_krylov)
+        print(f"[bench] packing: {res['packing']['mixed']['nr_iterations_per_s']:.2f} NR its/s "
+              f"mixed, detection {res['packing']['detect_ms']:.2f} ms",
+              file=sys.stderr, flush=True)
     if not args.no_batch and args.config == "c3":
         res["batch"] = run_batch(args.steps, args.warmup, args.mixed_rtol, args.mixed_krylov)
     if not args.no_cpu_baseline:

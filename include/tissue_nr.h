@@ -73,6 +73,9 @@ typedef struct tf_step_stats {
   int32_t cg_fallbacks;                    /* NR iterations whose CG solve failed
                                               (indefinite or stalled) and was
                                               redone with BiCGStab */
+  int32_t gmres_fallbacks;                 /* NR iterations whose BiCGStab solve
+                                              failed and was redone with FP64
+                                              GMRES (the last stage) */
 } tf_step_stats;
 
 typedef struct tf_handle tf_handle;
@@ -271,13 +274,16 @@ tf_status tf_nr_autotune(tf_handle* h, int32_t precision_mode, int32_t* slices_p
 tf_status tf_nr_set_tile_size(tf_handle* h, int32_t slices_per_tile);
 
 /* Krylov method of the linear solves of one precision mode (no reference
- * counterpart; the reference factorises with Pardiso).  BiCGStab handles any
- * Jacobian.  CG (one SpMV per iteration) suits Jacobians that are symmetric
- * positive definite up to the small viscous asymmetry; a tissue whose CG
- * solve meets non-positive curvature or stalls is solved again with BiCGStab
- * in the same NR iteration (tf_step_stats.cg_fallbacks).  Defaults: FP64
- * BiCGStab, mixed CG. */
-enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1 };
+ * counterpart; the reference factorises with Pardiso).  BiCGStab handles
+ * most Jacobians.  CG (one SpMV per iteration) suits Jacobians that are
+ * symmetric positive definite up to the small viscous asymmetry; a tissue
+ * whose CG solve meets non-positive curvature or stalls is solved again with
+ * BiCGStab in the same NR iteration (tf_step_stats.cg_fallbacks).  A tissue
+ * whose BiCGStab solve breaks down (typically on the indefinite Jacobians of
+ * node-node packing) is solved again with restarted FP64 GMRES(100) with
+ * block-Jacobi right preconditioning (tf_step_stats.gmres_fallbacks); GMRES
+ * may also be chosen for the FP64 mode.  Defaults: FP64 BiCGStab, mixed CG. */
+enum { TF_KRYLOV_BICGSTAB = 0, TF_KRYLOV_CG = 1, TF_KRYLOV_GMRES = 2 };
 tf_status tf_nr_set_krylov(tf_handle* h, int32_t precision_mode, int32_t method);
 
 /* Adjoint of n_steps implicit steps with respect to one growth parameter r_e

@@ -56,6 +56,7 @@ def _batch_physical(nr, orc, cs, dt, spt):
 
 @pytest.mark.parametrize("precision,rtol,spt,method", [
     ("fp64", 1e-10, 8, "bicgstab"), ("fp64", 1e-10, 2, "bicgstab"), ("fp64", 1e-10, 4, "cg"),
+    ("fp64", 1e-10, 4, "gmres"),
     ("mixed", 1e-3, 8, "cg"), ("mixed", 1e-5, 8, "bicgstab"), ("mixed", 1e-10, 4, "cg")])
 def test_batch_step_is_bitwise_each_tissue_alone(orc, nr, precision, rtol, spt, method):
     cs = _ellipses()

@@ -226,3 +226,85 @@ def test_packing_rejects_binding_and_the_adjoint(orc, nr):
     with pytest.raises(nr.TFError):  # no detection yet
         s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-12)
     s.close()
+
+
+def facing_ellipses(orc, gap=0.6):
+    """Two small ellipses (semi-axes 9 x 7 um, side 2 um, 4 layers over 20 um)
+    facing apical to apical, the wing-disc materials (E 1000 Pa apical layer,
+    100 Pa below, eta 10 Pa s, drag 16000 / 10), basal surfaces clamped,
+    growth from the seeded random field.  The couples' negative tangential
+    stiffness exceeds the apical layer's: the Jacobian is indefinite (102
+    negative eigenvalues)."""
+    ms = tissue.ellipse(9.0, 7.0, 2.0, 20.0, 4)
+    m = tissue.facing_slabs(ms, gap, (0.0, 0.0))
+    nn, ne = len(m["nodes"]), len(m["conn"])
+    place, spans = tissue.element_placement(m["conn"], m["node_placement"])
+    etype = tissue.element_type(m["conn"], m["node_type"])
+    top = m["layer"] == ms["n_layers"] - 1
+    E = np.where(top, 1000.0, 100.0)
+    ext = np.zeros((nn, 3))
+    ext[m["node_placement"] == 1] = 16000.0
+    ext[m["node_placement"] == 0] = 10.0
+    fx = np.zeros((nn, 3), np.uint8)
+    fx[m["node_placement"] == 0, :] = 1
+    cent = m["nodes"][m["conn"]].mean(1)[:, :2]
+    rate = tissue.gaussian_random_field(cent, 0.02, 0.01, 15.0, 2024)
+    Fg = tissue.growth_increment(rate, 60.0)
+    un = m["nodes"].reshape(-1).copy()
+    return dict(mesh=m, refpos=m["refpos"], Fg=Fg, invFg=orc.inv3_many(Fg.reshape(-1, 9)), E=E,
+                nu=np.full(ne, 0.3), eta=np.full(ne, 10.0), ext_visc=ext, place=place,
+                spans=spans, etype=etype, un=un, uk=un, fixed=fx, dt=60.0,
+                area=orc.visc_surfaces(nn, m["conn"], un, place, etype, spans))
+
+
+@pytest.mark.parametrize("method", ["bicgstab", "gmres"])
+def test_indefinite_contact_step_matches_oracle(orc, nr, method):
+    """The FP64 step on an indefinite contact Jacobian: BiCGStab breaks down
+    and the tissue is solved again with GMRES (or GMRES from the start); the
+    positions match the oracle's dense-LU step, and the mixed step (CG,
+    then its fallbacks) reaches the same root."""
+    p = facing_ellipses(orc)
+    pairs, mass, grid, thr, bb = oracle_packing(orc, p, p["un"])
+    assert len(pairs) == 51
+    print(f"indefinite contact step, FP64 {method}", flush=True)
+    s = gpu_solver(nr, p)
+    s.set_krylov("fp64", method)
+    s.set_node_mass(mass)
+    assert s.detect_packing(p["un"]) == len(pairs)
+    u, st = s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-12)
+    P = oracle_problem(orc, p)
+    with orc.packing(pairs, mass, thr):
+        ref, du, conv = P.nr_step_dense(p["un"], p["dt"], 1e-8, 20)
+    print(f"  NR {st['nr_iterations']} (oracle {len(du)}), gmres fallbacks "
+          f"{st['gmres_fallbacks']}, krylov {st['krylov_iterations']}", flush=True)
+    assert st["converged"] and conv
+    assert st["nr_iterations"] == len(du)
+    assert np.abs(u - ref).max() <= 1e-9 * np.abs(ref).max()
+    um, stm = s.step(p["un"], p["dt"], 1e-8, 20, "mixed", 1e-3)
+    print(f"  mixed NR {stm['nr_iterations']}, fallbacks cg {stm['cg_fallbacks']} fp64 "
+          f"{stm['fp64_fallbacks']} gmres {stm['gmres_fallbacks']}", flush=True)
+    assert stm["converged"]
+    assert np.abs(um - u).max() <= 1e-8 * np.abs(u).max()
+    s.close()
+
+
+def test_gmres_step_matches_oracle(orc, nr):
+    """FP64 GMRES as the step's solver on a plain (definite) tissue."""
+    import problems
+    p = problems.perturbed_problem(seed=6, visc=True, fixed=True)
+    m = p["mesh"]
+    s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], p["refpos"])
+    s.set_material(p["E"], p["nu"], p["eta"], p["ext_visc"], p["fixed"])
+    s.set_growth(p["invFg"])
+    s.set_visc_surface(p["area"])
+    s.set_krylov("fp64", "gmres")
+    u, st = s.step(p["un"], p["dt"], 1e-8, 20, "fp64", 1e-12)
+    lam, mu = orc.lame(p["E"], p["nu"])
+    P = orc.Problem(m["conn"], p["refpos"], p["invFg"], lam, mu, p["eta"], p["ext_visc"],
+                    p["area"], p["fixed"])
+    ref, du, conv = P.nr_step_dense(p["un"], p["dt"], 1e-8, 20)
+    assert st["converged"] and st["nr_iterations"] == len(du)
+    assert np.abs(u - ref).max() <= 1e-9 * np.abs(ref).max()
+    with pytest.raises(nr.TFError):
+        s.set_krylov("mixed", "gmres")
+    s.close()

@@ -168,12 +168,23 @@ struct Workspace {
   double* tnorm = nullptr;     // per tissue
   int32_t* host_tstate = nullptr;
   double* host_tnorm = nullptr;
+  // restarted GMRES (FP64): basis of kGmresRestart + 1 vectors, per-tile
+  // partials, per-tissue sums and host-computed coefficients
+  double* gV = nullptr;
+  double* gpart = nullptr;   // (kGmresRestart + 2) x tpart_stride
+  double* gsum = nullptr;    // 2 (kGmresRestart + 2) x tissues
+  double* gfac = nullptr;    // per tissue
+  double* gy = nullptr;      // kGmresRestart x tissues
+  int32_t* gact = nullptr;   // per tissue
+  int32_t* gk = nullptr;     // per tissue
+  int gpart_stride = 0;
   // captured iteration chunks of the tiled solver (hipGraph), see tf_tiled.hip
   void* graphs = nullptr;
   int graphs_on = 1;
 };
 
 constexpr int kBlock = 256;
+constexpr int kGmresRestart = 100;
 
 enum KState { K_RUN = 0, K_CONVERGED = 1, K_BREAKDOWN = 2 };
 // A Krylov residual above kDivergence |b| ends the solve as a breakdown.
@@ -388,7 +399,7 @@ hipError_t tiled_bicgstab(const KrylovArgs& k, const double* b, double* x, doubl
 // while rtol < inner_rtol.  A tissue whose FP32 solve breaks down, diverges
 // or reaches maxit ends with that state.
 // Krylov methods of the tiled solvers.
-enum KrylovMethod { KRYLOV_BICGSTAB = 0, KRYLOV_CG = 1 };
+enum KrylovMethod { KRYLOV_BICGSTAB = 0, KRYLOV_CG = 1, KRYLOV_GMRES = 2 };
 hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double rtol,
                        double inner_rtol, int maxit, int sync_every, int method,
                        TiledResult* res);
@@ -396,6 +407,10 @@ hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double r
 // or rho ends a tissue's solve as a breakdown.
 hipError_t tiled_cg(const KrylovArgs& k, const double* b, double* x, double rtol, int maxit,
                     int sync_every, TiledResult* res);
+// Restarted GMRES(kGmresRestart) with right block-Jacobi preconditioning on
+// the FP64 operator, all active tissues (workspace from the gV.. members).
+hipError_t tiled_gmres(const KrylovArgs& k, const double* b, double* x, double rtol, int maxit,
+                       TiledResult* res);
 // x_k += dx and per-tissue |dx|_2 into norms (host).
 hipError_t tiled_norm_update(const KrylovArgs& k, const double* dx, double* xk, double* norms);
 hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int method, int iters,

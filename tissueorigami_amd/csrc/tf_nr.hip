@@ -363,6 +363,26 @@ tf_status ensure_mixed(tf_handle* h) {
   return TF_OK;
 }
 
+// Workspace of the FP64 GMRES solver (allocated at its first use).
+tf_status ensure_gmres(tf_handle* h) {
+  Workspace& w = h->ws;
+  const int m = kGmresRestart, nt = h->n_tissues;
+  if (!w.gV) {
+    TF_CHECK(dalloc(h, &w.gV, (size_t)(m + 1) * 3 * h->Nn));
+    TF_CHECK(dalloc(h, &w.gsum, (size_t)2 * (m + 2) * nt));
+    TF_CHECK(dalloc(h, &w.gfac, nt));
+    TF_CHECK(dalloc(h, &w.gy, (size_t)m * nt));
+    TF_CHECK(dalloc(h, &w.gact, nt));
+    TF_CHECK(dalloc(h, &w.gk, nt));
+  }
+  if (w.gpart_stride < w.tpart_stride) {
+    dfree(h, &w.gpart);
+    TF_CHECK(dalloc(h, &w.gpart, (size_t)(m + 2) * w.tpart_stride));
+    w.gpart_stride = w.tpart_stride;
+  }
+  return TF_OK;
+}
+
 // Tiles of spt slices that never cross a tissue boundary (tissues start on
 // slice boundaries).  Rebuilding replaces the device tile table.
 tf_status build_tiles(tf_handle* h, int spt) {
@@ -1156,6 +1176,8 @@ namespace {
 // kMixedCap iterations, after which the tissue falls back to FP64.
 constexpr double kMixedInnerRtol = 1e-5;
 constexpr int kMixedCap = 5000;
+// GMRES iterations per solve (each synchronises with the host once)
+constexpr int kGmresCap = 5000;
 
 // The NR loop of Simulation::updateStepNR on device-resident state, run for
 // every tissue of the handle with the tissue's own convergence test: w.un
@@ -1189,6 +1211,10 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     const double inner = std::max(rtol, kMixedInnerRtol);
     if (mixed) {
       TF_CHECK(tiled_mixed(k, w.g, w.du, rtol, inner, kMixedCap, 16, method, &res));
+    } else if (method == KRYLOV_GMRES) {
+      s = ensure_gmres(h);
+      if (s != TF_OK) return s;
+      TF_CHECK(tiled_gmres(k, w.g, w.du, rtol, kGmresCap, &res));
     } else if (method == KRYLOV_CG) {
       TF_CHECK(tiled_cg(k, w.g, w.du, rtol, kmax, 16, &res));
     } else {
@@ -1211,19 +1237,29 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
       }
       return any_f;
     };
-    auto retry = [&](const std::vector<int32_t>& f, int fp64_operator) -> tf_status {
+    // the FP64 operator is current (FP64 step, or the mixed step's FP64 re-assembly)
+    bool have_fp64 = !mixed;
+    // stage 0: BiCGStab in the step's precision; 1: BiCGStab, FP64 operator;
+    // 2: GMRES, FP64 operator
+    auto retry = [&](const std::vector<int32_t>& f, int stage) -> tf_status {
       TF_CHECK(hipEventRecord(h->ev[4], h->st));
       TF_CHECK(hipMemcpyAsync(w.tactive, f.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
                               h->st));
-      if (fp64_operator) {
+      if (stage > 0 && !have_fp64) {
         tf_status as = assemble(h, dt, 0);
         if (as != TF_OK) return as;
+        have_fp64 = true;
       }
       KrylovArgs kr = krylov_args(h);
-      if (!fp64_operator && mixed)
+      if (stage == 0 && mixed) {
         TF_CHECK(tiled_mixed(kr, w.g, w.tmp, rtol, inner, kMixedCap, 16, KRYLOV_BICGSTAB, &fres));
-      else
+      } else if (stage == 2) {
+        tf_status gs = ensure_gmres(h);
+        if (gs != TF_OK) return gs;
+        TF_CHECK(tiled_gmres(kr, w.g, w.tmp, rtol, kGmresCap, &fres));
+      } else {
         TF_CHECK(tiled_bicgstab(kr, w.g, w.tmp, rtol, kmax, 16, &fres));
+      }
       TF_CHECK(tiled_copy_active(kr, w.tmp, w.du));
       TF_CHECK(hipMemcpyAsync(w.tactive, act.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
                               h->st));
@@ -1248,6 +1284,12 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
       tf_status rs = retry(fb, 1);
       if (rs != TF_OK) return rs;
       for (int T = 0; T < nt; ++T) S[T].fp64_fallbacks += fb[T];
+      k = krylov_args(h);
+    }
+    if (method != KRYLOV_GMRES && failed(fb)) {
+      tf_status rs = retry(fb, 2);
+      if (rs != TF_OK) return rs;
+      for (int T = 0; T < nt; ++T) S[T].gmres_fallbacks += fb[T];
       k = krylov_args(h);
     }
     TF_CHECK(launch_bind_increment(h->pairs, h->n_pairs, w.du, h->st));
@@ -2119,12 +2161,14 @@ tf_status tf_nr_autotune(tf_handle* h, int32_t precision_mode, int32_t* slices_p
 
 tf_status tf_nr_set_krylov(tf_handle* h, int32_t precision_mode, int32_t method) {
   if (!h || (precision_mode != TF_PREC_FP64 && precision_mode != TF_PREC_MIXED) ||
-      (method != TF_KRYLOV_BICGSTAB && method != TF_KRYLOV_CG)) {
-    set_error("tf_nr_set_krylov: invalid arguments");
+      (method != TF_KRYLOV_BICGSTAB && method != TF_KRYLOV_CG && method != TF_KRYLOV_GMRES) ||
+      (method == TF_KRYLOV_GMRES && precision_mode == TF_PREC_MIXED)) {
+    set_error("tf_nr_set_krylov: invalid arguments (GMRES is FP64 only)");
     return TF_ERR_ARG;
   }
   h->krylov_method[precision_mode == TF_PREC_MIXED ? 1 : 0] =
-      method == TF_KRYLOV_CG ? KRYLOV_CG : KRYLOV_BICGSTAB;
+      method == TF_KRYLOV_CG ? KRYLOV_CG
+                             : (method == TF_KRYLOV_GMRES ? KRYLOV_GMRES : KRYLOV_BICGSTAB);
   return TF_OK;
 }
 

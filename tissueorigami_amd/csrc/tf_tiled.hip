@@ -1162,6 +1162,322 @@ hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int method, int
   return e;
 }
 
+// ---- restarted GMRES(m), FP64, right block-Jacobi preconditioning ----
+// The last resort of the NR loop's solver chain: it needs no positive
+// definiteness and cannot break down short of exact convergence, so it solves
+// the indefinite Jacobians that node-node packing produces (a repulsive
+// central force has a negative tangential stiffness) where BiCGStab breaks
+// down.  Arnoldi with classical Gram-Schmidt applied twice, on the tiles and
+// per-tissue ordered sums of the other solvers; Givens rotations and the small
+// triangular solves of every tissue run on the host.  Partial slots: 0..m the
+// projections (v_i, w), m + 1 the squared norm.
+namespace {
+
+__global__ __launch_bounds__(kBlock) void tg_residual(TC c, const int32_t* gact,
+                                                      const double* __restrict__ val,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ b,
+                                                      double* __restrict__ r, int x_zero,
+                                                      int slot) {
+  TileCtx t;
+  if (!tile_ctx(c, t) || !gact[t.T]) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc[1] = {0.0};
+  for (int sl = t.s0 + wave; sl < t.s1; sl += kW) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (!x_zero) row_spmv<double>(sl, lane, c, val, x, a0, a1, a2);
+    const int row = sl * 64 + lane;
+    if (row < c.n_nodes) {
+      const double r0 = b[3 * row] - a0, r1 = b[3 * row + 1] - a1, r2 = b[3 * row + 2] - a2;
+      r[3 * row] = r0;
+      r[3 * row + 1] = r1;
+      r[3 * row + 2] = r2;
+      acc[0] += r0 * r0 + r1 * r1 + r2 * r2;
+    }
+  }
+  double* outs[1] = {TP(c, slot)};
+  reduce_store(acc, 1, outs, t.L);
+}
+
+// out[k * n_tissues + T] = ordered sum of slot k of tissue T, k < n_slots.
+__global__ __launch_bounds__(kBlock) void tg_tissue_sums(TC c, int k0, int n_slots, double* out) {
+  const int T = blockIdx.x;
+  if (T >= c.tm.n_tissues) return;
+  for (int k = k0; k < k0 + n_slots; ++k) {
+    const double s = tissue_sum(c, k, T);
+    if (threadIdx.x == 0) out[(size_t)k * c.tm.n_tissues + T] = s;
+  }
+}
+
+// out = in * fac[T]
+__global__ __launch_bounds__(kBlock) void tg_scale(TC c, const int32_t* gact, const double* in,
+                                                   const double* fac, double* out) {
+  TileCtx t;
+  if (!tile_ctx(c, t) || !gact[t.T]) return;
+  const double f = fac[t.T];
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[3 * i + d] = in[3 * i + d] * f;
+}
+
+// z = M^-1 v
+__global__ __launch_bounds__(kBlock) void tg_precond(TC c, const int32_t* gact,
+                                                     const double* minv, const double* v,
+                                                     double* z) {
+  TileCtx t;
+  if (!tile_ctx(c, t) || !gact[t.T]) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) minv_apply(minv, v, z, i);
+}
+
+// w = J z
+__global__ __launch_bounds__(kBlock) void tg_spmv(TC c, const int32_t* gact,
+                                                  const double* __restrict__ val,
+                                                  const double* __restrict__ z,
+                                                  double* __restrict__ w) {
+  TileCtx t;
+  if (!tile_ctx(c, t) || !gact[t.T]) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int sl = t.s0 + wave; sl < t.s1; sl += kW) {
+    double a0, a1, a2;
+    row_spmv<double>(sl, lane, c, val, z, a0, a1, a2);
+    const int row = sl * 64 + lane;
+    if (row < c.n_nodes) {
+      w[3 * row] = a0;
+      w[3 * row + 1] = a1;
+      w[3 * row + 2] = a2;
+    }
+  }
+}
+
+// partial (v_i, w) into slot i for i <= j
+__global__ __launch_bounds__(kBlock) void tg_dots(TC c, const int32_t* gact, const double* V,
+                                                  int64_t n3, int j, const double* w) {
+  TileCtx t;
+  if (!tile_ctx(c, t) || !gact[t.T]) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = 0; i <= j; ++i) {
+    const double* v = V + (size_t)i * n3;
+    double acc[1] = {0.0};
+    for (int q = t.s0 * 64 + threadIdx.x; q < end; q += kBlock)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[0] += v[3 * q + d] * w[3 * q + d];
+    double* outs[1] = {TP(c, i)};
+    reduce_store(acc, 1, outs, t.L);
+  }
+}
+
+// w -= sum_{i <= j} h[i, T] v_i; partial |w|^2 into slot norm_slot
+__global__ __launch_bounds__(kBlock) void tg_orth(TC c, const int32_t* gact, const double* V,
+                                                  int64_t n3, int j, const double* h, double* w,
+                                                  int norm_slot) {
+  TileCtx t;
+  if (!tile_ctx(c, t) || !gact[t.T]) return;
+  const int nt = c.tm.n_tissues;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  double acc[1] = {0.0};
+  for (int q = t.s0 * 64 + threadIdx.x; q < end; q += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double a = w[3 * q + d];
+      for (int i = 0; i <= j; ++i) a -= h[(size_t)i * nt + t.T] * V[(size_t)i * n3 + 3 * q + d];
+      w[3 * q + d] = a;
+      acc[0] += a * a;
+    }
+  double* outs[1] = {TP(c, norm_slot)};
+  reduce_store(acc, 1, outs, t.L);
+}
+
+// u = sum_{i < k[T]} y[i, T] v_i, then x += M^-1 u (fused per node)
+__global__ __launch_bounds__(kBlock) void tg_update(TC c, const int32_t* gupd, const double* V,
+                                                    int64_t n3, const int32_t* kT,
+                                                    const double* y, const double* minv,
+                                                    double* x) {
+  TileCtx t;
+  if (!tile_ctx(c, t) || !gupd[t.T]) return;
+  const int nt = c.tm.n_tissues;
+  const int kk = kT[t.T];
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int q = t.s0 * 64 + threadIdx.x; q < end; q += kBlock) {
+    double u[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < kk; ++i) {
+      const double yi = y[(size_t)i * nt + t.T];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) u[d] += yi * V[(size_t)i * n3 + 3 * q + d];
+    }
+    const double* M = minv + 9 * (size_t)q;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      x[3 * q + d] = x[3 * q + d] + (M[3 * d] * u[0] + M[3 * d + 1] * u[1] + M[3 * d + 2] * u[2]);
+  }
+}
+
+}  // namespace
+
+hipError_t tiled_gmres(const KrylovArgs& k, const double* b, double* x, double rtol, int maxit,
+                       TiledResult* res) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  const int G = k.tm.grid;
+  const int nt = k.tm.n_tissues;
+  const int m = kGmresRestart;
+  const int64_t n3 = 3 * (int64_t)k.n_nodes;
+  TC c = make_tc(k, rtol);
+  c.part = ws->gpart;
+  double* Vb = ws->gV;
+  double *r = ws->r, *z = ws->s, *w = ws->v;
+  hipError_t e;
+  // host arrays uploaded asynchronously live until the final synchronisation
+  std::vector<int32_t> act0(nt), gact(nt), kT(nt, 0), it(nt, 0), done(nt, 0), cyc(nt, 0);
+  if ((e = hipMemcpy(act0.data(), ws->tactive, sizeof(int32_t) * nt, hipMemcpyDeviceToHost)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipMemsetAsync(x, 0, sizeof(double) * n3, st)) != hipSuccess) return e;
+  std::vector<double> hs((size_t)(m + 2) * nt), hs2((size_t)(m + 2) * nt), bnorm(nt, 0.0);
+  std::vector<double> fac(nt), y((size_t)m * nt);
+  // per tissue: Hessenberg column-major (m+1) x m, rotations, rhs
+  std::vector<double> H((size_t)nt * (m + 1) * m), cs((size_t)nt * m), sn((size_t)nt * m),
+      g((size_t)nt * (m + 1));
+  auto upload = [&](const void* src, void* dst, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+  };
+  auto tissue_norms = [&](int slot, std::vector<double>& out) -> hipError_t {
+    hipLaunchKernelGGL(tg_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, slot, 1, ws->gsum);
+    hipError_t e2 = hipMemcpyAsync(out.data() + (size_t)slot * nt, ws->gsum + (size_t)slot * nt,
+                                   sizeof(double) * nt, hipMemcpyDeviceToHost, st);
+    if (e2 != hipSuccess) return e2;
+    return hipStreamSynchronize(st);
+  };
+  for (int T = 0; T < nt; ++T) {
+    gact[T] = act0[T];
+    res->state[T] = act0[T] ? K_RUN : K_CONVERGED;
+    res->iters[T] = 0;
+  }
+  for (int cycle = 0;; ++cycle) {
+    // r = b - J x and |r| per tissue
+    if ((e = upload(gact.data(), ws->gact, sizeof(int32_t) * nt)) != hipSuccess) return e;
+    hipLaunchKernelGGL(tg_residual, dim3(G), dim3(kBlock), 0, st, c, ws->gact, k.val, x, b, r,
+                       cycle == 0 ? 1 : 0, m + 1);
+    if ((e = tissue_norms(m + 1, hs)) != hipSuccess) return e;
+    bool any = false;
+    for (int T = 0; T < nt; ++T) {
+      if (!gact[T]) continue;
+      const double beta = sqrt(hs[(size_t)(m + 1) * nt + T]);
+      if (cycle == 0) bnorm[T] = beta;
+      if (!std::isfinite(beta)) {
+        res->state[T] = K_BREAKDOWN;
+        gact[T] = 0;
+      } else if (!(beta > rtol * bnorm[T])) {
+        res->state[T] = K_CONVERGED;
+        gact[T] = 0;
+      } else if (it[T] >= maxit) {
+        res->state[T] = K_RUN;
+        gact[T] = 0;
+      } else {
+        fac[T] = 1.0 / beta;
+        std::fill(g.begin() + (size_t)T * (m + 1), g.begin() + (size_t)(T + 1) * (m + 1), 0.0);
+        g[(size_t)T * (m + 1)] = beta;
+        kT[T] = 0;
+        any = true;
+      }
+      done[T] = !gact[T];
+    }
+    if (!any) break;
+    cyc = gact;  // tissues taking part in this cycle
+    if ((e = upload(gact.data(), ws->gact, sizeof(int32_t) * nt)) != hipSuccess) return e;
+    if ((e = upload(fac.data(), ws->gfac, sizeof(double) * nt)) != hipSuccess) return e;
+    hipLaunchKernelGGL(tg_scale, dim3(G), dim3(kBlock), 0, st, c, ws->gact, r, ws->gfac, Vb);
+    for (int j = 0; j < m; ++j) {
+      double* vj = Vb + (size_t)j * n3;
+      hipLaunchKernelGGL(tg_precond, dim3(G), dim3(kBlock), 0, st, c, ws->gact, k.minv, vj, z);
+      hipLaunchKernelGGL(tg_spmv, dim3(G), dim3(kBlock), 0, st, c, ws->gact, k.val, z, w);
+      // classical Gram-Schmidt, twice
+      for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(tg_dots, dim3(G), dim3(kBlock), 0, st, c, ws->gact, Vb, n3, j, w);
+        double* hdev = ws->gsum + (size_t)pass * (m + 2) * nt;
+        hipLaunchKernelGGL(tg_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, 0, j + 1, hdev);
+        hipLaunchKernelGGL(tg_orth, dim3(G), dim3(kBlock), 0, st, c, ws->gact, Vb, n3, j, hdev,
+                           w, m + 1);
+      }
+      hipLaunchKernelGGL(tg_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, m + 1, 1,
+                         ws->gsum + (size_t)(m + 2) * nt);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      if ((e = hipMemcpyAsync(hs.data(), ws->gsum, sizeof(double) * (m + 2) * nt,
+                              hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
+      if ((e = hipMemcpyAsync(hs2.data(), ws->gsum + (size_t)(m + 2) * nt,
+                              sizeof(double) * (m + 2) * nt, hipMemcpyDeviceToHost, st)) !=
+          hipSuccess)
+        return e;
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      bool running = false;
+      for (int T = 0; T < nt; ++T) {
+        if (!gact[T]) continue;
+        double* Hc = H.data() + ((size_t)T * m + j) * (m + 1);  // column j
+        for (int i = 0; i <= j; ++i) Hc[i] = hs[(size_t)i * nt + T] + hs2[(size_t)i * nt + T];
+        Hc[j + 1] = sqrt(hs2[(size_t)(m + 1) * nt + T]);
+        const double hnext = Hc[j + 1];
+        double* csT = cs.data() + (size_t)T * m;
+        double* snT = sn.data() + (size_t)T * m;
+        double* gT = g.data() + (size_t)T * (m + 1);
+        for (int i = 0; i < j; ++i) {
+          const double a = csT[i] * Hc[i] + snT[i] * Hc[i + 1];
+          Hc[i + 1] = -snT[i] * Hc[i] + csT[i] * Hc[i + 1];
+          Hc[i] = a;
+        }
+        const double den = std::hypot(Hc[j], Hc[j + 1]);
+        csT[j] = den > 0 ? Hc[j] / den : 1.0;
+        snT[j] = den > 0 ? Hc[j + 1] / den : 0.0;
+        Hc[j] = den;
+        Hc[j + 1] = 0.0;
+        gT[j + 1] = -snT[j] * gT[j];
+        gT[j] = csT[j] * gT[j];
+        kT[T] = j + 1;
+        it[T]++;
+        const double est = std::fabs(gT[j + 1]);
+        if (!std::isfinite(est) || !std::isfinite(den)) {
+          res->state[T] = K_BREAKDOWN;
+          gact[T] = 0;
+          cyc[T] = 0;
+          done[T] = 1;
+        } else if (est <= rtol * bnorm[T] || !(hnext > 0.0) || it[T] >= maxit) {
+          gact[T] = 0;  // ends this cycle; the true residual decides at the next
+        } else {
+          fac[T] = 1.0 / hnext;
+          running = true;
+        }
+      }
+      if (!running || j + 1 == m) break;
+      if ((e = upload(gact.data(), ws->gact, sizeof(int32_t) * nt)) != hipSuccess) return e;
+      if ((e = upload(fac.data(), ws->gfac, sizeof(double) * nt)) != hipSuccess) return e;
+      hipLaunchKernelGGL(tg_scale, dim3(G), dim3(kBlock), 0, st, c, ws->gact, w, ws->gfac,
+                         Vb + (size_t)(j + 1) * n3);
+    }
+    // y = H^-1 g per tissue, x += M^-1 V y
+    for (int T = 0; T < nt; ++T) {
+      if (!cyc[T]) continue;
+      const int kk = kT[T];
+      const double* HT = H.data() + (size_t)T * m * (m + 1);
+      const double* gT = g.data() + (size_t)T * (m + 1);
+      for (int i = kk - 1; i >= 0; --i) {
+        double sacc = gT[i];
+        for (int q = i + 1; q < kk; ++q) sacc -= HT[(size_t)q * (m + 1) + i] * y[(size_t)q * nt + T];
+        y[(size_t)i * nt + T] = sacc / HT[(size_t)i * (m + 1) + i];
+      }
+    }
+    if ((e = upload(cyc.data(), ws->gact, sizeof(int32_t) * nt)) != hipSuccess) return e;
+    if ((e = upload(kT.data(), ws->gk, sizeof(int32_t) * nt)) != hipSuccess) return e;
+    if ((e = upload(y.data(), ws->gy, sizeof(double) * m * nt)) != hipSuccess) return e;
+    hipLaunchKernelGGL(tg_update, dim3(G), dim3(kBlock), 0, st, c, ws->gact, Vb, n3, ws->gk,
+                       ws->gy, k.minv, x);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    for (int T = 0; T < nt; ++T) gact[T] = cyc[T];
+  }
+  for (int T = 0; T < nt; ++T) res->iters[T] = it[T];
+  return hipStreamSynchronize(st);
+}
+
 void tiled_release(Workspace* ws) {
   GraphCache* gc = static_cast<GraphCache*>(ws->graphs);
   if (!gc) return;

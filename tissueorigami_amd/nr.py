@@ -38,7 +38,8 @@ class StepStats(ctypes.Structure):
                 ("du_norms", ctypes.c_double * MAX_NR),
                 ("ms_element", ctypes.c_double), ("ms_assembly", ctypes.c_double),
                 ("ms_krylov", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("fp64_fallbacks", ctypes.c_int32), ("cg_fallbacks", ctypes.c_int32)]
+                ("fp64_fallbacks", ctypes.c_int32), ("cg_fallbacks", ctypes.c_int32),
+                ("gmres_fallbacks", ctypes.c_int32)]
 
     def as_dict(self):
         n = self.nr_iterations
@@ -48,7 +49,7 @@ class StepStats(ctypes.Structure):
                     du_norms=list(self.du_norms[:n]), ms_element=self.ms_element,
                     ms_assembly=self.ms_assembly, ms_krylov=self.ms_krylov,
                     ms_total=self.ms_total, fp64_fallbacks=self.fp64_fallbacks,
-                    cg_fallbacks=self.cg_fallbacks)
+                    cg_fallbacks=self.cg_fallbacks, gmres_fallbacks=self.gmres_fallbacks)
 
 
 class KernelTimes(ctypes.Structure):
@@ -328,6 +329,12 @@ class NewtonRaphsonSolver:
         _check(load().tf_nr_detect_packing(self._h, _p(u, _d), _p(n, _i)))
         return int(n[0])
 
+    def packing_count(self):
+        """Couples of the last detection."""
+        n = np.zeros(1, np.int32)
+        _check(load().tf_nr_get_packing(self._h, _p(n, _i), None, None, None))
+        return int(n[0])
+
     def packing(self):
         """(couples n x 2, forces n x 6 of the last residual, packing threshold,
         squared detection thresholds 10 x 5, bounding box 6)."""
@@ -353,9 +360,9 @@ class NewtonRaphsonSolver:
         _check(load().tf_nr_set_tile_size(self._h, int(slices_per_tile)))
 
     def set_krylov(self, precision, method):
-        """Krylov method ("bicgstab" or "cg") of one precision mode."""
+        """Krylov method ("bicgstab", "cg", or "gmres" for FP64) of one precision mode."""
         _check(load().tf_nr_set_krylov(self._h, PREC[precision],
-                                       {"bicgstab": 0, "cg": 1}[method]))
+                                       {"bicgstab": 0, "cg": 1, "gmres": 2}[method]))
 
     def set_coarse(self, cell_size=-1.0):
         """Two-level preconditioner of the mixed CG solve: aggregate cell size

@@ -117,3 +117,35 @@ def growth_steps(rate_per_hour, dt, n_steps, inv3=None):
         dinv[s, :, 0] = d
         dinv[s, :, 4] = d
     return inv, dinv
+
+
+def contact_workload(name="c2", gap=0.6, seed=2024):
+    """Two copies of a workload's ellipse facing each other apical to apical
+    (tissue.facing_slabs), `gap` um apart, with their basal surfaces clamped
+    (every DOF of basal nodes fixed), the workload's materials and drag, and
+    growth from the workload's random field over both copies.  Growth swells
+    the apical layers into each other; node-node packing
+    (Simulation::detectPacingNodes, Simulation.cpp:7997-8078) pushes them
+    apart."""
+    c = CONFIGS[name]
+    m1 = tissue.ellipse(c["a"], c["b"], c["side"], c["thickness"], c["layers"])
+    m = tissue.facing_slabs(m1, gap)
+    ne, nn = len(m["conn"]), len(m["nodes"])
+    top = m["layer"] == m1["n_layers"] - 1
+    E = np.where(top, c["E_ap"], c["E_other"])
+    ext = np.zeros((nn, 3))
+    ext[m["node_placement"] == 1] = 16000.0
+    ext[m["node_placement"] == 0] = 10.0
+    fixed = np.zeros((nn, 3), np.uint8)
+    fixed[m["node_placement"] == 0, :] = 1
+    cent = m["nodes"][m["conn"]].mean(1)[:, :2]
+    rate = tissue.gaussian_random_field(cent, 0.02, 0.01, 15.0, seed)
+    place, spans = tissue.element_placement(m["conn"], m["node_placement"])
+    etype = tissue.element_type(m["conn"], m["node_type"])
+    rates = np.zeros((ne, 3))
+    rates[:, 0] = rates[:, 1] = rate / 3600.0
+    return dict(mesh=m, E=E, nu=np.full(ne, c["nu"]), eta=np.full(ne, c["eta"]), ext=ext,
+                fixed=fixed, rates=rates, dt=c["dt"], place=place, spans=spans, etype=etype,
+                cfg=c, seed=seed, gap=gap,
+                desc=f"two {name} ellipses facing apical to apical, {gap} um apart, basal "
+                     "surfaces clamped")
