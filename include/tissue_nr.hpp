@@ -49,6 +49,8 @@ struct Node {
   double viscositySurface = 0.0;
   int tissuePlacement = 2;
   int tissueType = 0;
+  double mass = 0.0;                   // Node::mass (Simulation::updateNodeMasses)
+  bool hasLateralElementOwner = false;
 };
 
 // Element state read by the NR step (ShapeBase.h:53-69, 129-143).
@@ -60,6 +62,8 @@ class ShapeBase {
   double E = 10.0, v = 0.3, internalViscosity = 0.0;
   double InvFg[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   bool IsAblated = false;
+  int tissuePlacement = 2, tissueType = 0;  // ShapeBase.cpp:905-1037
+  bool spansWholeTissue = false;
   int getNodeNumber() const { return 6; }
   const int* getNodeIds() const { return NodeIds.data(); }
   // ShapeBase::updatePositions (ShapeBase.cpp:3279-3285): element positions
@@ -209,11 +213,51 @@ class NewtonRaphsonSolver {
     check(tf_nr_set_visc_surface(h, A.data()));
   }
 
+  // Node-node packing: Simulation::detectPacingNodes (Simulation.cpp:7997-8078)
+  // at the nodes' current positions, with their masses and placements and the
+  // elements' placement and type.  The couples found hold for the following
+  // updateStepNR calls, whose NR iterations then add the couples' forces and
+  // Jacobian blocks (Simulation.cpp:6680-6698).  Returns the number of couples.
+  int detectPacingNodes(const std::vector<std::unique_ptr<Node>>& Nodes,
+                        const std::vector<std::unique_ptr<ShapeBase>>& Elements,
+                        bool symmetricY = false) {
+    bind(Nodes, Elements);
+    const int ne = (int)Elements.size();
+    std::vector<int32_t> place(ne), type(ne);
+    std::vector<uint8_t> spans(ne);
+    for (int e = 0; e < ne; ++e) {
+      place[e] = Elements[e]->tissuePlacement;
+      type[e] = Elements[e]->tissueType;
+      spans[e] = Elements[e]->spansWholeTissue ? 1 : 0;
+    }
+    check(tf_nr_set_surface_flags(h, place.data(), type.data(), spans.data()));
+    if (!packingOn) {
+      std::vector<int32_t> npl(nNodes);
+      std::vector<uint8_t> ex(nNodes);
+      for (int i = 0; i < nNodes; ++i) {
+        npl[i] = Nodes[i]->tissuePlacement;
+        ex[i] = Nodes[i]->hasLateralElementOwner ? 1 : 0;
+      }
+      check(tf_nr_set_packing(h, npl.data(), ex.data(), symmetricY ? 1 : 0, 2));
+      packingOn = true;
+    }
+    std::vector<double> m(nNodes), u(3 * (size_t)nNodes);
+    for (int i = 0; i < nNodes; ++i) {
+      m[i] = Nodes[i]->mass;
+      for (int j = 0; j < 3; ++j) u[3 * i + j] = Nodes[i]->Position[j];
+    }
+    check(tf_nr_set_node_mass(h, m.data()));
+    int32_t n = 0;
+    check(tf_nr_detect_packing(h, u.data(), &n));
+    return n;
+  }
+
   tf_handle* handle() const { return h; }
 
  private:
   int nDim, nNodes;
   tf_handle* h = nullptr;
+  bool packingOn = false;
 };
 
 }  // namespace tissue_nr
