@@ -425,7 +425,11 @@ def main():
                     help="Krylov method of the FP32 inner solves (FP64 uses BiCGStab)")
     ap.add_argument("--coarse-cell", type=float, default=-1.0,
                     help="aggregate cell size of the two-level CG preconditioner (um; negative: "
-                         "automatic, about 400 aggregates; 0: block Jacobi only)")
+                         "automatic, about 280 cells; 0: block Jacobi only)")
+    ap.add_argument("--coarse-bands", type=int, default=3,
+                    help="bands through the thickness per aggregate cell of the coarse space")
+    ap.add_argument("--coarse-every-iteration", action="store_true",
+                    help="form the coarse inverse at every NR iteration (default: once per step)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -449,6 +453,8 @@ def main():
         solver.set_graphs(False)
     solver.set_krylov("mixed", args.mixed_krylov)
     solver.set_coarse(args.coarse_cell)
+    solver.set_coarse_bands(args.coarse_bands)
+    solver.set_coarse_lag(not args.coarse_every_iteration)
     n_blocks, n_slots = solver.pattern_info()
     print(f"[bench] setup {setup_s:.2f} s, {n_blocks} blocks, {n_slots} SELL slots",
           file=sys.stderr, flush=True)
@@ -500,8 +506,11 @@ def main():
                          krylov_rtol_mixed=args.mixed_rtol,
                          precond_fp64="node-block Jacobi",
                          precond_mixed=("node-block Jacobi + coarse correction (rigid-body modes "
-                                        f"of {n_agg} column aggregates)" if coarse_on
-                                        else "node-block Jacobi"),
+                                        f"of {n_agg} aggregates: (x, y) cells x "
+                                        f"{args.coarse_bands} thickness bands; inverse formed "
+                                        + ("every NR iteration)" if args.coarse_every_iteration
+                                           else "once per time step)")
+                                        if coarse_on else "node-block Jacobi"),
                          krylov_mixed=args.mixed_krylov, krylov_fp64="bicgstab",
                          growth_seed=w["seed"], setup_s=setup_s, slices_per_tile=spt)
     # roofline of the dominant kernel: the FP32 SpMV v = J ph of the mixed
@@ -551,8 +560,10 @@ def main():
             it_bytes = bicgstab_iteration_bytes(n_blocks, nn, sz)
         bnr = assembly_bytes(ne, n_blocks, nn, sz, visc, combined=key == "mixed") + k_per * it_bytes
         if nco:
-            # two Newton-Schulz steps (4 GEMMs) and a residual product per assembly
-            bnr += coarse_setup_bytes(n_blocks, nco, 5)
+            # two Newton-Schulz steps (4 GEMMs) and a residual product per coarse setup:
+            # one per time step (lagged) or one per assembly
+            per_nr = 1.0 if args.coarse_every_iteration else args.steps / t["nr_iterations"]
+            bnr += coarse_setup_bytes(n_blocks, nco, 5) * per_nr
         secs = (t["ms_element"] + t["ms_assembly"] + t["ms_krylov"]) / 1e3 / t["nr_iterations"]
         res.setdefault("roofline_nr_iteration", {})[key] = dict(
             algorithmic_bytes=bnr, krylov_iterations=k_per, device_s=secs,

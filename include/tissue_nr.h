@@ -345,14 +345,26 @@ tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable);
  * handles; no reference counterpart).  Block Jacobi is completed by a coarse
  * correction P A_c^-1 P^T r, where P holds the rigid-body modes (three
  * translations, three rotations) of node aggregates: the nodes whose
- * reference (x, y) lies in one square cell of side cell_size, through all
- * layers.  A_c = P^T J P is formed on the device and inverted (FP64) at every
- * mixed assembly.  cell_size < 0: automatic (about 400 aggregates over the
- * tissue's (x, y) extent; the default); 0: off (block Jacobi only).  The
+ * reference (x, y) lies in one square cell of side cell_size and whose
+ * reference z lies in one of z_bands equal bands of the thickness (default
+ * 3, tf_nr_set_coarse_bands).  A_c = P^T J P is formed on the device and
+ * inverted (FP64) at the first NR iteration of every mixed step
+ * (tf_nr_set_coarse_lag).  cell_size < 0: automatic (about 280 cells over the
+ * tissue's (x, y) bounding box; the default); 0: off (block Jacobi only).  The
  * converged positions do not depend on it, only the Krylov iteration count.
  * tf_nr_coarse_info reports the aggregates of the last build and whether the
  * last mixed solve used the correction. */
 tf_status tf_nr_set_coarse(tf_handle* h, double cell_size);
+/* Splits every aggregate cell into z_bands equal bands of the reference
+ * thickness (default 3; 1: whole columns), so the coarse space also holds
+ * through-thickness modes.  Handles with slave/master binding keep whole
+ * columns (and their automatic cell gives about 400 cells).  No reference
+ * counterpart. */
+tf_status tf_nr_set_coarse_bands(tf_handle* h, int32_t z_bands);
+/* lag != 0 (the default): A_c and its inverse are formed at the first NR
+ * iteration of a step and kept for the step's later iterations; 0: at every
+ * mixed assembly.  No reference counterpart. */
+tf_status tf_nr_set_coarse_lag(tf_handle* h, int32_t lag);
 tf_status tf_nr_coarse_info(tf_handle* h, int32_t* aggregates, int32_t* active);
 
 /* Size of the Jacobian pattern: stored 3x3 blocks (n_blocks, the structural

@@ -339,26 +339,31 @@ void coarse_release(CoarseOp& c) {
 }
 
 hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double cell,
-                        const std::vector<int32_t>& slice_base, const std::vector<int32_t>& col,
-                        const std::vector<int32_t>& row_len) {
+                        int z_bands, const std::vector<int32_t>& slice_base,
+                        const std::vector<int32_t>& col, const std::vector<int32_t>& row_len) {
   void* blas = c.blas;
   const int32_t gen = c.gen;
   c.blas = nullptr;
   coarse_release(c);
   c.blas = blas;
   c.gen = gen;
-  if (n_nodes <= 0 || !(cell > 0.0)) return hipErrorInvalidValue;
-  // square cells over the reference (x, y)
-  double x0 = X[0], y0 = X[1];
+  if (n_nodes <= 0 || !(cell > 0.0) || z_bands < 1) return hipErrorInvalidValue;
+  // square cells over the reference (x, y), each split into z_bands equal bands
+  // of the reference z range
+  double x0 = X[0], y0 = X[1], z0 = X[2], z1 = X[2];
   for (int i = 0; i < n_nodes; ++i) {
     x0 = std::min(x0, X[3 * i]);
     y0 = std::min(y0, X[3 * i + 1]);
+    z0 = std::min(z0, X[3 * i + 2]);
+    z1 = std::max(z1, X[3 * i + 2]);
   }
+  const double zb = z1 > z0 ? (z1 - z0) / z_bands : 1.0;
   std::vector<int64_t> key(n_nodes);
   for (int i = 0; i < n_nodes; ++i) {
     const int64_t ix = (int64_t)std::floor((X[3 * i] - x0) / cell);
     const int64_t iy = (int64_t)std::floor((X[3 * i + 1] - y0) / cell);
-    key[i] = (ix << 32) | iy;
+    const int64_t iz = std::min<int64_t>(z_bands - 1, (int64_t)std::floor((X[3 * i + 2] - z0) / zb));
+    key[i] = (ix << 40) | (iy << 16) | iz;
   }
   std::vector<int64_t> keys(key);
   std::sort(keys.begin(), keys.end());
@@ -390,7 +395,8 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double ce
       int bq = big[0];
       for (int b : big) {
         const double dx = cen[3 * q] - cen[3 * b], dy = cen[3 * q + 1] - cen[3 * b + 1];
-        const double d2 = dx * dx + dy * dy;
+        const double dz = cen[3 * q + 2] - cen[3 * b + 2];
+        const double d2 = dx * dx + dy * dy + dz * dz;
         if (d2 < best) { best = d2; bq = b; }
       }
       target[q] = bq;

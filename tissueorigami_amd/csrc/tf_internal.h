@@ -458,10 +458,11 @@ hipError_t launch_transpose_operator(int64_t n_slots, const int32_t* tslot, cons
 // ---- two-level preconditioner (tf_coarse.hip) ----
 // Aggregates and block lists for an operator pattern (host arrays of the
 // pattern: slice_base, col, row_len) and the node reference coordinates X
-// (3 per node); cell is the aggregate cell size (same units as X).
+// (3 per node); cell is the aggregate cell size (same units as X), and every
+// (x, y) cell is split into z_bands bands of the reference thickness.
 hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double cell,
-                        const std::vector<int32_t>& slice_base, const std::vector<int32_t>& col,
-                        const std::vector<int32_t>& row_len);
+                        int z_bands, const std::vector<int32_t>& slice_base,
+                        const std::vector<int32_t>& col, const std::vector<int32_t>& row_len);
 void coarse_release(CoarseOp& c);
 // A_c of the FP32 operator (col / val32 of the pattern the lists were built
 // for), its FP64 inverse and the FP32 copy; c.active reports success.

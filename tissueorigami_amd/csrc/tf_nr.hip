@@ -63,9 +63,12 @@ struct tf_handle {
   BoundOp bound;
   std::vector<void*> bound_allocs;
   // two-level preconditioner of the FP32 CG solve (single-tissue handles):
-  // cell size of the aggregates (< 0: automatic, about 400 aggregates; 0: off)
+  // cell size of the aggregates (< 0: automatic, about 280 cells over the (x, y)
+  // bounding box; 0: off)
   CoarseOp coarse;
   double coarse_cell = -1.0;
+  int32_t coarse_bands = 3;  // bands through the thickness per (x, y) cell
+  bool coarse_lag = true;    // coarse inverse of a step's first NR iteration kept for the step
   std::vector<double> h_X;  // node reference coordinates (first prism holding the node)
   std::vector<int32_t> bound_pairs;  // pairs the bound operator was built for
   bool any_visc = false, have_material = false, have_growth = false, have_area = false;
@@ -788,29 +791,41 @@ KrylovArgs krylov_args(tf_handle* h) {
 
 // The coarse space of the FP32 CG preconditioner for the operator just
 // assembled (mixed precision, one tissue): aggregates and block lists when
-// the operator's pattern changed, then A_c and its inverse.
-tf_status coarse_prepare(tf_handle* h) {
+// the operator's pattern changed, then A_c and its inverse.  With lagging on
+// (the default), A_c and its inverse are formed at the first NR iteration of
+// a step only and kept for the step's later iterations: the preconditioner
+// stays a fixed SPD operator within every solve, and a step repeated from the
+// same state repeats its bits.
+tf_status coarse_prepare(tf_handle* h, int nr_iteration) {
   CoarseOp& c = h->coarse;
+  const bool bound = h->bound.ready;
+  if (h->coarse_lag && nr_iteration > 0 && c.active && c.built && c.for_bound == bound)
+    return TF_OK;
   c.active = false;
   if (h->batch || h->coarse_cell == 0.0) return TF_OK;
-  const bool bound = h->bound.ready;
   if (!c.built || c.for_bound != bound) {
     double cell = h->coarse_cell;
-    if (cell < 0.0) {  // automatic: about 400 aggregates over the (x, y) bounding box
+    if (cell < 0.0) {  // automatic: about 280 cells over the (x, y) bounding box
       double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
       for (int i = 0; i < h->Nn; ++i) {
         x0 = std::min(x0, h->h_X[3 * i]); x1 = std::max(x1, h->h_X[3 * i]);
         y0 = std::min(y0, h->h_X[3 * i + 1]); y1 = std::max(y1, h->h_X[3 * i + 1]);
       }
-      cell = std::sqrt(std::max((x1 - x0) * (y1 - y0), 1e-30) / 400.0);
+      // bound systems: the whole-column aggregates of about 400 cells, whose
+      // coarse operator stays well conditioned under circumference binding
+      cell = std::sqrt(std::max((x1 - x0) * (y1 - y0), 1e-30) / (bound ? 400.0 : 280.0));
     }
     TF_CHECK(hipStreamSynchronize(h->st));
     tiled_release(&h->ws);  // captured chunks hold the previous coarse buffers
     const BoundOp& B = h->bound;
-    hipError_t e = bound ? coarse_build(c, h->Nn, h->h_X.data(), cell, B.h_slice_base, B.h_col,
-                                        B.h_row_len)
-                         : coarse_build(c, h->Nn, h->h_X.data(), cell, h->h_slice_base, h->h_col,
-                                        h->h_row_len);
+    // bound systems keep whole columns: binding ties a column's nodes to one
+    // master (circumference binding slaves x and y of every node above the
+    // basal one), which would leave the upper bands of such columns without
+    // the modes they are built from
+    hipError_t e = bound ? coarse_build(c, h->Nn, h->h_X.data(), cell, 1, B.h_slice_base,
+                                        B.h_col, B.h_row_len)
+                         : coarse_build(c, h->Nn, h->h_X.data(), cell, h->coarse_bands,
+                                        h->h_slice_base, h->h_col, h->h_row_len);
     if (e == hipErrorInvalidValue) {
       set_error("coarse space: cell size gives too many aggregates");
       return TF_ERR_ARG;
@@ -1202,7 +1217,7 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     if (s != TF_OK) return s;
     const int method = h->krylov_method[mixed];
     if (mixed && method == KRYLOV_CG) {
-      s = coarse_prepare(h);
+      s = coarse_prepare(h, it[0]);
       if (s != TF_OK) return s;
     } else {
       h->coarse.active = false;
@@ -2267,6 +2282,27 @@ tf_status tf_nr_set_coarse(tf_handle* h, double cell_size) {
   }
   TF_CHECK(hipStreamSynchronize(h->st));
   h->coarse_cell = cell_size < 0.0 ? -1.0 : cell_size;
+  tiled_release(&h->ws);
+  coarse_release(h->coarse);
+  return TF_OK;
+}
+
+tf_status tf_nr_set_coarse_lag(tf_handle* h, int32_t lag) {
+  if (!h) {
+    set_error("tf_nr_set_coarse_lag: invalid handle");
+    return TF_ERR_ARG;
+  }
+  h->coarse_lag = lag != 0;
+  return TF_OK;
+}
+
+tf_status tf_nr_set_coarse_bands(tf_handle* h, int32_t z_bands) {
+  if (!h || z_bands < 1 || z_bands > 64) {
+    set_error("tf_nr_set_coarse_bands: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  h->coarse_bands = z_bands;
   tiled_release(&h->ws);
   coarse_release(h->coarse);
   return TF_OK;
