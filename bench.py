@@ -425,7 +425,7 @@ def main():
                     help="Krylov method of the FP32 inner solves (FP64 uses BiCGStab)")
     ap.add_argument("--coarse-cell", type=float, default=-1.0,
                     help="aggregate cell size of the two-level CG preconditioner (um; negative: "
-                         "automatic, about 280 cells; 0: block Jacobi only)")
+                         "automatic, about sqrt(blocks) coarse DOF; 0: block Jacobi only)")
     ap.add_argument("--coarse-bands", type=int, default=3,
                     help="bands through the thickness per aggregate cell of the coarse space")
     ap.add_argument("--coarse-every-iteration", action="store_true",

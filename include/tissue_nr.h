@@ -349,8 +349,9 @@ tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable);
  * reference z lies in one of z_bands equal bands of the thickness (default
  * 3, tf_nr_set_coarse_bands).  A_c = P^T J P is formed on the device and
  * inverted (FP64) at the first NR iteration of every mixed step
- * (tf_nr_set_coarse_lag).  cell_size < 0: automatic (about 280 cells over the
- * tissue's (x, y) bounding box; the default); 0: off (block Jacobi only).  The
+ * (tf_nr_set_coarse_lag).  cell_size < 0: automatic (the default): about
+ * sqrt(stored blocks) coarse DOF, so that the dense coarse product stays near
+ * a tenth of an FP32 SpMV; 0: off (block Jacobi only).  The
  * converged positions do not depend on it, only the Krylov iteration count.
  * tf_nr_coarse_info reports the aggregates of the last build and whether the
  * last mixed solve used the correction. */
@@ -358,8 +359,8 @@ tf_status tf_nr_set_coarse(tf_handle* h, double cell_size);
 /* Splits every aggregate cell into z_bands equal bands of the reference
  * thickness (default 3; 1: whole columns), so the coarse space also holds
  * through-thickness modes.  Handles with slave/master binding keep whole
- * columns (and their automatic cell gives about 400 cells).  No reference
- * counterpart. */
+ * columns (and their automatic cell gives at most about 400 cells).  No
+ * reference counterpart. */
 tf_status tf_nr_set_coarse_bands(tf_handle* h, int32_t z_bands);
 /* lag != 0 (the default): A_c and its inverse are formed at the first NR
  * iteration of a step and kept for the step's later iterations; 0: at every

@@ -63,8 +63,8 @@ struct tf_handle {
   BoundOp bound;
   std::vector<void*> bound_allocs;
   // two-level preconditioner of the FP32 CG solve (single-tissue handles):
-  // cell size of the aggregates (< 0: automatic, about 280 cells over the (x, y)
-  // bounding box; 0: off)
+  // cell size of the aggregates (< 0: automatic, sized from the stored blocks;
+  // 0: off)
   CoarseOp coarse;
   double coarse_cell = -1.0;
   int32_t coarse_bands = 3;  // bands through the thickness per (x, y) cell
@@ -805,15 +805,21 @@ tf_status coarse_prepare(tf_handle* h, int nr_iteration) {
   if (h->batch || h->coarse_cell == 0.0) return TF_OK;
   if (!c.built || c.for_bound != bound) {
     double cell = h->coarse_cell;
-    if (cell < 0.0) {  // automatic: about 280 cells over the (x, y) bounding box
+    if (cell < 0.0) {
+      // automatic: n_c = 6 x aggregates close to sqrt(stored blocks), so that the
+      // dense coarse product (4 n_c^2 bytes) stays near a tenth of an FP32 SpMV
+      // (40 bytes per block); the (x, y) bounding box is cut into n_c / (6 bands)
+      // square cells.  Bound systems keep whole columns and at most about 400
+      // cells, whose coarse operator stays well conditioned under
+      // circumference binding.
       double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
       for (int i = 0; i < h->Nn; ++i) {
         x0 = std::min(x0, h->h_X[3 * i]); x1 = std::max(x1, h->h_X[3 * i]);
         y0 = std::min(y0, h->h_X[3 * i + 1]); y1 = std::max(y1, h->h_X[3 * i + 1]);
       }
-      // bound systems: the whole-column aggregates of about 400 cells, whose
-      // coarse operator stays well conditioned under circumference binding
-      cell = std::sqrt(std::max((x1 - x0) * (y1 - y0), 1e-30) / (bound ? 400.0 : 280.0));
+      const double nc = std::sqrt((double)h->h_nbr.size());
+      const double cells = bound ? std::min(400.0, nc / 6.0) : nc / (6.0 * h->coarse_bands);
+      cell = std::sqrt(std::max((x1 - x0) * (y1 - y0), 1e-30) / std::max(cells, 1.0));
     }
     TF_CHECK(hipStreamSynchronize(h->st));
     tiled_release(&h->ws);  // captured chunks hold the previous coarse buffers
