@@ -1,8 +1,9 @@
 """Batched tissues (tf_batch_create): one launch per solver stage for many
 independent meshes of uneven size.  Each tissue keeps its own Krylov scalars,
 convergence flag and NR test, so a tissue stepped in a batch must give the
-result of the same tissue stepped alone with the same tile size, bit for bit
-(alone with block Jacobi too: batches have no coarse correction).
+result of the same tissue stepped alone with the same tile size, bit for bit.
+That holds for the two-level CG of the mixed step too: every tissue of a
+batch has a coarse space of its own, the one it has alone.
 Every test here needs an MI355X."""
 import numpy as np
 import pytest
@@ -34,7 +35,6 @@ def _growth_inputs(orc, c, dt):
 def _alone_physical(nr, orc, c, dt, spt):
     m = c["mesh"]
     s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
-    s.set_coarse(0)  # batches solve with block Jacobi only
     s.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
     s.set_growth(_growth_inputs(orc, c, dt))
     s.update_visc_surface(m["nodes"].reshape(-1), c["place"], c["etype"], c["spans"])
@@ -89,7 +89,6 @@ HARD = [dict(seed=11, lx=4.0, ly=3.0, layers=2, visc=True, fixed=False),
 def _alone(nr, p, spt):
     m = p["mesh"]
     s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], p["refpos"])
-    s.set_coarse(0)
     s.set_material(p["E"], p["nu"], p["eta"], p["ext_visc"], p["fixed"])
     s.set_growth(p["invFg"])
     s.set_visc_surface(p["area"])
@@ -163,7 +162,6 @@ def test_batch_advance_is_bitwise_each_tissue_alone(nr):
     for t, c in enumerate(cfgs):
         m = c["m"]
         a = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
-        a.set_coarse(0)
         a.set_material(c["E"], c["nu"], c["eta"], c["ext"])
         a.set_growth_rates(c["rates"])
         a.set_surface_flags(c["place"], c["etype"], c["spans"])
@@ -175,6 +173,28 @@ def test_batch_advance_is_bitwise_each_tissue_alone(nr):
         assert np.array_equal(parts[t].reshape(-1), ua), t
         a.close()
     b.close()
+
+
+def test_batch_coarse_space_per_tissue(orc, nr):
+    # the per-tissue coarse correction is on by default and cuts the CG
+    # iterations of the mixed step against block Jacobi alone
+    cs = _ellipses()
+    dt = 60.0
+    its = {}
+    for cell in (-1.0, 0.0):
+        b, u0 = _batch_physical(nr, orc, cs, dt, 8)
+        b.set_coarse(cell)
+        ub, stats = b.step(u0, dt, 1e-8, 20, "mixed", 1e-5)
+        agg, active = b.coarse_info()
+        assert all(s["converged"] for s in stats)
+        if cell < 0:
+            assert agg >= len(cs) and active == 1
+        else:
+            assert agg == 0 and active == 0
+        its[cell] = [s["krylov_iterations"] for s in stats]
+        b.close()
+    big = int(np.argmax([len(c["mesh"]["nodes"]) for c in cs]))
+    assert its[-1.0][big] < its[0.0][big], its
 
 
 def test_autotune_picks_a_candidate_and_keeps_parity(orc, nr):

@@ -7,24 +7,27 @@
 // iterations per NR iteration at 1.6 M prisms).  The coarse space P holds,
 // for every aggregate of nodes, the six rigid-body modes of the aggregate
 // (three translations, three rotations about its centroid); an aggregate is
-// the set of nodes whose reference (x, y) lies in one square cell, through
-// all layers of the tissue.  Fixed and slave DOF have zero rows in P, so the
-// correction never moves them.
+// the set of nodes of one tissue whose reference (x, y) lies in one square
+// cell and whose reference z lies in one band of the tissue's thickness.
+// Fixed and slave DOF have zero rows in P, so the correction never moves them.
 //
-// Per assembly: A_c is formed block by block from the FP32 operator (one
+// Every tissue of a handle is a segment with its own aggregates, so A_c is
+// block-diagonal: one dense block, inverse and Newton-Schulz state per
+// segment, and a tissue's correction does not depend on the other tissues.
+//
+// Per setup: A_c is formed block by block from the FP32 operator (one
 // workgroup per aggregate pair, a fixed summation order).  A reference
-// inverse X_ref comes from rocSOLVER (getrf, getri) on the first assembly;
-// the operator changes little from one assembly to the next, so every
-// assembly starts from X = X_ref and takes Newton-Schulz steps
-// X <- X (2I - A_c X) (two rocBLAS GEMMs each, quadratic convergence) until
-// max|I - A_c X| < 1e-5 (a preconditioner needs no more).  When the steps do
-// not converge (residual above 0.25, or seven steps), X_ref is refactorised
-// from the current A_c.  The inverse is kept as a dense
-// FP32 matrix.  Per CG iteration: w = P^T r (one workgroup per aggregate),
-// y = A_c^-1 w (one wavefront per row), z += P y (one thread per node), and
-// the partial products (w, y) that complete (r, z).  Every sum has a fixed
-// order and rocBLAS runs with atomics off, so the solve stays bitwise
-// reproducible.
+// inverse X_ref comes from rocSOLVER (getrf, getri) on the first setup; the
+// operator changes little from one setup to the next, so every setup starts
+// from X = X_ref and takes Newton-Schulz steps X <- X (2I - A_c X) (two
+// rocBLAS GEMMs each, quadratic convergence) until max|I - A_c X| < 1e-5 (a
+// preconditioner needs no more).  When the steps do not converge (residual
+// above 0.25, or seven steps), X_ref is refactorised from the current A_c.
+// The inverse is kept as a dense FP32 matrix.  Per CG iteration: w = P^T r
+// (kSplit workgroups per aggregate), y = A_c^-1 w (one wavefront per row),
+// z += P y (one thread per node), and the partial products (w, y) that
+// complete (r, z).  Every sum has a fixed order and rocBLAS runs with atomics
+// off, so the solve stays bitwise reproducible.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -39,7 +42,7 @@ namespace tf {
 namespace {
 
 constexpr int kMinAggNodes = 16;   // smaller cells are merged into the nearest larger one
-constexpr int kMaxAggregates = 8192;
+constexpr int kMaxAggregates = 8192;  // per segment
 constexpr int kGemvRows = kBlock / 64;  // rows of A_c^-1 per product workgroup
 constexpr int kSplit = 4;               // workgroups per aggregate in the restriction
 
@@ -140,14 +143,18 @@ __global__ __launch_bounds__(kBlock) void coarse_fill_kernel(double* a, int64_t 
     a[i] = v;
 }
 
-// Pair blocks into the dense column-major A_c (each pair owns its 6x6 block).
-__global__ void coarse_scatter_kernel(int32_t n_pairs, int32_t n_c, const int32_t* cp_IK,
-                                      const double* blk, double* dense) {
+// Pair blocks into the dense column-major A_c block of their segment (each
+// pair owns its 6x6 block; both aggregates of a pair lie in one segment).
+__global__ void coarse_scatter_kernel(int32_t n_pairs, const int32_t* cp_IK,
+                                      const int32_t* agg_tissue, const int32_t* segd,
+                                      const int64_t* segm, const double* blk, double* dense) {
   const int pr = blockIdx.x, q = threadIdx.x;
   if (pr >= n_pairs || q >= 36) return;
   const int I = cp_IK[2 * pr], K = cp_IK[2 * pr + 1];
+  const int T = agg_tissue[I];
+  const int a0 = segd[2 * T], nc = segd[2 * T + 1];
   const int a = q / 6, b = q % 6;
-  dense[(size_t)(6 * K + b) * n_c + 6 * I + a] = blk[36 * (size_t)pr + q];
+  dense[segm[T] + (int64_t)(6 * (K - a0) + b) * nc + 6 * (I - a0) + a] = blk[36 * (size_t)pr + q];
 }
 
 // A mode with no free DOF (all its rows masked) has a zero row and column:
@@ -214,13 +221,16 @@ __global__ __launch_bounds__(kBlock) void coarse_twoI_kernel(int32_t n_c, double
 // part of the aggregate's node list; the offsets and masks are stored in list
 // order, only r is gathered.
 __global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
-    int32_t n_agg, int32_t ld, const int32_t* __restrict__ agg_ptr,
-    const int32_t* __restrict__ agg_nodes, const float* __restrict__ list_d,
-    const uint8_t* __restrict__ list_mask, const float* __restrict__ r,
-    const int32_t* __restrict__ state, float* __restrict__ w) {
+    int32_t n_agg, int32_t vlen, const int32_t* __restrict__ agg_ptr,
+    const int32_t* __restrict__ agg_nodes, const int32_t* __restrict__ agg_tissue,
+    const int32_t* __restrict__ agg_cbase, const int32_t* __restrict__ seg_on,
+    const float* __restrict__ list_d, const uint8_t* __restrict__ list_mask,
+    const float* __restrict__ r, const int32_t* __restrict__ state, float* __restrict__ w) {
   __shared__ double red[6][kBlock / 64];
   const int a = blockIdx.x / kSplit, part = blockIdx.x % kSplit;
-  if (a >= n_agg || state[0] != K_RUN) return;
+  if (a >= n_agg) return;
+  const int T = agg_tissue[a];
+  if (state[2 * T] != K_RUN || !seg_on[T]) return;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int a0 = agg_ptr[a], len = agg_ptr[a + 1] - a0;
   const int beg = a0 + (int)((int64_t)len * part / kSplit);
@@ -243,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
   block_sum<6>(acc, red);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) w[(size_t)part * ld + 6 * a + q] = (float)acc[q];
+    for (int q = 0; q < 6; ++q) w[(size_t)part * vlen + agg_cbase[a] + q] = (float)acc[q];
 }
 
 // Fixed / slave bits of every listed node (list order), from the DOF flags.
@@ -266,33 +276,35 @@ __global__ __launch_bounds__(kBlock) void coarse_mask_kernel(int32_t n_list,
 // y = A_c^-1 w, one wavefront per row (16-byte loads over the padded row
 // stride ld); partial (w, y) per workgroup.
 typedef float v4f __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(int32_t n_c, int32_t ld,
-                                                             const float* __restrict__ inv32,
-                                                             const float* __restrict__ w,
-                                                             const int32_t* __restrict__ state,
-                                                             float* __restrict__ y,
-                                                             double* __restrict__ cpart) {
+__global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
+    const int32_t* __restrict__ gw, const int32_t* __restrict__ sgi,
+    const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on, int32_t vlen,
+    const float* __restrict__ inv32, const float* __restrict__ w,
+    const int32_t* __restrict__ state, float* __restrict__ y, double* __restrict__ cpart) {
   __shared__ double wy[kGemvRows];
-  if (state[0] != K_RUN) return;
+  const int T = gw[2 * blockIdx.x], row0 = gw[2 * blockIdx.x + 1];
+  if (state[2 * T] != K_RUN) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int row = blockIdx.x * kGemvRows + wave;
+  const int nc = sgi[3 * T], ld = sgi[3 * T + 1], off = sgi[3 * T + 2];
+  const int row = row0 + wave;
   double dot = 0.0;
-  if (row < n_c) {
-    const v4f* a = (const v4f*)(inv32 + (size_t)row * ld);
+  if (seg_on[T] && row < nc) {
+    const v4f* a = (const v4f*)(inv32 + sgio[T] + (size_t)row * ld);
+    const float* ws = w + off;
     float s = 0.f;
     for (int j = lane; j < ld / 4; j += 64) {
-      v4f v = ((const v4f*)w)[j];
+      v4f v = ((const v4f*)ws)[j];
 #pragma unroll
-      for (int p = 1; p < kSplit; ++p) v += ((const v4f*)(w + (size_t)p * ld))[j];
+      for (int p = 1; p < kSplit; ++p) v += ((const v4f*)(ws + (size_t)p * vlen))[j];
       const v4f u = a[j];
       s += u.x * v.x + u.y * v.y + u.z * v.z + u.w * v.w;
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) y[row] = s;
-    float wr = w[row];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) y[off + row] = s;
+    float wr = ws[row];
 #pragma unroll
-    for (int p = 1; p < kSplit; ++p) wr += w[(size_t)p * ld + row];
+    for (int p = 1; p < kSplit; ++p) wr += ws[(size_t)p * vlen + row];
     dot = (double)wr * (double)s;
   }
   if (lane == 0) wy[wave] = dot;
@@ -307,13 +319,16 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(int32_t n_c, int32_
 
 // z += P y.
 __global__ __launch_bounds__(kBlock) void coarse_prolong_kernel(
-    int32_t n_nodes, const int32_t* __restrict__ node_agg, const float* __restrict__ d,
-    const uint8_t* __restrict__ flags, const float* __restrict__ y,
+    int32_t n_nodes, const int32_t* __restrict__ node_agg, const int32_t* __restrict__ agg_tissue,
+    const int32_t* __restrict__ agg_cbase, const int32_t* __restrict__ seg_on,
+    const float* __restrict__ d, const uint8_t* __restrict__ flags, const float* __restrict__ y,
     const int32_t* __restrict__ state, float* __restrict__ z) {
-  if (state[0] != K_RUN) return;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
     const int a = node_agg[i];
-    const float* ya = y + 6 * a;
+    if (a < 0) continue;
+    const int T = agg_tissue[a];
+    if (state[2 * T] != K_RUN || !seg_on[T]) continue;
+    const float* ya = y + agg_cbase[a];
     const float d0 = d[3 * i], d1 = d[3 * i + 1], d2 = d[3 * i + 2];
     // t + omega x d
     const float v[3] = {ya[0] + (ya[4] * d2 - ya[5] * d1), ya[1] + (ya[5] * d0 - ya[3] * d2),
@@ -338,106 +353,187 @@ void coarse_release(CoarseOp& c) {
   c = CoarseOp();
 }
 
-hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double cell,
-                        int z_bands, const std::vector<int32_t>& slice_base,
-                        const std::vector<int32_t>& col, const std::vector<int32_t>& row_len) {
+hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
+                        const std::vector<CoarseSeg>& segs, int z_bands,
+                        const std::vector<int32_t>& slice_base, const std::vector<int32_t>& col,
+                        const std::vector<int32_t>& row_len) {
   void* blas = c.blas;
   const int32_t gen = c.gen;
   c.blas = nullptr;
   coarse_release(c);
   c.blas = blas;
   c.gen = gen;
-  if (n_nodes <= 0 || !(cell > 0.0) || z_bands < 1) return hipErrorInvalidValue;
-  // square cells over the reference (x, y), each split into z_bands equal bands
-  // of the reference z range
-  double x0 = X[0], y0 = X[1], z0 = X[2], z1 = X[2];
-  for (int i = 0; i < n_nodes; ++i) {
-    x0 = std::min(x0, X[3 * i]);
-    y0 = std::min(y0, X[3 * i + 1]);
-    z0 = std::min(z0, X[3 * i + 2]);
-    z1 = std::max(z1, X[3 * i + 2]);
-  }
-  const double zb = z1 > z0 ? (z1 - z0) / z_bands : 1.0;
-  std::vector<int64_t> key(n_nodes);
-  for (int i = 0; i < n_nodes; ++i) {
-    const int64_t ix = (int64_t)std::floor((X[3 * i] - x0) / cell);
-    const int64_t iy = (int64_t)std::floor((X[3 * i + 1] - y0) / cell);
-    const int64_t iz = std::min<int64_t>(z_bands - 1, (int64_t)std::floor((X[3 * i + 2] - z0) / zb));
-    key[i] = (ix << 40) | (iy << 16) | iz;
-  }
-  std::vector<int64_t> keys(key);
-  std::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  const int n_cells = (int)keys.size();
-  std::vector<int32_t> cell_of(n_nodes), cnt(n_cells, 0);
-  for (int i = 0; i < n_nodes; ++i) {
-    cell_of[i] = (int32_t)(std::lower_bound(keys.begin(), keys.end(), key[i]) - keys.begin());
-    ++cnt[cell_of[i]];
-  }
-  std::vector<double> cen(3 * (size_t)n_cells, 0.0);
-  for (int i = 0; i < n_nodes; ++i)
-    for (int k = 0; k < 3; ++k) cen[3 * cell_of[i] + k] += X[3 * i + k];
-  for (int q = 0; q < n_cells; ++q)
-    for (int k = 0; k < 3; ++k) cen[3 * q + k] /= cnt[q];
-  // small cells join the nearest cell of kMinAggNodes nodes or more (all one
-  // aggregate when there is none)
-  std::vector<int32_t> target(n_cells), agg_of_cell(n_cells, -1);
-  std::vector<int32_t> big;
-  for (int q = 0; q < n_cells; ++q)
-    if (cnt[q] >= kMinAggNodes) big.push_back(q);
-  for (int q = 0; q < n_cells; ++q) {
-    if (cnt[q] >= kMinAggNodes) {
-      target[q] = q;
-    } else if (big.empty()) {
-      target[q] = 0;
-    } else {
-      double best = 1e300;
-      int bq = big[0];
-      for (int b : big) {
-        const double dx = cen[3 * q] - cen[3 * b], dy = cen[3 * q + 1] - cen[3 * b + 1];
-        const double dz = cen[3 * q + 2] - cen[3 * b + 2];
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 < best) { best = d2; bq = b; }
+  if (n_nodes <= 0 || segs.empty() || z_bands < 1) return hipErrorInvalidValue;
+  const int n_seg = (int)segs.size();
+  std::vector<int32_t> agg(n_nodes, -1);
+  std::vector<float> dn(3 * (size_t)n_nodes, 0.f);
+  std::vector<int32_t> seg_agg(n_seg + 1, 0), na_all, agg_tissue;
+  for (int T = 0; T < n_seg; ++T) {
+    const int b0 = segs[T].begin, b1 = segs[T].end;
+    const double cell = segs[T].cell;
+    if (b1 <= b0 || !(cell > 0.0)) return hipErrorInvalidValue;
+    const int nn = b1 - b0;
+    // square cells over the segment's reference (x, y), each split into
+    // z_bands equal bands of its reference z range
+    double x0 = X[3 * b0], y0 = X[3 * b0 + 1], z0 = X[3 * b0 + 2], z1 = X[3 * b0 + 2];
+    for (int i = b0; i < b1; ++i) {
+      x0 = std::min(x0, X[3 * i]);
+      y0 = std::min(y0, X[3 * i + 1]);
+      z0 = std::min(z0, X[3 * i + 2]);
+      z1 = std::max(z1, X[3 * i + 2]);
+    }
+    const double zb = z1 > z0 ? (z1 - z0) / z_bands : 1.0;
+    std::vector<int64_t> key(nn);
+    for (int q = 0; q < nn; ++q) {
+      const double* P = X + 3 * (size_t)(b0 + q);
+      const int64_t ix = (int64_t)std::floor((P[0] - x0) / cell);
+      const int64_t iy = (int64_t)std::floor((P[1] - y0) / cell);
+      const int64_t iz = std::min<int64_t>(z_bands - 1, (int64_t)std::floor((P[2] - z0) / zb));
+      key[q] = (ix << 40) | (iy << 16) | iz;
+    }
+    std::vector<int64_t> keys(key);
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int n_cells = (int)keys.size();
+    std::vector<int32_t> cell_of(nn), cnt(n_cells, 0);
+    for (int q = 0; q < nn; ++q) {
+      cell_of[q] = (int32_t)(std::lower_bound(keys.begin(), keys.end(), key[q]) - keys.begin());
+      ++cnt[cell_of[q]];
+    }
+    std::vector<double> cen(3 * (size_t)n_cells, 0.0);
+    for (int q = 0; q < nn; ++q)
+      for (int k = 0; k < 3; ++k) cen[3 * cell_of[q] + k] += X[3 * (size_t)(b0 + q) + k];
+    for (int g = 0; g < n_cells; ++g)
+      for (int k = 0; k < 3; ++k) cen[3 * g + k] /= cnt[g];
+    // small cells join the nearest cell of kMinAggNodes nodes or more (all one
+    // aggregate when there is none)
+    std::vector<int32_t> target(n_cells), agg_of_cell(n_cells, -1);
+    std::vector<int32_t> big;
+    for (int g = 0; g < n_cells; ++g)
+      if (cnt[g] >= kMinAggNodes) big.push_back(g);
+    for (int g = 0; g < n_cells; ++g) {
+      if (cnt[g] >= kMinAggNodes) {
+        target[g] = g;
+      } else if (big.empty()) {
+        target[g] = 0;
+      } else {
+        double best = 1e300;
+        int bq = big[0];
+        for (int bb : big) {
+          const double dx = cen[3 * g] - cen[3 * bb], dy = cen[3 * g + 1] - cen[3 * bb + 1];
+          const double dz = cen[3 * g + 2] - cen[3 * bb + 2];
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          if (d2 < best) { best = d2; bq = bb; }
+        }
+        target[g] = bq;
       }
-      target[q] = bq;
+    }
+    int n_loc = 0;
+    for (int g = 0; g < n_cells; ++g)
+      if (agg_of_cell[target[g]] < 0) agg_of_cell[target[g]] = n_loc++;
+    const int a_base = seg_agg[T];
+    if (n_loc > kMaxAggregates) return hipErrorInvalidValue;
+    seg_agg[T + 1] = a_base + n_loc;
+    for (int q = 0; q < nn; ++q) agg[b0 + q] = a_base + agg_of_cell[target[cell_of[q]]];
+    // centroid and radius per aggregate; scaled offsets per node
+    std::vector<double> ac(3 * (size_t)n_loc, 0.0), rad(n_loc, 0.0);
+    std::vector<int32_t> na(n_loc, 0);
+    for (int q = 0; q < nn; ++q) {
+      const int a = agg[b0 + q] - a_base;
+      ++na[a];
+      for (int k = 0; k < 3; ++k) ac[3 * a + k] += X[3 * (size_t)(b0 + q) + k];
+    }
+    for (int a = 0; a < n_loc; ++a)
+      for (int k = 0; k < 3; ++k) ac[3 * a + k] /= na[a];
+    for (int q = 0; q < nn; ++q) {
+      const int a = agg[b0 + q] - a_base;
+      double sq = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double t = X[3 * (size_t)(b0 + q) + k] - ac[3 * a + k];
+        sq += t * t;
+      }
+      rad[a] = std::max(rad[a], std::sqrt(sq));
+    }
+    for (int q = 0; q < nn; ++q) {
+      const int a = agg[b0 + q] - a_base;
+      const double L = rad[a] > 0.0 ? rad[a] : 1.0;
+      for (int k = 0; k < 3; ++k)
+        dn[3 * (size_t)(b0 + q) + k] = (float)((X[3 * (size_t)(b0 + q) + k] - ac[3 * a + k]) / L);
+    }
+    for (int a = 0; a < n_loc; ++a) {
+      na_all.push_back(na[a]);
+      agg_tissue.push_back(T);
     }
   }
-  int n_agg = 0;
-  for (int q = 0; q < n_cells; ++q)
-    if (agg_of_cell[target[q]] < 0) agg_of_cell[target[q]] = n_agg++;
-  if (n_agg > kMaxAggregates) return hipErrorInvalidValue;
-  std::vector<int32_t> agg(n_nodes);
-  for (int i = 0; i < n_nodes; ++i) agg[i] = agg_of_cell[target[cell_of[i]]];
-  // centroid and radius per aggregate; scaled offsets per node
-  std::vector<double> ac(3 * (size_t)n_agg, 0.0), rad(n_agg, 0.0);
-  std::vector<int32_t> na(n_agg, 0);
-  for (int i = 0; i < n_nodes; ++i) {
-    ++na[agg[i]];
-    for (int k = 0; k < 3; ++k) ac[3 * agg[i] + k] += X[3 * i + k];
-  }
-  for (int a = 0; a < n_agg; ++a)
-    for (int k = 0; k < 3; ++k) ac[3 * a + k] /= na[a];
-  for (int i = 0; i < n_nodes; ++i) {
-    double s = 0.0;
-    for (int k = 0; k < 3; ++k) {
-      const double t = X[3 * i + k] - ac[3 * agg[i] + k];
-      s += t * t;
-    }
-    rad[agg[i]] = std::max(rad[agg[i]], std::sqrt(s));
-  }
-  std::vector<float> dn(3 * (size_t)n_nodes);
-  for (int i = 0; i < n_nodes; ++i) {
-    const double L = rad[agg[i]] > 0.0 ? rad[agg[i]] : 1.0;
-    for (int k = 0; k < 3; ++k) dn[3 * i + k] = (float)((X[3 * i + k] - ac[3 * agg[i] + k]) / L);
-  }
-  std::vector<int32_t> aptr(n_agg + 1, 0), anodes(n_nodes);
-  for (int a = 0; a < n_agg; ++a) aptr[a + 1] = aptr[a] + na[a];
+  const int n_agg = seg_agg[n_seg];
+  std::vector<int32_t> aptr(n_agg + 1, 0);
+  for (int a = 0; a < n_agg; ++a) aptr[a + 1] = aptr[a] + na_all[a];
+  const int n_list = aptr[n_agg];
+  std::vector<int32_t> anodes(n_list);
   {
     std::vector<int32_t> fill(aptr.begin(), aptr.end() - 1);
-    for (int i = 0; i < n_nodes; ++i) anodes[fill[agg[i]]++] = i;
+    for (int i = 0; i < n_nodes; ++i)
+      if (agg[i] >= 0) anodes[fill[agg[i]]++] = i;
+  }
+  // segment layout: coarse DOF, vector offsets (multiples of 64), dense and
+  // FP32 matrix offsets (multiples of 64 elements), product workgroups
+  c.n_seg = n_seg;
+  c.seg_agg = seg_agg;
+  c.seg_nc.assign(n_seg, 0);
+  c.seg_ld.assign(n_seg, 0);
+  c.seg_coff.assign(n_seg, 0);
+  c.seg_moff.assign(n_seg, 0);
+  c.seg_ioff.assign(n_seg, 0);
+  c.seg_goff.assign(n_seg + 1, 0);
+  c.seg_poff.assign(n_seg, 0);
+  c.seg_have_inv.assign(n_seg, 0);
+  c.seg_active.assign(n_seg, 0);
+  int32_t vlen = 0, poff = 0;
+  int64_t moff = 0, ioff = 0;
+  std::vector<int32_t> gw, segd(2 * (size_t)n_seg), cbase(n_agg), sgi(3 * (size_t)n_seg);
+  std::vector<int64_t> segm(n_seg), sgio(n_seg);
+  for (int T = 0; T < n_seg; ++T) {
+    const int nc = 6 * (seg_agg[T + 1] - seg_agg[T]);
+    const int ld = (nc + 63) / 64 * 64;
+    c.seg_nc[T] = nc;
+    c.seg_ld[T] = ld;
+    c.seg_coff[T] = vlen;
+    c.seg_moff[T] = moff;
+    c.seg_ioff[T] = ioff;
+    c.seg_poff[T] = poff;
+    for (int a = seg_agg[T]; a < seg_agg[T + 1]; ++a) cbase[a] = vlen + 6 * (a - seg_agg[T]);
+    segd[2 * T] = seg_agg[T];
+    segd[2 * T + 1] = nc;
+    segm[T] = moff;
+    sgi[3 * T] = nc;
+    sgi[3 * T + 1] = ld;
+    sgi[3 * T + 2] = vlen;
+    sgio[T] = ioff;
+    for (int r0 = 0; r0 < nc; r0 += kGemvRows) {
+      gw.push_back(T);
+      gw.push_back(r0);
+    }
+    c.seg_goff[T + 1] = (int32_t)(gw.size() / 2);
+    vlen += ld;
+    moff += ((int64_t)nc * nc + 63) / 64 * 64;
+    ioff += (int64_t)nc * ld;
+    poff += nc;
   }
   // operator slots grouped by aggregate pair, ascending within each pair
-  std::vector<int32_t> pair_id((size_t)n_agg * n_agg, -1), pIK, pcnt;
+  std::vector<int32_t> pIK, pcnt;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> pair_of(n_agg);  // (K, pair id) per I
+  auto pair_id = [&](int I, int K, bool create) -> int32_t {
+    auto& v = pair_of[I];
+    for (auto& e : v)
+      if (e.first == K) return e.second;
+    if (!create) return -1;
+    const int32_t id = (int32_t)pcnt.size();
+    v.push_back({K, id});
+    pcnt.push_back(0);
+    pIK.push_back(I);
+    pIK.push_back(K);
+    return id;
+  };
   const int n_slices = (int)slice_base.size() - 1;
   auto for_slots = [&](auto f) {
     for (int sl = 0; sl < n_slices; ++sl) {
@@ -447,122 +543,227 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double ce
           const int R = sl * kSlice + lane;
           if (R >= n_nodes || k >= row_len[R]) continue;
           const int slot = slice_base[sl] + k * kSlice + lane;
-          f(slot, R, col[slot]);
+          const int C = col[slot];
+          // A_c keeps one block per segment: couplings between tissues (none
+          // in a batch of separate tissues) stay out of it
+          if (agg[R] < 0 || agg[C] < 0 || agg_tissue[agg[R]] != agg_tissue[agg[C]]) continue;
+          f(slot, R, C);
         }
     }
   };
   int64_t listed = 0;
-  for_slots([&](int slot, int R, int C) {
-    int32_t& id = pair_id[(size_t)agg[R] * n_agg + agg[C]];
-    if (id < 0) {
-      id = (int32_t)pcnt.size();
-      pcnt.push_back(0);
-      pIK.push_back(agg[R]);
-      pIK.push_back(agg[C]);
-    }
-    ++pcnt[id];
+  for_slots([&](int, int R, int C) {
+    ++pcnt[pair_id(agg[R], agg[C], true)];
     ++listed;
   });
   const int n_pairs = (int)pcnt.size();
   std::vector<int32_t> pptr(n_pairs + 1, 0), pslot(listed), prow(listed);
-  for (int p = 0; p < n_pairs; ++p) pptr[p + 1] = pptr[p] + pcnt[p];
+  for (int q = 0; q < n_pairs; ++q) pptr[q + 1] = pptr[q] + pcnt[q];
   {
     std::vector<int32_t> fill(pptr.begin(), pptr.end() - 1);
     for_slots([&](int slot, int R, int C) {
-      const int32_t id = pair_id[(size_t)agg[R] * n_agg + agg[C]];
+      const int32_t id = pair_id(agg[R], agg[C], false);
       pslot[fill[id]] = slot;
       prow[fill[id]] = R;
       ++fill[id];
     });
   }
   // node offsets in list order
-  std::vector<float> ldv(3 * (size_t)n_nodes);
-  for (int t = 0; t < n_nodes; ++t)
+  std::vector<float> ldv(3 * (size_t)std::max(n_list, 1));
+  for (int t = 0; t < n_list; ++t)
     for (int k = 0; k < 3; ++k) ldv[3 * (size_t)t + k] = dn[3 * (size_t)anodes[t] + k];
   c.n_agg = n_agg;
   c.n_c = 6 * n_agg;
   c.n_cpairs = n_pairs;
-  c.n_gemv = (c.n_c + kGemvRows - 1) / kGemvRows;
-  c.ld32 = (c.n_c + 63) / 64 * 64;
-  c.agg_total = n_nodes;
+  c.n_gemv = (int32_t)(gw.size() / 2);
+  c.vlen = vlen;
+  c.agg_total = n_list;
   hipError_t e;
   if ((e = upload(c, &c.node_agg, agg)) != hipSuccess) return e;
   if ((e = upload(c, &c.node_d, dn)) != hipSuccess) return e;
   if ((e = upload(c, &c.agg_ptr, aptr)) != hipSuccess) return e;
   if ((e = upload(c, &c.agg_nodes, anodes)) != hipSuccess) return e;
+  if ((e = upload(c, &c.agg_tissue, agg_tissue)) != hipSuccess) return e;
+  if ((e = upload(c, &c.agg_cbase, cbase)) != hipSuccess) return e;
   if ((e = upload(c, &c.list_d, ldv)) != hipSuccess) return e;
-  if ((e = upload(c, &c.list_mask, std::vector<uint8_t>((size_t)n_nodes, 0))) != hipSuccess) return e;
+  if ((e = upload(c, &c.list_mask, std::vector<uint8_t>((size_t)std::max(n_list, 1), 0))) != hipSuccess) return e;
   if ((e = upload(c, &c.cp_ptr, pptr)) != hipSuccess) return e;
   if ((e = upload(c, &c.cp_slot, pslot)) != hipSuccess) return e;
   if ((e = upload(c, &c.cp_row, prow)) != hipSuccess) return e;
   if ((e = upload(c, &c.cp_IK, pIK)) != hipSuccess) return e;
-  const size_t nc = (size_t)c.n_c, ld = (size_t)c.ld32;
+  if ((e = upload(c, &c.gw, gw)) != hipSuccess) return e;
+  if ((e = upload(c, &c.cpart_off, c.seg_goff)) != hipSuccess) return e;
+  if ((e = upload(c, &c.seg_on, std::vector<int32_t>(n_seg, 0))) != hipSuccess) return e;
+  if ((e = upload(c, &c.segd, segd)) != hipSuccess) return e;
+  if ((e = upload(c, &c.segm, segm)) != hipSuccess) return e;
+  if ((e = upload(c, &c.sgi, sgi)) != hipSuccess) return e;
+  if ((e = upload(c, &c.sgio, sgio)) != hipSuccess) return e;
   if ((e = upload(c, &c.pairblk, std::vector<double>(), 36 * (size_t)n_pairs)) != hipSuccess) return e;
-  if ((e = upload(c, &c.Ac, std::vector<double>(), nc * nc)) != hipSuccess) return e;
-  if ((e = upload(c, &c.X, std::vector<double>(), nc * nc)) != hipSuccess) return e;
-  if ((e = upload(c, &c.Xref, std::vector<double>(), nc * nc)) != hipSuccess) return e;
-  if ((e = upload(c, &c.X2, std::vector<double>(), nc * nc)) != hipSuccess) return e;
-  if ((e = upload(c, &c.T, std::vector<double>(), nc * nc)) != hipSuccess) return e;
-  if ((e = upload(c, &c.colmax, std::vector<double>(nc, 0.0))) != hipSuccess) return e;
-  if ((e = upload(c, &c.ipiv, std::vector<int32_t>(), nc)) != hipSuccess) return e;
-  if ((e = upload(c, &c.info, std::vector<int32_t>(), 2)) != hipSuccess) return e;
-  if ((e = upload(c, &c.inv32, std::vector<float>(nc * ld, 0.f))) != hipSuccess) return e;
-  if ((e = upload(c, &c.w, std::vector<float>(kSplit * ld, 0.f))) != hipSuccess) return e;
-  if ((e = upload(c, &c.y, std::vector<float>(nc, 0.f))) != hipSuccess) return e;
-  if ((e = upload(c, &c.cpart, std::vector<double>((size_t)c.n_gemv, 0.0))) != hipSuccess) return e;
+  if ((e = upload(c, &c.Ac, std::vector<double>(), (size_t)moff)) != hipSuccess) return e;
+  if ((e = upload(c, &c.X, std::vector<double>(), (size_t)moff)) != hipSuccess) return e;
+  if ((e = upload(c, &c.Xref, std::vector<double>(), (size_t)moff)) != hipSuccess) return e;
+  if ((e = upload(c, &c.X2, std::vector<double>(), (size_t)moff)) != hipSuccess) return e;
+  if ((e = upload(c, &c.T, std::vector<double>(), (size_t)moff)) != hipSuccess) return e;
+  c.colmax_len = std::max(poff, 1);
+  if ((e = upload(c, &c.colmax, std::vector<double>((size_t)c.colmax_len, 0.0))) != hipSuccess) return e;
+  if ((e = upload(c, &c.ipiv, std::vector<int32_t>(), (size_t)std::max(poff, 1))) != hipSuccess) return e;
+  if ((e = upload(c, &c.info, std::vector<int32_t>(), 2 * (size_t)n_seg)) != hipSuccess) return e;
+  if ((e = upload(c, &c.inv32, std::vector<float>((size_t)std::max<int64_t>(ioff, 1), 0.f))) != hipSuccess) return e;
+  if ((e = upload(c, &c.w, std::vector<float>(kSplit * (size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
+  if ((e = upload(c, &c.y, std::vector<float>((size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
+  if ((e = upload(c, &c.cpart, std::vector<double>((size_t)std::max(c.n_gemv, 1), 0.0))) != hipSuccess) return e;
   c.built = true;
   c.active = false;
-  c.have_inv = false;
   return hipSuccess;
 }
 
 namespace {
 
-// max |I - A_c X| on the host (one synchronisation).
-hipError_t residual(CoarseOp& c, rocblas_handle hb, hipStream_t st, double* out) {
-  const int nc = c.n_c;
-  const int64_t n2 = (int64_t)nc * nc;
+// Pointers to one segment's dense matrices.
+struct SegMats {
+  int nc;
+  double *Ac, *X, *Xref, *X2, *T;
+  int32_t *ipiv, *info;
+};
+
+SegMats seg_mats(CoarseOp& c, int T) {
+  const int64_t m = c.seg_moff[T];
+  return SegMats{c.seg_nc[T], c.Ac + m, c.X + m, c.Xref + m, c.X2 + m, c.T + m,
+                 c.ipiv + c.seg_poff[T], c.info + 2 * T};
+}
+
+// T = A_c X of one segment and the column maxima of |I - T| into its part of
+// c.colmax (no synchronisation).
+hipError_t residual_launch(CoarseOp& c, const SegMats& S, const double* X, rocblas_handle hb,
+                           hipStream_t st, int32_t coff) {
+  const int nc = S.nc;
   const double one = 1.0, zero = 0.0;
-  if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, c.Ac,
-                    nc, c.X, nc, &zero, c.T, nc) != rocblas_status_success)
+  if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, S.Ac,
+                    nc, X, nc, &zero, S.T, nc) != rocblas_status_success)
     return hipErrorUnknown;
-  hipLaunchKernelGGL(coarse_resid_kernel, dim3(nc), dim3(kBlock), 0, st, nc, c.T, c.colmax);
-  std::vector<double> cm(nc);
+  hipLaunchKernelGGL(coarse_resid_kernel, dim3(nc), dim3(kBlock), 0, st, nc, S.T,
+                     c.colmax + coff);
+  return hipGetLastError();
+}
+
+// X_ref = A_c^-1 of every listed segment by LU (rocSOLVER), one
+// synchronisation for all; ok[T] false when a segment's A_c is singular.
+hipError_t refactor(CoarseOp& c, const std::vector<int>& segs, rocblas_handle hb, hipStream_t st,
+                    std::vector<char>& ok) {
   hipError_t e;
-  if ((e = hipMemcpyAsync(cm.data(), c.colmax, sizeof(double) * nc, hipMemcpyDeviceToHost,
-                          st)) != hipSuccess)
+  std::vector<char> launched(c.n_seg, 0);
+  for (int T : segs) {
+    const SegMats S = seg_mats(c, T);
+    const int nc = S.nc;
+    ok[T] = 0;
+    if ((e = hipMemcpyAsync(S.Xref, S.Ac, sizeof(double) * (size_t)nc * nc,
+                            hipMemcpyDeviceToDevice, st)) != hipSuccess)
+      return e;
+    launched[T] = rocsolver_dgetrf(hb, nc, nc, S.Xref, nc, S.ipiv, S.info) ==
+                      rocblas_status_success &&
+                  rocsolver_dgetri(hb, nc, S.Xref, nc, S.ipiv, S.info + 1) ==
+                      rocblas_status_success;
+  }
+  std::vector<int32_t> info(2 * (size_t)c.n_seg, 1);
+  if ((e = hipMemcpyAsync(info.data(), c.info, sizeof(int32_t) * info.size(),
+                          hipMemcpyDeviceToHost, st)) != hipSuccess)
     return e;
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-  double r = 0.0;
-  for (double v : cm) r = std::max(r, v);
-  *out = r;
+  for (int T : segs) {
+    ++c.refactors;
+    ok[T] = launched[T] && info[2 * T] == 0 && info[2 * T + 1] == 0;
+  }
   return hipSuccess;
 }
 
-// X_ref = A_c^-1 by LU (rocSOLVER); false when A_c is singular.
-hipError_t refactor(CoarseOp& c, rocblas_handle hb, hipStream_t st, bool* ok) {
-  const int nc = c.n_c;
-  *ok = false;
+// The inverse of the A_c of every listed segment: Newton-Schulz steps from
+// its X_ref, which is refactorised when they do not converge; ok[T] when the
+// segment's inverse is usable.  The segments step together, with one
+// synchronisation per round of residuals, and each segment's arithmetic is
+// that of the segment alone.
+hipError_t factor_segments(CoarseOp& c, const std::vector<int>& segs, rocblas_handle hb,
+                           hipStream_t st, std::vector<char>& ok) {
   hipError_t e;
-  if ((e = hipMemcpyAsync(c.Xref, c.Ac, sizeof(double) * (size_t)nc * nc,
-                          hipMemcpyDeviceToDevice, st)) != hipSuccess)
-    return e;
-  if (rocsolver_dgetrf(hb, nc, nc, c.Xref, nc, c.ipiv, c.info) != rocblas_status_success ||
-      rocsolver_dgetri(hb, nc, c.Xref, nc, c.ipiv, c.info + 1) != rocblas_status_success)
-    return hipSuccess;
-  int32_t info[2] = {1, 1};
-  if ((e = hipMemcpyAsync(info, c.info, sizeof(info), hipMemcpyDeviceToHost, st)) != hipSuccess)
-    return e;
-  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-  ++c.refactors;
-  *ok = info[0] == 0 && info[1] == 0;
+  for (int T : segs) {
+    const int nc = c.seg_nc[T];
+    ok[T] = 0;
+    hipLaunchKernelGGL(coarse_diag_kernel, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       nc, c.Ac + c.seg_moff[T]);
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // X from the reference inverse by Newton-Schulz steps against the current
+  // A_c.  The result depends only on (A_c, X_ref), never on earlier solves,
+  // so a repeated step repeats its bits; X_ref is refactorised (rocSOLVER)
+  // only when the steps do not converge from it.
+  std::vector<char> cur(c.n_seg, 0);   // 0: X holds the iterate, 1: X2
+  std::vector<char> dead(c.n_seg, 0);  // singular A_c: the segment keeps block Jacobi
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    std::vector<int> redo, run;
+    for (int T : segs)
+      if (!ok[T] && !dead[T] && (!c.seg_have_inv[T] || attempt == 1)) redo.push_back(T);
+    if (!redo.empty()) {
+      std::vector<char> fok(c.n_seg, 0);
+      if ((e = refactor(c, redo, hb, st, fok)) != hipSuccess) return e;
+      for (int T : redo) {
+        c.seg_have_inv[T] = fok[T];
+        dead[T] = !fok[T];
+      }
+    }
+    for (int T : segs) {
+      if (ok[T] || dead[T]) continue;
+      const SegMats S = seg_mats(c, T);
+      if ((e = hipMemcpyAsync(S.X, S.Xref, sizeof(double) * (size_t)S.nc * S.nc,
+                              hipMemcpyDeviceToDevice, st)) != hipSuccess)
+        return e;
+      cur[T] = 0;
+      run.push_back(T);
+    }
+    const double one = 1.0, zero = 0.0;
+    for (int it = 0; !run.empty(); ++it) {
+      for (int T : run) {
+        const SegMats S = seg_mats(c, T);
+        if ((e = residual_launch(c, S, cur[T] ? S.X2 : S.X, hb, st, c.seg_poff[T])) !=
+            hipSuccess)
+          return e;
+      }
+      std::vector<double> cm(c.colmax_len);
+      if ((e = hipMemcpyAsync(cm.data(), c.colmax, sizeof(double) * cm.size(),
+                              hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      std::vector<int> next;
+      for (int T : run) {
+        const SegMats S = seg_mats(c, T);
+        const int nc = S.nc;
+        double r = 0.0;
+        for (int q = 0; q < nc; ++q) r = std::max(r, cm[c.seg_poff[T] + q]);
+        double* X = cur[T] ? S.X2 : S.X;
+        if (r < 1e-5) {
+          ok[T] = 1;
+          hipLaunchKernelGGL(coarse_convert_kernel, dim3((nc + 31) / 32, (nc + 31) / 32),
+                             dim3(kBlock), 0, st, nc, c.seg_ld[T], X, c.inv32 + c.seg_ioff[T]);
+          if ((e = hipGetLastError()) != hipSuccess) return e;
+          continue;
+        }
+        if (!(r < 0.25) || it == 6) continue;  // too far from X_ref (or NaN)
+        hipLaunchKernelGGL(coarse_twoI_kernel, dim3((nc + kBlock - 1) / kBlock, nc), dim3(kBlock),
+                           0, st, nc, S.T);
+        if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, X,
+                          nc, S.T, nc, &zero, cur[T] ? S.X : S.X2, nc) != rocblas_status_success)
+          return hipErrorUnknown;
+        cur[T] ^= 1;
+        next.push_back(T);
+      }
+      run.swap(next);
+    }
+  }
   return hipSuccess;
 }
 
 }  // namespace
 
 hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, int64_t n_slots,
-                         const uint8_t* dof_flags, hipStream_t st) {
+                         const uint8_t* dof_flags, const int32_t* run, hipStream_t st) {
   c.active = false;
   if (!c.built) return hipSuccess;
   if (!c.blas) {
@@ -575,67 +776,49 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, in
   }
   rocblas_handle hb = (rocblas_handle)c.blas;
   if (rocblas_set_stream(hb, st) != rocblas_status_success) return hipErrorUnknown;
-  const int nc = c.n_c;
-  const int64_t n2 = (int64_t)nc * nc;
-  hipLaunchKernelGGL(coarse_pairs_kernel, dim3(c.n_cpairs), dim3(kBlock), 0, st, c.n_cpairs,
-                     c.cp_ptr, c.cp_slot, c.cp_row, col, val32, n_slots, c.node_d, dof_flags,
-                     c.pairblk);
-  hipLaunchKernelGGL(coarse_fill_kernel, dim3(grid_nodes(n2)), dim3(kBlock), 0, st, c.Ac, n2,
-                     0.0);
-  hipLaunchKernelGGL(coarse_scatter_kernel, dim3(c.n_cpairs), dim3(64), 0, st, c.n_cpairs, nc,
-                     c.cp_IK, c.pairblk, c.Ac);
-  hipLaunchKernelGGL(coarse_diag_kernel, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                     nc, c.Ac);
+  int64_t total = 0;
+  for (int T = 0; T < c.n_seg; ++T) total = c.seg_moff[T] + (int64_t)c.seg_nc[T] * c.seg_nc[T];
+  hipLaunchKernelGGL(coarse_pairs_kernel, dim3(std::max(c.n_cpairs, 1)), dim3(kBlock), 0, st,
+                     c.n_cpairs, c.cp_ptr, c.cp_slot, c.cp_row, col, val32, n_slots, c.node_d,
+                     dof_flags, c.pairblk);
+  hipLaunchKernelGGL(coarse_fill_kernel, dim3(grid_nodes(std::max<int64_t>(total, 1))),
+                     dim3(kBlock), 0, st, c.Ac, total, 0.0);
+  hipLaunchKernelGGL(coarse_scatter_kernel, dim3(std::max(c.n_cpairs, 1)), dim3(64), 0, st,
+                     c.n_cpairs, c.cp_IK, c.agg_tissue, c.segd, c.segm, c.pairblk, c.Ac);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // X from the reference inverse by Newton-Schulz steps against the current
-  // A_c.  The result depends only on (A_c, X_ref), never on earlier solves,
-  // so a repeated step repeats its bits; X_ref is refactorised (rocSOLVER)
-  // only when the steps do not converge from it.
-  const size_t bytes = sizeof(double) * (size_t)nc * nc;
-  bool ok = false;
-  for (int attempt = 0; attempt < 2 && !ok; ++attempt) {
-    if (!c.have_inv || attempt == 1) {
-      bool fok = false;
-      if ((e = refactor(c, hb, st, &fok)) != hipSuccess) return e;
-      c.have_inv = fok;
-      if (!fok) return hipSuccess;  // singular: the solve keeps block Jacobi
-    }
-    if ((e = hipMemcpyAsync(c.X, c.Xref, bytes, hipMemcpyDeviceToDevice, st)) != hipSuccess)
-      return e;
-    const double one = 1.0, zero = 0.0;
-    for (int it = 0;; ++it) {
-      double r = 0.0;
-      if ((e = residual(c, hb, st, &r)) != hipSuccess) return e;  // T = A_c X
-      if (r < 1e-5) { ok = true; break; }
-      if (!(r < 0.25) || it == 6) break;  // too far from X_ref (or NaN)
-      hipLaunchKernelGGL(coarse_twoI_kernel, dim3((nc + kBlock - 1) / kBlock, nc), dim3(kBlock), 0,
-                         st, nc, c.T);
-      if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, nc, nc, nc, &one, c.X,
-                        nc, c.T, nc, &zero, c.X2, nc) != rocblas_status_success)
-        return hipErrorUnknown;
-      std::swap(c.X, c.X2);
-    }
+  // segments not iterating keep their coarse state
+  std::vector<int> segs;
+  for (int T = 0; T < c.n_seg; ++T)
+    if (!run || run[T]) segs.push_back(T);
+  std::vector<char> ok(c.n_seg, 0);
+  if ((e = factor_segments(c, segs, hb, st, ok)) != hipSuccess) return e;
+  for (int T : segs) c.seg_active[T] = ok[T];
+  std::vector<int32_t> on(c.n_seg, 0);
+  for (int T = 0; T < c.n_seg; ++T) {
+    on[T] = c.seg_active[T] ? 1 : 0;
+    c.active |= c.seg_active[T] != 0;
   }
-  if (!ok) return hipSuccess;
-  hipLaunchKernelGGL(coarse_convert_kernel, dim3((nc + 31) / 32, (nc + 31) / 32), dim3(kBlock), 0,
-                     st, nc, c.ld32, c.X, c.inv32);
-  hipLaunchKernelGGL(coarse_mask_kernel, dim3(grid_nodes(c.agg_total)), dim3(kBlock), 0, st,
-                     c.agg_total, c.agg_nodes, dof_flags, c.list_mask);
+  if ((e = hipMemcpyAsync(c.seg_on, on.data(), sizeof(int32_t) * c.n_seg, hipMemcpyHostToDevice,
+                          st)) != hipSuccess)
+    return e;
+  hipLaunchKernelGGL(coarse_mask_kernel, dim3(grid_nodes(std::max(c.agg_total, 1))), dim3(kBlock),
+                     0, st, c.agg_total, c.agg_nodes, dof_flags, c.list_mask);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  c.active = true;
-  return hipSuccess;
+  // the activity flags are pageable host memory: wait for their copy
+  return hipStreamSynchronize(st);
 }
 
 void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, float* z,
                   int32_t n_nodes, const uint8_t* dof_flags, hipStream_t st) {
   hipLaunchKernelGGL(coarse_restrict_kernel, dim3(c.n_agg * kSplit), dim3(kBlock), 0, st,
-                     c.n_agg, c.ld32, c.agg_ptr, c.agg_nodes, c.list_d, c.list_mask, r, state,
-                     c.w);
-  hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.n_c, c.ld32,
-                     c.inv32, c.w, state, c.y, c.cpart);
+                     c.n_agg, c.vlen, c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on,
+                     c.list_d, c.list_mask, r, state, c.w);
+  hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
+                     c.sgio, c.seg_on, c.vlen, c.inv32, c.w, state, c.y, c.cpart);
   hipLaunchKernelGGL(coarse_prolong_kernel, dim3(grid_nodes(n_nodes)), dim3(kBlock), 0, st,
-                     n_nodes, c.node_agg, c.node_d, dof_flags, c.y, state, z);
+                     n_nodes, c.node_agg, c.agg_tissue, c.agg_cbase, c.seg_on, c.node_d, dof_flags,
+                     c.y, state, z);
 }
 
 }  // namespace tf

@@ -84,37 +84,65 @@ struct BoundOp {
 struct CoarseOp {
   bool built = false;    // aggregates and block lists match the operator pattern
   bool for_bound = false;
-  bool active = false;   // the factorisation of the current operator succeeded
+  bool active = false;   // the factorisation of some segment's operator succeeded
+  // One segment per tissue: its own aggregates, its own dense A_c block, inverse
+  // and Newton-Schulz state.  A single-tissue handle has one segment.
+  int32_t n_seg = 0;
   int32_t n_agg = 0, n_c = 0, n_cpairs = 0, n_gemv = 0, agg_total = 0;
-  int32_t* node_agg = nullptr;  // per node
+  int32_t vlen = 0;             // padded coarse vector length (sum of the segments' ld)
+  std::vector<int32_t> seg_agg;   // n_seg + 1: aggregates of each segment
+  std::vector<int32_t> seg_nc;    // coarse DOF per segment (6 per aggregate)
+  std::vector<int32_t> seg_ld;    // row stride of the segment's inv32 (n_c rounded up to 64)
+  std::vector<int32_t> seg_coff;  // offset of the segment in the coarse vectors
+  std::vector<int64_t> seg_moff;  // offset of the segment's dense FP64 matrices
+  std::vector<int64_t> seg_ioff;  // offset of the segment's FP32 inverse
+  std::vector<int32_t> seg_goff;  // n_seg + 1: product workgroups of each segment
+  std::vector<int32_t> seg_poff;  // offset of the segment's pivots
+  std::vector<char> seg_have_inv;  // Xref holds an inverse of the segment
+  std::vector<char> seg_active;    // the segment's last factorisation succeeded
+  int32_t* node_agg = nullptr;  // per node (-1: no aggregate)
   float* node_d = nullptr;      // per node: (X - centroid) / radius, 3 floats
   int32_t* agg_ptr = nullptr;   // nodes of each aggregate, ascending
   int32_t* agg_nodes = nullptr;
+  int32_t* agg_tissue = nullptr;  // per aggregate: its segment (tissue)
+  int32_t* agg_cbase = nullptr;   // per aggregate: its first coarse DOF in the coarse vectors
   int32_t* cp_ptr = nullptr;    // operator slots of each aggregate pair, ascending
   int32_t* cp_slot = nullptr;
   int32_t* cp_row = nullptr;    // row (node) of each listed slot
   int32_t* cp_IK = nullptr;     // the pair's aggregates (I, K)
+  int32_t* gw = nullptr;        // per product workgroup: segment, first row
+  int32_t* cpart_off = nullptr; // n_seg + 1: product workgroups of each segment (device)
+  int32_t* seg_on = nullptr;    // per segment: 1 when its inverse is in use
+  int32_t* segd = nullptr;      // per segment: first aggregate, n_c (device, for the scatter)
+  int64_t* segm = nullptr;      // per segment: dense offset (device)
+  int32_t* sgi = nullptr;       // per segment: n_c, ld, vector offset (device, for the product)
+  int64_t* sgio = nullptr;      // per segment: FP32 inverse offset (device)
   float* list_d = nullptr;      // node offsets in aggregate-list order (3 per entry)
   uint8_t* list_mask = nullptr; // fixed / slave bits of the listed node (bit p: DOF p)
   double* pairblk = nullptr;    // 36 per pair: (P_I^T A P_K), row-major 6x6
-  double* Ac = nullptr;         // n_c x n_c column-major
+  double* Ac = nullptr;         // per segment n_c x n_c column-major
   double* Xref = nullptr;       // reference inverse (column-major, rocSOLVER)
   double* X = nullptr;          // inverse of the current A_c, and two work matrices
   double* X2 = nullptr;
   double* T = nullptr;
-  double* colmax = nullptr;     // per column: max |I - A_c X|
-  bool have_inv = false;        // Xref holds an inverse
+  double* colmax = nullptr;     // per column of every segment: max |I - A_c X|
+  int32_t colmax_len = 0;
   int32_t refactors = 0;        // rocSOLVER factorisations so far
   int32_t* ipiv = nullptr;
-  int32_t* info = nullptr;      // 2: getrf, getri
-  int32_t ld32 = 0;             // row stride of inv32 (n_c rounded up to 64)
-  float* inv32 = nullptr;       // A_c^-1, row-major FP32
-  float* w = nullptr;           // P^T r as partial vectors of ld32 entries (zero past n_c)
+  int32_t* info = nullptr;      // 2 per segment: getrf, getri
+  float* inv32 = nullptr;       // per segment A_c^-1, row-major FP32
+  float* w = nullptr;           // P^T r as partial vectors of vlen entries (zero padding)
   float* y = nullptr;           // A_c^-1 P^T r
   double* cpart = nullptr;      // per workgroup of the product: partial (w, y)
   std::vector<void*> allocs;
   void* blas = nullptr;         // rocblas_handle
   int32_t gen = 0;              // incremented by every build (captured graphs key on it)
+};
+// One segment of a coarse build: the nodes [begin, end) of one tissue and the
+// cell size of its aggregates.
+struct CoarseSeg {
+  int32_t begin, end;
+  double cell;
 };
 // A_b from the assembled J (FP64, or FP32 with mixed) and its block-Jacobi
 // inverse.
@@ -458,19 +486,24 @@ hipError_t launch_transpose_operator(int64_t n_slots, const int32_t* tslot, cons
 // ---- two-level preconditioner (tf_coarse.hip) ----
 // Aggregates and block lists for an operator pattern (host arrays of the
 // pattern: slice_base, col, row_len) and the node reference coordinates X
-// (3 per node); cell is the aggregate cell size (same units as X), and every
-// (x, y) cell is split into z_bands bands of the reference thickness.
-hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X, double cell,
-                        int z_bands, const std::vector<int32_t>& slice_base,
-                        const std::vector<int32_t>& col, const std::vector<int32_t>& row_len);
+// (3 per node), one segment (tissue) at a time: its nodes fall in square
+// (x, y) cells of the segment's cell size (same units as X), each split into
+// z_bands bands of the segment's reference thickness.  Aggregates never cross
+// segments, so A_c is block-diagonal with one dense block per segment.
+hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
+                        const std::vector<CoarseSeg>& segs, int z_bands,
+                        const std::vector<int32_t>& slice_base, const std::vector<int32_t>& col,
+                        const std::vector<int32_t>& row_len);
 void coarse_release(CoarseOp& c);
 // A_c of the FP32 operator (col / val32 of the pattern the lists were built
-// for), its FP64 inverse and the FP32 copy; c.active reports success.
+// for), its FP64 inverse and the FP32 copy, for every segment T with run[T]
+// set (all segments when run is null; host array); c.active reports that
+// some segment's inverse is in use.
 hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, int64_t n_slots,
-                         const uint8_t* dof_flags, hipStream_t st);
-// z += P A_c^-1 P^T r (FP32, one tissue), with the partial (w, y) per product
-// workgroup in c.cpart; nothing when the tissue's state (state[0]) is not
-// running.
+                         const uint8_t* dof_flags, const int32_t* run, hipStream_t st);
+// z += P A_c^-1 P^T r (FP32), with the partial (w, y) per product workgroup in
+// c.cpart; nothing for a tissue whose state (state[2 T]) is not running, and
+// no correction (zero partials) for a segment whose factorisation failed.
 void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, float* z,
                   int32_t n_nodes, const uint8_t* dof_flags, hipStream_t st);
 

@@ -790,36 +790,44 @@ KrylovArgs krylov_args(tf_handle* h) {
 }
 
 // The coarse space of the FP32 CG preconditioner for the operator just
-// assembled (mixed precision, one tissue): aggregates and block lists when
-// the operator's pattern changed, then A_c and its inverse.  With lagging on
-// (the default), A_c and its inverse are formed at the first NR iteration of
-// a step only and kept for the step's later iterations: the preconditioner
-// stays a fixed SPD operator within every solve, and a step repeated from the
-// same state repeats its bits.
-tf_status coarse_prepare(tf_handle* h, int nr_iteration) {
+// assembled (mixed precision): one segment per tissue, each with its own
+// aggregates and dense coarse inverse, so that a tissue of a batch gets the
+// correction it gets when stepped alone.  Aggregates and block lists are
+// built when the operator's pattern changed, then A_c and its inverse.  With
+// lagging on (the default), A_c and its inverse are formed at the first NR
+// iteration of a step only and kept for the step's later iterations: the
+// preconditioner stays a fixed SPD operator within every solve, and a step
+// repeated from the same state repeats its bits.
+tf_status coarse_prepare(tf_handle* h, int nr_iteration, const int32_t* run) {
   CoarseOp& c = h->coarse;
   const bool bound = h->bound.ready;
   if (h->coarse_lag && nr_iteration > 0 && c.active && c.built && c.for_bound == bound)
     return TF_OK;
   c.active = false;
-  if (h->batch || h->coarse_cell == 0.0) return TF_OK;
+  if (h->coarse_cell == 0.0) return TF_OK;
   if (!c.built || c.for_bound != bound) {
-    double cell = h->coarse_cell;
-    if (cell < 0.0) {
-      // automatic: n_c = 6 x aggregates close to sqrt(stored blocks), so that the
-      // dense coarse product (4 n_c^2 bytes) stays near a tenth of an FP32 SpMV
-      // (40 bytes per block); the (x, y) bounding box is cut into n_c / (6 bands)
-      // square cells.  Bound systems keep whole columns and at most about 400
-      // cells, whose coarse operator stays well conditioned under
-      // circumference binding.
-      double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
-      for (int i = 0; i < h->Nn; ++i) {
-        x0 = std::min(x0, h->h_X[3 * i]); x1 = std::max(x1, h->h_X[3 * i]);
-        y0 = std::min(y0, h->h_X[3 * i + 1]); y1 = std::max(y1, h->h_X[3 * i + 1]);
+    std::vector<CoarseSeg> segs(h->n_tissues);
+    for (int T = 0; T < h->n_tissues; ++T) {
+      CoarseSeg& g = segs[T];
+      g.begin = h->int_off[T];
+      g.end = h->int_off[T] + (h->user_off[T + 1] - h->user_off[T]);
+      g.cell = h->coarse_cell;
+      if (g.cell < 0.0) {
+        // automatic: n_c = 6 x aggregates close to sqrt(stored blocks of the
+        // tissue), so that the dense coarse product (4 n_c^2 bytes) stays near
+        // a tenth of an FP32 SpMV (40 bytes per block); the (x, y) bounding
+        // box is cut into n_c / (6 bands) square cells.  Bound systems keep
+        // whole columns and at most about 400 cells, whose coarse operator
+        // stays well conditioned under circumference binding.
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        for (int i = g.begin; i < g.end; ++i) {
+          x0 = std::min(x0, h->h_X[3 * i]); x1 = std::max(x1, h->h_X[3 * i]);
+          y0 = std::min(y0, h->h_X[3 * i + 1]); y1 = std::max(y1, h->h_X[3 * i + 1]);
+        }
+        const double nc = std::sqrt((double)(h->h_rowptr[g.end] - h->h_rowptr[g.begin]));
+        const double cells = bound ? std::min(400.0, nc / 6.0) : nc / (6.0 * h->coarse_bands);
+        g.cell = std::sqrt(std::max((x1 - x0) * (y1 - y0), 1e-30) / std::max(cells, 1.0));
       }
-      const double nc = std::sqrt((double)h->h_nbr.size());
-      const double cells = bound ? std::min(400.0, nc / 6.0) : nc / (6.0 * h->coarse_bands);
-      cell = std::sqrt(std::max((x1 - x0) * (y1 - y0), 1e-30) / std::max(cells, 1.0));
     }
     TF_CHECK(hipStreamSynchronize(h->st));
     tiled_release(&h->ws);  // captured chunks hold the previous coarse buffers
@@ -828,9 +836,9 @@ tf_status coarse_prepare(tf_handle* h, int nr_iteration) {
     // master (circumference binding slaves x and y of every node above the
     // basal one), which would leave the upper bands of such columns without
     // the modes they are built from
-    hipError_t e = bound ? coarse_build(c, h->Nn, h->h_X.data(), cell, 1, B.h_slice_base,
+    hipError_t e = bound ? coarse_build(c, h->Nn, h->h_X.data(), segs, 1, B.h_slice_base,
                                         B.h_col, B.h_row_len)
-                         : coarse_build(c, h->Nn, h->h_X.data(), cell, h->coarse_bands,
+                         : coarse_build(c, h->Nn, h->h_X.data(), segs, h->coarse_bands,
                                         h->h_slice_base, h->h_col, h->h_row_len);
     if (e == hipErrorInvalidValue) {
       set_error("coarse space: cell size gives too many aggregates");
@@ -842,9 +850,10 @@ tf_status coarse_prepare(tf_handle* h, int nr_iteration) {
   }
   if (bound)
     TF_CHECK(coarse_factor(c, h->bound.pat.col, h->bound.val32, h->bound.pat.n_slots,
-                           h->dof_flags, h->st));
+                           h->dof_flags, run, h->st));
   else
-    TF_CHECK(coarse_factor(c, h->pat.col, h->ws.val32, h->pat.n_slots, h->dof_flags, h->st));
+    TF_CHECK(coarse_factor(c, h->pat.col, h->ws.val32, h->pat.n_slots, h->dof_flags, run,
+                           h->st));
   return TF_OK;
 }
 
@@ -1223,7 +1232,7 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     if (s != TF_OK) return s;
     const int method = h->krylov_method[mixed];
     if (mixed && method == KRYLOV_CG) {
-      s = coarse_prepare(h, it[0]);
+      s = coarse_prepare(h, it[0], act.data());
       if (s != TF_OK) return s;
     } else {
       h->coarse.active = false;

@@ -95,9 +95,10 @@ struct TC {
   int stride;
   const int32_t* active; // per tissue: 0 = skip the solve (x = 0)
   double rtol;
-  // two-level CG (one tissue): per-workgroup partial (w, y) of the coarse
-  // product, which completes (r, z) = (r, D^-1 r) + (P^T r, A_c^-1 P^T r)
+  // two-level CG: per-workgroup partial (w, y) of the coarse product, which
+  // completes (r, z) = (r, D^-1 r) + (P^T r, A_c^-1 P^T r)
   const double* cpart;
+  const int32_t* cpart_off;  // per tissue: its range of product workgroups
   int n_cpart;
 };
 
@@ -464,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void tzc_spmv(TC c, int par, const T* __res
   if (!tile_ctx(c, t)) return;
   if (c.state[2 * t.T] != K_RUN) return;
   double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
-  if (c.n_cpart) rho += ordered_sum(c.cpart, 0, c.n_cpart);
+  if (c.n_cpart) rho += ordered_sum(c.cpart, c.cpart_off[t.T], c.cpart_off[t.T + 1]);
   const double rr = tissue_sum(c, par ? TC_RR1 : TC_RR0, t.T);
   const double rnorm = sqrt(rr);
   const bool first_it = (c.state[2 * t.T + 1] == 0);
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void tzc_update(TC c, int par, T* x, T* r, 
   if (!tile_ctx(c, t)) return;
   if (c.state[2 * t.T] != K_RUN) return;
   double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
-  if (c.n_cpart) rho += ordered_sum(c.cpart, 0, c.n_cpart);
+  if (c.n_cpart) rho += ordered_sum(c.cpart, c.cpart_off[t.T], c.cpart_off[t.T + 1]);
   const double pq = tissue_sum(c, TC_PQ, t.T);
   const double alpha = rho / pq;
   if (!(pq > 0.0) || !isfinite(alpha)) {
@@ -870,6 +871,7 @@ hipError_t tiled_cg_impl(const KrylovArgs& k, const CV<T>& V, const double* b, d
   TC c = make_tc(k, rtol);
   if (V.co) {
     c.cpart = V.co->cpart;
+    c.cpart_off = V.co->cpart_off;
     c.n_cpart = V.co->n_gemv;
   }
   hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, c, b, V.r, V.z, V.x, V.p0,
@@ -1124,6 +1126,7 @@ hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int method, int
     TC cc = c;
     if (V.co) {
       cc.cpart = V.co->cpart;
+      cc.cpart_off = V.co->cpart_off;
       cc.n_cpart = V.co->n_gemv;
     }
     hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, cc, (const double*)ws->b, V.r,
