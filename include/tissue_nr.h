@@ -341,8 +341,9 @@ tf_status tf_copy_bandwidth(int64_t bytes, int32_t reps, double* gb_per_s);
  * (default on; profiling launches directly).  No reference counterpart. */
 tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable);
 
-/* Two-level preconditioner of the mixed-precision CG solve (single-tissue
- * handles; no reference counterpart).  Block Jacobi is completed by a coarse
+/* Two-level preconditioner of the mixed-precision CG solve (no reference
+ * counterpart; in a batch handle every tissue has a coarse space of its own,
+ * the one it has alone).  Block Jacobi is completed by a coarse
  * correction P A_c^-1 P^T r, where P holds the rigid-body modes (three
  * translations, three rotations) of node aggregates: the nodes whose
  * reference (x, y) lies in one square cell of side cell_size and whose
@@ -350,7 +351,7 @@ tf_status tf_nr_set_graphs(tf_handle* h, int32_t enable);
  * 3, tf_nr_set_coarse_bands).  A_c = P^T J P is formed on the device and
  * inverted (FP64) at the first NR iteration of every mixed step
  * (tf_nr_set_coarse_lag).  cell_size < 0: automatic (the default): about
- * sqrt(stored blocks) coarse DOF, so that the dense coarse product stays near
+ * sqrt(stored blocks) coarse DOF per tissue, so that the dense coarse product stays near
  * a tenth of an FP32 SpMV; 0: off (block Jacobi only).  The
  * converged positions do not depend on it, only the Krylov iteration count.
  * tf_nr_coarse_info reports the aggregates of the last build and whether the
