@@ -206,6 +206,40 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
                         int32_t maxit, int32_t precision_mode, double krylov_rtol, double* u_out,
                         tf_step_stats* stats);
 
+/* Young's modulus time series (YoungsModulusModifier, YoungsModulusModifier.cpp:1-177,
+ * with its base TimeSeriesPhysicalProperties), single-tissue handles.
+ * tf_nr_add_stiffness_series replaces the YoungsModulusModifier constructor
+ * (ModelInputObject.cpp:1989-1990): n_times grids of ny x nx multipliers
+ * (grids[t][iy][ix], the rows of the reference's grid files) applied at
+ * times[t] (s); linear != 0 for a linear change of the multiplier, 0 for an
+ * exponential one; method 0 (nearest grid point), 1 (bilinear) or 2 (lower
+ * corner) for the rate at a prism's position.  applicable (n_prisms) holds
+ * YoungsModulusModifier::IsItApplicable per prism (the placement / type / ECM
+ * test of YoungsModulusModifier.cpp:74-100); rel_pos (n_prisms x 2) the
+ * prism's relative (x, y) position in the tissue's bounding box
+ * (ShapeBase::relativePosInBoundingBox, after the apical positioning of
+ * Simulation.cpp:6147-6167).  Needs tf_nr_set_material first.
+ * tf_nr_apply_stiffness_series replaces Simulation::ApplyAllYoungsModulusModifiers
+ *   (Simulation.cpp:878-884) at time t: each series in the order added, over
+ *   the interval [times[k], times[k+1]) that holds t (none when t lies past
+ *   the last time or before the first), updates the time-series multiplier
+ *   of every applicable prism by rate * dt (linear) or exp(rate * dt), and the
+ *   prism's Lame parameters (Prism::updateElasticProperties, Prism.cpp:526-527:
+ *   multiplier x E as set by tf_nr_set_material).
+ * tf_nr_advance applies the series at the start of every step, at the handle's
+ *   time (tf_nr_set_time, default 0; the time advances by dt per step, as
+ *   Simulation::currSimTimeSec does, Simulation.cpp:6129).
+ * tf_nr_get_stiffness_multiplier: the multiplier per prism
+ *   (ShapeBase::StiffnessTimeSeriesMultiplier, 1 until a series changes it). */
+tf_status tf_nr_add_stiffness_series(tf_handle* h, int32_t n_times, const double* times,
+                                     int32_t nx, int32_t ny, const double* grids, int32_t linear,
+                                     int32_t method, const uint8_t* applicable,
+                                     const double* rel_pos);
+tf_status tf_nr_apply_stiffness_series(tf_handle* h, double t, double dt);
+tf_status tf_nr_set_time(tf_handle* h, double t);
+tf_status tf_nr_get_time(tf_handle* h, double* t);
+tf_status tf_nr_get_stiffness_multiplier(tf_handle* h, double* out);
+
 /* Node-node packing (volume exclusion between surface nodes), the contact
  * terms Simulation::updateStepNR adds inside every NR iteration
  * (Simulation.cpp:6680-6698).  Single-tissue handles without slave/master

@@ -64,6 +64,12 @@ class ShapeBase {
   bool IsAblated = false;
   int tissuePlacement = 2, tissueType = 0;  // ShapeBase.cpp:905-1037
   bool spansWholeTissue = false;
+  // Young's modulus time series (ShapeBase.h:147): the multiplier the series
+  // set, the ECM flags its applicability test reads, and the relative (x, y)
+  // position in the tissue's bounding box it looks the rate up at
+  double StiffnessTimeSeriesMultiplier = 1.0;
+  bool isECMMimicing = false, isECMMimimcingAtCircumference = false, atBasalBorderOfECM = false;
+  double relativePosInBoundingBox[2] = {0.0, 0.0};
   int getNodeNumber() const { return 6; }
   const int* getNodeIds() const { return NodeIds.data(); }
   // ShapeBase::updatePositions (ShapeBase.cpp:3279-3285): element positions
@@ -250,6 +256,60 @@ class NewtonRaphsonSolver {
     int32_t n = 0;
     check(tf_nr_detect_packing(h, u.data(), &n));
     return n;
+  }
+
+  // The YoungsModulusModifier constructor (YoungsModulusModifier.cpp:5-13):
+  // TimeSeriesGrid[t][iy][ix] applied at TimeToApply[t], the linear or
+  // exponential change, the rate method (0 nearest, 1 bilinear, 2 lower
+  // corner) and WhereToApply (TimeSeriesPhysicalProperties.cpp:112-120).
+  // The prisms it applies to follow YoungsModulusModifier::IsItApplicable
+  // (YoungsModulusModifier.cpp:74-100).
+  void addYoungsModulusModifier(const std::vector<std::vector<std::vector<double>>>& TimeSeriesGrid,
+                                const std::vector<double>& TimeToApply, bool linear, int method,
+                                const std::array<bool, 7>& WhereToApply,
+                                const std::vector<std::unique_ptr<Node>>& Nodes,
+                                const std::vector<std::unique_ptr<ShapeBase>>& Elements) {
+    bind(Nodes, Elements);
+    const int nt = (int)TimeSeriesGrid.size();
+    const int ny = nt ? (int)TimeSeriesGrid[0].size() : 0;
+    const int nx = ny ? (int)TimeSeriesGrid[0][0].size() : 0;
+    std::vector<double> g;
+    for (const auto& grid : TimeSeriesGrid)
+      for (const auto& row : grid) g.insert(g.end(), row.begin(), row.end());
+    if (g.size() != (size_t)nt * nx * ny) throw std::runtime_error("ragged time-series grid");
+    const int ne = (int)Elements.size();
+    std::vector<uint8_t> app(ne);
+    std::vector<double> rel(2 * (size_t)ne);
+    const bool cl = WhereToApply[0], pm = WhereToApply[1], ap = WhereToApply[2],
+               mid = WhereToApply[3], bas = WhereToApply[4], cecm = WhereToApply[6];
+    for (int e = 0; e < ne; ++e) {
+      const ShapeBase& el = *Elements[e];
+      const int pl = el.tissuePlacement;
+      bool on = false;
+      if ((cl && el.tissueType == 0) || (pm && el.tissueType == 1)) {
+        if (el.isECMMimicing)
+          on = (cecm && pl == 0) || (cecm && el.isECMMimimcingAtCircumference && pl != 0);
+        else
+          on = (ap && (pl == 1 || pl == 4)) || (mid && pl == 2) ||
+               (bas && (pl == 0 || el.atBasalBorderOfECM));
+      }
+      app[e] = on ? 1 : 0;
+      rel[2 * e] = el.relativePosInBoundingBox[0];
+      rel[2 * e + 1] = el.relativePosInBoundingBox[1];
+    }
+    check(tf_nr_add_stiffness_series(h, nt, TimeToApply.data(), nx, ny, g.data(), linear ? 1 : 0,
+                                     method, app.data(), rel.data()));
+  }
+
+  // Simulation::ApplyAllYoungsModulusModifiers (Simulation.cpp:878-884) at
+  // currSimTimeSec: the multipliers come back into the elements.
+  void ApplyAllYoungsModulusModifiers(double currSimTimeSec, double dt,
+                                      const std::vector<std::unique_ptr<ShapeBase>>& Elements) {
+    if (!h) throw std::runtime_error("ApplyAllYoungsModulusModifiers: no modifier added");
+    check(tf_nr_apply_stiffness_series(h, currSimTimeSec, dt));
+    std::vector<double> m(Elements.size());
+    check(tf_nr_get_stiffness_multiplier(h, m.data()));
+    for (size_t e = 0; e < Elements.size(); ++e) Elements[e]->StiffnessTimeSeriesMultiplier = m[e];
   }
 
   tf_handle* handle() const { return h; }

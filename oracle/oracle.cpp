@@ -24,6 +24,8 @@
 //   dense transposed solve -> numpy.linalg.solve
 //   NR step                -> run07007 per-step replay (positions, |du| log)
 //   BSR + BiCGStab path    -> dense transposed solve on small meshes
+//   Young's modulus series -> closed-form rates, known-answer grid lookups,
+//                             multi-step products against the closed form
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -1491,6 +1493,106 @@ int orc_detect_packing(int n_nodes, int n_el, const int32_t* conn, const double*
     }
   }
   return count;
+}
+
+// ---- Young's modulus time series (YoungsModulusModifier.cpp) ----
+
+// YoungsModulusModifier::CalculateYoungsModulusChangeRate
+// (YoungsModulusModifier.cpp:15-46): rate[t][i][j] from the grids
+// grids[t][i][j] (t < n_times, i < ny, j < nx).
+void orc_stiffness_rates(int n_times, const double* times, int nx, int ny, const double* grids,
+                         int linear, double* rate) {
+  for (int t = 0; t + 1 < n_times; ++t) {
+    const double deltaT = times[t + 1] - times[t];
+    for (int i = 0; i < ny; ++i)
+      for (int j = 0; j < nx; ++j) {
+        const double g0 = grids[((size_t)t * ny + i) * nx + j];
+        const double g1 = grids[((size_t)(t + 1) * ny + i) * nx + j];
+        rate[((size_t)t * ny + i) * nx + j] = linear ? ((g1 / g0) - 1) / deltaT : log(g1 / g0) / deltaT;
+      }
+  }
+}
+
+// YoungsModulusModifier::calculateCurrentRateGridIndex
+// (YoungsModulusModifier.cpp:48-72): the interval holding t, -1 when none.
+int orc_stiffness_interval(int n_times, const double* times, double t) {
+  if (t > times[n_times - 1]) return -1;
+  for (int i = 0; i < n_times - 1; ++i)
+    if (t >= times[i] && t < times[i + 1]) return i;
+  return -1;
+}
+
+// ShapeBase::convertRelativePosToGridIndex (ShapeBase.cpp:834-860).
+static void orc_grid_index(const double* rel, int nx, int ny, int* ix, int* iy, double* fx,
+                           double* fy) {
+  double r0 = rel[0], r1 = rel[1];
+  r0 *= (float)(nx - 1);
+  r1 *= (float)(ny - 1);
+  *ix = (int)floor(r0);
+  *fx = r0 - *ix;
+  *iy = (int)floor(r1);
+  *fy = r1 - *iy;
+  if (*ix >= nx - 1) {
+    *ix = nx - 2;
+    *fx = 1.0;
+  } else if (*ix < 0) {
+    *ix = 0;
+    *fx = 0.0;
+  }
+  if (*iy >= ny - 1) {
+    *iy = ny - 2;
+    *fy = 1.0;
+  } else if (*iy < 0) {
+    *iy = 0;
+    *fy = 0.0;
+  }
+}
+
+// YoungsModulusModifier::GetCurrentMultiplierChangeRate
+// (YoungsModulusModifier.cpp:106-156) for one position.
+double orc_stiffness_rate_at(const double* rate_t, int nx, int ny, int method, const double* rel) {
+  int ix, iy;
+  double fx, fy;
+  orc_grid_index(rel, nx, ny, &ix, &iy, &fx, &fy);
+  double nb[4];
+  nb[0] = rate_t[(size_t)iy * nx + ix];
+  nb[1] = rate_t[(size_t)iy * nx + ix + 1];
+  nb[2] = rate_t[(size_t)(iy + 1) * nx + ix];
+  nb[3] = rate_t[(size_t)(iy + 1) * nx + ix + 1];
+  double out = 0.0;
+  if (method == 0) {
+    if (fx > 0.5) out = fy > 0.5 ? nb[3] : nb[1];
+    else out = fy > 0.5 ? nb[2] : nb[0];
+  } else if (method == 1) {
+    out = nb[0] * (1.0 - fx) * (1.0 - fy) + nb[1] * fx * (1.0 - fy) +
+          nb[2] * (1.0 - fx) * fy + nb[3] * fx * fy;
+  } else if (method == 2) {
+    out = nb[0];
+  }
+  return out;
+}
+
+// YoungsModulusModifier::updateTimeSeriesStiffnessMultiplier
+// (YoungsModulusModifier.cpp:158-177) over every prism, then
+// Prism::updateElasticProperties (Prism.cpp:526-527) with E the prism's E
+// times its stiffness multiplier.  stm, lambda and mu are updated in place for
+// the applicable prisms; returns the interval used (-1: nothing applied).
+int orc_stiffness_apply(int n_el, double t, double dt, int n_times, const double* times, int nx,
+                        int ny, const double* rate, int linear, int method,
+                        const uint8_t* applicable, const double* rel_pos, const double* E,
+                        const double* nu, double* stm, double* lambda, double* mu) {
+  const int k = orc_stiffness_interval(n_times, times, t);
+  if (k < 0) return -1;
+  const double* rt = rate + (size_t)k * nx * ny;
+  for (int e = 0; e < n_el; ++e) {
+    if (!applicable[e]) continue;
+    const double r = orc_stiffness_rate_at(rt, nx, ny, method, rel_pos + 2 * e);
+    stm[e] = linear ? stm[e] + r * dt : stm[e] * exp(r * dt);
+    const double v = nu[e];
+    lambda[e] = stm[e] * E[e] * v / (1 + v) / (1 - 2.0 * v);
+    mu[e] = stm[e] * E[e] / 2.0 / (1 + v);
+  }
+  return k;
 }
 
 }  // extern "C"

@@ -63,6 +63,16 @@ def lib():
         L.orc_reference_volumes.argtypes = [ctypes.c_int, _d, _d]
         L.orc_node_masses.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d]
         L.orc_bounding_box.argtypes = [ctypes.c_int, _d, ctypes.c_int, _d]
+        L.orc_stiffness_rates.argtypes = [ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, _d,
+                                          ctypes.c_int, _d]
+        L.orc_stiffness_interval.argtypes = [ctypes.c_int, _d, ctypes.c_double]
+        L.orc_stiffness_interval.restype = ctypes.c_int
+        L.orc_stiffness_rate_at.argtypes = [_d, ctypes.c_int, ctypes.c_int, ctypes.c_int, _d]
+        L.orc_stiffness_rate_at.restype = ctypes.c_double
+        L.orc_stiffness_apply.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, _d, ctypes.c_int, ctypes.c_int, _d,
+                                          ctypes.c_int, ctypes.c_int, _u8, _d, _d, _d, _d, _d, _d]
+        L.orc_stiffness_apply.restype = ctypes.c_int
         L.orc_packing_thresholds.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _i, _i, _u8,
                                              ctypes.c_int, _d, _d, _d]
         L.orc_detect_packing.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _i, _u8, _d, _d, _d,
@@ -312,3 +322,45 @@ def detect_packing(n_nodes, conn, u, node_placement, mass, grid_sq, bb, node_exc
     pairs = np.zeros((max(n, 1), 2), np.int32)
     lib().orc_detect_packing(*args, _p(pairs, _i), n)
     return pairs[:n]
+
+
+# ---- Young's modulus time series (YoungsModulusModifier.cpp)
+def stiffness_rates(times, grids, linear=True):
+    """Change-rate grids, (n_times - 1, ny, nx), of grids (n_times, ny, nx)."""
+    times = _f64(times); grids = _f64(grids)
+    nt, ny, nx = grids.shape
+    out = np.empty((nt - 1, ny, nx))
+    lib().orc_stiffness_rates(nt, _p(times, _d), nx, ny, _p(grids, _d), int(bool(linear)),
+                              _p(out, _d))
+    return out
+
+
+def stiffness_interval(times, t):
+    times = _f64(times)
+    return lib().orc_stiffness_interval(times.size, _p(times, _d), float(t))
+
+
+def stiffness_rate_at(rate_t, method, rel):
+    rate_t = _f64(rate_t); rel = _f64(rel).reshape(2)
+    ny, nx = rate_t.shape
+    return lib().orc_stiffness_rate_at(_p(rate_t, _d), nx, ny, int(method), _p(rel, _d))
+
+
+class StiffnessSeries:
+    """One Young's modulus series over a mesh's prisms, stepped on the CPU."""
+
+    def __init__(self, times, grids, linear, method, applicable, rel_pos):
+        self.times = _f64(times); self.grids = _f64(grids)
+        self.linear = int(bool(linear)); self.method = int(method)
+        self.rate = stiffness_rates(self.times, self.grids, linear)
+        self.applicable = np.ascontiguousarray(applicable, np.uint8).reshape(-1)
+        self.rel = _f64(rel_pos).reshape(-1, 2)
+
+    def apply(self, t, dt, E, nu, stm, lam, mu):
+        """Updates stm, lam, mu in place; returns the interval used (-1: none)."""
+        E = _f64(E); nu = _f64(nu)
+        nt, ny, nx = self.grids.shape
+        return lib().orc_stiffness_apply(
+            E.size, float(t), float(dt), nt, _p(self.times, _d), nx, ny, _p(self.rate, _d),
+            self.linear, self.method, _p(self.applicable, _u8), _p(self.rel, _d), _p(E, _d),
+            _p(nu, _d), _p(stm, _d), _p(lam, _d), _p(mu, _d))

@@ -1,0 +1,134 @@
+"""Young's modulus time series (YoungsModulusModifier.cpp): the oracle pinned
+by closed forms and known answers, the per-prism host helpers and the grid
+file format.  CPU only."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from tissueorigami_amd import io, tissue
+
+
+def test_rates_closed_form(orc):
+    g = np.stack([np.full((3, 4), 2.0), np.full((3, 4), 4.0), np.full((3, 4), 1.0)])
+    lin = orc.stiffness_rates([0.0, 100.0, 300.0], g, linear=True)
+    ex = orc.stiffness_rates([0.0, 100.0, 300.0], g, linear=False)
+    assert lin.shape == (2, 3, 4)
+    assert np.all(lin[0] == 0.01) and np.all(lin[1] == -0.75 / 200.0)
+    assert np.allclose(ex[0], math.log(2.0) / 100.0, rtol=1e-15)
+    assert np.allclose(ex[1], -math.log(4.0) / 200.0, rtol=1e-15)
+
+
+def test_interval_semantics(orc):
+    t = [0.0, 100.0, 300.0]
+    assert orc.stiffness_interval(t, 0.0) == 0
+    assert orc.stiffness_interval(t, 99.9) == 0
+    assert orc.stiffness_interval(t, 100.0) == 1
+    assert orc.stiffness_interval(t, 299.0) == 1
+    # the last time point opens no interval; nothing past it or before the first
+    assert orc.stiffness_interval(t, 300.0) == -1
+    assert orc.stiffness_interval(t, 301.0) == -1
+    assert orc.stiffness_interval(t, -1.0) == -1
+
+
+def test_rate_lookup_known_answers(orc):
+    ny, nx = 4, 5
+    iy, ix = np.mgrid[0:ny, 0:nx]
+    plane = 0.25 + 0.5 * ix - 0.125 * iy   # bilinear interpolation reproduces a plane
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        rel = rng.uniform(0.0, 0.999, 2)
+        x, y = rel[0] * (nx - 1), rel[1] * (ny - 1)
+        assert abs(orc.stiffness_rate_at(plane, 1, rel) - (0.25 + 0.5 * x - 0.125 * y)) < 1e-14
+    # nearest grid point (ties go to the lower corner), lower corner, clamps
+    grid = np.arange(ny * nx, dtype=float).reshape(ny, nx)
+    at = lambda m, rx, ry: orc.stiffness_rate_at(grid, m, [rx, ry])
+    assert at(0, 0.30 / 4, 0.70 / 3) == grid[1, 0]      # cell (0, 0), fracs 0.3 / 0.7
+    assert at(0, 0.70 / 4, 0.70 / 3) == grid[1, 1]
+    assert at(0, 0.70 / 4, 0.20 / 3) == grid[0, 1]
+    assert at(0, 0.20 / 4, 0.80 / 3) == grid[1, 0]
+    assert at(0, 0.50 / 4, 0.50 / 3) == grid[0, 0]
+    assert at(2, 2.9 / 4, 1.9 / 3) == grid[1, 2]
+    assert at(1, 1.0, 1.0) == grid[ny - 1, nx - 1]      # upper edge: last cell, frac 1
+    assert at(1, 1.7, 2.0) == grid[ny - 1, nx - 1]
+    assert at(1, -0.3, -2.0) == grid[0, 0]              # below the box: first cell, frac 0
+    assert at(0, 1.0, 0.0) == grid[0, nx - 1]
+
+
+@pytest.mark.parametrize("linear", [True, False])
+def test_multiplier_reaches_the_next_grid(orc, linear):
+    # stepping through an interval in steps of dt takes the multiplier of an
+    # applicable prism from 1 to G1 / G0 at the grid point (exactly for the
+    # linear change, to rounding for the exponential one); prisms the series
+    # does not apply to keep 1, and the Lame parameters follow (multiplier x E)
+    ny, nx = 3, 3
+    g0 = np.ones((ny, nx))
+    g1 = np.array([[1.0, 1.5, 2.0], [0.5, 1.0, 3.0], [1.25, 2.5, 0.8]])
+    times = [0.0, 600.0, 1200.0]
+    grids = np.stack([g0, g1, g1])
+    rel = np.array([[0.0, 0.0], [0.5, 0.0], [1.0, 1.0], [0.5, 0.5], [1.0, 0.5]])
+    app = np.array([1, 1, 1, 0, 1], np.uint8)
+    s = orc.StiffnessSeries(times, grids, linear, 0 if linear else 1, app, rel)
+    E = np.array([100.0, 200.0, 300.0, 400.0, 500.0]); nu = np.full(5, 0.3)
+    stm = np.ones(5); lam = np.zeros(5); mu = np.zeros(5)
+    dt = 60.0
+    for k in range(10):
+        assert s.apply(k * dt, dt, E, nu, stm, lam, mu) == 0
+    want = np.array([g1[0, 0], g1[0, 1], g1[2, 2], 1.0, g1[1, 2]])
+    if linear:
+        assert np.allclose(stm, want, rtol=0, atol=4e-16 * 10)
+    else:
+        assert np.allclose(stm, want, rtol=1e-14)
+    assert stm[3] == 1.0 and lam[3] == 0.0
+    L, M = orc.lame(stm * E, nu)
+    sel = app.astype(bool)
+    assert np.array_equal(lam[sel], L[sel]) and np.array_equal(mu[sel], M[sel])
+    # the second interval has rate 0: nothing moves; past the end nothing applies
+    before = stm.copy()
+    assert s.apply(600.0, dt, E, nu, stm, lam, mu) == 1
+    assert np.array_equal(stm, before)
+    assert s.apply(1200.0, dt, E, nu, stm, lam, mu) == -1
+
+
+def test_applicable_follows_the_reference_rule():
+    # columnar apical, basal, mid, lateral; peripodial apical; columnar basal ECM;
+    # lateral ECM at the circumference; the basal-lateral corner ECM prism
+    tt = [0, 0, 0, 0, 1, 0, 0, 0]
+    pl = [1, 0, 2, 4, 1, 0, 4, 0]
+    ecm = [0, 0, 0, 0, 0, 1, 1, 1]
+    circ = [0, 0, 0, 0, 0, 0, 1, 1]
+    w = lambda *f: tissue.stiffness_applicable(f, tt, pl, ecm, circ).tolist()
+    assert w(1, 0, 1, 0, 0, 0, 0) == [1, 0, 0, 1, 0, 0, 0, 0]
+    assert w(1, 0, 0, 1, 1, 0, 0) == [0, 1, 1, 0, 0, 0, 0, 0]
+    assert w(1, 0, 0, 0, 0, 0, 1) == [0, 0, 0, 0, 0, 1, 1, 1]
+    assert w(0, 1, 1, 1, 1, 1, 1) == [0, 0, 0, 0, 1, 0, 0, 0]
+    border = [0, 0, 1, 0, 0, 0, 0, 0]
+    assert tissue.stiffness_applicable((1, 0, 0, 0, 1, 0, 0), tt, pl, ecm, circ,
+                                       border).tolist() == [0, 1, 1, 0, 0, 0, 0, 0]
+
+
+def test_relative_positions_and_columns():
+    m = tissue.rectangle(lx=8.0, ly=4.0, side=1.0, thickness=2.0, n_layers=2)
+    u = m["nodes"]
+    rel = tissue.relative_positions(u, m["conn"])
+    c = u[m["conn"]].mean(axis=1)
+    assert np.allclose(rel[:, 0], (c[:, 0] - u[:, 0].min()) / np.ptp(u[:, 0]), rtol=1e-14)
+    assert np.allclose(rel[:, 1], (c[:, 1] - u[:, 1].min()) / np.ptp(u[:, 1]), rtol=1e-14)
+    assert rel.min() > 0.0 and rel.max() < 1.0
+    ne = len(m["conn"]); half = ne // 2
+    ap = np.full(ne, -1); ap[:half] = np.arange(half) + half
+    rel2 = tissue.relative_positions(u, m["conn"], apical_of=ap)
+    assert np.array_equal(rel2[:half], rel[half:2 * half])
+    assert np.array_equal(rel2[half:], rel[half:])
+
+
+def test_property_grid_file_round_trip(tmp_path):
+    p = os.path.join(tmp_path, "ym_t0.txt")
+    with open(p, "w") as f:   # "nX nY", then nY rows of nX values
+        f.write("4 2\n1 2 3 4\n5 6 7 8.5\n")
+    g = io.read_property_grid(p)
+    assert g.shape == (2, 4) and g[1, 3] == 8.5 and g[0, 1] == 2.0
+    q = os.path.join(tmp_path, "out.txt")
+    io.write_property_grid(q, g)
+    assert np.array_equal(io.read_property_grid(q), g)

@@ -24,6 +24,18 @@ void set_error(const std::string& s) { g_err = s; }
 
 using namespace tf;
 
+// One Young's modulus time series (YoungsModulusModifier.h:9-40): grids at
+// n_t time points, the change rate of each interval (host, (n_t - 1) x ny x nx
+// and on the device), the prisms it applies to, and their relative (x, y)
+// position in the tissue's bounding box.
+struct StiffSeries {
+  int32_t n_t = 0, nx = 0, ny = 0, method = 0, linear = 1;
+  std::vector<double> times;
+  double* rate = nullptr;
+  uint8_t* mask = nullptr;
+  double* relpos = nullptr;
+};
+
 struct tf_handle {
   int32_t Nn = 0, Ne = 0;  // internal node count (tissues padded to 64-node slices) and prisms
   // tissues of the handle: one for tf_nr_create, several for tf_batch_create
@@ -53,6 +65,14 @@ struct tf_handle {
   double* lam = nullptr;
   double* mu = nullptr;
   double* eta = nullptr;
+  // Young's modulus time series (YoungsModulusModifier): E and nu as set, the
+  // per-prism time-series multiplier, the series in application order
+  double* E_d = nullptr;
+  double* nu_d = nullptr;
+  double* stm = nullptr;
+  bool stm_touched = false;
+  std::vector<StiffSeries> series;
+  double sim_time = 0.0;
   double* extv = nullptr;
   double* area = nullptr;
   uint8_t* dof_flags = nullptr;
@@ -727,7 +747,8 @@ tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos
       dalloc(h, &h->invFg, Ne * 9) != hipSuccess || dalloc(h, &h->lam, Ne) != hipSuccess ||
       dalloc(h, &h->mu, Ne) != hipSuccess || dalloc(h, &h->eta, Ne) != hipSuccess ||
       dalloc(h, &h->extv, Nn * 3) != hipSuccess || dalloc(h, &h->area, Nn) != hipSuccess ||
-      dalloc(h, &h->dof_flags, Nn * 3) != hipSuccess) {
+      dalloc(h, &h->dof_flags, Nn * 3) != hipSuccess || dalloc(h, &h->E_d, Ne) != hipSuccess ||
+      dalloc(h, &h->nu_d, Ne) != hipSuccess || dalloc(h, &h->stm, Ne) != hipSuccess) {
     set_error("hipMalloc failed");
     return fail(TF_ERR_HIP);
   }
@@ -1011,15 +1032,23 @@ tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, co
     return TF_ERR_ARG;
   }
   const int Ne = h->Ne, Nn = h->Nn;
-  std::vector<double> lam(Ne), mu(Ne);
+  std::vector<double> lam(Ne), mu(Ne), stm(Ne, 1.0);
+  if (h->stm_touched)
+    TF_CHECK(hipMemcpy(stm.data(), h->stm, sizeof(double) * Ne, hipMemcpyDeviceToHost));
   bool any_visc = false;
   for (int e = 0; e < Ne; ++e) {
-    // Prism::setElasticProperties (Prism.cpp:512-513)
+    // Prism::updateElasticProperties (Prism.cpp:526-527), E the prism's E
+    // times its stiffness multiplier; the time-series multiplier is 1 until
+    // a Young's modulus series changes it
     const double v = nu[e];
-    lam[e] = E[e] * v / (1 + v) / (1 - 2.0 * v);
-    mu[e] = E[e] / 2.0 / (1 + v);
+    lam[e] = stm[e] * E[e] * v / (1 + v) / (1 - 2.0 * v);
+    mu[e] = stm[e] * E[e] / 2.0 / (1 + v);
     if (eta[e] != 0) any_visc = true;
   }
+  if (!h->stm_touched)
+    TF_CHECK(hipMemcpy(h->stm, stm.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->E_d, E, sizeof(double) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->nu_d, nu, sizeof(double) * Ne, hipMemcpyHostToDevice));
   std::vector<uint8_t> fu((size_t)h->n_user * 3, 0);
   if (fixed)
     for (size_t i = 0; i < fu.size(); ++i) fu[i] = fixed[i] ? 1 : 0;
@@ -1200,6 +1229,65 @@ tf_status tf_nr_solve(tf_handle* h, const double* b, double* x, double rtol, int
 }  // extern "C"
 
 namespace {
+
+// One Young's modulus series over every prism it applies to
+// (YoungsModulusModifier::updateTimeSeriesStiffnessMultiplier,
+// YoungsModulusModifier.cpp:158-177): the change rate at the prism's grid
+// position (GetElementPosition with ShapeBase::convertRelativePosToGridIndex,
+// ShapeBase.cpp:834-860; GetCurrentMultiplierChangeRate, :106-156), the
+// multiplier's linear or exponential update, and the Lame parameters
+// (Prism::updateElasticProperties, Prism.cpp:526-527).  Contraction is off so
+// that every product and sum rounds where the reference's does.
+__global__ __launch_bounds__(256) void stiffness_series_kernel(
+    int32_t n_e, const uint8_t* __restrict__ mask, const double* __restrict__ relpos,
+    const double* __restrict__ rate, int32_t nx, int32_t ny, int32_t method, int32_t linear,
+    double dt, const double* __restrict__ E, const double* __restrict__ nu, double* stm,
+    double* lam, double* mu) {
+#pragma clang fp contract(off)
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n_e || !mask[e]) return;
+  double rx = relpos[2 * e] * (double)(float)(nx - 1);
+  double ry = relpos[2 * e + 1] * (double)(float)(ny - 1);
+  int ix = (int)floor(rx), iy = (int)floor(ry);
+  double fx = rx - ix, fy = ry - iy;
+  if (ix >= nx - 1) { ix = nx - 2; fx = 1.0; }
+  else if (ix < 0) { ix = 0; fx = 0.0; }
+  if (iy >= ny - 1) { iy = ny - 2; fy = 1.0; }
+  else if (iy < 0) { iy = 0; fy = 0.0; }
+  const double n0 = rate[(size_t)iy * nx + ix], n1 = rate[(size_t)iy * nx + ix + 1];
+  const double n2 = rate[(size_t)(iy + 1) * nx + ix], n3 = rate[(size_t)(iy + 1) * nx + ix + 1];
+  double r;
+  if (method == 0) r = fx > 0.5 ? (fy > 0.5 ? n3 : n1) : (fy > 0.5 ? n2 : n0);
+  else if (method == 1)
+    r = n0 * (1.0 - fx) * (1.0 - fy) + n1 * fx * (1.0 - fy) + n2 * (1.0 - fx) * fy + n3 * fx * fy;
+  else r = n0;
+  const double m = linear ? stm[e] + r * dt : stm[e] * exp(r * dt);
+  stm[e] = m;
+  const double v = nu[e];
+  lam[e] = m * E[e] * v / (1 + v) / (1 - 2.0 * v);
+  mu[e] = m * E[e] / 2.0 / (1 + v);
+}
+
+// Simulation::ApplyAllYoungsModulusModifiers (Simulation.cpp:878-884) at time
+// t: every series in the order added, each over its interval of t
+// (YoungsModulusModifier::calculateCurrentRateGridIndex,
+// YoungsModulusModifier.cpp:48-72; none past the last time point or before
+// the first).
+tf_status apply_series(tf_handle* h, double t, double dt) {
+  for (StiffSeries& q : h->series) {
+    if (t > q.times[q.n_t - 1]) continue;
+    int k = -1;
+    for (int i = 0; i < q.n_t - 1; ++i)
+      if (t >= q.times[i] && t < q.times[i + 1]) { k = i; break; }
+    if (k < 0) continue;
+    hipLaunchKernelGGL(stiffness_series_kernel, dim3((h->Ne + 255) / 256), dim3(256), 0, h->st,
+                       h->Ne, q.mask, q.relpos, q.rate + (size_t)k * q.ny * q.nx, q.nx, q.ny,
+                       q.method, q.linear, dt, h->E_d, h->nu_d, h->stm, h->lam, h->mu);
+    TF_CHECK(hipGetLastError());
+    h->stm_touched = true;
+  }
+  return TF_OK;
+}
 
 // Mixed precision: FP32 inner solves are never asked for less than this
 // relative residual (FP64 refinement sweeps go further), and stop after
@@ -1475,6 +1563,103 @@ tf_status tf_nr_set_surface_flags(tf_handle* h, const int32_t* elem_placement,
   return TF_OK;
 }
 
+tf_status tf_nr_add_stiffness_series(tf_handle* h, int32_t n_times, const double* times,
+                                     int32_t nx, int32_t ny, const double* grids, int32_t linear,
+                                     int32_t method, const uint8_t* applicable,
+                                     const double* rel_pos) {
+  if (!h || n_times < 2 || !times || nx < 2 || ny < 2 || !grids || method < 0 || method > 2 ||
+      !applicable || !rel_pos) {
+    set_error("tf_nr_add_stiffness_series: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (h->batch) {
+    set_error("tf_nr_add_stiffness_series: single-tissue handles only");
+    return TF_ERR_ARG;
+  }
+  if (!h->have_material) {
+    set_error("tf_nr_add_stiffness_series: set the material first");
+    return TF_ERR_ARG;
+  }
+  StiffSeries q;
+  q.n_t = n_times; q.nx = nx; q.ny = ny; q.method = method; q.linear = linear ? 1 : 0;
+  q.times.assign(times, times + n_times);
+  // YoungsModulusModifier::CalculateYoungsModulusChangeRate
+  // (YoungsModulusModifier.cpp:15-46): per interval and grid point,
+  // (G1 / G0 - 1) / dT (linear) or log(G1 / G0) / dT (exponential)
+  const size_t g = (size_t)nx * ny;
+  std::vector<double> rate((size_t)(n_times - 1) * g);
+  for (int t = 0; t < n_times - 1; ++t) {
+    const double dT = times[t + 1] - times[t];
+    for (size_t k = 0; k < g; ++k) {
+      const double ratio = grids[(size_t)(t + 1) * g + k] / grids[(size_t)t * g + k];
+      rate[(size_t)t * g + k] = q.linear ? (ratio - 1) / dT : std::log(ratio) / dT;
+    }
+  }
+  std::vector<uint8_t> m(h->Ne);
+  for (int e = 0; e < h->Ne; ++e) {
+    m[e] = applicable[e] ? 1 : 0;
+    if (!std::isfinite(rel_pos[2 * e]) || !std::isfinite(rel_pos[2 * e + 1])) {
+      set_error("tf_nr_add_stiffness_series: relative positions must be finite");
+      return TF_ERR_ARG;
+    }
+  }
+  for (size_t k = 0; k < (size_t)n_times * g; ++k)
+    if (!std::isfinite(grids[k]) || !(grids[k] > 0.0)) {
+      set_error("tf_nr_add_stiffness_series: grid values must be positive and finite");
+      return TF_ERR_ARG;
+    }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  if (dalloc(h, &q.rate, rate.size()) != hipSuccess || dalloc(h, &q.mask, (size_t)h->Ne) != hipSuccess ||
+      dalloc(h, &q.relpos, 2 * (size_t)h->Ne) != hipSuccess) {
+    set_error("hipMalloc failed");
+    return TF_ERR_HIP;
+  }
+  TF_CHECK(hipMemcpy(q.rate, rate.data(), sizeof(double) * rate.size(), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(q.mask, m.data(), m.size(), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(q.relpos, rel_pos, sizeof(double) * 2 * h->Ne, hipMemcpyHostToDevice));
+  h->series.push_back(std::move(q));
+  return TF_OK;
+}
+
+tf_status tf_nr_set_time(tf_handle* h, double t) {
+  if (!h || !std::isfinite(t)) {
+    set_error("tf_nr_set_time: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  h->sim_time = t;
+  return TF_OK;
+}
+
+tf_status tf_nr_get_time(tf_handle* h, double* t) {
+  if (!h || !t) {
+    set_error("tf_nr_get_time: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  *t = h->sim_time;
+  return TF_OK;
+}
+
+tf_status tf_nr_apply_stiffness_series(tf_handle* h, double t, double dt) {
+  if (!h || !std::isfinite(t) || !std::isfinite(dt)) {
+    set_error("tf_nr_apply_stiffness_series: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  tf_status s = apply_series(h, t, dt);
+  if (s != TF_OK) return s;
+  TF_CHECK(hipStreamSynchronize(h->st));
+  return TF_OK;
+}
+
+tf_status tf_nr_get_stiffness_multiplier(tf_handle* h, double* out) {
+  if (!h || !out) {
+    set_error("tf_nr_get_stiffness_multiplier: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  TF_CHECK(hipMemcpy(out, h->stm, sizeof(double) * h->Ne, hipMemcpyDeviceToHost));
+  return TF_OK;
+}
+
 tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double dt, double tol,
                         int32_t maxit, int32_t precision_mode, double krylov_rtol, double* u_out,
                         tf_step_stats* stats) {
@@ -1499,8 +1684,14 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
   tf_status worst = TF_OK;
   std::vector<tf_step_stats> S(h->n_tissues);
   for (int s = 0; s < n_steps; ++s) {
-    // pre-step producers on the device: growth, then the viscosity surfaces
-    // from the positions at the start of the step
+    // pre-step producers on the device: the Young's modulus series at the
+    // step's start time (Simulation.cpp:5994), growth, then the viscosity
+    // surfaces from the positions at the start of the step
+    if (!h->series.empty()) {
+      tf_status ss = apply_series(h, h->sim_time, dt);
+      if (ss != TF_OK) return ss;
+    }
+    h->sim_time += dt;
     TF_CHECK(launch_grow(h->Ne, h->rates, dt, h->Fg, h->invFg, h->st));
     TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, w.uk, h->pat.node_ptr, h->pat.node_src,
                                  h->surf_place, h->surf_type, h->surf_spans, h->area, h->st));

@@ -315,3 +315,25 @@ def read_du_log(path):
         if m and cur is not None:
             cur.append(float(m.group(1)))
     return steps
+
+
+def read_property_grid(path):
+    """A physical-property grid file of a time series (TimeSeriesPhysicalProperties::ReadGrid,
+    TimeSeriesPhysicalProperties.cpp:34-92): "nX nY", then nY rows of nX values.
+    Returns (ny, nx)."""
+    with open(path) as f:
+        tok = f.read().split()
+    nx, ny = int(tok[0]), int(tok[1])
+    v = np.array([float(t) for t in tok[2:2 + nx * ny]])
+    if v.size != nx * ny:
+        raise ValueError(f"{path}: expected {nx * ny} values, found {v.size}")
+    return v.reshape(ny, nx)
+
+
+def write_property_grid(path, grid):
+    g = np.asarray(grid, np.float64)
+    ny, nx = g.shape
+    with open(path, "w") as f:
+        f.write(f"{nx} {ny}\n")
+        for row in g:
+            f.write(" ".join(repr(float(x)) for x in row) + "\n")

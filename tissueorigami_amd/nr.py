@@ -28,6 +28,8 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_copy_bandwidth", "tf_nr_set_coarse", "tf_nr_set_coarse_bands", "tf_nr_set_coarse_lag", "tf_nr_coarse_info",
            "tf_nr_set_packing", "tf_nr_set_node_mass", "tf_nr_update_node_mass",
            "tf_nr_detect_packing", "tf_nr_get_packing",
+           "tf_nr_add_stiffness_series", "tf_nr_apply_stiffness_series", "tf_nr_set_time",
+           "tf_nr_get_time", "tf_nr_get_stiffness_multiplier",
            "tf_nr_destroy", "tf_last_error"]
 
 
@@ -122,6 +124,13 @@ def load():
     L.tf_nr_update_node_mass.argtypes = [_h, _d, _d]
     L.tf_nr_detect_packing.argtypes = [_h, _d, _i]
     L.tf_nr_get_packing.argtypes = [_h, _i, _i, _d, _d]
+    L.tf_nr_add_stiffness_series.argtypes = [_h, ctypes.c_int32, _d, ctypes.c_int32,
+                                             ctypes.c_int32, _d, ctypes.c_int32, ctypes.c_int32,
+                                             _u8, _d]
+    L.tf_nr_apply_stiffness_series.argtypes = [_h, ctypes.c_double, ctypes.c_double]
+    L.tf_nr_set_time.argtypes = [_h, ctypes.c_double]
+    L.tf_nr_get_time.argtypes = [_h, _d]
+    L.tf_nr_get_stiffness_multiplier.argtypes = [_h, _d]
     L.tf_nr_destroy.argtypes = [_h]
     L.tf_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
@@ -286,6 +295,39 @@ class NewtonRaphsonSolver:
         ty = np.ascontiguousarray(elem_type, dtype=np.int32)
         sp = np.ascontiguousarray(spans, dtype=np.uint8)
         _check(load().tf_nr_set_surface_flags(self._h, _p(pl, _i), _p(ty, _i), _p(sp, _u8)))
+
+    # ---- Young's modulus time series (YoungsModulusModifier)
+    def add_stiffness_series(self, times, grids, linear, method, applicable, rel_pos):
+        """One series: grids (n_times, ny, nx) applied at times (s); method 0 nearest,
+        1 bilinear, 2 lower corner; applicable (n_prisms) and rel_pos (n_prisms x 2)
+        per prism (tissue.stiffness_applicable, tissue.relative_positions)."""
+        times = _f64(times).reshape(-1)
+        grids = _f64(grids)
+        if grids.ndim != 3 or grids.shape[0] != times.size:
+            raise ValueError("grids must be (n_times, ny, nx)")
+        app = np.ascontiguousarray(applicable, np.uint8).reshape(self.n_prisms)
+        rel = _f64(rel_pos).reshape(self.n_prisms, 2)
+        _check(load().tf_nr_add_stiffness_series(
+            self._h, times.size, _p(times, _d), grids.shape[2], grids.shape[1], _p(grids, _d),
+            int(bool(linear)), int(method), _p(app, _u8), _p(rel, _d)))
+
+    def apply_stiffness_series(self, t, dt):
+        """Simulation::ApplyAllYoungsModulusModifiers at time t (s)."""
+        _check(load().tf_nr_apply_stiffness_series(self._h, float(t), float(dt)))
+
+    def set_time(self, t):
+        """Time (s) of the next step of advance(), which applies the series first."""
+        _check(load().tf_nr_set_time(self._h, float(t)))
+
+    def time(self):
+        out = np.zeros(1)
+        _check(load().tf_nr_get_time(self._h, _p(out, _d)))
+        return float(out[0])
+
+    def stiffness_multiplier(self):
+        out = np.empty(self.n_prisms)
+        _check(load().tf_nr_get_stiffness_multiplier(self._h, _p(out, _d)))
+        return out
 
     def advance(self, n_steps, dt, u0=None, tol=1e-8, maxit=20, precision="fp64",
                 krylov_rtol=1e-10, want_positions=True):

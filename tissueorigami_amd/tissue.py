@@ -259,3 +259,66 @@ def facing_slabs(m, gap, shift=(0.0, 0.0), upper=None):
     out["n_lower_nodes"] = nn
     out["kind"] = m.get("kind", "mesh") + "-facing-slabs"
     return out
+
+
+def stiffness_applicable(where, tissue_type, placement, is_ecm=None, ecm_at_circumference=None,
+                         at_basal_border_of_ecm=None):
+    """Per prism: whether a Young's modulus series applies to it
+    (YoungsModulusModifier::IsItApplicable, YoungsModulusModifier.cpp:74-100).
+
+    where: the seven flags of TimeSeriesPhysicalProperties::DefineWhereToApply
+    (TimeSeriesPhysicalProperties.cpp:112-120): columnar layer, peripodial
+    membrane, apical, mid and basal layers, peripodial ECM, columnar ECM.
+    tissue_type: 0 columnar, 1 peripodial; placement: ShapeBase::tissuePlacement
+    (0 basal, 1 apical, 2 middle, 4 lateral)."""
+    cl, pm, ap, mid, bas, _pecm, cecm = [bool(w) for w in where]
+    tt = np.asarray(tissue_type).reshape(-1)
+    pl = np.asarray(placement).reshape(-1)
+    n = tt.size
+    z = np.zeros(n, bool)
+    ecm = z if is_ecm is None else np.asarray(is_ecm, bool).reshape(-1)
+    circ = z if ecm_at_circumference is None else np.asarray(ecm_at_circumference, bool).reshape(-1)
+    border = z if at_basal_border_of_ecm is None else np.asarray(at_basal_border_of_ecm, bool).reshape(-1)
+    layer = (cl & (tt == 0)) | (pm & (tt == 1))
+    on_ecm = (cecm & (pl == 0)) | (cecm & circ & (pl != 0))
+    on_cells = (ap & ((pl == 1) | (pl == 4))) | (mid & (pl == 2)) | (bas & ((pl == 0) | border))
+    return (layer & np.where(ecm, on_ecm, on_cells)).astype(np.uint8)
+
+
+def relative_positions(u, conn, apical_of=None, node_excluded=None, symmetric_y=False):
+    """Per prism: its centre's relative (x, y) position in the tissue's bounding
+    box (ShapeBase::calculateRelativePosInBoundingBox, ShapeBase.cpp:2091-2095,
+    with Simulation::calculateBoundingBox, Simulation.cpp:8388-8428).
+
+    apical_of (optional, per prism): the apical prism of its column, whose
+    position every prism of the column takes
+    (Simulation::updateRelativePositionsToApicalPositioning,
+    Simulation.cpp:6147-6167); -1 keeps the prism's own.  node_excluded
+    (optional, per node): nodes left out of the bounding box (those all of
+    whose owners mimic ECM)."""
+    u = np.asarray(u, np.float64).reshape(-1, 3)
+    conn = np.asarray(conn).reshape(-1, 6)
+    keep = np.ones(len(u), bool) if node_excluded is None else ~np.asarray(node_excluded, bool)
+    # the reference's loop: a coordinate lowers the minimum, or else raises the
+    # maximum, from the initial (1e5, -1e5)
+    lo = np.full(3, 100000.0)
+    hi = np.full(3, -100000.0)
+    for p in u[keep]:
+        for i in range(3):
+            if p[i] < lo[i]:
+                lo[i] = p[i]
+            elif p[i] > hi[i]:
+                hi[i] = p[i]
+    if symmetric_y:
+        lo[1] = -hi[1]
+    size = hi - lo
+    c = np.zeros((len(conn), 3))
+    for a in range(6):
+        c += u[conn[:, a]]
+    c /= 6
+    rel = np.stack([(c[:, 0] - lo[0]) / size[0], (c[:, 1] - lo[1]) / size[1]], axis=1)
+    if apical_of is not None:
+        ap = np.asarray(apical_of).reshape(-1)
+        sel = ap >= 0
+        rel[sel] = rel[ap[sel]]
+    return rel
