@@ -207,7 +207,9 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
                         tf_step_stats* stats);
 
 /* Young's modulus time series (YoungsModulusModifier, YoungsModulusModifier.cpp:1-177,
- * with its base TimeSeriesPhysicalProperties), single-tissue handles.
+ * with its base TimeSeriesPhysicalProperties).  On a batch handle the per-prism
+ * arrays run over the tissues' prisms concatenated, each tissue's relative
+ * positions taken in its own bounding box.
  * tf_nr_add_stiffness_series replaces the YoungsModulusModifier constructor
  * (ModelInputObject.cpp:1989-1990): n_times grids of ny x nx multipliers
  * (grids[t][iy][ix], the rows of the reference's grid files) applied at

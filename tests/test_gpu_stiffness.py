@@ -124,6 +124,57 @@ def test_advance_applies_series_each_step(orc, nr):
     a.close()
 
 
+def test_batch_series_matches_each_tissue_alone(orc, nr):
+    # a series over a batch handle (prisms concatenated) gives every tissue the
+    # multipliers and the step it gets alone, bit for bit (linear series)
+    ts = tissue.batch_ellipses(3, (300, 1500), seed=5)
+    dt = 60.0
+    times = np.array([0.0, 300.0, 900.0])
+    rng = np.random.default_rng(11)
+    grids = rng.uniform(0.6, 1.8, (3, 4, 6))
+    per = []
+    for c in ts:
+        m = c["mesh"]
+        ne = len(m["conn"])
+        per.append((tissue.relative_positions(m["nodes"], m["conn"]),
+                    (rng.uniform(size=ne) < 0.7).astype(np.uint8)))
+    b = nr.TissueBatch([(len(c["mesh"]["nodes"]), c["mesh"]["conn"], c["mesh"]["refpos"])
+                        for c in ts])
+    cat = lambda k: np.concatenate([c[k] for c in ts])
+    b.set_material(cat("E"), cat("nu"), cat("eta"), cat("ext_visc"))
+    b.set_tile_size(8)
+    b.add_stiffness_series(times, grids, True, 1, np.concatenate([a for _, a in per]),
+                           np.concatenate([r for r, _ in per]))
+    inc = [orc.inv3_many(tissue.growth_increment(c["rate"], dt)) for c in ts]
+    b.set_growth(np.concatenate(inc))
+    u0 = np.concatenate([c["mesh"]["nodes"].reshape(-1) for c in ts])
+    b.update_visc_surface(u0, cat("place"), cat("etype"), cat("spans"))
+    for k in range(6):
+        b.apply_stiffness_series(k * dt, dt)
+    ub, bst = b.step(u0, dt, 1e-8, 20, "fp64", 1e-10)
+    mb = b.stiffness_multiplier()
+    parts = b.split_nodes(ub)
+    off = 0
+    for t, c in enumerate(ts):
+        m = c["mesh"]
+        ne = len(m["conn"])
+        a = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
+        a.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
+        a.set_tile_size(8)
+        a.add_stiffness_series(times, grids, True, 1, per[t][1], per[t][0])
+        a.set_growth(inc[t])
+        a.update_visc_surface(m["nodes"].reshape(-1), c["place"], c["etype"], c["spans"])
+        for k in range(6):
+            a.apply_stiffness_series(k * dt, dt)
+        ua, ast = a.step(m["nodes"].reshape(-1), dt, 1e-8, 20, "fp64", 1e-10)
+        assert np.array_equal(mb[off:off + ne], a.stiffness_multiplier()), t
+        assert ast["du_norms"] == bst[t]["du_norms"], t
+        assert np.array_equal(parts[t].reshape(-1), ua), t
+        off += ne
+        a.close()
+    b.close()
+
+
 def test_series_rejects_bad_arguments(nr):
     c = problems.c1_config()
     s = _setup(nr, c)

@@ -1572,10 +1572,6 @@ tf_status tf_nr_add_stiffness_series(tf_handle* h, int32_t n_times, const double
     set_error("tf_nr_add_stiffness_series: invalid arguments");
     return TF_ERR_ARG;
   }
-  if (h->batch) {
-    set_error("tf_nr_add_stiffness_series: single-tissue handles only");
-    return TF_ERR_ARG;
-  }
   if (!h->have_material) {
     set_error("tf_nr_add_stiffness_series: set the material first");
     return TF_ERR_ARG;
