@@ -413,8 +413,9 @@ class NewtonRaphsonSolver:
         (negative: automatic, the default; 0: block Jacobi only)."""
         _check(load().tf_nr_set_coarse(self._h, float(cell_size)))
 
-    def set_coarse_bands(self, z_bands=1):
-        """Bands through the thickness per aggregate cell of the coarse space."""
+    def set_coarse_bands(self, z_bands=3):
+        """Bands through the thickness per aggregate cell of the coarse space (default 3;
+        1: whole columns)."""
         _check(load().tf_nr_set_coarse_bands(self._h, int(z_bands)))
 
     def set_coarse_lag(self, lag=True):
