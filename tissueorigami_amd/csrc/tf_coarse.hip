@@ -776,8 +776,9 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, in
   }
   rocblas_handle hb = (rocblas_handle)c.blas;
   if (rocblas_set_stream(hb, st) != rocblas_status_success) return hipErrorUnknown;
-  int64_t total = 0;
-  for (int T = 0; T < c.n_seg; ++T) total = c.seg_moff[T] + (int64_t)c.seg_nc[T] * c.seg_nc[T];
+  // the segments' dense blocks lie in order: A_c ends with the last one
+  const int last = c.n_seg - 1;
+  const int64_t total = c.seg_moff[last] + (int64_t)c.seg_nc[last] * c.seg_nc[last];
   hipLaunchKernelGGL(coarse_pairs_kernel, dim3(std::max(c.n_cpairs, 1)), dim3(kBlock), 0, st,
                      c.n_cpairs, c.cp_ptr, c.cp_slot, c.cp_row, col, val32, n_slots, c.node_d,
                      dof_flags, c.pairblk);
