@@ -132,3 +132,17 @@ def test_property_grid_file_round_trip(tmp_path):
     q = os.path.join(tmp_path, "out.txt")
     io.write_property_grid(q, g)
     assert np.array_equal(io.read_property_grid(q), g)
+
+
+@pytest.mark.parametrize("symmetric_y", [False, True])
+def test_relative_positions_use_the_oracle_bounding_box(orc, symmetric_y):
+    # the helper's bounding box is Simulation::calculateBoundingBox, which the
+    # oracle restates for packing
+    m = tissue.ellipse(a=20.0, b=12.0, side=2.0, thickness=6.0, n_layers=2)
+    u = m["nodes"] + np.random.default_rng(2).normal(0, 0.05, m["nodes"].shape)
+    bb = orc.bounding_box(u, symmetric_y)
+    lo, hi = bb[:3], bb[3:]
+    rel = tissue.relative_positions(u, m["conn"], symmetric_y=symmetric_y)
+    c = u[m["conn"]].sum(axis=1) / 6
+    assert np.allclose(rel[:, 0], (c[:, 0] - lo[0]) / (hi[0] - lo[0]), rtol=1e-14, atol=1e-15)
+    assert np.allclose(rel[:, 1], (c[:, 1] - lo[1]) / (hi[1] - lo[1]), rtol=1e-14, atol=1e-15)
