@@ -233,8 +233,9 @@ def test_c1_rectangle_50_steps_match_oracle(orc, nr):
 
 
 REPLAY = [("run07007", k) for k in range(10)] + [("run07008", k) for k in range(-1, 10)]
-# propagated Save_Frame / Save_Force rounding, as in test_oracle_fixtures.py
-POS_TOL = {"run07007": 5e-9, "run07008": 2e-7}
+# the oracle's worst replay errors with a 1.5x margin (positions) and the
+# propagated Save_Force rounding, as in test_oracle_fixtures.py
+POS_TOL = {"run07007": 1.5e-9, "run07008": 7e-8}
 FORCE_TOL = {"run07007": 2e-5, "run07008": 5e-3}
 
 

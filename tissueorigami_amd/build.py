@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtissue_nr.so")
 DRIVER = os.path.join(HERE, "nr_driver")
-SOURCES = ["tf_element.hip", "tf_assemble.hip", "tf_krylov.hip", "tf_tiled.hip", "tf_coarse.hip",
+SOURCES = ["tf_element.hip", "tf_assemble.hip", "tf_tiled.hip", "tf_coarse.hip",
            "tf_adjoint.hip", "tf_packing.hip", "tf_nr.hip"]
 # rocSOLVER (getrf / getri of the coarse matrix) and rocBLAS under it
 LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]

@@ -1,5 +1,5 @@
 // Internal declarations shared by the host driver (tf_nr.hip) and the device
-// code (tf_element.hip, tf_assemble.hip, tf_krylov.hip).
+// code (tf_element.hip, tf_assemble.hip, tf_tiled.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -196,6 +196,8 @@ struct Workspace {
   double* tnorm = nullptr;     // per tissue
   int32_t* host_tstate = nullptr;
   double* host_tnorm = nullptr;
+  double* gnorm = nullptr;       // per tissue: |g|_2 of the Newton residual
+  double* host_gnorm = nullptr;
   // restarted GMRES (FP64): basis of kGmresRestart + 1 vectors, per-tile
   // partials, per-tissue sums and host-computed coefficients
   double* gV = nullptr;
@@ -383,7 +385,7 @@ hipError_t launch_grow(int32_t n_prisms, const double* rates, double dt, double*
                        double* invFg, hipStream_t st);
 hipError_t launch_inverse(int32_t n_prisms, const double* Fg, double* invFg, hipStream_t st);
 
-// ---- Krylov (tf_krylov.hip) ----
+// ---- Krylov (tf_tiled.hip) ----
 // Optional event timing of the SpMV launches ([0] FP64 operator, [1] FP32).
 constexpr int kProfPairs = 128;
 // Event timing of the SpMV launches.  kind = 2 * which + (FP32 ? 1 : 0),
@@ -441,6 +443,8 @@ hipError_t tiled_gmres(const KrylovArgs& k, const double* b, double* x, double r
                        TiledResult* res);
 // x_k += dx and per-tissue |dx|_2 into norms (host).
 hipError_t tiled_norm_update(const KrylovArgs& k, const double* dx, double* xk, double* norms);
+// Per-tissue |g|_2 into ws->host_gnorm (asynchronous; read after a sync).
+hipError_t tiled_residual_norms(const KrylovArgs& k, const double* g);
 hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int method, int iters,
                                  float* ms);
 // Plain copy of n4 16-byte vectors (bandwidth reference); kCopyVariants

@@ -461,6 +461,8 @@ tf_status alloc_tissue_state(tf_handle* h) {
   TF_CHECK(dalloc(h, &w.tnorm, nt));
   TF_CHECK(hipHostMalloc((void**)&w.host_tstate, 2 * sizeof(int32_t) * nt, hipHostMallocDefault));
   TF_CHECK(hipHostMalloc((void**)&w.host_tnorm, sizeof(double) * nt, hipHostMallocDefault));
+  TF_CHECK(dalloc(h, &w.gnorm, nt));
+  TF_CHECK(hipHostMalloc((void**)&w.host_gnorm, sizeof(double) * nt, hipHostMallocDefault));
   std::vector<int32_t> ones(nt, 1);
   TF_CHECK(hipMemcpy(w.tactive, ones.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice));
   return TF_OK;
@@ -822,8 +824,27 @@ KrylovArgs krylov_args(tf_handle* h) {
 tf_status coarse_prepare(tf_handle* h, int nr_iteration, const int32_t* run) {
   CoarseOp& c = h->coarse;
   const bool bound = h->bound.ready;
-  if (h->coarse_lag && nr_iteration > 0 && c.active && c.built && c.for_bound == bound)
+  if (h->coarse_lag && nr_iteration > 0 && c.built && c.for_bound == bound &&
+      h->coarse_cell != 0.0) {
+    // lagged: a segment whose factorisation succeeded at the step's first
+    // iteration keeps it; a running segment whose factorisation failed (a
+    // singular A_c, or Newton-Schulz steps that do not converge) tries again,
+    // as its tissue does when stepped alone
+    std::vector<int32_t> retry(h->n_tissues, 0);
+    bool any = false;
+    for (int T = 0; T < h->n_tissues; ++T) {
+      retry[T] = (!run || run[T]) && !c.seg_active[T] ? 1 : 0;
+      any |= retry[T] != 0;
+    }
+    if (!any) return TF_OK;
+    if (bound)
+      TF_CHECK(coarse_factor(c, h->bound.pat.col, h->bound.val32, h->bound.pat.n_slots,
+                             h->dof_flags, retry.data(), h->st));
+    else
+      TF_CHECK(coarse_factor(c, h->pat.col, h->ws.val32, h->pat.n_slots, h->dof_flags,
+                             retry.data(), h->st));
     return TF_OK;
+  }
   c.active = false;
   if (h->coarse_cell == 0.0) return TF_OK;
   if (!c.built || c.for_bound != bound) {
@@ -1310,7 +1331,7 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
   std::vector<double> norms(nt, 0.0);
   for (int T = 0; T < nt; ++T) std::memset(&S[T], 0, sizeof(tf_step_stats));
   static thread_local TiledResult res, fres;
-  while (true) {
+  for (int pass = 0;; ++pass) {
     bool any = false;
     for (int T = 0; T < nt; ++T) any |= act[T] != 0;
     if (!any) break;
@@ -1320,12 +1341,16 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     if (s != TF_OK) return s;
     const int method = h->krylov_method[mixed];
     if (mixed && method == KRYLOV_CG) {
-      s = coarse_prepare(h, it[0], act.data());
+      s = coarse_prepare(h, pass, act.data());
       if (s != TF_OK) return s;
     } else {
       h->coarse.active = false;
     }
     KrylovArgs k = krylov_args(h);
+    // |g| per tissue, read after the solve's synchronisation: a NaN or Inf in
+    // the residual (the reference only prints it, NewtonRaphsonSolver.cpp:453-455)
+    // ends that tissue's step with TF_ERR_NAN
+    TF_CHECK(tiled_residual_norms(k, w.g));
     const double inner = std::max(rtol, kMixedInnerRtol);
     if (mixed) {
       TF_CHECK(tiled_mixed(k, w.g, w.du, rtol, inner, kMixedCap, 16, method, &res));
@@ -1344,13 +1369,16 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     TF_CHECK(hipEventElapsedTime(&t_el, h->ev[0], h->ev[1]));
     TF_CHECK(hipEventElapsedTime(&t_as, h->ev[1], h->ev[2]));
     TF_CHECK(hipEventElapsedTime(&t_kr, h->ev[2], h->ev[3]));
+    // tissues with a non-finite residual take no fallback solve
+    std::vector<int32_t> bad(nt, 0);
+    for (int T = 0; T < nt; ++T) bad[T] = act[T] && !std::isfinite(w.host_gnorm[T]);
     // Tissues whose solve failed are solved again, each on its own, by the
     // next method of the chain CG -> BiCGStab (same precision) -> BiCGStab
     // with the FP64 operator; the other tissues keep their increments.
     auto failed = [&](std::vector<int32_t>& f) {
       bool any_f = false;
       for (int T = 0; T < nt; ++T) {
-        f[T] = (act[T] && res.state[T] != K_CONVERGED) ? 1 : 0;
+        f[T] = (act[T] && !bad[T] && res.state[T] != K_CONVERGED) ? 1 : 0;
         any_f |= f[T] != 0;
       }
       return any_f;
@@ -1431,7 +1459,11 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
       }
       St.krylov_iterations_total += kit;
       tf_status result = TF_OK;
-      if (res.state[T] == K_BREAKDOWN) {
+      if (bad[T]) {
+        set_error("NaN/Inf in the NR residual at iteration " + std::to_string(it[T]) +
+                  (nt > 1 ? " of tissue " + std::to_string(T) : std::string()));
+        result = TF_ERR_NAN;
+      } else if (res.state[T] == K_BREAKDOWN) {
         set_error("BiCGStab breakdown in NR iteration " + std::to_string(it[T]) +
                   (nt > 1 ? " of tissue " + std::to_string(T) : std::string()));
         result = TF_ERR_KRYLOV;
@@ -2557,6 +2589,7 @@ tf_status tf_nr_destroy(tf_handle* h) {
   if (h->ws.tpart) (void)hipFree(h->ws.tpart);
   if (h->ws.host_tstate) (void)hipHostFree(h->ws.host_tstate);
   if (h->ws.host_tnorm) (void)hipHostFree(h->ws.host_tnorm);
+  if (h->ws.host_gnorm) (void)hipHostFree(h->ws.host_gnorm);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->st) (void)hipStreamDestroy(h->st);

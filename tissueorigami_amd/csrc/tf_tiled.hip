@@ -9,8 +9,8 @@
 // tissues of the batch, so a tissue solved in a batch gives bitwise the result
 // of the same tissue solved alone with the same tile size.
 //
-// Per iteration (see the derivation in tf_krylov.hip): w = J^T rh once per
-// solve makes (rh, v) = (w, ph) available before the SpMV that computes v.
+// Per iteration: w = J^T rh once per solve makes (rh, v) = (w, ph) = (rh, J ph)
+// available before the SpMV that computes v (since (J^T rh, ph) = (rh, J ph)).
 //   tz_spmv_v:   convergence test, alpha = rho / (w, ph); v = J ph;
 //                s = r - alpha v; sh = M^-1 s
 //   tz_spmv_t:   t = J sh; partials (t, s), (t, t)
@@ -1479,6 +1479,49 @@ hipError_t tiled_gmres(const KrylovArgs& k, const double* b, double* x, double r
   }
   for (int T = 0; T < nt; ++T) res->iters[T] = it[T];
   return hipStreamSynchronize(st);
+}
+
+// Per-tissue |g|_2 of the Newton residual into ws->host_gnorm (pinned),
+// copied asynchronously: valid after the caller's next synchronisation.  A
+// NaN or Inf here is reported as TF_ERR_NAN by the NR loop.
+hipError_t tiled_residual_norms(const KrylovArgs& k, const double* g) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  TC c = make_tc(k, 0.0);
+  const int nt = k.tm.n_tissues;
+  hipLaunchKernelGGL(tz_sqnorm<double>, dim3(k.tm.grid), dim3(kBlock), 0, st, c, g, 1);
+  hipLaunchKernelGGL(tz_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, 1, ws->gnorm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(ws->host_gnorm, ws->gnorm, sizeof(double) * nt, hipMemcpyDeviceToHost,
+                        st);
+}
+
+// The linear solve of the NR step on a single-tissue handle: the replacement
+// of the Pardiso call of NewtonRaphsonSolver::solveWithPardiso
+// (NewtonRaphsonSolver.cpp:498-763).  A bound system arrives as its explicit
+// operator N^T J N + I_slave, so every solve takes the same path.
+hipError_t krylov_bicgstab(const KrylovArgs& k, const double* b, double* x, double rtol,
+                           int maxit, int* iters, int* state, int sync_every) {
+  static thread_local TiledResult res;
+  hipError_t e = tiled_bicgstab(k, b, x, rtol, maxit, sync_every, &res);
+  *iters = res.iters[0];
+  *state = res.state[0];
+  return e;
+}
+
+// Adds the durations of the profiled launches (HIP event pairs recorded by the
+// runtime at each kernel's start and end) to the per-kind sums.
+void prof_harvest(Prof* p) {
+  if (!p || !p->on) return;
+  for (int i = 0; i < p->used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p->ev[2 * i], p->ev[2 * i + 1]) == hipSuccess) {
+      p->ms[p->kind[i]] += ms;
+      p->count[p->kind[i]]++;
+    }
+  }
+  p->used = 0;
 }
 
 void tiled_release(Workspace* ws) {
