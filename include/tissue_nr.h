@@ -123,6 +123,21 @@ tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u,
                                     const int32_t* elem_type, const uint8_t* spans,
                                     double* area_out);
 
+/* tf_nr_update_visc_surface for callers that cannot allocate device memory
+ * (loops under capture, serving loops, harnesses that forbid hipMalloc): the
+ * same kernel and the same results, with the per-prism flags staged in the
+ * library's static device scratch array (32 MiB per device, held until the
+ * process ends) instead of buffers allocated and freed per call; it calls no
+ * hipMalloc or hipFree.  Handles of up to about 3.7 million prisms (9 bytes
+ * of scratch per prism); larger ones get TF_ERR_ARG.  All calls on one device
+ * share the array: calls that overlap on one device (from several threads)
+ * race and must be serialised by the caller.  No counterpart in the
+ * reference; it exists for the callers above. */
+tf_status tf_nr_update_visc_surface_noalloc(tf_handle* h, const double* u,
+                                            const int32_t* elem_placement,
+                                            const int32_t* elem_type, const uint8_t* spans,
+                                            double* area_out);
+
 /* One implicit time step: NR iterations from u_n until |du|_2 <= tol or
  * maxit (the reference uses tol 1e-8 and maxit 20,
  * NewtonRaphsonSolver.cpp:22, Simulation.cpp:6615).  Replaces

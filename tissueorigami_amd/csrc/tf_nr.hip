@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -52,6 +53,8 @@ struct tf_handle {
   // Krylov method per precision [FP64, mixed]
   int krylov_method[2] = {KRYLOV_BICGSTAB, KRYLOV_CG};
   int device = 0;
+  // this device's copy of the library's static scratch array (see scratch_resolve)
+  void* scratch = nullptr;
   hipStream_t st = nullptr;
   DevPattern pat;
   std::vector<int32_t> h_slice_base, h_col, h_row_len, h_rowptr, h_nbr;
@@ -154,6 +157,42 @@ void dfree(tf_handle* h, T** p) {
 }
 
 tf_status layout_pattern(tf_handle* h, const int32_t* conn);
+
+// Scratch of the allocation-free entry point tf_nr_update_visc_surface_noalloc:
+// the per-prism placement, type and spans flags staged on the device (9 bytes
+// per prism, each array on a 256-byte boundary).  Every device the library runs
+// on holds its own copy of the array until the process ends.  Host code never
+// uses the array's name as an address: scratch_resolve gets this device's
+// address with hipGetSymbolAddress once, with the device current, and every
+// launch and copy uses that pointer.  All calls on one device share the array.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// This device's address of the scratch array, resolved on the first call for
+// the device (the caller has made it current) and kept per device.
+tf_status scratch_resolve(int dev, void** out) {
+  static std::mutex mu;
+  static void* table[kMaxDevices] = {};
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("device index " + std::to_string(dev) + " beyond the scratch table");
+    return TF_ERR_NO_DEVICE;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!table[dev]) {
+    void* p = nullptr;
+    TF_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("hipGetSymbolAddress returned no address for the scratch array");
+      return TF_ERR_HIP;
+    }
+    table[dev] = p;
+  }
+  *out = table[dev];
+  return TF_OK;
+}
+
+constexpr size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 
 tf_status check_device(int* dev) {
   int n = 0;
@@ -742,6 +781,7 @@ tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos
   }
   for (auto& e : h->ev)
     if (hipEventCreate(&e) != hipSuccess) return fail(TF_ERR_HIP);
+  if ((s = scratch_resolve(h->device, &h->scratch)) != TF_OK) return fail(s);
   if ((s = build_pattern(h, conn)) != TF_OK) return fail(s);
   if ((s = alloc_workspace(h)) != TF_OK) return fail(s);
   const size_t Ne = h->Ne, Nn = h->Nn;
@@ -1187,6 +1227,56 @@ tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u, const int32_t
   TF_CHECK(hipFree(ty));
   TF_CHECK(hipFree(sp));
   h->have_area = true;
+  return TF_OK;
+}
+
+tf_status tf_nr_update_visc_surface_noalloc(tf_handle* h, const double* u,
+                                            const int32_t* elem_placement,
+                                            const int32_t* elem_type, const uint8_t* spans,
+                                            double* area_out) {
+  if (!h || !u || !elem_placement || !elem_type || !spans) {
+    set_error("tf_nr_update_visc_surface_noalloc: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  const size_t Ne = h->Ne;
+  const size_t o_ty = align256(Ne * sizeof(int32_t)), o_sp = 2 * o_ty;
+  const size_t need = o_sp + align256(Ne);
+  if (need > kScratchBytes) {
+    set_error("tf_nr_update_visc_surface_noalloc: " + std::to_string(Ne) + " prisms need " +
+              std::to_string(need) + " bytes of scratch, more than its " +
+              std::to_string(kScratchBytes) + " (about 3.7 million prisms)");
+    return TF_ERR_ARG;
+  }
+  if (!h->scratch) {
+    set_error("tf_nr_update_visc_surface_noalloc: no scratch address for the device");
+    return TF_ERR_HIP;
+  }
+  unsigned char* base = static_cast<unsigned char*>(h->scratch);
+  int32_t* pl = reinterpret_cast<int32_t*>(base);
+  int32_t* ty = reinterpret_cast<int32_t*>(base + o_ty);
+  uint8_t* sp = base + o_sp;
+  // every staged byte is written before the kernel reads it: nothing of an
+  // earlier call's contents is used
+  TF_CHECK(hipMemcpyAsync(pl, elem_placement, Ne * sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+  TF_CHECK(hipMemcpyAsync(ty, elem_type, Ne * sizeof(int32_t), hipMemcpyHostToDevice, h->st));
+  TF_CHECK(hipMemcpyAsync(sp, spans, Ne, hipMemcpyHostToDevice, h->st));
+  {
+    tf_status up = upload_nodes(h, h->ws.tmp, u);
+    if (up != TF_OK) return up;
+  }
+  TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, h->ws.tmp, h->pat.node_ptr,
+                               h->pat.node_src, pl, ty, sp, h->area, h->st));
+  TF_CHECK(hipStreamSynchronize(h->st));
+  h->have_area = true;
+  if (area_out) {
+    if (!h->batch) {
+      TF_CHECK(hipMemcpy(area_out, h->area, sizeof(double) * h->Nn, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<double> a(h->Nn);
+      TF_CHECK(hipMemcpy(a.data(), h->area, sizeof(double) * h->Nn, hipMemcpyDeviceToHost));
+      from_internal<double>(h, a.data(), area_out, 1);
+    }
+  }
   return TF_OK;
 }
 

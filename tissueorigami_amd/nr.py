@@ -20,7 +20,7 @@ PREC = {"fp64": 0, "mixed": 1}
 MAX_NR = 32
 
 EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_visc_surface",
-           "tf_nr_update_visc_surface", "tf_nr_step", "tf_nr_assemble", "tf_nr_solve",
+           "tf_nr_update_visc_surface", "tf_nr_update_visc_surface_noalloc", "tf_nr_step", "tf_nr_assemble", "tf_nr_solve",
            "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_profile",
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
@@ -88,6 +88,7 @@ def load():
     L.tf_nr_set_growth.argtypes = [_h, _d]
     L.tf_nr_set_visc_surface.argtypes = [_h, _d]
     L.tf_nr_update_visc_surface.argtypes = [_h, _d, _i, _i, _u8, _d]
+    L.tf_nr_update_visc_surface_noalloc.argtypes = [_h, _d, _i, _i, _u8, _d]
     L.tf_nr_step.argtypes = [_h, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
                              ctypes.c_int32, ctypes.c_double, _d, ctypes.POINTER(StepStats)]
     L.tf_nr_assemble.argtypes = [_h, _d, _d, ctypes.c_double, _d, _d]
