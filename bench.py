@@ -270,6 +270,28 @@ def run_batch(steps, warmup, mixed_rtol, mixed_krylov="cg", seed=4, n=64, sequen
     cgfb = sum(st["cg_fallbacks"] for step in stats for st in step)
     # launches are shared: the batch's wall time per step is that of its
     # slowest tissue; the rate counts every tissue's NR iterations
+    # kernel evidence: one more step with every CG SpMV launch bracketed by HIP
+    # events on the handle's stream (direct launches); the achieved rate counts
+    # each tissue's algorithmic SpMV bytes once per CG iteration it ran
+    b.profile(True)
+    _, pst = b.advance(1, dt, None, 1e-8, 20, "mixed", mixed_rtol, want_positions=False)
+    kt = b.kernel_times()
+    b.profile(False)
+    its = [st["krylov_iterations_total"] for st in pst[0]]
+    blocks = [tissue.block_count(c["mesh"]["conn"], len(c["mesh"]["nodes"])) for c in ts]
+    algo = sum(k * cg_spmv_bytes(nb, len(c["mesh"]["nodes"]), 4)
+               for k, nb, c in zip(its, blocks, ts))
+    spmv_s = kt["spmv_ms_fp32"] / 1e3
+    kname = "tzc_spmv<float>" if mixed_krylov == "cg" else "tz_spmv_v<float>"
+    profiled = dict(kernel=kname, launches=kt["spmv_launches_fp32"],
+                    spmv_ms=kt["spmv_ms_fp32"], avg_launch_ms=kt["spmv_ms_fp32"] /
+                    max(kt["spmv_launches_fp32"], 1), tissue_iterations=int(sum(its)),
+                    max_tissue_iterations=int(max(its)), algorithmic_bytes=int(algo),
+                    achieved=algo / spmv_s / 1e9 if spmv_s > 0 else None,
+                    frac=(algo / spmv_s / 1e9 / HBM_PEAK_GBS) if spmv_s > 0 else None,
+                    element_ms=kt["element_ms"], assembly_ms=kt["assembly_ms"],
+                    assemblies=kt["assemblies"],
+                    fallbacks=sum(st["cg_fallbacks"] + st["fp64_fallbacks"] for st in pst[0]))
     b.close()
     # sequential reference: a sample of the tissues, each in its own handle,
     # same steps, extrapolated by prism count
@@ -297,7 +319,7 @@ def run_batch(steps, warmup, mixed_rtol, mixed_krylov="cg", seed=4, n=64, sequen
                steps=steps, slices_per_tile=spt, nr_iterations=nr_its,
                nr_iterations_per_s=nr_its / wall, time_steps_per_s=steps / wall, wall_s=wall,
                converged=conv, fp64_fallbacks=fb, cg_fallbacks=cgfb, krylov_mixed=mixed_krylov,
-               setup_s=setup_s,
+               setup_s=setup_s, profiled_step=profiled,
                sequential_estimate_s=seq_est, batch_speedup_vs_sequential=seq_est / wall,
                sequential_sample=f"{sequential_sample} of the {n} tissues, scaled by prism count")
     print(f"[bench] batch: {steps} steps {wall:.2f} s, {nr_its} NR its; sequential estimate "

@@ -113,6 +113,21 @@ tf_status tf_nr_set_growth(tf_handle* h, const double* inv_Fg);
  * (Simulation.cpp:7598-7621). */
 tf_status tf_nr_set_visc_surface(tf_handle* h, const double* area);
 
+/* Laser ablation: ablated_prisms (n_prisms flags, ShapeBase::IsAblated; NULL:
+ * none) contribute neither forces nor tangents to the step (their forces are
+ * not computed and their TriPointKe / TriPointKv are zero,
+ * NewtonRaphsonSolver.cpp:236, 344; ShapeBase.cpp:2850-2852, 2879-2882), no
+ * viscosity surfaces (Simulation.cpp:7608-7621) and do not grow in
+ * tf_nr_advance (Simulation.cpp:6055); ablated_nodes (Simulation::AblatedNodes,
+ * n_ablated_nodes ids, may be NULL) get 1 on every Jacobian diagonal entry that
+ * is exactly zero after the drag and packing terms
+ * (NewtonRaphsonSolver::checkJacobianForAblatedNodes, NewtonRaphsonSolver.cpp:425-439,
+ * called at Simulation.cpp:6691).  Each call replaces the previous flags.  On a
+ * batch handle both use the concatenated numbering.  Not supported together
+ * with node-node packing or by the adjoint (TF_ERR_ARG). */
+tf_status tf_nr_set_ablation(tf_handle* h, const uint8_t* ablated_prisms,
+                             const int32_t* ablated_nodes, int32_t n_ablated_nodes);
+
 /* Computes the node viscosity surfaces on the device from positions u
  * (3 x n_nodes) and per-prism tissue placement / type / spans-whole-tissue
  * flags (ShapeBase.cpp:905-1037), then stores them as tf_nr_set_visc_surface

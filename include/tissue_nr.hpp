@@ -60,8 +60,11 @@ class ShapeBase {
   std::array<int, 6> NodeIds{};
   double ReferencePositions[6][3] = {};
   double E = 10.0, v = 0.3, internalViscosity = 0.0;
+  // ShapeBase::stiffnessMultiplier (ShapeBase.h:146): the Lame parameters are
+  // formed from it times E (Prism::updateElasticProperties, Prism.cpp:526-527)
+  double stiffnessMultiplier = 1.0;
   double InvFg[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-  bool IsAblated = false;
+  bool IsAblated = false;  // ShapeBase::IsAblated: no forces, zero tangents
   int tissuePlacement = 2, tissueType = 0;  // ShapeBase.cpp:905-1037
   bool spansWholeTissue = false;
   // Young's modulus time series (ShapeBase.h:147): the multiplier the series
@@ -111,6 +114,8 @@ class NewtonRaphsonSolver {
   double threshold = 1E-8;                    // NewtonRaphsonSolver.cpp:22
   std::vector<double> un, uk, deltaU, gSum, displacementPerDt;
   std::vector<std::vector<int>> slaveMasterList;
+  // Simulation::AblatedNodes, the list checkJacobianForAblatedNodes applies
+  std::vector<int> AblatedNodes;
   bool boundNodesWithSlaveMasterDefinition = false;
   int precision = TF_PREC_FP64;
   double krylovTolerance = 1e-10;
@@ -138,6 +143,18 @@ class NewtonRaphsonSolver {
                                           double dt) {
     bind(Nodes, Elements);
     check(tf_nr_assemble(h, uk.data(), un.data(), dt, gSum.data(), nullptr));
+    lastDt = dt;
+  }
+  // checkJacobianForAblatedNodes (NewtonRaphsonSolver.cpp:425-439): a zero
+  // Jacobian diagonal entry of a listed node becomes 1.  The device assembly
+  // applies it to the list bound with the other inputs; called after
+  // calculateForcesAndJacobianMatrixNR with a list that differs from that one,
+  // it binds the list and assembles again at the same state.
+  void checkJacobianForAblatedNodes(const std::vector<int>& nodes) {
+    AblatedNodes = nodes;
+    if (!h || boundAblNodes == nodes) return;
+    bindAblation(ablFlags);
+    if (lastDt > 0) check(tf_nr_assemble(h, uk.data(), un.data(), lastDt, gSum.data(), nullptr));
   }
   // solveForDeltaU (NewtonRaphsonSolver.cpp:461-488) with the device Krylov
   // solver in place of Pardiso; slaves follow their masters.
@@ -176,8 +193,11 @@ class NewtonRaphsonSolver {
     return st;
   }
 
-  // Uploads the per-step inputs (material, growth, viscosity surfaces, fixing,
-  // binding) from the node and element objects.
+  // Uploads the per-step inputs (material with each element's stiffness
+  // multiplier, growth, viscosity surfaces, fixing, binding, ablation) from the
+  // node and element objects.  Each input goes to the device only when it
+  // differs from what was bound last, so the NR iterations of a step, which
+  // call this at every assembly, upload nothing after the first.
   void bind(const std::vector<std::unique_ptr<Node>>& Nodes,
             const std::vector<std::unique_ptr<ShapeBase>>& Elements) {
     const int ne = (int)Elements.size();
@@ -193,11 +213,12 @@ class NewtonRaphsonSolver {
     }
     std::vector<double> E(ne), nu(ne), eta(ne), iFg(9 * (size_t)ne), ev(3 * (size_t)nNodes),
         A(nNodes);
-    std::vector<uint8_t> fixed(3 * (size_t)nNodes);
+    std::vector<uint8_t> fixed(3 * (size_t)nNodes), abl(ne);
     for (int e = 0; e < ne; ++e) {
-      E[e] = Elements[e]->E;
+      E[e] = Elements[e]->stiffnessMultiplier * Elements[e]->E;
       nu[e] = Elements[e]->v;
       eta[e] = Elements[e]->internalViscosity;
+      abl[e] = Elements[e]->IsAblated ? 1 : 0;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) iFg[9 * e + 3 * i + j] = Elements[e]->InvFg[i][j];
     }
@@ -213,10 +234,23 @@ class NewtonRaphsonSolver {
       pairs.push_back(p[0]);
       pairs.push_back(p[1]);
     }
-    check(tf_nr_set_material(h, E.data(), nu.data(), eta.data(), ev.data(), fixed.data(),
-                             pairs.empty() ? nullptr : pairs.data(), (int32_t)(pairs.size() / 2)));
-    check(tf_nr_set_growth(h, iFg.data()));
-    check(tf_nr_set_visc_surface(h, A.data()));
+    if (!bound || E != bE || nu != bNu || eta != bEta || ev != bEv || fixed != bFixed ||
+        pairs != bPairs) {
+      check(tf_nr_set_material(h, E.data(), nu.data(), eta.data(), ev.data(), fixed.data(),
+                               pairs.empty() ? nullptr : pairs.data(),
+                               (int32_t)(pairs.size() / 2)));
+      bE = E; bNu = nu; bEta = eta; bEv = ev; bFixed = fixed; bPairs = pairs;
+    }
+    if (!bound || iFg != bFg) {
+      check(tf_nr_set_growth(h, iFg.data()));
+      bFg = iFg;
+    }
+    if (!bound || A != bA) {
+      check(tf_nr_set_visc_surface(h, A.data()));
+      bA = A;
+    }
+    if (abl != ablFlags || AblatedNodes != boundAblNodes) bindAblation(abl);
+    bound = true;
   }
 
   // Node-node packing: Simulation::detectPacingNodes (Simulation.cpp:7997-8078)
@@ -315,9 +349,30 @@ class NewtonRaphsonSolver {
   tf_handle* handle() const { return h; }
 
  private:
+  // the ablation flags and node list of the next assemblies
+  void bindAblation(const std::vector<uint8_t>& abl) {
+    std::vector<int32_t> nodes(AblatedNodes.begin(), AblatedNodes.end());
+    bool any = false;
+    for (uint8_t a : abl) any |= a != 0;
+    bool before = !boundAblNodes.empty();
+    for (uint8_t a : ablFlags) before |= a != 0;
+    if (any || !nodes.empty() || before)
+      check(tf_nr_set_ablation(h, any ? abl.data() : nullptr, nodes.empty() ? nullptr : nodes.data(),
+                             (int32_t)nodes.size()));
+    ablFlags = abl;
+    boundAblNodes = AblatedNodes;
+  }
+
   int nDim, nNodes;
   tf_handle* h = nullptr;
   bool packingOn = false;
+  double lastDt = 0.0;
+  // what bind() last uploaded
+  bool bound = false;
+  std::vector<double> bE, bNu, bEta, bEv, bFg, bA;
+  std::vector<uint8_t> bFixed, ablFlags;
+  std::vector<int32_t> bPairs;
+  std::vector<int> boundAblNodes;
 };
 
 }  // namespace tissue_nr

@@ -489,6 +489,33 @@ struct PackState {
 };
 PackState g_pack;
 
+// Laser ablation (ShapeBase::IsAblated per prism, Simulation::AblatedNodes),
+// set by orc_set_ablation.  While set, an ablated prism contributes neither
+// forces (NewtonRaphsonSolver.cpp:236, 344) nor tangents (its TriPointKe and
+// TriPointKv are zero, ShapeBase.cpp:2850-2852, 2879-2882) nor viscosity
+// surfaces (Simulation.cpp:7608-7621), and an ablated node whose Jacobian
+// diagonal entry is exactly zero after the drag and packing terms gets 1 there
+// (checkJacobianForAblatedNodes, NewtonRaphsonSolver.cpp:425-439, called at
+// Simulation.cpp:6691 before the fixed-DOF pass).
+struct AblationState {
+  std::vector<uint8_t> el;     // per prism
+  std::vector<int32_t> nodes;  // ablated node ids, in the reference's list order
+  bool on = false;
+};
+AblationState g_abl;
+
+bool ablated_el(int e) { return g_abl.on && !g_abl.el.empty() && g_abl.el[e]; }
+
+// checkJacobianForAblatedNodes on a dense K
+void ablated_node_diagonal(double* K, int n) {
+  if (!g_abl.on) return;
+  for (int nd : g_abl.nodes)
+    for (int aa = 0; aa < 3; ++aa) {
+      double& d = K[(size_t)(3 * nd + aa) * n + (3 * nd + aa)];
+      if (d == 0) d = 1;
+    }
+}
+
 // One couple of Simulation::calculatePackingForcesImplicit3D
 // (Simulation.cpp:7338-7400) and calculatePackingJacobian3D
 // (Simulation.cpp:7444-7490): force F on id0 (-F on id1) and the symmetric
@@ -574,12 +601,17 @@ void assemble_dense(const Problem& P, const double* uk, const double* un, double
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < P.n_el; ++e) {
     double pos[6][3], ref[6][3], iFg[3][3], vel[6][3];
+    if (ablated_el(e)) {
+      outs[e] = ElementOut();
+      continue;
+    }
     gather_element(P, e, uk, un, dt, pos, ref, iFg, vel);
     element_eval(pos, ref, iFg, P.lambda[e], P.mu[e], P.eta[e], vel, dt, tangent_mode, outs[e]);
   }
   std::vector<double> ge(n, 0.0), gvi(n, 0.0), gve(n, 0.0);
   std::memset(K, 0, sizeof(double) * (size_t)n * n);
   for (int e = 0; e < P.n_el; ++e) {
+    if (ablated_el(e)) continue;  // writeInternalForcesTogeAndgv is skipped
     for (int a = 0; a < 6; ++a) {
       const int nd = P.conn[6 * e + a];
       for (int i = 0; i < 3; ++i) {
@@ -649,6 +681,7 @@ void assemble_dense(const Problem& P, const double* uk, const double* un, double
         }
     }
   }
+  ablated_node_diagonal(K, n);
   for (int i = 0; i < n; ++i) {
     gsum[i] = ge[i] + gvi[i] + gve[i];
     // addPackingForces puts PackingForces + 0 into gExt (Simulation.cpp:8130-8151),
@@ -821,6 +854,10 @@ void assemble_bsr(const Problem& P, const double* uk, const double* un, double d
 #pragma omp parallel for schedule(static)
   for (int e = 0; e < P.n_el; ++e) {
     double pos[6][3], ref[6][3], iFg[3][3], vel[6][3];
+    if (ablated_el(e)) {
+      outs[e] = ElementOut();
+      continue;
+    }
     gather_element(P, e, uk, un, dt, pos, ref, iFg, vel);
     element_eval(pos, ref, iFg, P.lambda[e], P.mu[e], P.eta[e], vel, dt, 1, outs[e]);
   }
@@ -828,6 +865,7 @@ void assemble_bsr(const Problem& P, const double* uk, const double* un, double d
   std::vector<double> ge(n, 0.0), gvi(n, 0.0);
   for (int e = 0; e < P.n_el; ++e)
     for (int a = 0; a < 6; ++a) {
+      if (ablated_el(e)) break;
       const int nd = P.conn[6 * e + a];
       for (int i = 0; i < 3; ++i) {
         const bool fx = P.fixed && P.fixed[3 * nd + i];
@@ -860,6 +898,14 @@ void assemble_bsr(const Problem& P, const double* uk, const double* un, double d
       gsum[3 * i + j] = ge[3 * i + j] + gvi[3 * i + j] + (-(sv * disp));
     }
   }
+  if (g_abl.on)
+    for (int nd : g_abl.nodes) {
+      const int d = find_block(M, nd, nd);
+      for (int aa = 0; aa < 3; ++aa) {
+        double& v = M.val[(size_t)9 * d + 4 * aa];
+        if (v == 0) v = 1;
+      }
+    }
   if (P.fixed) {
     for (int r = 0; r < P.n_nodes; ++r)
       for (int k = M.rowptr[r]; k < M.rowptr[r + 1]; ++k) {
@@ -1120,6 +1166,7 @@ void orc_visc_surfaces(int n_nodes, int n_el, const int32_t* conn, const double*
   for (int e = 0; e < n_el; ++e) {
     const int tt = elem_type[e], tp = elem_placement[e];
     if (tt != 0 && tt != 1) continue;
+    if (ablated_el(e)) continue;
     int apical[3] = {3, 4, 5}, basal[3] = {0, 1, 2};
     if (tt == 1) { apical[0] = 0; apical[1] = 1; apical[2] = 2; basal[0] = 3; basal[1] = 4; basal[2] = 5; }
     const bool has_apical = (tp == 1) || (tp == 2 && spans[e]);
@@ -1292,6 +1339,14 @@ void orc_set_packing(const int32_t* pairs, int n_pairs, const double* mass, int 
   g_pack.on = true;
 }
 
+void orc_set_ablation(const uint8_t* ablated_el, int n_el, const int32_t* nodes, int n_nodes) {
+  g_abl = AblationState();
+  if (!ablated_el && !nodes) return;
+  if (ablated_el) g_abl.el.assign(ablated_el, ablated_el + n_el);
+  if (nodes) g_abl.nodes.assign(nodes, nodes + n_nodes);
+  g_abl.on = true;
+}
+
 void orc_packing_couple(const double* p0, const double* p1, double m0, double m1, double thr,
                         double* F, double* D) {
   packing_couple(p0, p1, m0, m1, thr, F, D);
@@ -1342,6 +1397,7 @@ void orc_node_masses(int n_nodes, int n_el, const int32_t* conn, const double* v
                      const double* Fg, double* mass) {
   for (int i = 0; i < n_nodes; ++i) mass[i] = 0;
   for (int e = 0; e < n_el; ++e) {
+    if (ablated_el(e)) continue;  // Simulation::updateNodeMasses skips ablated prisms
     double M[3][3];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) M[i][j] = Fg[9 * e + 3 * i + j];

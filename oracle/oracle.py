@@ -58,6 +58,7 @@ def lib():
         L.orc_assemble_bsr.restype = ctypes.c_long
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_packing.argtypes = [_i, ctypes.c_int, _d, ctypes.c_int, ctypes.c_double]
+        L.orc_set_ablation.argtypes = [_u8, ctypes.c_int, _i, ctypes.c_int]
         L.orc_packing_couple.argtypes = [_d, _d, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, _d, _d]
         L.orc_reference_volumes.argtypes = [ctypes.c_int, _d, _d]
@@ -269,6 +270,26 @@ class packing:
 
     def __exit__(self, *exc):
         lib().orc_set_packing(None, 0, None, 0, 0.0)
+        return False
+
+
+class ablation:
+    """Context in which the oracle's assemblies, NR steps and viscosity surfaces
+    treat the prisms flagged in `prisms` (n_prisms, ShapeBase::IsAblated) as
+    laser-ablated and apply checkJacobianForAblatedNodes to `nodes`
+    (Simulation::AblatedNodes)."""
+
+    def __init__(self, prisms=None, nodes=None):
+        self.prisms = None if prisms is None else np.ascontiguousarray(prisms, dtype=np.uint8)
+        self.nodes = None if nodes is None else _i32(nodes).reshape(-1)
+
+    def __enter__(self):
+        lib().orc_set_ablation(_p(self.prisms, _u8), 0 if self.prisms is None else self.prisms.size,
+                               _p(self.nodes, _i), 0 if self.nodes is None else self.nodes.size)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_ablation(None, 0, None, 0)
         return False
 
 

@@ -32,3 +32,19 @@ def test_sparse_step_with_binding_matches_dense(orc):
     us, _, _, cs = P.nr_step_sparse(p["un"], p["dt"], krylov_rtol=1e-13)
     assert cd and cs
     assert np.abs(us - ud).max() <= 1e-10 * np.abs(ud).max()
+
+
+@pytest.mark.parametrize("shape", ["rectangle", "ellipse"])
+def test_block_count_equals_the_oracle_bsr(orc, shape):
+    """tissue.block_count (the bench's per-tissue SpMV byte count) equals the
+    number of blocks of the oracle's BSR assembly of the same mesh."""
+    from tissueorigami_amd import tissue
+    m = tissue.rectangle(5.0, 4.0, 1.0, 2.0, 2) if shape == "rectangle" else \
+        tissue.ellipse(8.0, 6.0, 1.0, 5.0, 3)
+    nn, ne = len(m["nodes"]), len(m["conn"])
+    u = m["nodes"].reshape(-1)
+    lam, mu = orc.lame(np.full(ne, 1000.0), np.full(ne, 0.3))
+    P = orc.Problem(m["conn"], m["refpos"], np.tile(np.eye(3).reshape(9), (ne, 1)), lam, mu,
+                    np.zeros(ne), np.ones((nn, 3)), np.ones(nn))
+    rowptr, col, val, g = P.assemble_bsr(u, u, 1.0)
+    assert tissue.block_count(m["conn"], nn) == len(col)

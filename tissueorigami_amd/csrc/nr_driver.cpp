@@ -11,7 +11,11 @@
 //   n_prisms lines: 6 node ids, 18 reference coordinates, E nu eta, 9 InvFg entries
 //   optionally: n_pairs, then n_pairs lines "slave_dof master_dof"
 //     (NewtonRaphsonSolver::slaveMasterList, as Simulation::bindCircumferenceXY
-//     fills it)
+//     fills it), then any of these sections:
+//     "stiffness" and n_prisms values of ShapeBase::stiffnessMultiplier;
+//     "ablated_prisms" k and k prism ids (ShapeBase::IsAblated);
+//     "ablated_nodes" k and k node ids (Simulation::AblatedNodes, applied by
+//     checkJacobianForAblatedNodes after the assembly, Simulation.cpp:6691)
 // Output: one line per node with the converged x y z (%.17g), then a line
 //   "iterations <k>" and one "du <norm>" line per NR iteration.
 #include <cstdio>
@@ -81,6 +85,30 @@ int main(int argc, char** argv) {
       return 2;
     }
   }
+  std::vector<int> ablatedNodes;
+  in.clear();
+  std::string key;
+  while (in >> key) {
+    if (key == "stiffness") {
+      for (auto& el : Elements) in >> el->stiffnessMultiplier;
+    } else if (key == "ablated_prisms" || key == "ablated_nodes") {
+      int k = 0;
+      in >> k;
+      for (int i = 0; i < k; ++i) {
+        int id = -1;
+        in >> id;
+        if (key == "ablated_nodes") ablatedNodes.push_back(id);
+        else if (id >= 0 && id < ne) Elements[id]->IsAblated = true;
+      }
+    } else {
+      std::cerr << "unknown section " << key << "\n";
+      return 2;
+    }
+    if (!in) {
+      std::cerr << "bad section " << key << "\n";
+      return 2;
+    }
+  }
   std::vector<double> norms;
   try {
     tissue_nr::NewtonRaphsonSolver NRSolver(3, nn);
@@ -89,6 +117,7 @@ int main(int argc, char** argv) {
     NRSolver.slaveMasterList = slaveMasterList;
     NRSolver.boundNodesWithSlaveMasterDefinition = !slaveMasterList.empty();
     if (mode == "step") {
+      NRSolver.AblatedNodes = ablatedNodes;
       tf_step_stats st = NRSolver.updateStepNR(Nodes, Elements, dt, maxit);
       for (int k = 0; k < st.nr_iterations; ++k) norms.push_back(st.du_norms[k]);
     } else {
@@ -101,6 +130,7 @@ int main(int argc, char** argv) {
       while (!converged) {
         NRSolver.calculateDisplacementMatrix(dt);
         NRSolver.calculateForcesAndJacobianMatrixNR(Nodes, Elements, dt);
+        NRSolver.checkJacobianForAblatedNodes(ablatedNodes);
         NRSolver.solveForDeltaU();
         converged = NRSolver.checkConvergenceViaDeltaU();
         double s = 0.0;

@@ -172,6 +172,13 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
               for (int q = 0; q < 3; ++q) J[3 * p + q] = -D[3 * q + p] + J[3 * p + q];
           }
         }
+        // checkJacobianForAblatedNodes (NewtonRaphsonSolver.cpp:425-439), after the
+        // drag and packing terms and before the fixed-DOF pass (Simulation.cpp:6691)
+        if (a.abl_node && a.abl_node[R]) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+            if (J[4 * j] == 0.0) J[4 * j] = 1.0;
+        }
       }
       if (a.dof_flags) {
 #pragma unroll
@@ -332,7 +339,7 @@ __device__ double tri_area(const double* u, int n0, int n1, int n2) {
 __global__ __launch_bounds__(kBlock) void visc_surface_kernel(
     int32_t n_nodes, const int32_t* conn, const double* u, const int32_t* node_ptr,
     const int32_t* node_src, const int32_t* placement, const int32_t* type,
-    const uint8_t* spans, double* area) {
+    const uint8_t* spans, const uint8_t* ablated, double* area) {
   for (int R = blockIdx.x * kBlock + threadIdx.x; R < n_nodes; R += gridDim.x * kBlock) {
     double A = 0.0;
     for (int t = node_ptr[R]; t < node_ptr[R + 1]; ++t) {
@@ -340,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void visc_surface_kernel(
       const int e = s / 6, k = s % 6;
       const int tt = type[e], tp = placement[e];
       if (tt != 0 && tt != 1) continue;
+      if (ablated && ablated[e]) continue;  // Simulation.cpp:7608-7621 skips ablated prisms
       const bool has_apical = (tp == 1) || (tp == 2 && spans[e]);
       const bool has_basal = (tp == 0) || (tp == 2 && spans[e]);
       const int* c = conn + 6 * e;
@@ -364,8 +372,10 @@ __global__ __launch_bounds__(kBlock) void visc_surface_kernel(
 // increments: F_g <- inc F_g, then InvFg = inv(F_g) by the same LU as the
 // element kernel.
 __global__ __launch_bounds__(kBlock) void grow_kernel(int32_t n_prisms, const double* rates,
-                                                      double dt, double* Fg, double* invFg) {
+                                                      double dt, const uint8_t* ablated,
+                                                      double* Fg, double* invFg) {
   for (int e = blockIdx.x * kBlock + threadIdx.x; e < n_prisms; e += gridDim.x * kBlock) {
+    if (ablated && ablated[e]) continue;  // Simulation.cpp:6055: ablated prisms do not grow
     double inc[3];
     for (int i = 0; i < 3; ++i) inc[i] = exp(rates[3 * e + i] * dt);
     double F[9], X[9];
@@ -501,17 +511,48 @@ hipError_t launch_visc_surface(int32_t n_nodes, int32_t n_prisms, const int32_t*
                                const double* u, const int32_t* node_ptr,
                                const int32_t* node_src, const int32_t* placement,
                                const int32_t* type, const uint8_t* spans, double* area,
-                               hipStream_t st) {
+                               hipStream_t st, const uint8_t* ablated) {
   (void)n_prisms;
   hipLaunchKernelGGL(visc_surface_kernel, dim3(grid_for(n_nodes)), dim3(kBlock), 0, st, n_nodes,
-                     conn, u, node_ptr, node_src, placement, type, spans, area);
+                     conn, u, node_ptr, node_src, placement, type, spans, ablated, area);
   return hipGetLastError();
 }
 
 hipError_t launch_grow(int32_t n_prisms, const double* rates, double dt, double* Fg,
-                       double* invFg, hipStream_t st) {
+                       double* invFg, hipStream_t st, const uint8_t* ablated) {
   hipLaunchKernelGGL(grow_kernel, dim3(grid_for(n_prisms)), dim3(kBlock), 0, st, n_prisms, rates,
-                     dt, Fg, invFg);
+                     dt, ablated, Fg, invFg);
+  return hipGetLastError();
+}
+
+namespace {
+// One workgroup per ablated prism: its 324 tangent entries (in the buffers the
+// element mode writes) and its 18 + 18 force entries are set to zero.
+__global__ __launch_bounds__(kBlock) void ablate_zero_kernel(ElementArgs a, int mode,
+                                                             const int32_t* list, int n) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const size_t e = (size_t)list[i];
+  for (int q = threadIdx.x; q < 324; q += kBlock) {
+    if (mode == 0) {
+      a.Ke[e * 324 + q] = 0.0;
+      if (a.any_visc) a.Kv[e * 324 + q] = 0.0;
+    } else {
+      a.Ke32[e * 324 + q] = 0.f;
+      if (a.any_visc && mode == 1) a.Kv32[e * 324 + q] = 0.f;
+    }
+  }
+  for (int q = threadIdx.x; q < 18; q += kBlock) {
+    a.Fe[e * 18 + q] = 0.0;
+    a.Fv[e * 18 + q] = 0.0;
+  }
+}
+}  // namespace
+
+hipError_t launch_ablate_zero(const ElementArgs& a, int mode, const int32_t* list, int n,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(ablate_zero_kernel, dim3(n), dim3(kBlock), 0, st, a, mode, list, n);
   return hipGetLastError();
 }
 

@@ -65,7 +65,7 @@ __device__ inline void p_rows(const float* d, const uint8_t* flags, int i, doubl
   const double R[3][3] = {{0.0, d2, -d1}, {-d2, 0.0, d0}, {d1, -d0, 0.0}};
 #pragma unroll
   for (int p = 0; p < 3; ++p) {
-    const bool off = flags && (flags[3 * i + p] & 3);
+    const bool off = flags && (flags[3 * i + p] & kCoarseExcluded);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       P[p][a] = off ? 0.0 : (p == a ? 1.0 : 0.0);
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void coarse_mask_kernel(int32_t n_list,
     if (flags) {
 #pragma unroll
       for (int p = 0; p < 3; ++p)
-        if (flags[3 * i + p] & 3) m |= (uint8_t)(1 << p);
+        if (flags[3 * i + p] & kCoarseExcluded) m |= (uint8_t)(1 << p);
     }
     list_mask[t] = m;
   }
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void coarse_prolong_kernel(
                         ya[2] + (ya[3] * d1 - ya[4] * d0)};
 #pragma unroll
     for (int p = 0; p < 3; ++p)
-      if (!(flags && (flags[3 * i + p] & 3))) z[3 * i + p] += v[p];
+      if (!(flags && (flags[3 * i + p] & kCoarseExcluded))) z[3 * i + p] += v[p];
   }
 }
 

@@ -214,6 +214,13 @@ struct Workspace {
 };
 
 constexpr int kBlock = 256;
+// Per-DOF flags (dof_flags): bit 0 fixed, bit 1 slave of a binding pair, bit 2
+// a DOF of a laser-ablated node.  The coarse space of the two-level CG has
+// zero rows for all three, so its correction never moves them: an ablated node
+// without live prisms has a residual that does not depend on its position, and
+// only an exact solve of its identity rows keeps it in place.
+constexpr uint8_t kDofFixed = 1, kDofSlave = 2, kDofAblated = 4;
+constexpr uint8_t kCoarseExcluded = kDofFixed | kDofSlave | kDofAblated;
 constexpr int kGmresRestart = 100;
 
 enum KState { K_RUN = 0, K_CONVERGED = 1, K_BREAKDOWN = 2 };
@@ -263,6 +270,9 @@ struct AssembleArgs {
   const double* ext_visc;
   const double* area;
   const uint8_t* dof_flags;  // bit0 fixed, bit1 slave
+  // laser-ablated nodes (Simulation::AblatedNodes, nullptr: none): a zero
+  // diagonal Jacobian entry becomes 1 (checkJacobianForAblatedNodes)
+  const uint8_t* abl_node = nullptr;
   const double* uk;
   const double* un;
   double dt;
@@ -283,6 +293,12 @@ struct AssembleArgs {
   double* pk_force = nullptr;
 };
 hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st);
+// Zero outputs (tangents in the element mode's buffers, and forces) of the
+// laser-ablated prisms `list` (n of them) after the per-prism kernel: an
+// ablated prism contributes no forces and zero tangents
+// (NewtonRaphsonSolver.cpp:236, 344; ShapeBase.cpp:2850-2852, 2879-2882).
+hipError_t launch_ablate_zero(const ElementArgs& a, int mode, const int32_t* list, int n,
+                              hipStream_t st);
 
 // ---- node-node packing (tf_packing.hip) ----
 // One couple id0 < id1 of Simulation::calculatePackingForcesImplicit3D
@@ -379,10 +395,12 @@ hipError_t launch_visc_surface(int32_t n_nodes, int32_t n_prisms, const int32_t*
                                const double* u, const int32_t* node_ptr,
                                const int32_t* node_src, const int32_t* placement,
                                const int32_t* type, const uint8_t* spans, double* area,
-                               hipStream_t st);
+                               hipStream_t st,
+                               const uint8_t* ablated = nullptr);
 
 hipError_t launch_grow(int32_t n_prisms, const double* rates, double dt, double* Fg,
-                       double* invFg, hipStream_t st);
+                       double* invFg, hipStream_t st,
+                       const uint8_t* ablated = nullptr);
 hipError_t launch_inverse(int32_t n_prisms, const double* Fg, double* invFg, hipStream_t st);
 
 // ---- Krylov (tf_tiled.hip) ----

@@ -55,6 +55,14 @@ struct tf_handle {
   int device = 0;
   // this device's copy of the library's static scratch array (see scratch_resolve)
   void* scratch = nullptr;
+  // laser ablation (tf_nr_set_ablation): per-prism flags and their list, and
+  // per-node flags of the ablated nodes (internal numbering)
+  uint8_t* abl_el = nullptr;
+  int32_t* abl_list = nullptr;
+  uint8_t* abl_node = nullptr;
+  std::vector<uint8_t> h_abl_node;  // per internal node, for the DOF flags
+  int32_t n_abl = 0;
+  bool any_abl_node = false;
   hipStream_t st = nullptr;
   DevPattern pat;
   std::vector<int32_t> h_slice_base, h_col, h_row_len, h_rowptr, h_nbr;
@@ -79,6 +87,7 @@ struct tf_handle {
   double* extv = nullptr;
   double* area = nullptr;
   uint8_t* dof_flags = nullptr;
+  std::vector<uint8_t> h_flags_base;  // dof_flags without the ablation bits
   int32_t* pairs = nullptr;
   int32_t n_pairs = 0;
   std::vector<int32_t> h_pairs;
@@ -1023,6 +1032,7 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
   TF_CHECK(hipEventRecord(h->ev[0], h->st));
   // mixed: FP32 elastic + viscous blocks summed per K block (one gather each)
   TF_CHECK(launch_element(ea, mixed ? 2 : 0, h->st));
+  TF_CHECK(launch_ablate_zero(ea, mixed ? 2 : 0, h->abl_list, h->n_abl, h->st));
   TF_CHECK(hipEventRecord(h->ev[1], h->st));
   AssembleArgs aa;
   aa.n_nodes = h->Nn;
@@ -1032,6 +1042,7 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
   aa.any_visc = h->any_visc ? 1 : 0;
   aa.combined = mixed ? 1 : 0;
   aa.ext_visc = h->extv; aa.area = h->area; aa.dof_flags = h->dof_flags;
+  aa.abl_node = h->any_abl_node ? h->abl_node : nullptr;
   aa.uk = w.uk; aa.un = w.un; aa.dt = dt;
   aa.val = mixed ? nullptr : w.val;
   aa.val32 = w.val32;
@@ -1136,8 +1147,12 @@ tf_status tf_nr_set_material(tf_handle* h, const double* E, const double* nu, co
     const int shift = 3 * (h->int_off[ts] - h->user_off[ts]);
     h->h_pairs[2 * p] = s + shift;
     h->h_pairs[2 * p + 1] = m + shift;
-    flags[s + shift] |= 2;
+    flags[s + shift] |= kDofSlave;
   }
+  h->h_flags_base = flags;
+  for (size_t i = 0; i < h->h_abl_node.size(); ++i)
+    if (h->h_abl_node[i])
+      for (int d = 0; d < 3; ++d) flags[3 * i + d] |= kDofAblated;
   TF_CHECK(hipMemcpy(h->lam, lam.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(h->mu, mu.data(), sizeof(double) * Ne, hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(h->eta, eta, sizeof(double) * Ne, hipMemcpyHostToDevice));
@@ -1194,6 +1209,60 @@ tf_status tf_nr_set_visc_surface(tf_handle* h, const double* area) {
   return TF_OK;
 }
 
+tf_status tf_nr_set_ablation(tf_handle* h, const uint8_t* ablated_prisms,
+                            const int32_t* ablated_nodes, int32_t n_ablated_nodes) {
+  if (!h || n_ablated_nodes < 0 || (n_ablated_nodes > 0 && !ablated_nodes)) {
+    set_error("tf_nr_set_ablation: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (h->pack.on && (ablated_prisms || n_ablated_nodes > 0)) {
+    set_error("laser ablation together with node-node packing is not supported");
+    return TF_ERR_ARG;
+  }
+  std::vector<int32_t> list;
+  if (ablated_prisms)
+    for (int e = 0; e < h->Ne; ++e)
+      if (ablated_prisms[e]) list.push_back(e);
+  std::vector<uint8_t> nf((size_t)h->n_user, 0);
+  for (int i = 0; i < n_ablated_nodes; ++i) {
+    const int nd = ablated_nodes[i];
+    if (nd < 0 || nd >= h->n_user) {
+      set_error("tf_nr_set_ablation: ablated node " + std::to_string(nd) + " out of range");
+      return TF_ERR_ARG;
+    }
+    nf[nd] = 1;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  if (!h->abl_el) {
+    TF_CHECK(dalloc(h, &h->abl_el, (size_t)h->Ne));
+    TF_CHECK(dalloc(h, &h->abl_list, (size_t)h->Ne));
+    TF_CHECK(dalloc(h, &h->abl_node, (size_t)h->Nn));
+  }
+  std::vector<uint8_t> ef((size_t)h->Ne, 0);
+  for (int32_t e : list) ef[e] = 1;
+  std::vector<uint8_t> ni = to_internal<uint8_t>(h, nf.data(), 1, (uint8_t)0);
+  TF_CHECK(hipMemcpy(h->abl_el, ef.data(), ef.size(), hipMemcpyHostToDevice));
+  if (!list.empty())
+    TF_CHECK(hipMemcpy(h->abl_list, list.data(), sizeof(int32_t) * list.size(),
+                       hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->abl_node, ni.data(), ni.size(), hipMemcpyHostToDevice));
+  h->n_abl = (int32_t)list.size();
+  h->any_abl_node = n_ablated_nodes > 0;
+  h->h_abl_node = ni;
+  if (h->h_flags_base.empty()) {
+    // before tf_nr_set_material: padding nodes of a batch are fixed
+    std::vector<uint8_t> fu((size_t)h->n_user * 3, 0);
+    h->h_flags_base = to_internal<uint8_t>(h, fu.data(), 3, kDofFixed);
+  }
+  std::vector<uint8_t> flags = h->h_flags_base;
+  for (size_t i = 0; i < ni.size(); ++i)
+    if (ni[i])
+      for (int d = 0; d < 3; ++d) flags[3 * i + d] |= kDofAblated;
+  TF_CHECK(hipMemcpy(h->dof_flags, flags.data(), flags.size(), hipMemcpyHostToDevice));
+  h->assembled = false;
+  return TF_OK;
+}
+
 tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u, const int32_t* elem_placement,
                                     const int32_t* elem_type, const uint8_t* spans,
                                     double* area_out) {
@@ -1216,7 +1285,8 @@ tf_status tf_nr_update_visc_surface(tf_handle* h, const double* u, const int32_t
     TF_CHECK(hipStreamSynchronize(h->st));
   }
   TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, h->ws.tmp, h->pat.node_ptr,
-                               h->pat.node_src, pl, ty, sp, h->area, h->st));
+                               h->pat.node_src, pl, ty, sp, h->area, h->st,
+                               h->n_abl ? h->abl_el : nullptr));
   TF_CHECK(hipStreamSynchronize(h->st));
   if (area_out) {
     std::vector<double> a(h->Nn);
@@ -1265,7 +1335,8 @@ tf_status tf_nr_update_visc_surface_noalloc(tf_handle* h, const double* u,
     if (up != TF_OK) return up;
   }
   TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, h->ws.tmp, h->pat.node_ptr,
-                               h->pat.node_src, pl, ty, sp, h->area, h->st));
+                               h->pat.node_src, pl, ty, sp, h->area, h->st,
+                               h->n_abl ? h->abl_el : nullptr));
   TF_CHECK(hipStreamSynchronize(h->st));
   h->have_area = true;
   if (area_out) {
@@ -1810,9 +1881,10 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
       if (ss != TF_OK) return ss;
     }
     h->sim_time += dt;
-    TF_CHECK(launch_grow(h->Ne, h->rates, dt, h->Fg, h->invFg, h->st));
+    const uint8_t* abl = h->n_abl ? h->abl_el : nullptr;
+    TF_CHECK(launch_grow(h->Ne, h->rates, dt, h->Fg, h->invFg, h->st, abl));
     TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, w.uk, h->pat.node_ptr, h->pat.node_src,
-                                 h->surf_place, h->surf_type, h->surf_spans, h->area, h->st));
+                                 h->surf_place, h->surf_type, h->surf_spans, h->area, h->st, abl));
     h->have_area = true;
     if (h->pack.on) {
       // node masses from the grown volumes, then the couples at the step's
@@ -1897,8 +1969,9 @@ tf_status tf_adj_forward(tf_handle* h, int32_t n_steps, const double* u0,
     set_error("tf_adj_forward: invalid arguments");
     return TF_ERR_ARG;
   }
-  if (h->n_pairs > 0 || h->batch || h->pack.on) {
-    set_error("tf_adj_forward: binding, batches and packing are not supported by the adjoint");
+  if (h->n_pairs > 0 || h->batch || h->pack.on || h->n_abl > 0 || h->any_abl_node) {
+    set_error("tf_adj_forward: binding, batches, packing and ablation are not supported by the "
+              "adjoint");
     return TF_ERR_ARG;
   }
   const size_t n = 3 * (size_t)h->Nn, Ne = h->Ne;
@@ -2094,6 +2167,10 @@ tf_status tf_nr_set_packing(tf_handle* h, const int32_t* node_placement,
     return TF_ERR_ARG;
   }
   PackDev& p = h->pack;
+  if (node_placement && (h->n_abl > 0 || h->any_abl_node)) {
+    set_error("node-node packing together with laser ablation is not supported");
+    return TF_ERR_ARG;
+  }
   if (!node_placement) {  // off: every contact slot unused
     if (p.on) {
       TF_CHECK(hipMemsetAsync(p.cnt, 0, sizeof(int32_t) * h->Nn, h->st));
@@ -2424,6 +2501,7 @@ tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke
                    h->last_dt, h->any_visc ? 1 : 0, w.elemKe, w.elemKv, w.elemKe32, w.elemKv32,
                    w.elemFe, w.elemFv};
     TF_CHECK(launch_element(ea, 1, h->st));
+    TF_CHECK(launch_ablate_zero(ea, 1, h->abl_list, h->n_abl, h->st));
     TF_CHECK(hipStreamSynchronize(h->st));
   }
   if (ge) TF_CHECK(hipMemcpy(ge, h->ws.elemFe, sizeof(double) * Ne * 18, hipMemcpyDeviceToHost));

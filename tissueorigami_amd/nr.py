@@ -20,7 +20,7 @@ PREC = {"fp64": 0, "mixed": 1}
 MAX_NR = 32
 
 EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_visc_surface",
-           "tf_nr_update_visc_surface", "tf_nr_update_visc_surface_noalloc", "tf_nr_step", "tf_nr_assemble", "tf_nr_solve",
+           "tf_nr_update_visc_surface", "tf_nr_update_visc_surface_noalloc", "tf_nr_set_ablation", "tf_nr_step", "tf_nr_assemble", "tf_nr_solve",
            "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_profile",
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
@@ -89,6 +89,7 @@ def load():
     L.tf_nr_set_visc_surface.argtypes = [_h, _d]
     L.tf_nr_update_visc_surface.argtypes = [_h, _d, _i, _i, _u8, _d]
     L.tf_nr_update_visc_surface_noalloc.argtypes = [_h, _d, _i, _i, _u8, _d]
+    L.tf_nr_set_ablation.argtypes = [_h, _u8, _i, ctypes.c_int32]
     L.tf_nr_step.argtypes = [_h, _d, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
                              ctypes.c_int32, ctypes.c_double, _d, ctypes.POINTER(StepStats)]
     L.tf_nr_assemble.argtypes = [_h, _d, _d, ctypes.c_double, _d, _d]
@@ -213,6 +214,14 @@ class NewtonRaphsonSolver:
     def set_visc_surface(self, area):
         a = _f64(area).reshape(self.n_nodes)
         _check(load().tf_nr_set_visc_surface(self._h, _p(a, _d)))
+
+    def set_ablation(self, prisms=None, nodes=None):
+        """Laser ablation: per-prism flags (ShapeBase::IsAblated) and the
+        ablated node ids of checkJacobianForAblatedNodes; None clears."""
+        pr = None if prisms is None else np.ascontiguousarray(prisms, dtype=np.uint8)
+        nd = None if nodes is None else np.ascontiguousarray(nodes, dtype=np.int32).reshape(-1)
+        _check(load().tf_nr_set_ablation(self._h, _p(pr, _u8), _p(nd, _i),
+                                         0 if nd is None else int(nd.size)))
 
     def update_visc_surface(self, u, elem_placement, elem_type, spans):
         u = _f64(u).reshape(-1)

@@ -224,6 +224,15 @@ def batch_ellipses(n=64, prism_range=(2000, 40000), layer_range=(2, 6), E_range=
     return out
 
 
+def block_count(conn, n_nodes):
+    """Stored 3x3 blocks of the Jacobian of a mesh: the distinct (node, node)
+    pairs that share a prism, the diagonal included."""
+    c = np.asarray(conn, dtype=np.int64).reshape(-1, 6)
+    a = np.repeat(c, 6, axis=1).reshape(-1)
+    b = np.tile(c, (1, 6)).reshape(-1)
+    return int(len(np.unique(a * int(n_nodes) + b)))
+
+
 def rates_xyz(rate_per_hour):
     """Per-prism growth rates (x, y, z) in 1/s for in-plane isotropic growth."""
     r = np.zeros((len(rate_per_hour), 3))
