@@ -236,6 +236,75 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
                         int32_t maxit, int32_t precision_mode, double krylov_rtol, double* u_out,
                         tf_step_stats* stats);
 
+/* Growth functions on the device (Simulation::calculateGrowth,
+ * Simulation.cpp:8640-8660, with uniform and grid-based functions).  Once
+ * they are added, every step of tf_nr_advance runs, before the NR loop and in
+ * Simulation::runOneStep's order (Simulation.cpp:5974-6063): the relative
+ * positions of the prisms in the tissue's bounding box, the stiffness series,
+ * the pinning updates, each prism's growth rotation about z, the growth of
+ * every function active at the step's start time, the column volume
+ * conservation, and F_g / InvFg; fixed growth rates (tf_nr_set_growth_rates)
+ * are then not used.  On a batch handle the arrays run over the tissues'
+ * prisms (and nodes) concatenated; the functions apply to every tissue, each
+ * in its own bounding box.
+ * tf_nr_set_growth_prisms: per prism ShapeBase::tissueType (0 columnar, 1
+ *   peripodial, 2 linker), tissuePlacement (0 basal, 1 apical, 2 middle, 4
+ *   lateral), flags (1 isECMMimicing, 2 isECMMimimcingAtCircumference, 4
+ *   isActinMimicing), weights (n_prisms x 2 or NULL: columnarGrowthWeight,
+ *   peripodialGrowthWeight of linker prisms), rel_z
+ *   (initialRelativePositionInZ, ShapeBase.cpp:817-828); apical_of (NULL or
+ *   per prism: the prism whose relative position it takes,
+ *   Simulation::updateRelativePositionsToApicalPositioning,
+ *   Simulation.cpp:6147-6167, -1 for its own); node_excluded (NULL or per
+ *   node: Node::allOwnersECMMimicing, left out of the bounding box,
+ *   Simulation::calculateBoundingBox, Simulation.cpp:8388-8428);
+ *   symmetric_y (Simulation::symmetricY); explicit_ecm
+ *   (Simulation::thereIsExplicitECM: ECM prisms take no columnar or
+ *   peripodial growth).
+ * tf_nr_add_uniform_growth: UniformGrowthFunction (GrowthFunctionTypes.h:14-97;
+ *   ModelInputObject::readGrowthType1, ModelInputObject.cpp:403-517) active
+ *   for t0 <= t < t1 (kept as float, as GrowthFunctionBase.h:39-40 does);
+ *   apply: 1 columnar layer, 2 peripodial membrane, 4 basal ECM, 8 lateral
+ *   ECM; [zmin, zmax] the rel_z range; rate (x, y, z, 1/s) and angle
+ *   (radians) as the function holds them.
+ * tf_nr_add_grid_growth: GridBasedGrowthFunction (GrowthFunctionTypes.h:290-546;
+ *   ModelInputObject::readGrowthType3, ModelInputObject.cpp:684-870) on an
+ *   nx x ny grid: rates[(i * ny + j) * 3 + k] (1/s) and angles_deg[i * ny + j]
+ *   (degrees) at grid point (i, j), i along x.
+ * tf_nr_set_growth_options: Simulation::gridGrowthsInterpolationType (0
+ *   nearest corner, 1 bilinear), GridGrowthsPinnedOnInitialMesh and the
+ *   pinning update times (s; each done once, at the first step whose start
+ *   time reaches it, Simulation.cpp:6169-6187).
+ * tf_nr_set_column_volume_conservation: Simulation::conservingColumnVolumes
+ *   (Simulation.cpp:6040-6046, 6518-6600): columns[n_columns][layers] holds
+ *   the prisms of each column whose volume is conserved (the columnar, not
+ *   ECM, not actin prisms of an apical prism's stack; -1 pads).
+ * tf_nr_update_relative_positions: the relative positions at positions u
+ *   (NULL: the device-resident positions); the first call also sets the
+ *   pinned positions (Simulation::initiateSystem, Simulation.cpp:741-752).
+ *   Stiffness series use these from then on.
+ * tf_nr_relative_positions: current and pinned positions (n_prisms x 2; either
+ *   may be NULL).
+ * tf_nr_grow: one step's growth at time t from positions u (NULL: the
+ *   device-resident positions), with the current relative positions. */
+tf_status tf_nr_set_growth_prisms(tf_handle* h, const int32_t* tissue_type,
+                                  const int32_t* placement, const uint8_t* flags,
+                                  const double* weights, const double* rel_z,
+                                  const int32_t* apical_of, const uint8_t* node_excluded,
+                                  int32_t symmetric_y, int32_t explicit_ecm);
+tf_status tf_nr_add_uniform_growth(tf_handle* h, double t0, double t1, int32_t apply, double zmin,
+                                   double zmax, const double* rate, double angle);
+tf_status tf_nr_add_grid_growth(tf_handle* h, double t0, double t1, int32_t apply, double zmin,
+                                double zmax, int32_t nx, int32_t ny, const double* rates,
+                                const double* angles_deg);
+tf_status tf_nr_set_growth_options(tf_handle* h, int32_t interpolation, int32_t pinned,
+                                   int32_t n_pin_times, const double* pin_times);
+tf_status tf_nr_set_column_volume_conservation(tf_handle* h, int32_t n_columns, int32_t layers,
+                                               const int32_t* columns);
+tf_status tf_nr_update_relative_positions(tf_handle* h, const double* u);
+tf_status tf_nr_relative_positions(tf_handle* h, double* current, double* pinned);
+tf_status tf_nr_grow(tf_handle* h, const double* u, double t, double dt);
+
 /* Young's modulus time series (YoungsModulusModifier, YoungsModulusModifier.cpp:1-177,
  * with its base TimeSeriesPhysicalProperties).  On a batch handle the per-prism
  * arrays run over the tissues' prisms concatenated, each tissue's relative
@@ -250,7 +319,10 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
  * test of YoungsModulusModifier.cpp:74-100); rel_pos (n_prisms x 2) the
  * prism's relative (x, y) position in the tissue's bounding box
  * (ShapeBase::relativePosInBoundingBox, after the apical positioning of
- * Simulation.cpp:6147-6167).  Needs tf_nr_set_material first.
+ * Simulation.cpp:6147-6167), used until the device computes them
+ * (tf_nr_update_relative_positions, and every step of tf_nr_advance once the
+ * growth prisms are set); NULL when the growth prisms are set.  Needs
+ * tf_nr_set_material first.
  * tf_nr_apply_stiffness_series replaces Simulation::ApplyAllYoungsModulusModifiers
  *   (Simulation.cpp:878-884) at time t: each series in the order added, over
  *   the interval [times[k], times[k+1]) that holds t (none when t lies past

@@ -51,6 +51,37 @@ struct Node {
   int tissueType = 0;
   double mass = 0.0;                   // Node::mass (Simulation::updateNodeMasses)
   bool hasLateralElementOwner = false;
+  // Node::allOwnersECMMimicing: left out of the bounding box (Simulation.cpp:8397)
+  bool allOwnersECMMimicing = false;
+};
+
+// A growth function (GrowthFunctionBase.h:13-60 with UniformGrowthFunction,
+// type 1, GrowthFunctionTypes.h:14-97, and GridBasedGrowthFunction, type 3,
+// GrowthFunctionTypes.h:290-546), holding what the reference's objects hold
+// after ModelInputObject has read them: times in s, rates in 1/s, the uniform
+// angle in radians, the grid angles in degrees.
+struct GrowthFunction {
+  int Type = 1;
+  float initTime = 0.0f, endTime = 0.0f;
+  bool applyToColumnarLayer = false, applyToPeripodialMembrane = false,
+       applyToBasalECM = false, applyToLateralECM = false;
+  double zMin = 0.0, zMax = 1.0;
+  double GrowthRate[3] = {0.0, 0.0, 0.0};
+  double angle = 0.0;
+  int nGridX = 0, nGridY = 0;
+  std::vector<std::vector<std::array<double, 3>>> GrowthMatrix;  // [i][j][x, y, z]
+  std::vector<std::vector<double>> AngleMatrix;                  // [i][j]
+};
+
+// Growth options of Simulation (Simulation.h:349-360) the growth step reads.
+struct GrowthOptions {
+  int gridGrowthsInterpolationType = 1;
+  bool GridGrowthsPinnedOnInitialMesh = false;
+  std::vector<int> growthPinUpdateTime;  // s
+  bool thereIsExplicitECM = false;
+  bool conservingColumnVolumes = false;
+  bool symmetricY = false;
+  int TissueHeightDiscretisationLayers = 1;
 };
 
 // Element state read by the NR step (ShapeBase.h:53-69, 129-143).
@@ -73,6 +104,15 @@ class ShapeBase {
   double StiffnessTimeSeriesMultiplier = 1.0;
   bool isECMMimicing = false, isECMMimimcingAtCircumference = false, atBasalBorderOfECM = false;
   double relativePosInBoundingBox[2] = {0.0, 0.0};
+  // growth (ShapeBase.h:120-160): the growth tensor, the linker prisms'
+  // weights, the relative z position, actin, the stack of the prism's column
+  // (basal first) and the pinned relative position
+  double Fg[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  double columnarGrowthWeight = 0.0, peripodialGrowthWeight = 0.0;
+  double initialRelativePositionInZ = 0.0;
+  bool isActinMimicing = false;
+  std::vector<int> elementsIdsOnSameColumn;
+  double initialRelativePosInBoundingBox[2] = {0.0, 0.0};
   int getNodeNumber() const { return 6; }
   const int* getNodeIds() const { return NodeIds.data(); }
   // ShapeBase::updatePositions (ShapeBase.cpp:3279-3285): element positions
@@ -346,9 +386,159 @@ class NewtonRaphsonSolver {
     for (size_t e = 0; e < Elements.size(); ++e) Elements[e]->StiffnessTimeSeriesMultiplier = m[e];
   }
 
+  // The growth functions and options of a Simulation, with its elements'
+  // growth data: from now on calculateRelativePositions and calculateGrowth
+  // run them on the device, and the stiffness series take their relative
+  // positions from there.  The elements' Fg is the starting growth.
+  void setGrowthFunctions(const std::vector<GrowthFunction>& functions, const GrowthOptions& opt,
+                          const std::vector<std::unique_ptr<Node>>& Nodes,
+                          const std::vector<std::unique_ptr<ShapeBase>>& Elements) {
+    bind(Nodes, Elements);
+    const int ne = (int)Elements.size();
+    const int L = opt.TissueHeightDiscretisationLayers;
+    std::vector<int32_t> type(ne), place(ne), apical(ne, -1);
+    std::vector<uint8_t> flags(ne), excl(nNodes);
+    std::vector<double> w(2 * (size_t)ne), rz(ne), Fg(9 * (size_t)ne);
+    std::vector<int32_t> cols;
+    for (int e = 0; e < ne; ++e) {
+      const ShapeBase& el = *Elements[e];
+      type[e] = el.tissueType;
+      place[e] = el.tissuePlacement;
+      flags[e] = (el.isECMMimicing ? 1 : 0) | (el.isECMMimimcingAtCircumference ? 2 : 0) |
+                 (el.isActinMimicing ? 4 : 0);
+      w[2 * e] = el.columnarGrowthWeight;
+      w[2 * e + 1] = el.peripodialGrowthWeight;
+      rz[e] = el.initialRelativePositionInZ;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Fg[9 * e + 3 * i + j] = el.Fg[i][j];
+    }
+    for (int e = 0; e < ne; ++e) {
+      const ShapeBase& el = *Elements[e];
+      if (el.tissueType != 0 || el.tissuePlacement != 1 || (int)el.elementsIdsOnSameColumn.size() < L)
+        continue;
+      // Simulation::updateRelativePositionsToApicalPositioning (Simulation.cpp:6147-6167)
+      for (int i = 0; i < L - 2; ++i) apical[el.elementsIdsOnSameColumn[i]] = e;
+      // Simulation::conserveColumnVolume (Simulation.cpp:6518-6540)
+      for (int i = 0; i < L; ++i) {
+        const int m = el.elementsIdsOnSameColumn[i];
+        const ShapeBase& c = *Elements[m];
+        cols.push_back(c.tissueType == 0 && !c.isECMMimicing && !c.isActinMimicing ? m : -1);
+      }
+    }
+    for (int i = 0; i < nNodes; ++i) excl[i] = Nodes[i]->allOwnersECMMimicing ? 1 : 0;
+    check(tf_nr_set_growth_prisms(h, type.data(), place.data(), flags.data(), w.data(), rz.data(),
+                                  apical.data(), excl.data(), opt.symmetricY ? 1 : 0,
+                                  opt.thereIsExplicitECM ? 1 : 0));
+    for (const GrowthFunction& f : functions) {
+      const int apply = (f.applyToColumnarLayer ? 1 : 0) | (f.applyToPeripodialMembrane ? 2 : 0) |
+                        (f.applyToBasalECM ? 4 : 0) | (f.applyToLateralECM ? 8 : 0);
+      if (f.Type == 1) {
+        check(tf_nr_add_uniform_growth(h, f.initTime, f.endTime, apply, f.zMin, f.zMax,
+                                       f.GrowthRate, f.angle));
+      } else if (f.Type == 3) {
+        std::vector<double> r, a;
+        for (int i = 0; i < f.nGridX; ++i)
+          for (int j = 0; j < f.nGridY; ++j) {
+            for (int k = 0; k < 3; ++k) r.push_back(f.GrowthMatrix.at(i).at(j)[k]);
+            a.push_back(f.AngleMatrix.at(i).at(j));
+          }
+        check(tf_nr_add_grid_growth(h, f.initTime, f.endTime, apply, f.zMin, f.zMax, f.nGridX,
+                                    f.nGridY, r.data(), a.data()));
+      } else {
+        throw std::invalid_argument("growth function type " + std::to_string(f.Type) +
+                                    " is not supported");
+      }
+    }
+    std::vector<double> pins(opt.growthPinUpdateTime.begin(), opt.growthPinUpdateTime.end());
+    check(tf_nr_set_growth_options(h, opt.gridGrowthsInterpolationType,
+                                   opt.GridGrowthsPinnedOnInitialMesh ? 1 : 0, (int32_t)pins.size(),
+                                   pins.empty() ? nullptr : pins.data()));
+    if (opt.conservingColumnVolumes && !cols.empty())
+      check(tf_nr_set_column_volume_conservation(h, (int32_t)(cols.size() / L), L, cols.data()));
+    check(tf_nr_growth_tensor(h, nullptr, Fg.data()));
+  }
+
+  // Simulation::calculateBoundingBox with ShapeBase::calculateRelativePosInBoundingBox
+  // and updateRelativePositionsToApicalPositioning (Simulation.cpp:5976-5983) at
+  // the nodes' positions, on the device; the positions come back into the
+  // elements (the first call also sets the pinned positions).
+  void calculateRelativePositions(const std::vector<std::unique_ptr<Node>>& Nodes,
+                                  const std::vector<std::unique_ptr<ShapeBase>>& Elements) {
+    constructUnMatrix(Nodes);
+    check(tf_nr_update_relative_positions(h, un.data()));
+    const size_t ne = Elements.size();
+    std::vector<double> cur(2 * ne), pin(2 * ne);
+    check(tf_nr_relative_positions(h, cur.data(), pin.data()));
+    for (size_t e = 0; e < ne; ++e)
+      for (int i = 0; i < 2; ++i) {
+        Elements[e]->relativePosInBoundingBox[i] = cur[2 * e + i];
+        Elements[e]->initialRelativePosInBoundingBox[i] = pin[2 * e + i];
+      }
+  }
+
+  // The growth of one step at currSimTimeSec (the pinning updates,
+  // Simulation::updateGrowthRotationMatrices, calculateGrowth, the column
+  // volume conservation and growShapeByFg of Simulation::runOneStep,
+  // Simulation.cpp:6012-6063) from the nodes' positions.  Fg and InvFg come
+  // back into the elements; the device keeps its copy.
+  void calculateGrowth(const std::vector<std::unique_ptr<Node>>& Nodes,
+                       const std::vector<std::unique_ptr<ShapeBase>>& Elements,
+                       double currSimTimeSec, double dt) {
+    constructUnMatrix(Nodes);
+    check(tf_nr_grow(h, un.data(), currSimTimeSec, dt));
+    const size_t ne = Elements.size();
+    std::vector<double> Fg(9 * ne);
+    check(tf_nr_growth_tensor(h, Fg.data(), nullptr));
+    for (size_t e = 0; e < ne; ++e) {
+      ShapeBase& el = *Elements[e];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) el.Fg[i][j] = Fg[9 * e + 3 * i + j];
+      invert3(el.Fg, el.InvFg);
+      for (int q = 0; q < 9; ++q) bFg[9 * e + q] = el.InvFg[q / 3][q % 3];
+    }
+  }
+
   tf_handle* handle() const { return h; }
 
  private:
+  // 3x3 inverse by LU with partial pivoting, the device's (and GSL's
+  // LU_decomp / LU_invert) order of operations, so that the InvFg the elements
+  // get back equals the device's and bind() has nothing to upload.
+  static void invert3(const double A[3][3], double X[3][3]) {
+    double a[3][3];
+    int p[3] = {0, 1, 2};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) a[i][j] = A[i][j];
+    for (int c = 0; c < 2; ++c) {
+      int r = c;
+      double mx = std::fabs(a[c][c]);
+      for (int i = c + 1; i < 3; ++i)
+        if (std::fabs(a[i][c]) > mx) { mx = std::fabs(a[i][c]); r = i; }
+      if (r != c) {
+        for (int k = 0; k < 3; ++k) { const double t = a[c][k]; a[c][k] = a[r][k]; a[r][k] = t; }
+        const int t = p[c]; p[c] = p[r]; p[r] = t;
+      }
+      if (a[c][c] != 0.0)
+        for (int i = c + 1; i < 3; ++i) {
+          a[i][c] /= a[c][c];
+          for (int k = c + 1; k < 3; ++k) a[i][k] -= a[i][c] * a[c][k];
+        }
+    }
+    for (int col = 0; col < 3; ++col) {
+      double y[3];
+      for (int i = 0; i < 3; ++i) {
+        double s = p[i] == col ? 1.0 : 0.0;
+        for (int k = 0; k < i; ++k) s -= a[i][k] * y[k];
+        y[i] = s;
+      }
+      for (int i = 2; i >= 0; --i) {
+        double s = y[i];
+        for (int k = i + 1; k < 3; ++k) s -= a[i][k] * X[k][col];
+        X[i][col] = s / a[i][i];
+      }
+    }
+  }
+
   // the ablation flags and node list of the next assemblies
   void bindAblation(const std::vector<uint8_t>& abl) {
     std::vector<int32_t> nodes(AblatedNodes.begin(), AblatedNodes.end());

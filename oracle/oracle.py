@@ -79,6 +79,18 @@ def lib():
         L.orc_detect_packing.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _i, _u8, _d, _d, _d,
                                          _i, ctypes.c_int]
         L.orc_detect_packing.restype = ctypes.c_int
+        L.orc_grid_compatible.argtypes = [ctypes.c_int, ctypes.c_int, _d, _d, _d, _d, _u8]
+        L.orc_polar_rotation.argtypes = [_d, _d]
+        L.orc_polar_rotation.restype = ctypes.c_int
+        L.orc_growth_rotation.argtypes = [_d, _d, _d]
+        L.orc_growth_rotation.restype = ctypes.c_int
+        L.orc_relative_positions.argtypes = [ctypes.c_int, _d, _u8, ctypes.c_int, ctypes.c_int,
+                                             _i, _i, _d]
+        L.orc_growth_step.argtypes = [ctypes.c_int, _i, _d, _d, _i, _i, _u8, _d, _d,
+                                      ctypes.c_int, ctypes.c_int, _i, _i, _d, _d, _d, _d, _i,
+                                      _d, _d, _d, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, _u8, ctypes.c_int, ctypes.c_int, _i, _d,
+                                      _d, _d]
         _lib = L
     return _lib
 
@@ -385,3 +397,94 @@ class StiffnessSeries:
             E.size, float(t), float(dt), nt, _p(self.times, _d), nx, ny, _p(self.rate, _d),
             self.linear, self.method, _p(self.applicable, _u8), _p(self.rel, _d), _p(E, _d),
             _p(nu, _d), _p(stm, _d), _p(lam, _d), _p(mu, _d))
+
+
+# ---- growth producers (oracle_growth.cpp)
+
+def grid_compatible(rates, angles_deg):
+    """Compatible corner data of a growth grid: rates (nx, ny, 3), angles (nx, ny)
+    in degrees -> (growths (cells, 4, 3), angles (cells, 4), eliminated (cells, 4)),
+    cell ix * (ny - 1) + iy."""
+    r = _f64(rates)
+    a = _f64(angles_deg)
+    nx, ny = a.shape
+    cells = (nx - 1) * (ny - 1)
+    cg = np.zeros((cells, 4, 3)); ca = np.zeros((cells, 4)); ce = np.zeros((cells, 4), np.uint8)
+    lib().orc_grid_compatible(nx, ny, _p(r, _d), _p(a, _d), _p(cg, _d), _p(ca, _d), _p(ce, _u8))
+    return cg, ca, ce.astype(bool)
+
+
+def polar_rotation(F):
+    F = _f64(F).reshape(3, 3)
+    R = np.zeros((3, 3))
+    return R if lib().orc_polar_rotation(_p(F, _d), _p(R, _d)) else None
+
+
+def growth_rotation(ref18, pos18):
+    """GrowthStrainsRotMat of a prism (3 x 3), or None when its growth is not rotated."""
+    r = _f64(ref18).reshape(18)
+    q = _f64(pos18).reshape(18)
+    g = np.zeros((3, 3))
+    return g if lib().orc_growth_rotation(_p(r, _d), _p(q, _d), _p(g, _d)) else None
+
+
+def relative_positions(u, conn, excluded=None, symmetric_y=False, apical_of=None):
+    u = _f64(u).reshape(-1)
+    c = _i32(conn).reshape(-1, 6)
+    ex = None if excluded is None else np.ascontiguousarray(excluded, np.uint8)
+    ap = None if apical_of is None else _i32(apical_of)
+    rel = np.zeros((len(c), 2))
+    lib().orc_relative_positions(u.size // 3, _p(u, _d), _p(ex, _u8), int(bool(symmetric_y)),
+                                 len(c), _p(c, _i), _p(ap, _i), _p(rel, _d))
+    return rel
+
+
+def growth_step(conn, refpos, u, prisms, fns, rel, t, dt, interp, Fg, ablated=None,
+                columns=None):
+    """One step of Simulation::calculateGrowth and growShapeByFg.  prisms: dict of
+    type, place, flags, rel_z, weights (or None), explicit_ecm; fns: list of dicts
+    with type (1 or 3), t (t0, t1), apply (bits), z (zmin, zmax) and rate (1/s),
+    angle (radians) for type 1, rates (nx, ny, 3) in 1/s and angles_deg (nx, ny)
+    for type 3; columns (n, layers) or None.  Returns (Fg, invFg), n x 9."""
+    c = _i32(conn).reshape(-1, 6)
+    ne = len(c)
+    ref = _f64(refpos).reshape(ne, 18)
+    u = _f64(u).reshape(-1)
+    tt = _i32(prisms["type"]); pl = _i32(prisms["place"])
+    fl = np.ascontiguousarray(prisms["flags"], np.uint8)
+    w = prisms.get("weights")
+    w = None if w is None else _f64(w).reshape(ne, 2)
+    rz = _f64(prisms["rel_z"])
+    n = len(fns)
+    ftype = np.array([f["type"] for f in fns], np.int32)
+    fapply = np.array([f["apply"] for f in fns], np.int32)
+    ft = _f64([f["t"] for f in fns]).reshape(-1)
+    fz = _f64([f.get("z", (0.0, 1.0)) for f in fns]).reshape(-1)
+    frate = _f64([f.get("rate", (0.0, 0.0, 0.0)) for f in fns]).reshape(-1)
+    fang = _f64([f.get("angle", 0.0) for f in fns])
+    fgrid = np.zeros((max(n, 1), 3), np.int32)
+    grates, gang, off = [], [], 0
+    for k, f in enumerate(fns):
+        if f["type"] == 3:
+            a = _f64(f["angles_deg"])
+            fgrid[k] = (a.shape[0], a.shape[1], off)
+            grates.append(_f64(f["rates"]).reshape(-1))
+            gang.append(a.reshape(-1))
+            off += a.size
+    grates = _f64(np.concatenate(grates) if grates else np.zeros(1))
+    gang = _f64(np.concatenate(gang) if gang else np.zeros(1))
+    rel = _f64(rel).reshape(ne, 2)
+    ab = None if ablated is None else np.ascontiguousarray(ablated, np.uint8)
+    cols = None if columns is None else _i32(columns).reshape(len(columns), -1)
+    ncol, layers = (0, 0) if cols is None else cols.shape
+    vref = reference_volumes(ref) if cols is not None else None
+    Fo = _f64(Fg).reshape(ne, 9).copy()
+    inv = np.zeros((ne, 9))
+    lib().orc_growth_step(ne, _p(c, _i), _p(ref, _d), _p(u, _d), _p(tt, _i), _p(pl, _i),
+                          _p(fl, _u8), _p(w, _d), _p(rz, _d), int(bool(prisms.get("explicit_ecm"))),
+                          n, _p(ftype, _i), _p(fapply, _i), _p(ft, _d), _p(fz, _d),
+                          _p(frate, _d), _p(fang, _d), _p(fgrid, _i), _p(grates, _d),
+                          _p(gang, _d), _p(rel, _d), float(t), float(dt), int(interp),
+                          _p(ab, _u8), ncol, layers, _p(cols, _i),
+                          _p(None if vref is None else _f64(vref), _d), _p(Fo, _d), _p(inv, _d))
+    return Fo, inv

@@ -42,6 +42,56 @@ def sha256(path):
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
 
 
+def growth_options(path, inp):
+    """The GrowthOptions section of a model input file
+    (ModelInputObject::readGrowthOptions, ModelInputObject.cpp:290-400) and
+    its functions (types 1 and 3), as data: times, parts, z range, uniform
+    rates (per hour) and angle (degrees), grid files read by io.read_growth_grid;
+    with the explicit-ECM and column-conservation switches."""
+    tok = open(path).read().split()
+    def after(key, k=0, start=0):
+        return tok[tok.index(key, start) + 1 + k]
+    g0 = tok.index("GrowthOptions:")
+    n = int(after("NumberofGrowthFunctions(int):", 0, g0))
+    out = {"growth_pinned": int(after("GridGrowthsPinnedOnInitialMesh(bool):", 0, g0)),
+           "growth_interp": int(after("GridGrowthsInterpolationType(0=step,1=linear):", 0, g0))}
+    npin = int(after("PinningUpdateTimes(number-times(sec)):", 0, g0))
+    out["growth_pin_times"] = np.array([float(after("PinningUpdateTimes(number-times(sec)):", 1 + j, g0))
+                                        for j in range(npin)])
+    types, times, apply, zr, rate, angle = [], [], [], [], [], []
+    p = g0
+    for f in range(n):
+        p = tok.index("GrowthFunctionType(int-seeDocumentation):", p) + 1
+        t = int(tok[p])
+        types.append(t)
+        times.append([float(after("InitialTime(sec):", 0, p)), float(after("FinalTime(sec):", 0, p))])
+        apply.append([int(after(k, 0, p)) for k in ("ApplyToColumnarLayer(bool):",
+                                                   "ApplyToPeripodialMembrane(bool):",
+                                                   "ApplyToBasalECM(bool):",
+                                                   "ApplyToLateralECM(bool):")])
+        if t == 1:
+            q = tok.index("MaxValue(fractionPerHour-DV,AP,AB):", p)
+            rate.append([float(tok[q + 1 + k]) for k in range(3)])
+            angle.append(float(after("Angle(degrees):", 0, p)))
+            zr.append([0.0, 1.0])
+        elif t == 3:
+            rate.append([0.0, 0.0, 0.0])
+            angle.append(0.0)
+            q = tok.index("zRange:", p)
+            zr.append([float(tok[q + 1]), float(tok[q + 2])])
+            gr, ga = io.read_growth_grid(os.path.join(inp, after("Filename(full-path):", 0, p)))
+            out[f"growth_grid{f}_rates"], out[f"growth_grid{f}_angles"] = gr, ga
+        else:
+            raise ValueError(f"growth function type {t} not extracted")
+    out.update(growth_type=np.array(types, np.int32), growth_time=np.array(times),
+               growth_apply=np.array(apply, np.int32), growth_z=np.array(zr),
+               growth_rate_per_hour=np.array(rate), growth_angle_deg=np.array(angle))
+    out["explicit_ecm"] = int(after("ThereIsExplicitECM(bool):"))
+    key = "ThereIsColumnViseVolumeConservation(bool):"
+    out["column_volume_conservation"] = int(after(key)) if key in tok else 0
+    return out
+
+
 def make_run(run, mesh_name):
     """run07007 (rectangle, no symmetry, no binding) and run07008 (half wing
     disc, symmetric in y, circumference x/y bound to the basal node)."""
@@ -89,6 +139,7 @@ def make_run(run, mesh_name):
     dt = float(modelinput_value(mi, "TimeStep(sec):"))
     sym_y = int(modelinput_value(mi, "symmetricInY(bool):"))
     bind_circ = int(modelinput_value(mi, "bindCircumferenceXYToBasal(bool):"))
+    extra.update(growth_options(mi, inp))
     end = os.path.join(exp, "MeshFromEndPoint.mesh")
     end_nodes, end_info, end_conn, _ = io.read_mesh(end)
     assert np.array_equal(end_conn, conn)

@@ -195,3 +195,48 @@ def step_inputs(d, k, orc):
         row = d["du_log"][k + 1]
         out["du_log"] = row[~np.isnan(row)]
     return out
+
+
+def growth_setup(d):
+    """Per-prism data, functions and options of a regression run, from its
+    mesh and model input (tests/golden)."""
+    conn, info, nodes = d["conn"], d["mesh_node_info"], d["mesh_nodes"]
+    ne = len(conn)
+    place, _ = tissue.element_placement(conn, info[:, 0])
+    etype = tissue.element_type(conn, info[:, 1])
+    ecm = np.zeros(ne, bool); ecm[d["ecm"]] = True
+    act = np.zeros(ne, bool); act[d["actin"]] = True
+    flags = (ecm * 1 + act * 4).astype(np.uint8)
+    _, L = tissue.tissue_height(nodes, conn, info[:, 1], info[:, 0])
+    stacks = tissue.element_stacks(conn, etype, place, L)
+    P = dict(type=etype.astype(np.int32), place=place.astype(np.int32), flags=flags,
+             rel_z=tissue.initial_z_positions(nodes, conn, info[:, 1], info[:, 0]),
+             weights=None, explicit_ecm=bool(d["explicit_ecm"]))
+    fns = []
+    for k, t in enumerate(d["growth_type"]):
+        ap = d["growth_apply"][k]
+        f = dict(type=int(t), t=tuple(d["growth_time"][k]), z=tuple(d["growth_z"][k]),
+                 apply=int(ap[0] + 2 * ap[1] + 4 * ap[2] + 8 * ap[3]))
+        if t == 1:
+            f["rate"], f["angle"] = tissue.uniform_growth_input(d["growth_rate_per_hour"][k],
+                                                                d["growth_angle_deg"][k])
+        else:
+            f["rates"] = tissue.grid_growth_input(d[f"growth_grid{k}_rates"])
+            f["angles_deg"] = d[f"growth_grid{k}_angles"]
+        fns.append(f)
+    cols = None
+    if int(d["column_volume_conservation"]):
+        cols = tissue.conserved_columns(stacks, etype, place, ecm, act, L)
+    extra = dict(apical_of=tissue.apical_positioning(stacks, etype, place, ne, L),
+                 excluded=tissue.ecm_only_nodes(conn, len(nodes), ecm), columns=cols,
+                 ecm=ecm, actin=act, interp=int(d["growth_interp"]),
+                 pinned=bool(d["growth_pinned"]), symmetric_y=bool(d["symmetric_y"]))
+    return P, fns, extra
+
+
+def aligned_start(d, k):
+    """Positions at the start of the step that produced frame k, after the
+    rigid DV alignment of Simulation::runOneStep."""
+    dorsal, ventral = assign_tips(d["mesh_nodes"], d["mesh_node_info"])
+    start = d["mesh_nodes"] if k == 0 else d["frames"][k - 1]
+    return align_dv_to_x(start, dorsal, ventral)

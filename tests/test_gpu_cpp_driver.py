@@ -108,3 +108,73 @@ def test_cpp_driver_stiffness_multiplier_and_ablation(orc, tmp_path, mode):
     with orc.ablation(prisms=flags, nodes=ablated_nodes):
         u1, _, _ = P1.nr_step_dense(p["un"], p["dt"], 1e-8, 20)
     assert np.abs(u1 - uo).max() > 1e-6 * np.abs(uo).max()
+
+
+def test_cpp_driver_growth_steps_equal_the_c_abi(tmp_path):
+    """Growth through the C++ surface (setGrowthFunctions, then per step
+    calculateRelativePositions, calculateGrowth and updateStepNR): a grid and
+    a uniform function with column volume conservation, three steps; positions
+    and F_g bit for bit as the same sequence through the C ABI."""
+    from tissueorigami_amd import nr, tissue, workloads
+    nr.load()
+    c = workloads.c1_config()
+    m = c["mesh"]
+    conn = m["conn"]
+    nn, ne = len(m["nodes"]), len(conn)
+    place = c["place"].astype(np.int32)
+    etype = c["etype"].astype(np.int32)
+    _, L = tissue.tissue_height(m["nodes"], conn, m["node_type"], m["node_placement"])
+    stacks = tissue.element_stacks(conn, etype, place, L)
+    rz = tissue.initial_z_positions(m["nodes"], conn, m["node_type"], m["node_placement"])
+    rng = np.random.default_rng(21)
+    grid_r = rng.uniform(0.0, 3e-4, (4, 3, 3)) * np.array([1.0, 1.0, 0.1])
+    grid_a = rng.uniform(0.0, 180.0, (4, 3))
+    uni = (2e-5, 1e-5, 0.0)
+    area = np.asarray(c["ext_visc"][:, 2] > 0, float)
+    dt, steps = 60.0, 3
+    u0 = m["nodes"].reshape(-1)
+    s = dict(un=u0, fixed=np.zeros(3 * nn, np.uint8), ext_visc=c["ext_visc"], visc_surf=area,
+             conn=conn, refpos=m["refpos"], E=c["E"], nu=c["nu"], eta=c["eta"],
+             invFg=np.tile(np.eye(3).reshape(9), (ne, 1)), pairs=np.zeros((0, 2), int), dt=dt)
+    inp, out = tmp_path / "in.txt", tmp_path / "out.txt"
+    write_input(inp, s, nn)
+    with open(inp, "a") as f:
+        f.write("growth_prisms\n")
+        for e in range(ne):
+            col = stacks.get(e, [])
+            f.write(f"{etype[e]} {place[e]} 0 0 0 {float(rz[e])!r} {len(col)} " +
+                    " ".join(str(x) for x in col) + "\n")
+        f.write("growth_grid 0 1e9 1 0 1 4 3\n")
+        for i in range(4):
+            for j in range(3):
+                f.write(" ".join(repr(float(v)) for v in grid_r[i, j]) + f" {float(grid_a[i, j])!r}\n")
+        f.write("growth_uniform 60 1e9 1 0.2 1 " + " ".join(repr(v) for v in uni) + " 0.4\n")
+        f.write(f"growth_options 1 0 1 0 0 {L} 0\n")
+        f.write(f"steps {steps} 0\n")
+    subprocess.run([DRIVER, str(inp), str(out), "step"], check=True, timeout=120)
+    lines = open(out).read().splitlines()
+    u_cpp = np.array([[float(x) for x in l.split()] for l in lines[:nn]]).reshape(-1)
+    Fg_cpp = np.array([[float(x) for x in l.split()[1:]] for l in lines if l.startswith("Fg")])
+
+    h = nr.NewtonRaphsonSolver(nn, conn, m["refpos"])
+    h.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"], np.zeros(3 * nn, np.uint8))
+    h.set_visc_surface(area)
+    ap = tissue.apical_positioning(stacks, etype, place, ne, L)
+    h.set_growth_prisms(etype, place, np.zeros(ne, np.uint8), rz, np.zeros((ne, 2)), ap,
+                        np.zeros(nn, np.uint8))
+    h.add_grid_growth(0.0, 1e9, grid_r, grid_a, 1, (0.0, 1.0))
+    h.add_uniform_growth(60.0, 1e9, uni, 0.4, 1, (0.2, 1.0))
+    h.set_growth_options(1, False, ())
+    h.set_column_volume_conservation(tissue.conserved_columns(
+        stacks, etype, place, np.zeros(ne, bool), np.zeros(ne, bool), L))
+    h.growth_tensor(np.tile(np.eye(3).reshape(9), (ne, 1)))
+    u = u0.copy()
+    for k in range(steps):
+        h.update_relative_positions(u)
+        h.grow(k * dt, dt, u)
+        u, st = h.step(u, dt, 1e-8, 20, "fp64", 1e-13)
+        assert st["converged"]
+    assert np.array_equal(Fg_cpp, h.growth_tensor())
+    assert np.array_equal(u_cpp, u)
+    assert np.abs(Fg_cpp - np.eye(3).reshape(9)).max() > 1e-3
+    h.close()

@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libtissue_nr.so")
 DRIVER = os.path.join(HERE, "nr_driver")
 SOURCES = ["tf_element.hip", "tf_assemble.hip", "tf_tiled.hip", "tf_coarse.hip",
-           "tf_adjoint.hip", "tf_packing.hip", "tf_nr.hip"]
+           "tf_adjoint.hip", "tf_packing.hip", "tf_growth.hip", "tf_nr.hip"]
 # rocSOLVER (getrf / getri of the coarse matrix) and rocBLAS under it
 LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -15,9 +15,23 @@
 //     "stiffness" and n_prisms values of ShapeBase::stiffnessMultiplier;
 //     "ablated_prisms" k and k prism ids (ShapeBase::IsAblated);
 //     "ablated_nodes" k and k node ids (Simulation::AblatedNodes, applied by
-//     checkJacobianForAblatedNodes after the assembly, Simulation.cpp:6691)
+//     checkJacobianForAblatedNodes after the assembly, Simulation.cpp:6691);
+//     growth (GrowthFunction / GrowthOptions of tissue_nr.hpp):
+//     "growth_prisms" then per prism: tissueType tissuePlacement flags
+//       (1 ECM, 2 ECM at circumference, 4 actin) columnarGrowthWeight
+//       peripodialGrowthWeight initialRelativePositionInZ, the column length c
+//       and c prism ids (elementsIdsOnSameColumn);
+//     "ecm_nodes" k and k node ids (Node::allOwnersECMMimicing);
+//     "growth_uniform" t0 t1 apply zmin zmax rx ry rz angle;
+//     "growth_grid" t0 t1 apply zmin zmax nx ny, then per grid point (i, j),
+//       i outer: rx ry rz angle_deg;
+//     "growth_options" interpolation pinned conserve symmetricY explicitECM
+//       layers n_pins pin_times...;
+//     "steps" K t0: K steps of Simulation::runOneStep's growth path
+//       (calculateRelativePositions, calculateGrowth at t, updateStepNR).
 // Output: one line per node with the converged x y z (%.17g), then a line
-//   "iterations <k>" and one "du <norm>" line per NR iteration.
+//   "iterations <k>" and one "du <norm>" line per NR iteration (of the last
+//   step), and with growth one "Fg" line of 9 values per prism.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -86,10 +100,71 @@ int main(int argc, char** argv) {
     }
   }
   std::vector<int> ablatedNodes;
+  std::vector<tissue_nr::GrowthFunction> growth;
+  tissue_nr::GrowthOptions gopt;
+  int n_steps = 0;
+  double t0 = 0.0;
   in.clear();
   std::string key;
   while (in >> key) {
-    if (key == "stiffness") {
+    if (key == "growth_prisms") {
+      for (auto& el : Elements) {
+        int fl = 0, c = 0;
+        in >> el->tissueType >> el->tissuePlacement >> fl >> el->columnarGrowthWeight >>
+            el->peripodialGrowthWeight >> el->initialRelativePositionInZ >> c;
+        el->isECMMimicing = fl & 1;
+        el->isECMMimimcingAtCircumference = fl & 2;
+        el->isActinMimicing = fl & 4;
+        el->elementsIdsOnSameColumn.resize(c);
+        for (int& id : el->elementsIdsOnSameColumn) in >> id;
+      }
+    } else if (key == "ecm_nodes") {
+      int k = 0;
+      in >> k;
+      for (int i = 0; i < k; ++i) {
+        int id = -1;
+        in >> id;
+        if (id >= 0 && id < nn) Nodes[id]->allOwnersECMMimicing = true;
+      }
+    } else if (key == "growth_uniform" || key == "growth_grid") {
+      tissue_nr::GrowthFunction f;
+      int apply = 0;
+      in >> f.initTime >> f.endTime >> apply >> f.zMin >> f.zMax;
+      f.applyToColumnarLayer = apply & 1;
+      f.applyToPeripodialMembrane = apply & 2;
+      f.applyToBasalECM = apply & 4;
+      f.applyToLateralECM = apply & 8;
+      if (key == "growth_uniform") {
+        f.Type = 1;
+        in >> f.GrowthRate[0] >> f.GrowthRate[1] >> f.GrowthRate[2] >> f.angle;
+      } else {
+        f.Type = 3;
+        in >> f.nGridX >> f.nGridY;
+        if (f.nGridX < 2 || f.nGridY < 2 || f.nGridX > 4096 || f.nGridY > 4096) {
+          std::cerr << "bad grid size\n";
+          return 2;
+        }
+        f.GrowthMatrix.assign(f.nGridX, std::vector<std::array<double, 3>>(f.nGridY));
+        f.AngleMatrix.assign(f.nGridX, std::vector<double>(f.nGridY));
+        for (int i = 0; i < f.nGridX; ++i)
+          for (int j = 0; j < f.nGridY; ++j)
+            in >> f.GrowthMatrix[i][j][0] >> f.GrowthMatrix[i][j][1] >> f.GrowthMatrix[i][j][2] >>
+                f.AngleMatrix[i][j];
+      }
+      growth.push_back(f);
+    } else if (key == "growth_options") {
+      int pinned = 0, conserve = 0, sym = 0, ecm = 0, np = 0;
+      in >> gopt.gridGrowthsInterpolationType >> pinned >> conserve >> sym >> ecm >>
+          gopt.TissueHeightDiscretisationLayers >> np;
+      gopt.GridGrowthsPinnedOnInitialMesh = pinned;
+      gopt.conservingColumnVolumes = conserve;
+      gopt.symmetricY = sym;
+      gopt.thereIsExplicitECM = ecm;
+      gopt.growthPinUpdateTime.resize(np > 0 ? np : 0);
+      for (int& t : gopt.growthPinUpdateTime) in >> t;
+    } else if (key == "steps") {
+      in >> n_steps >> t0;
+    } else if (key == "stiffness") {
       for (auto& el : Elements) in >> el->stiffnessMultiplier;
     } else if (key == "ablated_prisms" || key == "ablated_nodes") {
       int k = 0;
@@ -116,7 +191,23 @@ int main(int argc, char** argv) {
     NRSolver.krylovTolerance = 1e-13;
     NRSolver.slaveMasterList = slaveMasterList;
     NRSolver.boundNodesWithSlaveMasterDefinition = !slaveMasterList.empty();
-    if (mode == "step") {
+    if (n_steps > 0) {
+      // Simulation::runOneStep's order: relative positions, growth at the
+      // step's start time, then the NR step
+      NRSolver.AblatedNodes = ablatedNodes;
+      if (!growth.empty()) NRSolver.setGrowthFunctions(growth, gopt, Nodes, Elements);
+      double t = t0;
+      for (int k = 0; k < n_steps; ++k) {
+        if (!growth.empty()) {
+          NRSolver.calculateRelativePositions(Nodes, Elements);
+          NRSolver.calculateGrowth(Nodes, Elements, t, dt);
+        }
+        tf_step_stats st = NRSolver.updateStepNR(Nodes, Elements, dt, maxit);
+        norms.clear();
+        for (int i = 0; i < st.nr_iterations; ++i) norms.push_back(st.du_norms[i]);
+        t += dt;
+      }
+    } else if (mode == "step") {
       NRSolver.AblatedNodes = ablatedNodes;
       tf_step_stats st = NRSolver.updateStepNR(Nodes, Elements, dt, maxit);
       for (int k = 0; k < st.nr_iterations; ++k) norms.push_back(st.du_norms[k]);
@@ -156,6 +247,13 @@ int main(int argc, char** argv) {
     std::fprintf(out, "%.17g %.17g %.17g\n", n->Position[0], n->Position[1], n->Position[2]);
   std::fprintf(out, "iterations %zu\n", norms.size());
   for (double d : norms) std::fprintf(out, "du %.17g\n", d);
+  if (!growth.empty())
+    for (auto& el : Elements) {
+      std::fprintf(out, "Fg");
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) std::fprintf(out, " %.17g", el->Fg[i][j]);
+      std::fprintf(out, "\n");
+    }
   std::fclose(out);
   return 0;
 }

@@ -403,6 +403,60 @@ hipError_t launch_grow(int32_t n_prisms, const double* rates, double dt, double*
                        const uint8_t* ablated = nullptr);
 hipError_t launch_inverse(int32_t n_prisms, const double* Fg, double* invFg, hipStream_t st);
 
+// ---- growth producers (tf_growth.hip) ----
+// Per-prism data of the growth functions (ShapeBase members): tissue type,
+// placement, flags, growth weights of linker prisms (columnar, peripodial),
+// relative z position in the tissue height (0 apical .. 1 basal).
+constexpr uint8_t kGrowEcm = 1, kGrowEcmCirc = 2, kGrowActin = 4;
+constexpr int kApplyColumnar = 1, kApplyPeripodial = 2, kApplyBasalEcm = 4, kApplyLateralEcm = 8;
+struct GrowthPrisms {
+  const int32_t* type = nullptr;
+  const int32_t* place = nullptr;
+  const uint8_t* flags = nullptr;
+  const double* weights = nullptr;  // 2 per prism
+  const double* rel_z = nullptr;
+  int explicit_ecm = 0;
+};
+// One growth function: type 1 (uniform: rates and the rotation matrix of its
+// angle) or 3 (grid of nx x ny points: its compatible corner data start at
+// cell cell_off of the handle's grid arrays); active for t0 <= t < t1, for the
+// tissue parts of `apply` and the relative z range [zmin, zmax].
+struct GrowthFn {
+  int type = 0, apply = 0, nx = 0, ny = 0;
+  int64_t cell_off = 0;
+  double t0 = 0, t1 = 0, zmin = 0, zmax = 1;
+  double rate[3] = {0, 0, 0};
+  double rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+};
+void grid_compatible(int nx, int ny, const double* rates, const double* angles_deg, double* cg,
+                     double* ca, uint8_t* ce);
+// Relative (x, y) positions of the prisms in their tissue's bounding box at
+// positions u (internal node numbering; tissue T owns nodes
+// [node_rng[2T], node_rng[2T+1]) and the prisms with prism_tissue == T).
+hipError_t launch_relative_positions(int32_t n_tissues, const int32_t* node_rng, int32_t n_prisms,
+                                     const int32_t* conn, const int32_t* prism_tissue,
+                                     const double* u, const uint8_t* excluded, int symmetric_y,
+                                     const int32_t* apical_of, double* bb, double* raw,
+                                     double* rel, hipStream_t st);
+// Buffers of the column volume conservation (on = 1): per prism the rotated
+// growth and shape-change increments, the growth rotation and whether it
+// applies, the grown and the reference volume.
+struct GrowthStage {
+  int on = 0;
+  double *G = nullptr, *SC = nullptr, *gsr = nullptr, *vol = nullptr;
+  const double* vref = nullptr;
+  uint8_t* rot = nullptr;
+};
+// The growth of one step at time t (Simulation::calculateGrowth and the
+// growShapeByFg loop of Simulation::runOneStep); columns[n_columns][layers]
+// (prism ids, -1 for none) when gs.on.
+hipError_t launch_growth(int32_t n_prisms, const int32_t* conn, const double* refpos,
+                         const double* u, const GrowthPrisms& gp, const GrowthFn* fns, int n_fn,
+                         const double* cg, const double* ca, const uint8_t* ce, const double* relpos,
+                         double t, double dt, int interp, const uint8_t* ablated, double* Fg,
+                         double* invFg, const GrowthStage& gs, int32_t n_columns, int32_t layers,
+                         const int32_t* columns, hipStream_t st);
+
 // ---- Krylov (tf_tiled.hip) ----
 // Optional event timing of the SpMV launches ([0] FP64 operator, [1] FP32).
 constexpr int kProfPairs = 128;

@@ -37,6 +37,37 @@ struct StiffSeries {
   double* relpos = nullptr;
 };
 
+// Growth functions evaluated on the device each step (Simulation::calculateGrowth
+// with uniform and grid-based functions): per-prism data, the relative
+// positions in the bounding box (current and pinned), the functions and the
+// compatible corner data of their grids, the pinning times and the column
+// volume conservation.
+struct GrowthState {
+  bool prisms = false;
+  GrowthPrisms gp;
+  int32_t *type = nullptr, *place = nullptr, *apical_of = nullptr, *prism_tissue = nullptr,
+          *node_rng = nullptr;
+  uint8_t *flags = nullptr, *node_excl = nullptr;
+  double *weights = nullptr, *rel_z = nullptr;
+  int symmetric_y = 0;
+  double *bb = nullptr, *raw = nullptr, *relcur = nullptr, *relinit = nullptr, *upos = nullptr;
+  bool have_cur = false, have_init = false;
+  std::vector<GrowthFn> fns;
+  GrowthFn* d_fns = nullptr;
+  std::vector<double> h_cg, h_ca;
+  std::vector<uint8_t> h_ce;
+  double *cg = nullptr, *ca = nullptr;
+  uint8_t* ce = nullptr;
+  int interp = 1;
+  bool pinned = false;
+  std::vector<double> pin_times;
+  std::vector<char> pin_done;
+  int32_t n_columns = 0, layers = 0;
+  int32_t* columns = nullptr;
+  GrowthStage stage;
+  double* vref = nullptr;
+};
+
 struct tf_handle {
   int32_t Nn = 0, Ne = 0;  // internal node count (tissues padded to 64-node slices) and prisms
   // tissues of the handle: one for tf_nr_create, several for tf_batch_create
@@ -123,6 +154,7 @@ struct tf_handle {
   int32_t *surf_place = nullptr, *surf_type = nullptr;
   uint8_t* surf_spans = nullptr;
   bool have_rates = false, have_flags = false, have_positions = false;
+  GrowthState gr;
   // adjoint state: position checkpoints of the last tf_adj_forward
   double* adj_ckpt = nullptr;
   int adj_cap = 0, adj_steps = 0;
@@ -1462,8 +1494,17 @@ tf_status apply_series(tf_handle* h, double t, double dt) {
     for (int i = 0; i < q.n_t - 1; ++i)
       if (t >= q.times[i] && t < q.times[i + 1]) { k = i; break; }
     if (k < 0) continue;
+    // the prisms' current positions in the bounding box when the device keeps
+    // them (tf_nr_update_relative_positions, every step of tf_nr_advance),
+    // else the positions given with the series
+    const double* rp = h->gr.have_cur ? h->gr.relcur : q.relpos;
+    if (!rp) {
+      set_error("stiffness series: no relative positions (give them with the series, or set the "
+                "growth prisms and update the relative positions)");
+      return TF_ERR_ARG;
+    }
     hipLaunchKernelGGL(stiffness_series_kernel, dim3((h->Ne + 255) / 256), dim3(256), 0, h->st,
-                       h->Ne, q.mask, q.relpos, q.rate + (size_t)k * q.ny * q.nx, q.nx, q.ny,
+                       h->Ne, q.mask, rp, q.rate + (size_t)k * q.ny * q.nx, q.nx, q.ny,
                        q.method, q.linear, dt, h->E_d, h->nu_d, h->stm, h->lam, h->mu);
     TF_CHECK(hipGetLastError());
     h->stm_touched = true;
@@ -1761,7 +1802,7 @@ tf_status tf_nr_add_stiffness_series(tf_handle* h, int32_t n_times, const double
                                      int32_t method, const uint8_t* applicable,
                                      const double* rel_pos) {
   if (!h || n_times < 2 || !times || nx < 2 || ny < 2 || !grids || method < 0 || method > 2 ||
-      !applicable || !rel_pos) {
+      !applicable || (!rel_pos && !h->gr.prisms)) {
     set_error("tf_nr_add_stiffness_series: invalid arguments");
     return TF_ERR_ARG;
   }
@@ -1787,7 +1828,7 @@ tf_status tf_nr_add_stiffness_series(tf_handle* h, int32_t n_times, const double
   std::vector<uint8_t> m(h->Ne);
   for (int e = 0; e < h->Ne; ++e) {
     m[e] = applicable[e] ? 1 : 0;
-    if (!std::isfinite(rel_pos[2 * e]) || !std::isfinite(rel_pos[2 * e + 1])) {
+    if (rel_pos && (!std::isfinite(rel_pos[2 * e]) || !std::isfinite(rel_pos[2 * e + 1]))) {
       set_error("tf_nr_add_stiffness_series: relative positions must be finite");
       return TF_ERR_ARG;
     }
@@ -1799,13 +1840,14 @@ tf_status tf_nr_add_stiffness_series(tf_handle* h, int32_t n_times, const double
     }
   TF_CHECK(hipStreamSynchronize(h->st));
   if (dalloc(h, &q.rate, rate.size()) != hipSuccess || dalloc(h, &q.mask, (size_t)h->Ne) != hipSuccess ||
-      dalloc(h, &q.relpos, 2 * (size_t)h->Ne) != hipSuccess) {
+      (rel_pos && dalloc(h, &q.relpos, 2 * (size_t)h->Ne) != hipSuccess)) {
     set_error("hipMalloc failed");
     return TF_ERR_HIP;
   }
   TF_CHECK(hipMemcpy(q.rate, rate.data(), sizeof(double) * rate.size(), hipMemcpyHostToDevice));
   TF_CHECK(hipMemcpy(q.mask, m.data(), m.size(), hipMemcpyHostToDevice));
-  TF_CHECK(hipMemcpy(q.relpos, rel_pos, sizeof(double) * 2 * h->Ne, hipMemcpyHostToDevice));
+  if (rel_pos)
+    TF_CHECK(hipMemcpy(q.relpos, rel_pos, sizeof(double) * 2 * h->Ne, hipMemcpyHostToDevice));
   h->series.push_back(std::move(q));
   return TF_OK;
 }
@@ -1849,6 +1891,361 @@ tf_status tf_nr_get_stiffness_multiplier(tf_handle* h, double* out) {
   return TF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// F_g and InvFg as identity when no growth tensor was given yet.
+tf_status ensure_Fg(tf_handle* h) {
+  if (h->Fg) return TF_OK;
+  const size_t Ne = h->Ne;
+  TF_CHECK(dalloc(h, &h->Fg, Ne * 9));
+  std::vector<double> I(Ne * 9, 0.0);
+  for (size_t e = 0; e < Ne; ++e) I[9 * e] = I[9 * e + 4] = I[9 * e + 8] = 1.0;
+  TF_CHECK(hipMemcpy(h->Fg, I.data(), sizeof(double) * Ne * 9, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(h->invFg, I.data(), sizeof(double) * Ne * 9, hipMemcpyHostToDevice));
+  h->have_growth = true;
+  return TF_OK;
+}
+
+// Device positions of the producers: u (user numbering) staged on the device,
+// or the handle's resident positions.
+tf_status producer_positions(tf_handle* h, const double* u, const double** out) {
+  if (u) {
+    tf_status up = upload_nodes(h, h->gr.upos, u);
+    if (up != TF_OK) return up;
+    *out = h->gr.upos;
+    return TF_OK;
+  }
+  if (!h->have_positions) {
+    set_error("no positions on the device: give u");
+    return TF_ERR_ARG;
+  }
+  *out = h->ws.uk;
+  return TF_OK;
+}
+
+// The relative positions of the prisms at positions u (device): the bounding
+// box of every tissue, the prism centres, the apical column copy; the first
+// time also the pinned positions (Simulation::initiateSystem sets
+// initialRelativePosInBoundingBox from the positions of the initial mesh).
+tf_status relpos_update(tf_handle* h, const double* u) {
+  GrowthState& g = h->gr;
+  TF_CHECK(launch_relative_positions(h->n_tissues, g.node_rng, h->Ne, h->conn, g.prism_tissue, u,
+                                     g.node_excl, g.symmetric_y, g.apical_of, g.bb, g.raw,
+                                     g.relcur, h->st));
+  g.have_cur = true;
+  if (!g.have_init) {
+    TF_CHECK(hipMemcpyAsync(g.relinit, g.relcur, sizeof(double) * 2 * h->Ne,
+                            hipMemcpyDeviceToDevice, h->st));
+    g.have_init = true;
+  }
+  return TF_OK;
+}
+
+// The growth of one step at time t from positions u (device): the pinning
+// updates due at t (Simulation::checkForPinningPositionsUpdate,
+// Simulation.cpp:6169-6187), then every active growth function, the rotation,
+// the column volume conservation and F_g.
+tf_status growth_run(tf_handle* h, const double* u, double t, double dt) {
+  GrowthState& g = h->gr;
+  if (!g.have_cur) {
+    set_error("growth: update the relative positions first");
+    return TF_ERR_ARG;
+  }
+  if (g.pinned)
+    for (size_t i = 0; i < g.pin_times.size(); ++i)
+      if (t >= g.pin_times[i] && !g.pin_done[i]) {
+        TF_CHECK(hipMemcpyAsync(g.relinit, g.relcur, sizeof(double) * 2 * h->Ne,
+                                hipMemcpyDeviceToDevice, h->st));
+        g.pin_done[i] = 1;
+      }
+  tf_status fs = ensure_Fg(h);
+  if (fs != TF_OK) return fs;
+  const uint8_t* abl = h->n_abl ? h->abl_el : nullptr;
+  TF_CHECK(launch_growth(h->Ne, h->conn, h->refpos, u, g.gp, g.d_fns, (int)g.fns.size(), g.cg,
+                         g.ca, g.ce, g.pinned ? g.relinit : g.relcur, t, dt, g.interp, abl, h->Fg,
+                         h->invFg, g.stage, g.n_columns, g.layers, g.columns, h->st));
+  h->have_growth = true;
+  return TF_OK;
+}
+
+tf_status add_growth_fn(tf_handle* h, GrowthFn f) {
+  GrowthState& g = h->gr;
+  g.fns.push_back(f);
+  TF_CHECK(hipStreamSynchronize(h->st));
+  dfree(h, &g.d_fns);
+  TF_CHECK(dalloc(h, &g.d_fns, g.fns.size()));
+  TF_CHECK(hipMemcpy(g.d_fns, g.fns.data(), sizeof(GrowthFn) * g.fns.size(),
+                     hipMemcpyHostToDevice));
+  return ensure_Fg(h);
+}
+
+tf_status check_fn_args(tf_handle* h, double t0, double t1, int32_t apply, double zmin,
+                        double zmax, const char* who) {
+  if (!h || !std::isfinite(t0) || !std::isfinite(t1) || apply < 0 || apply > 15 ||
+      !std::isfinite(zmin) || !std::isfinite(zmax)) {
+    set_error(std::string(who) + ": invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (!h->gr.prisms) {
+    set_error(std::string(who) + ": set the growth prisms first");
+    return TF_ERR_ARG;
+  }
+  return TF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tf_status tf_nr_set_growth_prisms(tf_handle* h, const int32_t* tissue_type,
+                                  const int32_t* placement, const uint8_t* flags,
+                                  const double* weights, const double* rel_z,
+                                  const int32_t* apical_of, const uint8_t* node_excluded,
+                                  int32_t symmetric_y, int32_t explicit_ecm) {
+  if (!h || !tissue_type || !placement || !flags || !rel_z) {
+    set_error("tf_nr_set_growth_prisms: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  const int Ne = h->Ne;
+  for (int e = 0; e < Ne; ++e) {
+    if (tissue_type[e] < 0 || tissue_type[e] > 2) {
+      set_error("tf_nr_set_growth_prisms: tissue type of prism " + std::to_string(e) +
+                " is not 0, 1 or 2");
+      return TF_ERR_ARG;
+    }
+    if (apical_of && (apical_of[e] < -1 || apical_of[e] >= Ne)) {
+      set_error("tf_nr_set_growth_prisms: apical prism of prism " + std::to_string(e) +
+                " out of range");
+      return TF_ERR_ARG;
+    }
+    if (!std::isfinite(rel_z[e]) || (weights && (!std::isfinite(weights[2 * e]) ||
+                                                 !std::isfinite(weights[2 * e + 1])))) {
+      set_error("tf_nr_set_growth_prisms: relative z and weights must be finite");
+      return TF_ERR_ARG;
+    }
+  }
+  GrowthState& g = h->gr;
+  TF_CHECK(hipStreamSynchronize(h->st));
+  if (!g.prisms) {
+    TF_CHECK(dalloc(h, &g.type, Ne));
+    TF_CHECK(dalloc(h, &g.place, Ne));
+    TF_CHECK(dalloc(h, &g.apical_of, Ne));
+    TF_CHECK(dalloc(h, &g.prism_tissue, Ne));
+    TF_CHECK(dalloc(h, &g.node_rng, 2 * (size_t)h->n_tissues));
+    TF_CHECK(dalloc(h, &g.flags, Ne));
+    TF_CHECK(dalloc(h, &g.node_excl, (size_t)h->Nn));
+    TF_CHECK(dalloc(h, &g.weights, 2 * (size_t)Ne));
+    TF_CHECK(dalloc(h, &g.rel_z, Ne));
+    TF_CHECK(dalloc(h, &g.bb, 6 * (size_t)h->n_tissues));
+    TF_CHECK(dalloc(h, &g.raw, 2 * (size_t)Ne));
+    TF_CHECK(dalloc(h, &g.relcur, 2 * (size_t)Ne));
+    TF_CHECK(dalloc(h, &g.relinit, 2 * (size_t)Ne));
+    TF_CHECK(dalloc(h, &g.upos, 3 * (size_t)h->Nn));
+  }
+  std::vector<int32_t> ap((size_t)Ne, -1), pt((size_t)Ne, 0), rng(2 * (size_t)h->n_tissues);
+  if (apical_of) std::copy(apical_of, apical_of + Ne, ap.begin());
+  for (int T = 0; T < h->n_tissues; ++T) {
+    for (int e = h->elem_off[T]; e < h->elem_off[T + 1]; ++e) pt[e] = T;
+    rng[2 * T] = h->int_off[T];
+    rng[2 * T + 1] = h->int_off[T] + (h->user_off[T + 1] - h->user_off[T]);
+  }
+  std::vector<double> wt(2 * (size_t)Ne, 0.0);
+  if (weights) std::copy(weights, weights + 2 * (size_t)Ne, wt.begin());
+  std::vector<uint8_t> fl(flags, flags + Ne);
+  std::vector<uint8_t> nx((size_t)h->n_user, 0);
+  if (node_excluded)
+    for (int i = 0; i < h->n_user; ++i) nx[i] = node_excluded[i] ? 1 : 0;
+  std::vector<uint8_t> nxi = to_internal<uint8_t>(h, nx.data(), 1, (uint8_t)1);
+  TF_CHECK(hipMemcpy(g.type, tissue_type, sizeof(int32_t) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.place, placement, sizeof(int32_t) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.apical_of, ap.data(), sizeof(int32_t) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.prism_tissue, pt.data(), sizeof(int32_t) * Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.node_rng, rng.data(), sizeof(int32_t) * rng.size(), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.flags, fl.data(), Ne, hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.node_excl, nxi.data(), nxi.size(), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.weights, wt.data(), sizeof(double) * wt.size(), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.rel_z, rel_z, sizeof(double) * Ne, hipMemcpyHostToDevice));
+  g.gp.type = g.type;
+  g.gp.place = g.place;
+  g.gp.flags = g.flags;
+  g.gp.weights = g.weights;
+  g.gp.rel_z = g.rel_z;
+  g.gp.explicit_ecm = explicit_ecm ? 1 : 0;
+  g.symmetric_y = symmetric_y ? 1 : 0;
+  g.prisms = true;
+  g.have_cur = g.have_init = false;
+  return TF_OK;
+}
+
+tf_status tf_nr_add_uniform_growth(tf_handle* h, double t0, double t1, int32_t apply, double zmin,
+                                   double zmax, const double* rate, double angle) {
+  tf_status a = check_fn_args(h, t0, t1, apply, zmin, zmax, "tf_nr_add_uniform_growth");
+  if (a != TF_OK) return a;
+  if (!rate || !std::isfinite(rate[0]) || !std::isfinite(rate[1]) || !std::isfinite(rate[2]) ||
+      !std::isfinite(angle)) {
+    set_error("tf_nr_add_uniform_growth: invalid rates or angle");
+    return TF_ERR_ARG;
+  }
+  GrowthFn f;
+  f.type = 1;
+  f.apply = apply;
+  // GrowthFunctionBase keeps the times as float (GrowthFunctionBase.h:39-40)
+  f.t0 = (double)(float)t0;
+  f.t1 = (double)(float)t1;
+  f.zmin = zmin;
+  f.zmax = zmax;
+  for (int k = 0; k < 3; ++k) f.rate[k] = rate[k];
+  // UniformGrowthFunction's ShearAngleRotationMatrix (GrowthFunctionTypes.h:22-44)
+  const double c = std::cos(angle), s = std::sin(angle);
+  const double R[9] = {c, -1.0 * s, 0.0, s, c, 0.0, 0.0, 0.0, 1.0};
+  for (int q = 0; q < 9; ++q) f.rot[q] = R[q];
+  return add_growth_fn(h, f);
+}
+
+tf_status tf_nr_add_grid_growth(tf_handle* h, double t0, double t1, int32_t apply, double zmin,
+                                double zmax, int32_t nx, int32_t ny, const double* rates,
+                                const double* angles_deg) {
+  tf_status a = check_fn_args(h, t0, t1, apply, zmin, zmax, "tf_nr_add_grid_growth");
+  if (a != TF_OK) return a;
+  if (nx < 2 || ny < 2 || (int64_t)nx * ny > (int64_t)1 << 24 || !rates || !angles_deg) {
+    set_error("tf_nr_add_grid_growth: the grid needs at least 2 x 2 points, rates and angles");
+    return TF_ERR_ARG;
+  }
+  const size_t np = (size_t)nx * ny;
+  for (size_t k = 0; k < np; ++k)
+    if (!std::isfinite(angles_deg[k]) || !std::isfinite(rates[3 * k]) ||
+        !std::isfinite(rates[3 * k + 1]) || !std::isfinite(rates[3 * k + 2])) {
+      set_error("tf_nr_add_grid_growth: rates and angles must be finite");
+      return TF_ERR_ARG;
+    }
+  GrowthState& g = h->gr;
+  const size_t cells = (size_t)(nx - 1) * (ny - 1), off = g.h_ca.size() / 4;
+  g.h_cg.resize(12 * (off + cells));
+  g.h_ca.resize(4 * (off + cells));
+  g.h_ce.resize(4 * (off + cells));
+  grid_compatible(nx, ny, rates, angles_deg, g.h_cg.data() + 12 * off, g.h_ca.data() + 4 * off,
+                  g.h_ce.data() + 4 * off);
+  TF_CHECK(hipStreamSynchronize(h->st));
+  dfree(h, &g.cg);
+  dfree(h, &g.ca);
+  dfree(h, &g.ce);
+  TF_CHECK(dalloc(h, &g.cg, g.h_cg.size()));
+  TF_CHECK(dalloc(h, &g.ca, g.h_ca.size()));
+  TF_CHECK(dalloc(h, &g.ce, g.h_ce.size()));
+  TF_CHECK(hipMemcpy(g.cg, g.h_cg.data(), sizeof(double) * g.h_cg.size(), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.ca, g.h_ca.data(), sizeof(double) * g.h_ca.size(), hipMemcpyHostToDevice));
+  TF_CHECK(hipMemcpy(g.ce, g.h_ce.data(), g.h_ce.size(), hipMemcpyHostToDevice));
+  GrowthFn f;
+  f.type = 3;
+  f.apply = apply;
+  f.t0 = (double)(float)t0;
+  f.t1 = (double)(float)t1;
+  f.zmin = zmin;
+  f.zmax = zmax;
+  f.nx = nx;
+  f.ny = ny;
+  f.cell_off = (int64_t)off;
+  return add_growth_fn(h, f);
+}
+
+tf_status tf_nr_set_growth_options(tf_handle* h, int32_t interpolation, int32_t pinned,
+                                   int32_t n_pin_times, const double* pin_times) {
+  if (!h || interpolation < 0 || interpolation > 1 || n_pin_times < 0 ||
+      (n_pin_times > 0 && !pin_times)) {
+    set_error("tf_nr_set_growth_options: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  GrowthState& g = h->gr;
+  g.interp = interpolation;
+  g.pinned = pinned != 0;
+  g.pin_times.assign(pin_times, pin_times + n_pin_times);
+  g.pin_done.assign(n_pin_times, 0);
+  return TF_OK;
+}
+
+tf_status tf_nr_set_column_volume_conservation(tf_handle* h, int32_t n_columns, int32_t layers,
+                                               const int32_t* columns) {
+  if (!h || n_columns < 0 || (n_columns > 0 && (layers < 1 || !columns))) {
+    set_error("tf_nr_set_column_volume_conservation: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  GrowthState& g = h->gr;
+  std::vector<char> seen((size_t)h->Ne, 0);
+  for (int64_t k = 0; k < (int64_t)n_columns * layers; ++k) {
+    const int e = columns[k];
+    if (e < -1 || e >= h->Ne || (e >= 0 && seen[e])) {
+      set_error("tf_nr_set_column_volume_conservation: prism " + std::to_string(e) +
+                " out of range or in two columns");
+      return TF_ERR_ARG;
+    }
+    if (e >= 0) seen[e] = 1;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  dfree(h, &g.columns);
+  g.n_columns = n_columns;
+  g.layers = layers;
+  g.stage.on = n_columns > 0 ? 1 : 0;
+  if (!g.stage.on) return TF_OK;
+  TF_CHECK(dalloc(h, &g.columns, (size_t)n_columns * layers));
+  TF_CHECK(hipMemcpy(g.columns, columns, sizeof(int32_t) * n_columns * layers,
+                     hipMemcpyHostToDevice));
+  if (!g.vref) {
+    const size_t Ne = h->Ne;
+    TF_CHECK(dalloc(h, &g.vref, Ne));
+    TF_CHECK(dalloc(h, &g.stage.G, 9 * Ne));
+    TF_CHECK(dalloc(h, &g.stage.SC, 9 * Ne));
+    TF_CHECK(dalloc(h, &g.stage.gsr, 9 * Ne));
+    TF_CHECK(dalloc(h, &g.stage.vol, Ne));
+    TF_CHECK(dalloc(h, &g.stage.rot, Ne));
+    TF_CHECK(launch_ref_volume(h->Ne, h->refpos, g.vref, h->st));
+    TF_CHECK(hipStreamSynchronize(h->st));
+    g.stage.vref = g.vref;
+  }
+  return TF_OK;
+}
+
+tf_status tf_nr_update_relative_positions(tf_handle* h, const double* u) {
+  if (!h || !h->gr.prisms) {
+    set_error("tf_nr_update_relative_positions: set the growth prisms first");
+    return TF_ERR_ARG;
+  }
+  const double* ud = nullptr;
+  tf_status p = producer_positions(h, u, &ud);
+  if (p != TF_OK) return p;
+  tf_status r = relpos_update(h, ud);
+  if (r != TF_OK) return r;
+  TF_CHECK(hipStreamSynchronize(h->st));
+  return TF_OK;
+}
+
+tf_status tf_nr_relative_positions(tf_handle* h, double* current, double* pinned) {
+  if (!h || !h->gr.have_cur || (!current && !pinned)) {
+    set_error("tf_nr_relative_positions: no relative positions yet");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  const size_t b = sizeof(double) * 2 * h->Ne;
+  if (current) TF_CHECK(hipMemcpy(current, h->gr.relcur, b, hipMemcpyDeviceToHost));
+  if (pinned) TF_CHECK(hipMemcpy(pinned, h->gr.relinit, b, hipMemcpyDeviceToHost));
+  return TF_OK;
+}
+
+tf_status tf_nr_grow(tf_handle* h, const double* u, double t, double dt) {
+  if (!h || !std::isfinite(t) || !std::isfinite(dt) || h->gr.fns.empty()) {
+    set_error("tf_nr_grow: invalid arguments or no growth functions");
+    return TF_ERR_ARG;
+  }
+  const double* ud = nullptr;
+  tf_status p = producer_positions(h, u, &ud);
+  if (p != TF_OK) return p;
+  tf_status r = growth_run(h, ud, t, dt);
+  if (r != TF_OK) return r;
+  TF_CHECK(hipStreamSynchronize(h->st));
+  return TF_OK;
+}
+
 tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double dt, double tol,
                         int32_t maxit, int32_t precision_mode, double krylov_rtol, double* u_out,
                         tf_step_stats* stats) {
@@ -1857,8 +2254,9 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
     set_error("tf_nr_advance: invalid arguments");
     return TF_ERR_ARG;
   }
-  if (!h->have_rates || !h->have_flags || (!u0 && !h->have_positions)) {
-    set_error("tf_nr_advance needs growth rates, surface flags and positions");
+  const bool fns = !h->gr.fns.empty();
+  if ((!h->have_rates && !fns) || !h->have_flags || (!u0 && !h->have_positions)) {
+    set_error("tf_nr_advance needs growth rates or growth functions, surface flags and positions");
     return TF_ERR_ARG;
   }
   const size_t n = 3 * (size_t)h->Nn;
@@ -1873,16 +2271,27 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
   tf_status worst = TF_OK;
   std::vector<tf_step_stats> S(h->n_tissues);
   for (int s = 0; s < n_steps; ++s) {
-    // pre-step producers on the device: the Young's modulus series at the
-    // step's start time (Simulation.cpp:5994), growth, then the viscosity
-    // surfaces from the positions at the start of the step
+    // pre-step producers on the device, in Simulation::runOneStep's order
+    // (Simulation.cpp:5974-6063): the relative positions in the bounding box,
+    // the Young's modulus series at the step's start time, growth (functions
+    // at the start time, or fixed rates), then the viscosity surfaces from
+    // the positions at the start of the step
+    if (h->gr.prisms) {
+      tf_status rs = relpos_update(h, w.uk);
+      if (rs != TF_OK) return rs;
+    }
     if (!h->series.empty()) {
       tf_status ss = apply_series(h, h->sim_time, dt);
       if (ss != TF_OK) return ss;
     }
-    h->sim_time += dt;
     const uint8_t* abl = h->n_abl ? h->abl_el : nullptr;
-    TF_CHECK(launch_grow(h->Ne, h->rates, dt, h->Fg, h->invFg, h->st, abl));
+    if (fns) {
+      tf_status gs = growth_run(h, w.uk, h->sim_time, dt);
+      if (gs != TF_OK) return gs;
+    } else {
+      TF_CHECK(launch_grow(h->Ne, h->rates, dt, h->Fg, h->invFg, h->st, abl));
+    }
+    h->sim_time += dt;
     TF_CHECK(launch_visc_surface(h->Nn, h->Ne, h->conn, w.uk, h->pat.node_ptr, h->pat.node_src,
                                  h->surf_place, h->surf_type, h->surf_spans, h->area, h->st, abl));
     h->have_area = true;

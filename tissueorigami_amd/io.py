@@ -337,3 +337,40 @@ def write_property_grid(path, grid):
         f.write(f"{nx} {ny}\n")
         for row in g:
             f.write(" ".join(repr(float(x)) for x in row) + "\n")
+
+
+def read_growth_grid(path):
+    """A growth-rate grid file of a grid-based growth function
+    (ModelInputObject::readGrowthType3, ModelInputObject.cpp:746-812): "nX nY",
+    then the rates (fraction per hour) of rows j = nY-1 .. 0, each with x, y, z
+    of the nX points, then the angles (degrees) of rows j = nY-1 .. 0.  The
+    rates go through a float as the reference reads them.  Returns
+    (rates (nx, ny, 3) per hour, angles (nx, ny) in degrees)."""
+    with open(path) as f:
+        tok = f.read().split()
+    nx, ny = int(tok[0]), int(tok[1])
+    p = 2
+    rates = np.zeros((nx, ny, 3))
+    for j in range(ny - 1, -1, -1):
+        for i in range(nx):
+            for k in range(3):
+                rates[i, j, k] = float(np.float32(tok[p]))
+                p += 1
+    angles = np.zeros((nx, ny))
+    for j in range(ny - 1, -1, -1):
+        for i in range(nx):
+            angles[i, j] = float(tok[p])
+            p += 1
+    return rates, angles
+
+
+def write_growth_grid(path, rates_per_hour, angles_deg):
+    r = np.asarray(rates_per_hour, np.float64)
+    a = np.asarray(angles_deg, np.float64)
+    nx, ny = a.shape
+    with open(path, "w") as f:
+        f.write(f"{nx} {ny}\n")
+        for j in range(ny - 1, -1, -1):
+            f.write(" ".join(repr(float(r[i, j, k])) for i in range(nx) for k in range(3)) + "\n")
+        for j in range(ny - 1, -1, -1):
+            f.write(" ".join(repr(float(a[i, j])) for i in range(nx)) + "\n")

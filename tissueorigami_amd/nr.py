@@ -30,6 +30,9 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_nr_detect_packing", "tf_nr_get_packing",
            "tf_nr_add_stiffness_series", "tf_nr_apply_stiffness_series", "tf_nr_set_time",
            "tf_nr_get_time", "tf_nr_get_stiffness_multiplier",
+           "tf_nr_set_growth_prisms", "tf_nr_add_uniform_growth", "tf_nr_add_grid_growth",
+           "tf_nr_set_growth_options", "tf_nr_set_column_volume_conservation",
+           "tf_nr_update_relative_positions", "tf_nr_relative_positions", "tf_nr_grow",
            "tf_nr_destroy", "tf_last_error"]
 
 
@@ -133,6 +136,18 @@ def load():
     L.tf_nr_set_time.argtypes = [_h, ctypes.c_double]
     L.tf_nr_get_time.argtypes = [_h, _d]
     L.tf_nr_get_stiffness_multiplier.argtypes = [_h, _d]
+    L.tf_nr_set_growth_prisms.argtypes = [_h, _i, _i, _u8, _d, _d, _i, _u8, ctypes.c_int32,
+                                          ctypes.c_int32]
+    L.tf_nr_add_uniform_growth.argtypes = [_h, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                           ctypes.c_double, ctypes.c_double, _d, ctypes.c_double]
+    L.tf_nr_add_grid_growth.argtypes = [_h, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                        ctypes.c_int32, _d, _d]
+    L.tf_nr_set_growth_options.argtypes = [_h, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _d]
+    L.tf_nr_set_column_volume_conservation.argtypes = [_h, ctypes.c_int32, ctypes.c_int32, _i]
+    L.tf_nr_update_relative_positions.argtypes = [_h, _d]
+    L.tf_nr_relative_positions.argtypes = [_h, _d, _d]
+    L.tf_nr_grow.argtypes = [_h, _d, ctypes.c_double, ctypes.c_double]
     L.tf_nr_destroy.argtypes = [_h]
     L.tf_last_error.restype = ctypes.c_char_p
     for name in EXPORTS[:-1]:
@@ -307,19 +322,92 @@ class NewtonRaphsonSolver:
         _check(load().tf_nr_set_surface_flags(self._h, _p(pl, _i), _p(ty, _i), _p(sp, _u8)))
 
     # ---- Young's modulus time series (YoungsModulusModifier)
-    def add_stiffness_series(self, times, grids, linear, method, applicable, rel_pos):
+    def add_stiffness_series(self, times, grids, linear, method, applicable, rel_pos=None):
         """One series: grids (n_times, ny, nx) applied at times (s); method 0 nearest,
         1 bilinear, 2 lower corner; applicable (n_prisms) and rel_pos (n_prisms x 2)
-        per prism (tissue.stiffness_applicable, tissue.relative_positions)."""
+        per prism (tissue.stiffness_applicable, tissue.relative_positions); rel_pos
+        None once the growth prisms are set (the device keeps them current)."""
         times = _f64(times).reshape(-1)
         grids = _f64(grids)
         if grids.ndim != 3 or grids.shape[0] != times.size:
             raise ValueError("grids must be (n_times, ny, nx)")
         app = np.ascontiguousarray(applicable, np.uint8).reshape(self.n_prisms)
-        rel = _f64(rel_pos).reshape(self.n_prisms, 2)
+        rel = None if rel_pos is None else _f64(rel_pos).reshape(self.n_prisms, 2)
         _check(load().tf_nr_add_stiffness_series(
             self._h, times.size, _p(times, _d), grids.shape[2], grids.shape[1], _p(grids, _d),
             int(bool(linear)), int(method), _p(app, _u8), _p(rel, _d)))
+
+    # ---- growth functions (Simulation::calculateGrowth) on the device
+    def set_growth_prisms(self, tissue_type, placement, flags, rel_z, weights=None,
+                          apical_of=None, node_excluded=None, symmetric_y=False,
+                          explicit_ecm=False):
+        """Per-prism data of the growth functions (tissue.growth_prisms builds them):
+        tissue type, placement, flags (1 ECM, 2 ECM at circumference, 4 actin),
+        rel_z, linker weights (n x 2), apical prism of the column (-1: own),
+        nodes left out of the bounding box."""
+        ne = self.n_prisms
+        tt = np.ascontiguousarray(tissue_type, np.int32).reshape(ne)
+        pl = np.ascontiguousarray(placement, np.int32).reshape(ne)
+        fl = np.ascontiguousarray(flags, np.uint8).reshape(ne)
+        rz = _f64(rel_z).reshape(ne)
+        w = None if weights is None else _f64(weights).reshape(ne, 2)
+        ap = None if apical_of is None else np.ascontiguousarray(apical_of, np.int32).reshape(ne)
+        nx = None if node_excluded is None else \
+            np.ascontiguousarray(node_excluded, np.uint8).reshape(self.n_nodes)
+        _check(load().tf_nr_set_growth_prisms(self._h, _p(tt, _i), _p(pl, _i), _p(fl, _u8),
+                                              _p(w, _d), _p(rz, _d), _p(ap, _i), _p(nx, _u8),
+                                              int(bool(symmetric_y)), int(bool(explicit_ecm))))
+
+    def add_uniform_growth(self, t0, t1, rate, angle, apply=1, z_range=(0.0, 1.0)):
+        """UniformGrowthFunction: rate (x, y, z) in 1/s, angle in radians; apply
+        1 columnar, 2 peripodial, 4 basal ECM, 8 lateral ECM."""
+        r = _f64(rate).reshape(3)
+        _check(load().tf_nr_add_uniform_growth(self._h, float(t0), float(t1), int(apply),
+                                               float(z_range[0]), float(z_range[1]), _p(r, _d),
+                                               float(angle)))
+
+    def add_grid_growth(self, t0, t1, rates, angles_deg, apply=1, z_range=(0.0, 1.0)):
+        """GridBasedGrowthFunction: rates (nx, ny, 3) in 1/s and angles (nx, ny) in
+        degrees at the grid points, the first index along x."""
+        r = _f64(rates)
+        a = _f64(angles_deg)
+        if r.ndim != 3 or r.shape[2] != 3 or a.shape != r.shape[:2]:
+            raise ValueError("rates must be (nx, ny, 3) and angles (nx, ny)")
+        _check(load().tf_nr_add_grid_growth(self._h, float(t0), float(t1), int(apply),
+                                            float(z_range[0]), float(z_range[1]), r.shape[0],
+                                            r.shape[1], _p(r, _d), _p(a, _d)))
+
+    def set_growth_options(self, interpolation=1, pinned=False, pin_times=()):
+        t = _f64(pin_times).reshape(-1)
+        _check(load().tf_nr_set_growth_options(self._h, int(interpolation), int(bool(pinned)),
+                                               t.size, _p(t, _d) if t.size else None))
+
+    def set_column_volume_conservation(self, columns):
+        """columns (n_columns, layers): the prisms whose column volume is conserved
+        (-1 pads); an empty array turns it off."""
+        c = np.ascontiguousarray(columns, np.int32)
+        if c.size == 0:
+            _check(load().tf_nr_set_column_volume_conservation(self._h, 0, 0, None))
+            return
+        c = c.reshape(c.shape[0], -1)
+        _check(load().tf_nr_set_column_volume_conservation(self._h, c.shape[0], c.shape[1],
+                                                           _p(c, _i)))
+
+    def update_relative_positions(self, u=None):
+        u = None if u is None else _f64(u).reshape(-1)
+        _check(load().tf_nr_update_relative_positions(self._h, _p(u, _d)))
+
+    def relative_positions(self):
+        """(current, pinned) relative positions, each n_prisms x 2."""
+        cur = np.empty((self.n_prisms, 2))
+        pin = np.empty((self.n_prisms, 2))
+        _check(load().tf_nr_relative_positions(self._h, _p(cur, _d), _p(pin, _d)))
+        return cur, pin
+
+    def grow(self, t, dt, u=None):
+        """One step's growth at time t (s) from positions u (None: the device's)."""
+        u = None if u is None else _f64(u).reshape(-1)
+        _check(load().tf_nr_grow(self._h, _p(u, _d), float(t), float(dt)))
 
     def apply_stiffness_series(self, t, dt):
         """Simulation::ApplyAllYoungsModulusModifiers at time t (s)."""

@@ -331,3 +331,146 @@ def relative_positions(u, conn, apical_of=None, node_excluded=None, symmetric_y=
         sel = ap >= 0
         rel[sel] = rel[ap[sel]]
     return rel
+
+
+# ---- per-prism data of the growth functions (the reference's set-up, on the host)
+
+def tissue_height(u, conn, node_type, node_placement):
+    """Simulation::calculateTissueHeight (Simulation.cpp:3275-3330) for a prism
+    mesh: from the first basal columnar node, up through the first prism that
+    has the current node as a basal node, adding that prism's height
+    (Prism::getElementHeight, Prism.cpp:1445-1450: |node 0 - node 3|) until an
+    apical node.  Returns (TissueHeight, TissueHeightDiscretisationLayers)."""
+    u = np.asarray(u, np.float64).reshape(-1, 3)
+    conn = np.asarray(conn).reshape(-1, 6)
+    nt = np.asarray(node_type).reshape(-1)
+    npl = np.asarray(node_placement).reshape(-1)
+    basal = np.where((nt == 0) & (npl == 0))[0]
+    if basal.size == 0:
+        raise ValueError("no basal columnar node")
+    node, h, layers = int(basal[0]), 0.0, 0
+    while npl[node] != 1:
+        owners = np.where((conn[:, :3] == node).any(axis=1))[0]
+        if owners.size == 0:
+            raise ValueError(f"no prism has node {node} as a basal node")
+        e = int(owners[0])
+        d = u[conn[e, 0]] - u[conn[e, 3]]
+        h += (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) ** 0.5
+        node = int(conn[e, 3 + list(conn[e, :3]).index(node)])
+        layers += 1
+    return h, layers
+
+
+def initial_z_positions(u, conn, node_type, node_placement):
+    """ShapeBase::initialRelativePositionInZ per prism
+    (Simulation::assignInitialZPositions, Simulation.cpp:3365-3369, after the
+    bounding box of all nodes; ShapeBase::setInitialZPosition,
+    ShapeBase.cpp:812-828): 1 - (centre z - lowest z) / tissue height,
+    clamped to [0, 1]; 0 apical, 1 basal."""
+    u = np.asarray(u, np.float64).reshape(-1, 3)
+    conn = np.asarray(conn).reshape(-1, 6)
+    H, _ = tissue_height(u, conn, node_type, node_placement)
+    zmin = 100000.0
+    for z in u[:, 2]:
+        if z < zmin:
+            zmin = z
+    out = np.empty(len(conn))
+    for e in range(len(conn)):
+        c = 0.0
+        for a in range(6):
+            c += u[conn[e, a], 2]
+        c /= 6
+        r = 1.0 - ((c - zmin) / H)
+        out[e] = min(max(r, 0.0), 1.0)
+    return out
+
+
+def element_stacks(conn, elem_type, elem_place, layers, ablated=None):
+    """Prism::constructElementStackList (Prism.cpp:1605-1647) from every
+    columnar basal prism that is not ablated (Simulation::fillInElementColumnLists,
+    Simulation.cpp:7657-7674): the stack, basal first, each next prism the
+    first other prism whose basal nodes hold the three apical nodes reached so
+    far.  Returns {prism: stack} for every prism of a stack."""
+    conn = np.asarray(conn).reshape(-1, 6)
+    et = np.asarray(elem_type).reshape(-1)
+    ep = np.asarray(elem_place).reshape(-1)
+    abl = np.zeros(len(conn), bool) if ablated is None else np.asarray(ablated, bool)
+    basal_sets = [set(conn[e, :3].tolist()) for e in range(len(conn))]
+    stacks = {}
+    for b in range(len(conn)):
+        if abl[b] or et[b] != 0 or ep[b] != 0:
+            continue
+        stack = [b]
+        cur, top = b, [int(x) for x in conn[b, 3:]]
+        for _ in range(1, layers):
+            nxt = next((e for e in range(len(conn))
+                        if e != cur and all(n in basal_sets[e] for n in top)), None)
+            if nxt is None:
+                break
+            stack.append(nxt)
+            top = [int(conn[nxt, 3 + list(conn[nxt, :3]).index(n)]) for n in top]
+            cur = nxt
+        for e in stack:
+            stacks[e] = stack
+    return stacks
+
+
+def apical_positioning(stacks, elem_type, elem_place, n_prisms, layers):
+    """Simulation::updateRelativePositionsToApicalPositioning
+    (Simulation.cpp:6147-6167) as a per-prism map: every columnar apical prism
+    gives its relative position to the first layers - 2 prisms of its stack
+    (basal first; the prism just below the apical one keeps its own).  -1:
+    the prism keeps its own."""
+    et = np.asarray(elem_type).reshape(-1)
+    ep = np.asarray(elem_place).reshape(-1)
+    out = np.full(n_prisms, -1, np.int32)
+    for e in range(n_prisms):
+        if ep[e] == 1 and et[e] == 0 and e in stacks:
+            for m in stacks[e][:max(layers - 2, 0)]:
+                out[m] = e
+    return out
+
+
+def conserved_columns(stacks, elem_type, elem_place, is_ecm, is_actin, layers):
+    """The prisms whose column volume Simulation::conserveColumnVolume
+    (Simulation.cpp:6518-6600) conserves: per columnar apical prism, its
+    stack's columnar prisms that mimic neither ECM nor actin.  Returns
+    (n_columns, layers) prism ids, -1 padded."""
+    et = np.asarray(elem_type).reshape(-1)
+    ep = np.asarray(elem_place).reshape(-1)
+    ecm = np.asarray(is_ecm, bool).reshape(-1)
+    act = np.asarray(is_actin, bool).reshape(-1)
+    cols = []
+    for e in range(len(et)):
+        if et[e] == 0 and ep[e] == 1 and e in stacks:
+            m = [x for x in stacks[e][:layers] if et[x] == 0 and not ecm[x] and not act[x]]
+            cols.append(m + [-1] * (layers - len(m)))
+    return np.array(cols, np.int32).reshape(-1, layers)
+
+
+def ecm_only_nodes(conn, n_nodes, is_ecm):
+    """Node::allOwnersECMMimicing (Simulation::setUpECMMimicingElements,
+    Simulation.cpp:8866-8920): nodes every owner of which mimics ECM."""
+    conn = np.asarray(conn).reshape(-1, 6)
+    ecm = np.asarray(is_ecm, bool).reshape(-1)
+    owned = np.zeros(n_nodes, bool)
+    other = np.zeros(n_nodes, bool)
+    owned[conn.reshape(-1)] = True
+    other[conn[~ecm].reshape(-1)] = True
+    return (owned & ~other).astype(np.uint8)
+
+
+def uniform_growth_input(rate_per_hour, angle_deg):
+    """The rates (1/s) and angle (radians) a UniformGrowthFunction holds after
+    ModelInputObject::readGrowthType1 (ModelInputObject.cpp:474-501): each
+    value read into a float, scaled by 1/3600 (the angle by pi/180) and stored
+    back into the float."""
+    r = [float(np.float32(float(np.float32(v)) * (1.0 / 3600.0))) for v in rate_per_hour]
+    a = float(np.float32(float(np.float32(angle_deg)) * (np.pi / 180.0)))
+    return np.array(r), a
+
+
+def grid_growth_input(rates_per_hour):
+    """Grid growth rates in 1/s as GridBasedGrowthFunction holds them
+    (ModelInputObject.cpp:800-812: the float read times 1/3600 in double)."""
+    return np.asarray(rates_per_hour, np.float64) * (1.0 / 3600.0)
