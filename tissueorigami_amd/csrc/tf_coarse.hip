@@ -24,10 +24,10 @@
 // preconditioner needs no more).  When the steps do not converge (residual
 // above 0.25, or seven steps), X_ref is refactorised from the current A_c.
 // The inverse is kept as a dense FP32 matrix.  Per CG iteration: w = P^T r
-// (kSplit workgroups per aggregate), y = A_c^-1 w (one wavefront per row),
-// z += P y (one thread per node), and the partial products (w, y) that
-// complete (r, z).  Every sum has a fixed order and rocBLAS runs with atomics
-// off, so the solve stays bitwise reproducible.
+// (kSplit workgroups per aggregate), y = A_c^-1 w (one wavefront per row) and
+// the partial products (w, y) that complete (r, z); the CG's search-direction
+// update adds P y (tzc_pupdate).  Every sum has a fixed order and rocBLAS runs
+// with atomics off, so the solve stays bitwise reproducible.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -314,28 +314,6 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
 #pragma unroll
     for (int k = 0; k < kGemvRows; ++k) t += wy[k];
     cpart[blockIdx.x] = t;
-  }
-}
-
-// z += P y.
-__global__ __launch_bounds__(kBlock) void coarse_prolong_kernel(
-    int32_t n_nodes, const int32_t* __restrict__ node_agg, const int32_t* __restrict__ agg_tissue,
-    const int32_t* __restrict__ agg_cbase, const int32_t* __restrict__ seg_on,
-    const float* __restrict__ d, const uint8_t* __restrict__ flags, const float* __restrict__ y,
-    const int32_t* __restrict__ state, float* __restrict__ z) {
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
-    const int a = node_agg[i];
-    if (a < 0) continue;
-    const int T = agg_tissue[a];
-    if (state[2 * T] != K_RUN || !seg_on[T]) continue;
-    const float* ya = y + agg_cbase[a];
-    const float d0 = d[3 * i], d1 = d[3 * i + 1], d2 = d[3 * i + 2];
-    // t + omega x d
-    const float v[3] = {ya[0] + (ya[4] * d2 - ya[5] * d1), ya[1] + (ya[5] * d0 - ya[3] * d2),
-                        ya[2] + (ya[3] * d1 - ya[4] * d0)};
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-      if (!(flags && (flags[3 * i + p] & kCoarseExcluded))) z[3 * i + p] += v[p];
   }
 }
 
@@ -810,16 +788,12 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, in
   return hipStreamSynchronize(st);
 }
 
-void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, float* z,
-                  int32_t n_nodes, const uint8_t* dof_flags, hipStream_t st) {
+void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, hipStream_t st) {
   hipLaunchKernelGGL(coarse_restrict_kernel, dim3(c.n_agg * kSplit), dim3(kBlock), 0, st,
                      c.n_agg, c.vlen, c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on,
                      c.list_d, c.list_mask, r, state, c.w);
   hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
                      c.sgio, c.seg_on, c.vlen, c.inv32, c.w, state, c.y, c.cpart);
-  hipLaunchKernelGGL(coarse_prolong_kernel, dim3(grid_nodes(n_nodes)), dim3(kBlock), 0, st,
-                     n_nodes, c.node_agg, c.agg_tissue, c.agg_cbase, c.seg_on, c.node_d, dof_flags,
-                     c.y, state, z);
 }
 
 }  // namespace tf

@@ -227,6 +227,81 @@ enum KState { K_RUN = 0, K_CONVERGED = 1, K_BREAKDOWN = 2 };
 // A Krylov residual above kDivergence |b| ends the solve as a breakdown.
 constexpr double kDivergence = 1e8;
 
+// ---- per-tissue scalars of the tiled solvers ----
+// Ordered sum of part[beg..end), identical in every thread of the workgroup.
+__device__ inline double wg_ordered_sum(const double* part, int beg, int end) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ double red[kWaves];
+  __shared__ double result;
+  double s = 0.0;
+  for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) s += part[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += red[w];
+    result = t;
+  }
+  __syncthreads();
+  return result;
+}
+
+// Slots of the per-tissue scalars (8 per tissue) of the tiled solvers.  A
+// slot is written only by the first workgroup of its tissue, and only in a
+// launch whose workgroups do not read it.  CG: TS_RHO holds rho of the last
+// iteration (read by the start-of-iteration step), TS_RHON the current rho
+// (written by that step, read by the update, which then moves it to TS_RHO).
+enum { TS_RHO = 0, TS_ALPHA, TS_OMEGA, TS_BNORM, TS_RNORM, TS_BETA, TS_RHON };
+
+// What the preconditioned CG's start-of-iteration step reads and writes for
+// one tissue: the tiles' partials (r, z) and (r, r), optionally the coarse
+// product's partials (w, y), the scalars and the state.
+struct CgS1 {
+  const double* rz = nullptr;
+  const double* rr = nullptr;
+  const int32_t* tissue_tile = nullptr;
+  const double* cpart = nullptr;   // null: no coarse partials
+  const int32_t* cpart_off = nullptr;
+  double* scal = nullptr;
+  int32_t* state = nullptr;
+  double rtol = 0.0;
+};
+
+// rho = (r, z) [+ (w, y)], the convergence test on |r| (against |b| = |r| of
+// the first step, before any update), beta = rho / rho_old.  Every thread of
+// the workgroup calls it and gets beta and the state; with `write`, thread 0
+// stores |b| (first step), |r|, beta, rho (TS_RHON) and a state that ends the
+// solve.
+__device__ inline int cg_s1(const CgS1& s, int T, bool write, double* beta_out) {
+  const int t0 = s.tissue_tile[T], t1 = s.tissue_tile[T + 1];
+  double rho = wg_ordered_sum(s.rz, t0, t1);
+  if (s.cpart) rho += wg_ordered_sum(s.cpart, s.cpart_off[T], s.cpart_off[T + 1]);
+  const double rr = wg_ordered_sum(s.rr, t0, t1);
+  double* sc = s.scal + 8 * T;
+  const bool first = s.state[2 * T + 1] == 0;
+  const double rnorm = sqrt(rr);
+  const double bnorm = first ? rnorm : sc[TS_BNORM];
+  const double beta = rho / sc[TS_RHO];
+  int state = K_RUN;
+  if (!isfinite(rnorm) || rnorm > kDivergence * bnorm) state = K_BREAKDOWN;
+  else if (rnorm <= s.rtol * bnorm) state = K_CONVERGED;
+  else if (!(rho > 0.0) || !isfinite(beta)) state = K_BREAKDOWN;
+  if (write && threadIdx.x == 0) {
+    if (first) sc[TS_BNORM] = bnorm;
+    sc[TS_RNORM] = rnorm;
+    sc[TS_BETA] = beta;
+    sc[TS_RHON] = rho;
+    if (state != K_RUN) s.state[2 * T] = state;
+  }
+  *beta_out = beta;
+  return state;
+}
+
 struct Launch {
   hipStream_t stream;
 };
@@ -577,11 +652,11 @@ void coarse_release(CoarseOp& c);
 // some segment's inverse is in use.
 hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, int64_t n_slots,
                          const uint8_t* dof_flags, const int32_t* run, hipStream_t st);
-// z += P A_c^-1 P^T r (FP32), with the partial (w, y) per product workgroup in
-// c.cpart; nothing for a tissue whose state (state[2 T]) is not running, and
-// no correction (zero partials) for a segment whose factorisation failed.
-void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, float* z,
-                  int32_t n_nodes, const uint8_t* dof_flags, hipStream_t st);
+// y = A_c^-1 P^T r (FP32) in c.y, with the partial (w, y) per product
+// workgroup in c.cpart; nothing for a tissue whose state (state[2 T]) is not
+// running, and zero partials for a segment whose factorisation failed.  The
+// correction P y is added where the search direction is formed.
+void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, hipStream_t st);
 
 void set_error(const std::string& s);
 

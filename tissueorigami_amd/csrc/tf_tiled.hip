@@ -60,28 +60,6 @@ __device__ inline void reduce_store(double* vals, int n, double* const* outs, in
     }
 }
 
-// Ordered sum of part[beg..end), identical in every thread.
-__device__ inline double ordered_sum(const double* part, int beg, int end) {
-  __shared__ double red[kW];
-  __shared__ double result;
-  double s = 0.0;
-  for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) s += part[i];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  __syncthreads();
-  if (lane == 0) red[wave] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-#pragma unroll
-    for (int w = 0; w < kW; ++w) t += red[w];
-    result = t;
-  }
-  __syncthreads();
-  return result;
-}
-
 struct TC {
   TileMap tm;
   int32_t n_nodes;
@@ -95,14 +73,8 @@ struct TC {
   int stride;
   const int32_t* active; // per tissue: 0 = skip the solve (x = 0)
   double rtol;
-  // two-level CG: per-workgroup partial (w, y) of the coarse product, which
-  // completes (r, z) = (r, D^-1 r) + (P^T r, A_c^-1 P^T r)
-  const double* cpart;
-  const int32_t* cpart_off;  // per tissue: its range of product workgroups
-  int n_cpart;
 };
 
-enum { TS_RHO = 0, TS_ALPHA, TS_OMEGA, TS_BNORM, TS_RNORM };
 
 __device__ inline double* TP(const TC& c, int k) { return c.part + (size_t)k * c.stride; }
 
@@ -121,7 +93,7 @@ __device__ inline bool tile_ctx(const TC& c, TileCtx& t) {
 }
 
 __device__ inline double tissue_sum(const TC& c, int k, int T) {
-  return ordered_sum(TP(c, k), c.tm.tissue_tile[T], c.tm.tissue_tile[T + 1]);
+  return wg_ordered_sum(TP(c, k), c.tm.tissue_tile[T], c.tm.tissue_tile[T + 1]);
 }
 
 template <typename T>
@@ -404,20 +376,25 @@ __global__ __launch_bounds__(kBlock) void tz_update_p(TC c, const T* r, T* p, T*
 }
 
 // ---------------------------------------------------------------------------
-// Preconditioned CG for nearly symmetric Jacobians, two launches per
-// iteration:
-//   tzc_spmv:   convergence test on |r|; beta = rho / rho_old;
-//               p = z + beta p_old formed on the fly for every gathered column
-//               and stored for the tile's own rows (into the other p buffer);
-//               q = J p; partial (p, q)
-//   tzc_update: alpha = rho / (p, q); x += alpha p; r -= alpha q; z = M^-1 r;
-//               partials (r, z), (r, r) for the next iteration
-// (r, z) and (r, r) alternate between partial slots 0/3 and 1/4 with the
-// iteration's parity, which the host passes, and (p, q) uses slot 2, so no
-// launch reads a slot its own workgroups write.  rho_old is written by the
-// update kernel only.  A non-positive (p, q) or rho (indefinite operator) is a
-// breakdown: the caller falls back to BiCGStab.
-enum { TC_RZ0 = 0, TC_RR0 = 1, TC_PQ = 2, TC_RZ1 = 3, TC_RR1 = 4 };
+// Preconditioned CG for nearly symmetric Jacobians.  Every workgroup forms its
+// tissue's scalars from the ordered sums of the tiles' partials (the same
+// value in every workgroup), and the tissue's first workgroup stores them:
+//   S1 (cg_s1): rho = (r, z), the convergence test on |r|, beta = rho / rho_old;
+//   S2:         alpha = rho / (p, q).
+// Without a coarse space, two launches per iteration:
+//   tzc_spmv<FUSED>: S1; p = z + beta p_old formed on the fly for every
+//                    gathered column and stored for the tile's own rows (into
+//                    the other p buffer); q = J p; partial (p, q)
+//   tzc_update:      S2; x += alpha p; r -= alpha q; z = M^-1 r; partials
+//                    (r, z), (r, r)
+// With the coarse space (z = M^-1 r + P A_c^-1 P^T r), five:
+//   tzc_spmv:        q = J p; partial (p, q)
+//   tzc_update:      S2; x += alpha p; r -= alpha q; z_D = M^-1 r; partials
+//   coarse restriction; coarse product y = A_c^-1 P^T r with partials (w, y)
+//   tzc_pupdate:     S1 (with the (w, y) partials); p = (z_D + P y) + beta p
+// A non-positive (p, q) or rho (indefinite operator) is a breakdown: the caller
+// falls back to BiCGStab.
+enum { TC_RZ0 = 0, TC_RR0 = 1, TC_PQ = 2 };
 
 template <typename T>
 __device__ inline T fma_t(T a, T b, T c) {
@@ -425,7 +402,19 @@ __device__ inline T fma_t(T a, T b, T c) {
   else return fma(a, b, c);
 }
 
-// r = b, x = 0, p_old = 0, z = M^-1 r; partials (r, z), (r, r); rho_old = 1.
+__device__ inline CgS1 make_s1(const TC& c) {
+  CgS1 s;
+  s.rz = TP(c, TC_RZ0);
+  s.rr = TP(c, TC_RR0);
+  s.tissue_tile = c.tm.tissue_tile;
+  s.scal = c.scal;
+  s.state = c.state;
+  s.rtol = c.rtol;
+  return s;
+}
+
+// r = b, x = 0, p = 0, z = M^-1 r; partials (r, z), (r, r); rho_old = 1.  A
+// tissue not solved (active 0) is converged with x = 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void tzc_init(TC c, const double* b, T* r, T* z, T* x,
                                                    T* p, const T* minv) {
@@ -456,50 +445,49 @@ __global__ __launch_bounds__(kBlock) void tzc_init(TC c, const double* b, T* r, 
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void tzc_spmv(TC c, int par, const T* __restrict__ val,
+// q = J p, partial (p, q).  FUSED: S1, then p = z + beta p_old on the fly
+// (p_old and z gathered, p stored into pnew for the tile's rows); else p is
+// gathered as it is (pold) and pnew is not used.
+template <typename T, bool FUSED>
+__global__ __launch_bounds__(kBlock) void tzc_spmv(TC c, const T* __restrict__ val,
                                                    const T* __restrict__ z,
                                                    const T* __restrict__ pold,
                                                    T* __restrict__ pnew, T* __restrict__ q) {
   TileCtx t;
   if (!tile_ctx(c, t)) return;
   if (c.state[2 * t.T] != K_RUN) return;
-  double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
-  if (c.n_cpart) rho += ordered_sum(c.cpart, c.cpart_off[t.T], c.cpart_off[t.T + 1]);
-  const double rr = tissue_sum(c, par ? TC_RR1 : TC_RR0, t.T);
-  const double rnorm = sqrt(rr);
-  const bool first_it = (c.state[2 * t.T + 1] == 0);
-  double* sc = c.scal + 8 * t.T;
-  const double bnorm = first_it ? rnorm : sc[TS_BNORM];
-  const double beta = rho / sc[TS_RHO];
-  int state = K_RUN;
-  if (!isfinite(rnorm) || rnorm > kDivergence * bnorm) state = K_BREAKDOWN;
-  else if (rnorm <= c.rtol * bnorm) state = K_CONVERGED;
-  else if (!(rho > 0.0) || !isfinite(beta)) state = K_BREAKDOWN;
-  if (t.first && threadIdx.x == 0) {
-    if (first_it) sc[TS_BNORM] = bnorm;
-    sc[TS_RNORM] = rnorm;
-    if (state != K_RUN) c.state[2 * t.T] = state;
+  T tb = (T)0;
+  if (FUSED) {
+    double beta;
+    if (cg_s1(make_s1(c), t.T, t.first, &beta) != K_RUN) return;
+    tb = (T)beta;
   }
-  if (state != K_RUN) return;
-  const T tb = (T)beta;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double acc[1] = {0.0};
   for (int sl = t.s0 + wave; sl < t.s1; sl += kW) {
     T a0, a1, a2;
-    row_spmv_g<T>(sl, lane, c, val,
-                  [z, pold, tb](int cc, T& x0, T& x1, T& x2) {
-                    x0 = fma_t(tb, pold[3 * cc], z[3 * cc]);
-                    x1 = fma_t(tb, pold[3 * cc + 1], z[3 * cc + 1]);
-                    x2 = fma_t(tb, pold[3 * cc + 2], z[3 * cc + 2]);
-                  },
-                  a0, a1, a2);
+    if (FUSED) {
+      row_spmv_g<T>(sl, lane, c, val,
+                    [z, pold, tb](int cc, T& x0, T& x1, T& x2) {
+                      x0 = fma_t(tb, pold[3 * cc], z[3 * cc]);
+                      x1 = fma_t(tb, pold[3 * cc + 1], z[3 * cc + 1]);
+                      x2 = fma_t(tb, pold[3 * cc + 2], z[3 * cc + 2]);
+                    },
+                    a0, a1, a2);
+    } else {
+      row_spmv<T>(sl, lane, c, val, pold, a0, a1, a2);
+    }
     const int row = sl * 64 + lane;
     if (row < c.n_nodes) {
-      const T p0 = fma_t(tb, pold[3 * row], z[3 * row]);
-      const T p1 = fma_t(tb, pold[3 * row + 1], z[3 * row + 1]);
-      const T p2 = fma_t(tb, pold[3 * row + 2], z[3 * row + 2]);
-      pnew[3 * row] = p0; pnew[3 * row + 1] = p1; pnew[3 * row + 2] = p2;
+      T p0, p1, p2;
+      if (FUSED) {
+        p0 = fma_t(tb, pold[3 * row], z[3 * row]);
+        p1 = fma_t(tb, pold[3 * row + 1], z[3 * row + 1]);
+        p2 = fma_t(tb, pold[3 * row + 2], z[3 * row + 2]);
+        pnew[3 * row] = p0; pnew[3 * row + 1] = p1; pnew[3 * row + 2] = p2;
+      } else {
+        p0 = pold[3 * row]; p1 = pold[3 * row + 1]; p2 = pold[3 * row + 2];
+      }
       q[3 * row] = a0; q[3 * row + 1] = a1; q[3 * row + 2] = a2;
       acc[0] += (double)p0 * (double)a0 + (double)p1 * (double)a1 + (double)p2 * (double)a2;
     }
@@ -508,14 +496,14 @@ __global__ __launch_bounds__(kBlock) void tzc_spmv(TC c, int par, const T* __res
   reduce_store(acc, 1, outs, t.L);
 }
 
+// S2; x += alpha p; r -= alpha q; z = M^-1 r; partials (r, z), (r, r).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void tzc_update(TC c, int par, T* x, T* r, T* z, const T* p,
+__global__ __launch_bounds__(kBlock) void tzc_update(TC c, T* x, T* r, T* z, const T* p,
                                                      const T* q, const T* minv) {
   TileCtx t;
   if (!tile_ctx(c, t)) return;
   if (c.state[2 * t.T] != K_RUN) return;
-  double rho = tissue_sum(c, par ? TC_RZ1 : TC_RZ0, t.T);
-  if (c.n_cpart) rho += ordered_sum(c.cpart, c.cpart_off[t.T], c.cpart_off[t.T + 1]);
+  const double rho = c.scal[8 * t.T + TS_RHON];
   const double pq = tissue_sum(c, TC_PQ, t.T);
   const double alpha = rho / pq;
   if (!(pq > 0.0) || !isfinite(alpha)) {
@@ -539,11 +527,54 @@ __global__ __launch_bounds__(kBlock) void tzc_update(TC c, int par, T* x, T* r, 
       acc[1] += (double)r[3 * i + d] * (double)r[3 * i + d];
     }
   }
-  double* outs[2] = {TP(c, par ? TC_RZ0 : TC_RZ1), TP(c, par ? TC_RR0 : TC_RR1)};
+  double* outs[2] = {TP(c, TC_RZ0), TP(c, TC_RR0)};
   reduce_store(acc, 2, outs, t.L);
   if (t.first && threadIdx.x == 0) {
+    c.scal[8 * t.T + TS_ALPHA] = alpha;
     c.scal[8 * t.T + TS_RHO] = rho;
     c.state[2 * t.T + 1] = c.state[2 * t.T + 1] + 1;
+  }
+}
+
+// Coarse data of the search-direction update (FP32 CG with a coarse space).
+struct CoarseP {
+  const int32_t* node_agg;
+  const int32_t* agg_cbase;
+  const int32_t* seg_on;
+  const float* node_d;
+  const uint8_t* flags;
+  const float* y;
+};
+
+// S1 (with the coarse partials (w, y)); p = (z_D + P y) + beta p, in place;
+// the coarse term t + omega x d of the node's aggregate on the DOF that take
+// part in the coarse space.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void tzc_pupdate(TC c, CgS1 s1, CoarseP cp,
+                                                      const T* __restrict__ z,
+                                                      T* __restrict__ p) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (c.state[2 * t.T] != K_RUN) return;
+  double beta;
+  if (cg_s1(s1, t.T, t.first, &beta) != K_RUN) return;
+  const T tb = (T)beta;
+  const bool on = cp.seg_on[t.T] != 0;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) {
+    T zi[3] = {z[3 * i], z[3 * i + 1], z[3 * i + 2]};
+    const int a = on ? cp.node_agg[i] : -1;
+    if (a >= 0) {
+      const float* ya = cp.y + cp.agg_cbase[a];
+      const float d0 = cp.node_d[3 * i], d1 = cp.node_d[3 * i + 1], d2 = cp.node_d[3 * i + 2];
+      const float v[3] = {ya[0] + (ya[4] * d2 - ya[5] * d1), ya[1] + (ya[5] * d0 - ya[3] * d2),
+                          ya[2] + (ya[3] * d1 - ya[4] * d0)};
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (!(cp.flags && (cp.flags[3 * i + d] & kCoarseExcluded))) zi[d] += (T)v[d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) p[3 * i + d] = fma_t(tb, p[3 * i + d], zi[d]);
   }
 }
 
@@ -821,17 +852,44 @@ struct CV {
   T *r, *z, *x, *p0, *p1, *q;
   const T* val;
   const T* minv;
-  const CoarseOp* co = nullptr;  // FP32 only: z += P A_c^-1 P^T r after every z = D^-1 r
+  const CoarseOp* co = nullptr;  // FP32 only: z = D^-1 r + P A_c^-1 P^T r
   const uint8_t* flags = nullptr;
   int32_t n_nodes = 0;
 };
 
-// The coarse correction of z (FP32 CG with an active coarse space).
+CgS1 host_s1(const TC& c) {
+  CgS1 s;
+  s.rz = c.part + (size_t)TC_RZ0 * c.stride;
+  s.rr = c.part + (size_t)TC_RR0 * c.stride;
+  s.tissue_tile = c.tm.tissue_tile;
+  s.scal = c.scal;
+  s.state = c.state;
+  s.rtol = c.rtol;
+  return s;
+}
+
+// The coarse half of a CG iteration (FP32 with a coarse space): restriction,
+// product, then S1 and p = (z_D + P y) + beta p.
 template <typename T>
-void coarse_step(const TC& c, const CV<T>& V, hipStream_t st) {
+void coarse_half(const TC& c, const CV<T>& V, int G, hipStream_t st) {
   if constexpr (std::is_same<T, float>::value) {
-    if (V.co) coarse_apply(*V.co, c.state, V.r, V.z, V.n_nodes, V.flags, st);
+    coarse_apply(*V.co, c.state, V.r, st);
+    CgS1 s1 = host_s1(c);
+    s1.cpart = V.co->cpart;
+    s1.cpart_off = V.co->cpart_off;
+    const CoarseP cp{V.co->node_agg, V.co->agg_cbase, V.co->seg_on, V.co->node_d, V.flags,
+                     V.co->y};
+    hipLaunchKernelGGL(tzc_pupdate<T>, dim3(G), dim3(kBlock), 0, st, c, s1, cp, (const T*)V.z,
+                       V.p0);
   }
+}
+
+// r = b, x = 0, p = 0, z; with the coarse space also the first S1 and p.
+template <typename T>
+void cg_start(const TC& c, const CV<T>& V, const double* b, int G, hipStream_t st) {
+  hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, c, b, V.r, V.z, V.x, V.p0,
+                     V.minv);
+  if (V.co) coarse_half<T>(c, V, G, st);
 }
 
 template <typename T>
@@ -842,9 +900,6 @@ void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chu
       (void)hipStreamSynchronize(st);
       prof_harvest(pf);
     }
-    const int par = (it0 + it) & 1;
-    const T* pold = par ? V.p1 : V.p0;
-    T* pnew = par ? V.p0 : V.p1;
     const bool on = pf && pf->on && pf->used < kProfPairs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (on) {
@@ -853,11 +908,21 @@ void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chu
       ev1 = pf->ev[2 * pf->used + 1];
       pf->used++;
     }
-    hipExtLaunchKernelGGL(tzc_spmv<T>, dim3(G), dim3(kBlock), 0, st, ev0, ev1, 0, c, par, V.val,
-                          (const T*)V.z, pold, pnew, V.q);
-    hipLaunchKernelGGL(tzc_update<T>, dim3(G), dim3(kBlock), 0, st, c, par, V.x, V.r, V.z,
-                       (const T*)pnew, (const T*)V.q, V.minv);
-    coarse_step<T>(c, V, st);
+    if (V.co) {
+      hipExtLaunchKernelGGL(tzc_spmv<T, false>, dim3(G), dim3(kBlock), 0, st, ev0, ev1, 0, c,
+                            V.val, (const T*)V.z, (const T*)V.p0, (T*)nullptr, V.q);
+      hipLaunchKernelGGL(tzc_update<T>, dim3(G), dim3(kBlock), 0, st, c, V.x, V.r, V.z,
+                         (const T*)V.p0, (const T*)V.q, V.minv);
+      coarse_half<T>(c, V, G, st);
+    } else {
+      const int par = (it0 + it) & 1;
+      const T* pold = par ? V.p1 : V.p0;
+      T* pnew = par ? V.p0 : V.p1;
+      hipExtLaunchKernelGGL(tzc_spmv<T, true>, dim3(G), dim3(kBlock), 0, st, ev0, ev1, 0, c,
+                            V.val, (const T*)V.z, pold, pnew, V.q);
+      hipLaunchKernelGGL(tzc_update<T>, dim3(G), dim3(kBlock), 0, st, c, V.x, V.r, V.z,
+                         (const T*)pnew, (const T*)V.q, V.minv);
+    }
   }
 }
 
@@ -868,15 +933,8 @@ hipError_t tiled_cg_impl(const KrylovArgs& k, const CV<T>& V, const double* b, d
   hipStream_t st = k.L.stream;
   const int G = k.tm.grid;
   const int nt = k.tm.n_tissues;
-  TC c = make_tc(k, rtol);
-  if (V.co) {
-    c.cpart = V.co->cpart;
-    c.cpart_off = V.co->cpart_off;
-    c.n_cpart = V.co->n_gemv;
-  }
-  hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, c, b, V.r, V.z, V.x, V.p0,
-                     V.minv);
-  coarse_step<T>(c, V, st);
+  const TC c = make_tc(k, rtol);
+  cg_start<T>(c, V, b, G, st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const void* ptrs[12] = {V.r, V.z, V.x, V.p0, V.p1, V.q, V.val, V.minv,
@@ -886,17 +944,18 @@ hipError_t tiled_cg_impl(const KrylovArgs& k, const CV<T>& V, const double* b, d
   int done = 0;
   while (true) {
     const int chunk = std::min(sync_every, maxit - done);
-    const GraphKey key = make_key(1, (int)sizeof(T), chunk, done & 1, c, ptrs, 12);
+    // the p buffers alternate only without the coarse space
+    const GraphKey key = make_key(1, (int)sizeof(T), chunk, V.co ? 0 : (done & 1), c, ptrs, 12);
     const int it0 = done;
     e = run_captured(k, key, [&](Prof* pf) { launch_cg_chunk<T>(c, V, G, st, chunk, it0, pf); });
     if (e != hipSuccess) return e;
     done += chunk;
-    if (done >= maxit) {  // one more convergence test on the last residuals
+    if (done >= maxit && !V.co) {  // one more convergence test on the last residuals
       const int par = done & 1;
-      hipLaunchKernelGGL(tzc_spmv<T>, dim3(G), dim3(kBlock), 0, st, c, par, V.val,
+      hipLaunchKernelGGL((tzc_spmv<T, true>), dim3(G), dim3(kBlock), 0, st, c, V.val,
                          (const T*)V.z, (const T*)(par ? V.p1 : V.p0), par ? V.p0 : V.p1, V.q);
     }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // with the coarse space every iteration ends with the next one's S1
     if ((e = hipMemcpyAsync(ws->host_tstate, ws->tstate, 2 * sizeof(int32_t) * nt,
                             hipMemcpyDeviceToHost, st)) != hipSuccess)
       return e;
@@ -1123,17 +1182,9 @@ hipError_t tiled_time_iterations(const KrylovArgs& k, int mixed, int method, int
   };
   auto run_cg = [&](auto V) {
     using T = std::remove_pointer_t<decltype(V.r)>;
-    TC cc = c;
-    if (V.co) {
-      cc.cpart = V.co->cpart;
-      cc.cpart_off = V.co->cpart_off;
-      cc.n_cpart = V.co->n_gemv;
-    }
-    hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, cc, (const double*)ws->b, V.r,
-                       V.z, V.x, V.p0, V.minv);
-    coarse_step<T>(cc, V, st);
+    cg_start<T>(c, V, (const double*)ws->b, G, st);
     (void)hipEventRecord(e0, st);
-    launch_cg_chunk<T>(cc, V, G, st, iters, 0, nullptr);
+    launch_cg_chunk<T>(c, V, G, st, iters, 0, nullptr);
     (void)hipEventRecord(e1, st);
   };
   if (method == KRYLOV_CG) {
