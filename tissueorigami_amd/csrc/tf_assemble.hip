@@ -11,7 +11,10 @@
 // precomputed list sorted by element id, so the sum of every entry is formed
 // in exactly the reference's order and the result is bitwise reproducible.
 // The same thread applies the external viscosity and the fixed-DOF masks and,
-// for diagonal blocks, writes the block-Jacobi inverse.
+// for diagonal blocks, writes the block-Jacobi inverse.  After its slice's
+// blocks, the workgroup forms the residual of the slice's 64 nodes (one
+// component per wavefront), summing each node's element forces in element
+// order.
 #include "tf_internal.h"
 
 #pragma clang fp contract(off)
@@ -82,6 +85,51 @@ __device__ inline void gather_blocks(const KT* __restrict__ Kel, const int32_t* 
 #pragma unroll
       for (int q = 0; q < 3; ++q) J[3 * p + q] = J[3 * p + q] + k[3 * q + p];
   }
+}
+
+// Node residual, component i of node R: gSum = ge + gvInternal + gvExternal +
+// gExt (packing), fixed DOF zeroed (NewtonRaphsonSolver.cpp:362-397, 441-457,
+// 209); element forces are summed in element order (ShapeBase.cpp:2362-2376).
+// internal receives SystemForces (elastic + viscous per element, non-fixed
+// DOF only).
+__device__ inline void node_force(const AssembleArgs& a, int R, int i) {
+  const double rdt = 1.0 / a.dt;
+  const int beg = a.pat.node_ptr[R], end = a.pat.node_ptr[R + 1];
+  const bool fx = a.dof_flags ? (a.dof_flags[3 * R + i] & 1) : false;
+  double ge = 0.0, gv = 0.0, sf = 0.0;
+#pragma unroll 4
+  for (int t = beg; t < end; ++t) {
+    const int s = a.pat.node_src[t];  // e*6 + a
+    const double fe_raw = a.Fe[3 * (size_t)s + i], fv_raw = a.Fv[3 * (size_t)s + i];
+    const double fe = fx ? 0.0 : fe_raw;
+    const double fv = fx ? 0.0 : fv_raw;
+    ge = ge + fe;
+    gv = gv + fv;
+    if (!fx) sf = sf + (fe_raw + fv_raw);
+  }
+  // PackingForces of R, couple by couple (Simulation.cpp:7396-7401)
+  double fp = 0.0;
+  if (a.pk_cnt) {
+    const int nc = min(a.pk_cnt[R], a.pk_cap);
+    for (int e = 0; e < nc; ++e) {
+      const int o = a.pk_inc[(size_t)R * a.pk_cap + e];
+      const int lo = min(R, o), hi = max(R, o);
+      double F[3], D[9];
+      pack_couple(a.uk + 3 * lo, a.uk + 3 * hi, a.pk_mass[lo], a.pk_mass[hi], a.pk_par[0], F, D);
+      fp = (R == lo) ? fp + F[i] : fp - F[i];
+    }
+    if (a.pk_force) a.pk_force[3 * R + i] = fp;
+  }
+  const double sv = a.area[R] * a.ext_visc[3 * R + i];
+  const double disp = (a.uk[3 * R + i] - a.un[3 * R + i]) * rdt;
+  const double gve = -(sv * disp);
+  double g = ge + gv + gve;
+  // gExt = PackingForces + 0 (Simulation.cpp:8130-8151), added by
+  // NewtonRaphsonSolver::addExernalForces
+  g = g + (fp + 0.0);
+  if (fx) g = 0.0;
+  a.g[3 * R + i] = g;
+  if (a.internal) a.internal[3 * R + i] = sf;
 }
 
 template <bool VISC, bool MIXED>
@@ -219,63 +267,8 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
       for (int q = 0; q < 9; ++q) a.val[(size_t)q * n_slots + slot] = J[q];
     }
     }
-  }
-}
-
-// Node residual: gSum = ge + gvInternal + gvExternal + gExt (packing), fixed DOF
-// zeroed (NewtonRaphsonSolver.cpp:362-397, 441-457, 209); element forces
-// are summed in element order (ShapeBase.cpp:2362-2376).  internal receives
-// SystemForces (elastic + viscous per element, non-fixed DOF only).
-__global__ __launch_bounds__(kBlock) void node_force_kernel(AssembleArgs a) {
-  const double rdt = 1.0 / a.dt;
-  for (int R = blockIdx.x * kBlock + threadIdx.x; R < a.n_nodes; R += gridDim.x * kBlock) {
-    const int beg = a.pat.node_ptr[R], end = a.pat.node_ptr[R + 1];
-    double ge[3] = {0.0, 0.0, 0.0}, gv[3] = {0.0, 0.0, 0.0}, sf[3] = {0.0, 0.0, 0.0};
-    bool fx[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) fx[i] = a.dof_flags ? (a.dof_flags[3 * R + i] & 1) : false;
-    for (int t = beg; t < end; ++t) {
-      const int s = a.pat.node_src[t];  // e*6 + a
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const double fe = fx[i] ? 0.0 : a.Fe[3 * (size_t)s + i];
-        const double fv = fx[i] ? 0.0 : a.Fv[3 * (size_t)s + i];
-        ge[i] = ge[i] + fe;
-        gv[i] = gv[i] + fv;
-        if (!fx[i]) sf[i] = sf[i] + (a.Fe[3 * (size_t)s + i] + a.Fv[3 * (size_t)s + i]);
-      }
-    }
-    // PackingForces of R, couple by couple (Simulation.cpp:7396-7401)
-    double fp[3] = {0.0, 0.0, 0.0};
-    if (a.pk_cnt) {
-      const int nc = min(a.pk_cnt[R], a.pk_cap);
-      for (int e = 0; e < nc; ++e) {
-        const int o = a.pk_inc[(size_t)R * a.pk_cap + e];
-        const int lo = min(R, o), hi = max(R, o);
-        double F[3], D[9];
-        pack_couple(a.uk + 3 * lo, a.uk + 3 * hi, a.pk_mass[lo], a.pk_mass[hi], a.pk_par[0], F,
-                    D);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) fp[i] = (R == lo) ? fp[i] + F[i] : fp[i] - F[i];
-      }
-      if (a.pk_force) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) a.pk_force[3 * R + i] = fp[i];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const double sv = a.area[R] * a.ext_visc[3 * R + i];
-      const double disp = (a.uk[3 * R + i] - a.un[3 * R + i]) * rdt;
-      const double gve = -(sv * disp);
-      double g = ge[i] + gv[i] + gve;
-      // gExt = PackingForces + 0 (Simulation.cpp:8130-8151), added by
-      // NewtonRaphsonSolver::addExernalForces
-      g = g + (fp[i] + 0.0);
-      if (fx[i]) g = 0.0;
-      a.g[3 * R + i] = g;
-      if (a.internal) a.internal[3 * R + i] = sf[i];
-    }
+    // the residual of the slice's nodes: three wavefronts, one component each
+    if (kq < 3 && R < a.n_nodes) node_force(a, R, kq);
   }
 }
 
@@ -420,9 +413,6 @@ hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st) {
     if (mixed) hipLaunchKernelGGL((assemble_kernel<false, true>), dim3(g1), dim3(kBlock), 0, st, a);
     else hipLaunchKernelGGL((assemble_kernel<false, false>), dim3(g1), dim3(kBlock), 0, st, a);
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(node_force_kernel, dim3(grid_for(a.n_nodes)), dim3(kBlock), 0, st, a);
   return hipGetLastError();
 }
 
