@@ -204,9 +204,10 @@ tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja
                         int64_t* nnz);
 
 /* Per-prism outputs of the last assembly, for element-level parity (the
- * tangents are the FP32 ones after a mixed tf_nr_assemble_ex, recomputed from
- * that call's inputs: the mixed assembly itself keeps only their per-block
- * sums, so the FP32 tangents need tf_nr_assemble_ex as the last assembly):
+ * tangents are the FP32 ones after a mixed tf_nr_assemble_ex, recomputed with
+ * the forces from that call's inputs: the mixed assembly itself keeps only
+ * the per-block sums of the tangents and the summed forces, so after a mixed
+ * assembly every output needs tf_nr_assemble_ex as the last assembly):
  * elastic and internal viscous element forces (n_prisms x 18,
  * ElementalElasticSystemForces / ElementalInternalViscousSystemForces before
  * fixed-DOF masking) and the 18 x 18 tangents TriPointKe / TriPointKv in the

@@ -100,7 +100,9 @@ __device__ inline void node_force(const AssembleArgs& a, int R, int i) {
 #pragma unroll 4
   for (int t = beg; t < end; ++t) {
     const int s = a.pat.node_src[t];  // e*6 + a
-    const double fe_raw = a.Fe[3 * (size_t)s + i], fv_raw = a.Fv[3 * (size_t)s + i];
+    // combined: Fe holds elastic + viscous, and gv stays 0
+    const double fe_raw = a.Fe[3 * (size_t)s + i];
+    const double fv_raw = a.combined ? 0.0 : a.Fv[3 * (size_t)s + i];
     const double fe = fx ? 0.0 : fe_raw;
     const double fv = fx ? 0.0 : fv_raw;
     ge = ge + fe;

@@ -669,6 +669,286 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision per-prism kernel (mode 2): the FP32 K blocks of the summed
+// elastic + viscous tangent and the FP64 nodal forces (elastic + viscous
+// summed), which the mixed NR step needs.  FP64 results agree with the
+// reference-order evaluation to rounding, not bit for bit:
+//   - sigma = (lambda ln J I + mu (b - I)) / J with b = Fe Fe^T (equal to
+//     Fe PK2 Fe^T / J since Fe C^-1 Fe^T = I), which needs no inverse of C;
+//   - the inverses of dX/dxi and of the current Jacobian are adj(M) / det(M).
+// Phase 1 (one thread per (prism, Gauss point)) leaves in LDS both the FP64
+// data of the forces and the FP32 data of the tangent, already scaled by the
+// Gauss weight and the volume; phase 2 (one thread per (prism, node a))
+// forms the six K blocks (a, 0..5) and node a's force.
+#ifndef TF_MIX_EPB
+#define TF_MIX_EPB 8
+#endif
+constexpr int kMEPB = TF_MIX_EPB;
+constexpr int kMGP = kMEPB * 6;
+constexpr int kMThreads = (kMGP + 63) / 64 * 64;
+
+struct GPMix {
+  double fN[6][3];  // grad N * det F * det dX/dxi
+  double s6[6];     // sigma: xx, yy, zz, xy, yz, xz
+  double v6[6];     // eta d, same order (viscous)
+  float N[6][3];    // grad N
+  float sg[9];      // sigma
+  float clw, cmw, volw;
+  float Sp[9], Pp[9];  // w vol eta d and (w vol eta / 2) (I/dt - L)
+  // padding to 110 dwords: the Gauss points that the threads of a wavefront
+  // read together then fall in different LDS banks
+  float pad[2];
+};
+static_assert(sizeof(GPMix) == 440, "GPMix layout");
+
+// adj(M) / det(M), and det(M)
+__device__ inline double inv3_adj(const double M[3][3], double X[3][3]) {
+  const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1];
+  const double c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2];
+  const double c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+  const double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+  const double r = 1.0 / det;
+  X[0][0] = c00 * r;
+  X[1][0] = c01 * r;
+  X[2][0] = c02 * r;
+  X[0][1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * r;
+  X[1][1] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * r;
+  X[2][1] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * r;
+  X[0][2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * r;
+  X[1][2] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * r;
+  X[2][2] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * r;
+  return det;
+}
+
+template <bool VISC>
+__global__ __launch_bounds__(kMThreads) void element_kernel_mix(ElemConst c, ElementArgs a) {
+#pragma clang fp contract(fast)
+  __shared__ GPMix sd[kMGP];
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * kMEPB;
+  const double w = c.w;
+  const float wf = (float)w;
+
+  // ---------------- phase 1: one thread per (prism, Gauss point) ----------
+  if (tid < kMGP) {
+    const int el = tid / 6, g = tid % 6;
+    const int e = e0 + el;
+    if (e < a.n_prisms) {
+      int nd[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) nd[k] = a.conn[6 * e + k];
+      double pos[6][3], ref[6][3], iFg[3][3];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          pos[k][i] = a.uk[3 * nd[k] + i];
+          ref[k][i] = a.refpos[18 * e + 3 * k + i];
+        }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) iFg[i][j] = a.invFg[9 * e + 3 * i + j];
+      const double lambda = a.lambda[e], mu = a.mu[e];
+      const double(*SFD)[6] = c.sfd[g];
+      double dXde[3][3], InvdXde[3][3], Jac[3][3], InvJ[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double s = 0.0, t = 0.0;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            s += SFD[j][k] * ref[k][i];
+            t += SFD[j][k] * pos[k][i];
+          }
+          dXde[i][j] = s;
+          Jac[i][j] = t;
+        }
+      const double detdXde = inv3_adj(dXde, InvdXde);
+      double F[3][3], Fe[3][3];
+      matmul3(Jac, InvdXde, F);
+      matmul3(F, iFg, Fe);
+      const double detFe = det3(Fe);
+      const double lnJ = log(detFe);
+      const double rJ = 1.0 / detFe;
+      double sig[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) {
+          const double b = Fe[i][0] * Fe[j][0] + Fe[i][1] * Fe[j][1] + Fe[i][2] * Fe[j][2];
+          const double v = (i == j ? lambda * lnJ + mu * (b - 1.0) : mu * b) * rJ;
+          sig[3 * i + j] = v;
+          sig[3 * j + i] = v;
+        }
+      inv3_adj(Jac, InvJ);
+      double dN[6][3];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+          dN[k][m] = InvJ[0][m] * SFD[0][k] + InvJ[1][m] * SFD[1][k] + InvJ[2][m] * SFD[2][k];
+      const double detF = det3(F);
+      const double vol = detF * detdXde;
+      const double cc = vol * rJ;
+      GPMix& o = sd[tid];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          o.fN[k][m] = dN[k][m] * vol;
+          o.N[k][m] = (float)dN[k][m];
+        }
+      o.s6[0] = sig[0]; o.s6[1] = sig[4]; o.s6[2] = sig[8];
+      o.s6[3] = sig[1]; o.s6[4] = sig[5]; o.s6[5] = sig[2];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) o.sg[q] = (float)sig[q];
+      o.clw = (float)(cc * lambda * w);
+      o.cmw = (float)(cc * (mu - lambda * lnJ) * w);
+      o.volw = (float)(vol * w);
+      if (VISC) {
+        const double eta = a.eta[e];
+        double sv[9], P[9];
+        if (eta != 0) {
+          const double rdt = 1.0 / a.dt;
+          double vel[6][3];
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+              vel[k][i] = (a.uk[3 * nd[k] + i] - a.un[3 * nd[k] + i]) * rdt;
+          // velocity gradient L (ShapeBase.cpp:2908-2916, 3097-3121), its
+          // symmetric part d and (I/dt - L)
+          double L[9];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) L[q] = 0.0;
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) L[3 * i + j] += vel[k][i] * dN[k][j];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+              sv[3 * i + j] = 0.5 * (L[3 * i + j] + L[3 * j + i]) * eta;
+              P[3 * i + j] = (i == j ? rdt : 0.0) - L[3 * i + j];
+            }
+        } else {
+#pragma unroll
+          for (int q = 0; q < 9; ++q) { sv[q] = 0.0; P[q] = 0.0; }
+        }
+        o.v6[0] = sv[0]; o.v6[1] = sv[4]; o.v6[2] = sv[8];
+        o.v6[3] = sv[1]; o.v6[4] = sv[5]; o.v6[5] = sv[2];
+        const double vw = vol * w, pe = 0.5 * vol * w * eta;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          o.Sp[q] = (float)(vw * sv[q]);
+          o.Pp[q] = (float)(pe * P[q]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---------------- phase 2: one thread per (prism, node a) ----------------
+  if (tid >= kMGP) return;
+  const int el = tid / 6, ia = tid % 6;
+  const int e = e0 + el;
+  if (e >= a.n_prisms) return;
+  const bool has_eta = VISC && a.eta[e] != 0;
+  float K[6][9];
+#pragma unroll
+  for (int b = 0; b < 6; ++b)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) K[b][q] = 0.f;
+  double fe[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int g = 0; g < 6; ++g) {
+    const GPMix& d = sd[el * 6 + g];
+    // the force of node a: (sigma + eta d) grad N_a vol w
+    {
+      const double bx = d.fN[ia][0], by = d.fN[ia][1], bz = d.fN[ia][2];
+      double s0 = d.s6[0], s1 = d.s6[1], s2 = d.s6[2], s3 = d.s6[3], s4 = d.s6[4], s5 = d.s6[5];
+      if (VISC) {
+        s0 += d.v6[0]; s1 += d.v6[1]; s2 += d.v6[2];
+        s3 += d.v6[3]; s4 += d.v6[4]; s5 += d.v6[5];
+      }
+      fe[0] += (bx * s0 + by * s3 + bz * s5) * w;
+      fe[1] += (by * s1 + bx * s3 + bz * s4) * w;
+      fe[2] += (bz * s2 + by * s4 + bx * s5) * w;
+    }
+    // the Gauss point's FP32 data in registers, loaded once
+    float N[6][3], sg[9], Sp[9], Pp[9];
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) N[b][j] = d.N[b][j];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) sg[q] = d.sg[q];
+    if (has_eta) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) { Sp[q] = d.Sp[q]; Pp[q] = d.Pp[q]; }
+    }
+    float Na[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Na[j] = N[ia][j];
+    const float clw = d.clw, cmw = d.cmw, volw = d.volw;
+    float sTNa[3], V[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      sTNa[j] = sg[j] * Na[0] + sg[3 + j] * Na[1] + sg[6 + j] * Na[2];
+      V[j] = cmw * Na[j];
+    }
+    if (has_eta) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        V[j] -= Sp[3 * j] * Na[0] + Sp[3 * j + 1] * Na[1] + Sp[3 * j + 2] * Na[2];
+    }
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      float Nb[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Nb[j] = N[b][j];
+      const float dot = Na[0] * Nb[0] + Na[1] * Nb[1] + Na[2] * Nb[2];
+      const float diag = volw * (sTNa[0] * Nb[0] + sTNa[1] * Nb[1] + sTNa[2] * Nb[2]) + cmw * dot;
+      float U[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) U[j] = clw * Nb[j];
+      if (has_eta) {
+        float W[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          U[j] += Sp[3 * j] * Nb[0] + Sp[3 * j + 1] * Nb[1] + Sp[3 * j + 2] * Nb[2];
+          W[j] = Pp[j] * Nb[0] + Pp[3 + j] * Nb[1] + Pp[6 + j] * Nb[2];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc)
+            K[b][3 * r + cc] += Na[r] * U[cc] + Nb[r] * V[cc] + Na[cc] * W[r] + dot * Pp[3 * cc + r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) K[b][3 * r + cc] += Na[r] * U[cc] + Nb[r] * V[cc];
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) K[b][4 * r] += diag;
+    }
+  }
+  float* out = a.Ke32 + ((size_t)e * 36 + ia * 6) * 9;
+#pragma unroll
+  for (int b = 0; b < 6; ++b)
+#pragma unroll
+    for (int q = 0; q < 9; ++q) out[9 * b + q] = K[b][q];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) a.Fe[18 * e + 3 * ia + i] = 0.0 - fe[i];
+}
+
 ElemConst make_const() {
   ElemConst c;
   // Gauss rule (Prism.cpp:99-131) and shape-function derivative table
@@ -703,13 +983,20 @@ hipError_t launch_element(const ElementArgs& a, int mode, hipStream_t st) {
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, st, c, a);
   };
+  if (mode == 2) {
+    // the mixed NR step: summed FP32 blocks and summed FP64 forces (in Fe)
+    const int gm = (a.n_prisms + kMEPB - 1) / kMEPB;
+    if (a.any_visc)
+      hipLaunchKernelGGL(element_kernel_mix<true>, dim3(gm), dim3(kMThreads), 0, st, c, a);
+    else
+      hipLaunchKernelGGL(element_kernel_mix<false>, dim3(gm), dim3(kMThreads), 0, st, c, a);
+    return hipGetLastError();
+  }
   if (a.any_visc) {
-    if (mode == 2) go(element_kernel<true, 2>);
-    else if (mode == 1) go(element_kernel<true, 1>);
+    if (mode == 1) go(element_kernel<true, 1>);
     else go(element_kernel<true, 0>);
   } else {
-    if (mode == 2) go(element_kernel<false, 2>);
-    else if (mode == 1) go(element_kernel<false, 1>);
+    if (mode == 1) go(element_kernel<false, 1>);
     else go(element_kernel<false, 0>);
   }
   return hipGetLastError();

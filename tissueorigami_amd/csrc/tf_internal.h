@@ -327,7 +327,8 @@ struct ElementArgs {
   double* Fv;
 };
 // mode 0: FP64 tangents; 1: FP32 elastic and viscous tangents apart;
-// 2: FP32 elastic + viscous summed per K block into Ke32
+// 2: FP32 elastic + viscous summed per K block into Ke32, and the elastic +
+// viscous forces summed into Fe (Fv not written)
 hipError_t launch_element(const ElementArgs& a, int mode, hipStream_t st);
 
 // ---- assembly (tf_assemble.hip) ----
@@ -341,7 +342,7 @@ struct AssembleArgs {
   const double* Fe;
   const double* Fv;
   int any_visc;
-  int combined;  // mixed: Ke32 already holds elastic + viscous (element mode 2)
+  int combined;  // mixed: Ke32 holds elastic + viscous blocks and Fe the summed forces (element mode 2)
   const double* ext_visc;
   const double* area;
   const uint8_t* dof_flags;  // bit0 fixed, bit1 slave

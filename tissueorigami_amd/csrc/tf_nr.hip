@@ -2898,11 +2898,12 @@ tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke
     return TF_ERR_ARG;
   }
   const size_t Ne = h->Ne;
-  if (h->last_mixed && (Ke || Kv)) {
-    // the mixed assembly keeps only the summed FP32 blocks: the separate FP32
-    // tangents are recomputed from the assembly's inputs
+  if (h->last_mixed) {
+    // the mixed assembly keeps only the summed FP32 blocks and the summed
+    // forces: the separate forces and FP32 tangents are recomputed from the
+    // assembly's inputs
     if (!h->assembly_inputs_kept) {
-      set_error("tf_nr_element_outputs: mixed tangents need a preceding tf_nr_assemble_ex");
+      set_error("tf_nr_element_outputs: after a mixed assembly it needs tf_nr_assemble_ex");
       return TF_ERR_ARG;
     }
     Workspace& w = h->ws;
