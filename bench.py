@@ -101,38 +101,43 @@ def assembly_bytes(n_prisms, n_blocks, n_nodes, s, visc, combined=False):
     18x18 tangents and 18-forces written, then each block gathered once with
     its source index, the operator (9 values per block) and the block-Jacobi
     inverse written, node forces read and the residual written.  With viscosity
-    the FP64 path writes and gathers an elastic and a viscous tangent; the
-    mixed path (combined) one FP32 tangent holding both."""
+    the FP64 path writes and gathers an elastic and a viscous tangent and an
+    elastic and a viscous force; the mixed path (combined) one FP32 tangent
+    and one force holding both."""
     nk = 2 if visc and not combined else 1
-    elem = n_prisms * (24 + 144 + 72 + 24 + 2 * 144 + nk * 324 * s + 2 * 144)
+    nf = 1 if combined else 2
+    elem = n_prisms * (24 + 144 + 72 + 24 + 2 * 144 + nk * 324 * s + nf * 144)
     asm = n_prisms * 36 * (4 + nk * 9 * s) + n_blocks * 9 * s + n_nodes * 9 * (s + 8)
-    forces = n_prisms * 2 * 144 + n_nodes * (3 * 8 * 4)
+    forces = n_prisms * nf * 144 + n_nodes * (3 * 8 * 4)
     return elem + asm + forces
 
 
-def cg_spmv_bytes(n_blocks, n_nodes, s):
-    """tzc_spmv (q = J p with p = z + beta p_old formed on the fly, dot (p, q)):
-    block values and columns; per node z and p_old gathered once each, p and q
-    written."""
-    return n_blocks * (9 * s + 4) + n_nodes * s * (3 + 3 + 3 + 3)
+def cg_spmv_bytes(n_blocks, n_nodes, s, fused=True):
+    """tzc_spmv: block values and columns, and per node: with the search
+    direction formed on the fly (fused: p = z + beta p_old, no coarse space) z
+    and p_old gathered once each, p and q written; with the coarse space p
+    gathered once and q written."""
+    return n_blocks * (9 * s + 4) + n_nodes * s * ((3 + 3 + 3 + 3) if fused else (3 + 3))
 
 
 def cg_iteration_bytes(n_blocks, n_nodes, s, n_coarse=0):
     """One CG iteration: tzc_spmv and tzc_update (reads x, p, r, q and the
     block-Jacobi inverse; writes x, r, z), and with a coarse space of n_coarse
-    DOF the correction z += P A_c^-1 P^T r: restriction (list index, offsets,
-    mask, r gathered), the dense FP32 inverse read once, prolongation
-    (aggregate, offsets, flags, z read and written)."""
-    b = cg_spmv_bytes(n_blocks, n_nodes, s) + n_nodes * s * (6 + 3 + 6 + 3 + 9 + 3)
+    DOF: restriction (list index, offsets, mask, r gathered), the dense inverse
+    (FP32, or FP64 for the FP64 CG) read once, and the search-direction update
+    tzc_pupdate (aggregate, offsets, flags; z read, p read and written)."""
+    b = (cg_spmv_bytes(n_blocks, n_nodes, s, fused=not n_coarse)
+         + n_nodes * s * (6 + 3 + 6 + 3 + 9 + 3))
     if n_coarse:
-        b += n_nodes * (4 + 12 + 1 + 12) + 4 * n_coarse * n_coarse + n_nodes * (4 + 12 + 3 + 24)
+        b += (n_nodes * (4 + 12 + 1 + 3 * s) + s * n_coarse * n_coarse
+              + n_nodes * (4 + 12 + 3) + n_nodes * s * 9)
     return b
 
 
-def coarse_setup_bytes(n_blocks, n_coarse, gemms):
-    """Per assembly: A_c from the FP32 operator (values, slot and row lists)
-    and `gemms` dense FP64 products of the Newton-Schulz refinement."""
-    return n_blocks * (36 + 4 + 4) + gemms * 3 * 8 * n_coarse * n_coarse
+def coarse_setup_bytes(n_blocks, n_coarse, gemms, s=4):
+    """Per assembly: A_c from the operator (values of s bytes, slot and row
+    lists) and `gemms` dense FP64 products of the Newton-Schulz refinement."""
+    return n_blocks * (9 * s + 4 + 4) + gemms * 3 * 8 * n_coarse * n_coarse
 
 
 def profile_step(solver, w, u, Fg, precision, krylov_rtol):
@@ -501,7 +506,7 @@ def main():
     res["value"] = tm["nr_iterations"] / wall_m
     res["ms_per_step"] = 1000.0 * wall_m / args.steps
     fp64 = None
-    td = None
+    td = tc = None
     if not args.no_fp64:
         u_d, _, td, wall_d = run_steps(solver, w, u_w, Fg_w, args.steps, "fp64",
                                        krylov_rtol=args.fp64_rtol)
@@ -516,6 +521,24 @@ def main():
         res["mixed_vs_fp64_speedup"] = res["value"] / fp64["value"]
         res["mixed_vs_fp64_time_steps_speedup"] = wall_d / wall_m
         res["mixed_rel_pos_error_vs_fp64"] = err
+        # the same two-level CG and forcing term as the mixed step, in FP64
+        # (FP64 operator, residual, vectors and coarse inverse): what precision
+        # alone buys
+        solver.set_krylov("fp64", "cg")
+        u_c, _, tc, wall_c = run_steps(solver, w, u_w, Fg_w, args.steps, "fp64",
+                                       krylov_rtol=args.mixed_rtol)
+        solver.set_krylov("fp64", "bicgstab")
+        print(f"[bench] fp64 two-level CG: {tc['nr_iterations']} NR its, "
+              f"{tc['krylov_iterations']} Krylov its, {wall_c:.2f} s", file=sys.stderr, flush=True)
+        res["fp64_cg"] = dict(value=tc["nr_iterations"] / wall_c, nr_iterations=tc["nr_iterations"],
+                              krylov_iterations=tc["krylov_iterations"], wall_s=wall_c,
+                              ms_element=tc["ms_element"], ms_assembly=tc["ms_assembly"],
+                              ms_krylov=tc["ms_krylov"], converged=tc["converged"],
+                              krylov_rtol=args.mixed_rtol,
+                              rel_pos_error_vs_fp64=float(np.abs(u_c - u_d).max()
+                                                          / np.abs(u_d).max()))
+        res["mixed_vs_fp64_cg_time_steps_speedup"] = wall_c / wall_m
+        res["mixed_vs_best_fp64_time_steps_speedup"] = min(wall_c, wall_d) / wall_m
     res["mixed"] = dict(nr_iterations=tm["nr_iterations"], krylov_iterations=tm["krylov_iterations"],
                         wall_s=wall_m, ms_element=tm["ms_element"], ms_assembly=tm["ms_assembly"],
                         ms_krylov=tm["ms_krylov"], converged=tm["converged"],
@@ -544,7 +567,7 @@ def main():
     if kt_m["spmv_launches_fp32"] > 0:
         avg_s = kt_m["spmv_ms_fp32"] / kt_m["krylov_iterations"] / 1000.0
         if args.mixed_krylov == "cg":
-            kname, b = "tzc_spmv<float>", cg_spmv_bytes(n_blocks, nn, 4)
+            kname, b = "tzc_spmv<float>", cg_spmv_bytes(n_blocks, nn, 4, fused=not coarse_on)
         else:
             kname, b = "tz_spmv_v<float>", spmv_v_bytes(n_blocks, nn, 4)
         ach = b / avg_s / 1e9
@@ -571,21 +594,23 @@ def main():
     # whole NR iteration: element kernel + assembly (+ coarse setup) + k Krylov iterations
     visc = bool(np.any(w["eta"] != 0))
     for key, t, sz, meth in (("mixed", tm, 4, args.mixed_krylov),
-                             ("fp64", td if fp64 else None, 8, "bicgstab")):
+                             ("fp64", td if fp64 else None, 8, "bicgstab"),
+                             ("fp64_cg", tc if fp64 else None, 8, "cg")):
         if t is None:
             continue
         k_per = t["krylov_iterations"] / t["nr_iterations"]
-        nco = 6 * n_agg if (key == "mixed" and coarse_on and meth == "cg") else 0
+        nco = 6 * n_agg if (key != "fp64" and coarse_on and meth == "cg") else 0
         if meth == "cg":
             it_bytes = cg_iteration_bytes(n_blocks, nn, sz, nco)
         else:
             it_bytes = bicgstab_iteration_bytes(n_blocks, nn, sz)
-        bnr = assembly_bytes(ne, n_blocks, nn, sz, visc, combined=key == "mixed") + k_per * it_bytes
+        bnr = (assembly_bytes(ne, n_blocks, nn, sz, visc, combined=key == "mixed")
+               + k_per * it_bytes)
         if nco:
             # two Newton-Schulz steps (4 GEMMs) and a residual product per coarse setup:
             # one per time step (lagged) or one per assembly
             per_nr = 1.0 if args.coarse_every_iteration else args.steps / t["nr_iterations"]
-            bnr += coarse_setup_bytes(n_blocks, nco, 5) * per_nr
+            bnr += coarse_setup_bytes(n_blocks, nco, 5, sz) * per_nr
         secs = (t["ms_element"] + t["ms_assembly"] + t["ms_krylov"]) / 1e3 / t["nr_iterations"]
         res.setdefault("roofline_nr_iteration", {})[key] = dict(
             algorithmic_bytes=bnr, krylov_iterations=k_per, device_s=secs,

@@ -100,10 +100,11 @@ __device__ inline void block_sum(double (&v)[N], double (*red)[kBlock / 64]) {
 
 // (P_I^T A P_K) for every aggregate pair (I, K), from the operator slots
 // listed for the pair.
+template <typename V>
 __global__ __launch_bounds__(kBlock) void coarse_pairs_kernel(
     int32_t n_pairs, const int32_t* __restrict__ cp_ptr, const int32_t* __restrict__ cp_slot,
     const int32_t* __restrict__ cp_row, const int32_t* __restrict__ col,
-    const float* __restrict__ val, int64_t n_slots, const float* __restrict__ d,
+    const V* __restrict__ val, int64_t n_slots, const float* __restrict__ d,
     const uint8_t* __restrict__ flags, double* __restrict__ out) {
   __shared__ double red[36][kBlock / 64];
   const int pr = blockIdx.x;
@@ -164,16 +165,17 @@ __global__ __launch_bounds__(kBlock) void coarse_diag_kernel(int32_t n_c, double
   if (r < n_c && dense[(size_t)r * n_c + r] == 0.0) dense[(size_t)r * n_c + r] = 1.0;
 }
 
-// FP32 row-major copy (row stride ld) of the column-major FP64 inverse,
-// through 32 x 32 LDS tiles (grid: tiles of rows x tiles of columns).
+// Row-major copy (row stride ld; FP32 or FP64) of the column-major FP64
+// inverse, through 32 x 32 LDS tiles (grid: tiles of rows x tiles of columns).
+template <typename O>
 __global__ __launch_bounds__(kBlock) void coarse_convert_kernel(int32_t n_c, int32_t ld,
-                                                                const double* X, float* inv32) {
-  __shared__ float tile[32][33];
+                                                                const double* X, O* inv32) {
+  __shared__ O tile[32][33];
   const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int k = ty; k < 32; k += 8) {
     const int r = r0 + tx, cc = c0 + k;  // read down a column of X
-    tile[k][tx] = (r < n_c && cc < n_c) ? (float)X[(size_t)cc * n_c + r] : 0.f;
+    tile[k][tx] = (r < n_c && cc < n_c) ? (O)X[(size_t)cc * n_c + r] : (O)0;
   }
   __syncthreads();
   for (int k = ty; k < 32; k += 8) {
@@ -220,12 +222,13 @@ __global__ __launch_bounds__(kBlock) void coarse_twoI_kernel(int32_t n_c, double
 // the product kernel): kSplit workgroups per aggregate, each over a fixed
 // part of the aggregate's node list; the offsets and masks are stored in list
 // order, only r is gathered.
+template <typename R>
 __global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
     int32_t n_agg, int32_t vlen, const int32_t* __restrict__ agg_ptr,
     const int32_t* __restrict__ agg_nodes, const int32_t* __restrict__ agg_tissue,
     const int32_t* __restrict__ agg_cbase, const int32_t* __restrict__ seg_on,
     const float* __restrict__ list_d, const uint8_t* __restrict__ list_mask,
-    const float* __restrict__ r, const int32_t* __restrict__ state, float* __restrict__ w) {
+    const R* __restrict__ r, const int32_t* __restrict__ state, R* __restrict__ w) {
   __shared__ double red[6][kBlock / 64];
   const int a = blockIdx.x / kSplit, part = blockIdx.x % kSplit;
   if (a >= n_agg) return;
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
   block_sum<6>(acc, red);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) w[(size_t)part * vlen + agg_cbase[a] + q] = (float)acc[q];
+    for (int q = 0; q < 6; ++q) w[(size_t)part * vlen + agg_cbase[a] + q] = (R)acc[q];
 }
 
 // Fixed / slave bits of every listed node (list order), from the DOF flags.
@@ -306,6 +309,49 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
 #pragma unroll
     for (int p = 1; p < kSplit; ++p) wr += ws[(size_t)p * vlen + row];
     dot = (double)wr * (double)s;
+  }
+  if (lane == 0) wy[wave] = dot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int k = 0; k < kGemvRows; ++k) t += wy[k];
+    cpart[blockIdx.x] = t;
+  }
+}
+
+// The FP64 product for the FP64 CG: y = A_c^-1 w with the FP64 inverse.
+typedef double v2d __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(kBlock) void coarse_gemv64_kernel(
+    const int32_t* __restrict__ gw, const int32_t* __restrict__ sgi,
+    const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on, int32_t vlen,
+    const double* __restrict__ inv64, const double* __restrict__ w,
+    const int32_t* __restrict__ state, double* __restrict__ y, double* __restrict__ cpart) {
+  __shared__ double wy[kGemvRows];
+  const int T = gw[2 * blockIdx.x], row0 = gw[2 * blockIdx.x + 1];
+  if (state[2 * T] != K_RUN) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nc = sgi[3 * T], ld = sgi[3 * T + 1], off = sgi[3 * T + 2];
+  const int row = row0 + wave;
+  double dot = 0.0;
+  if (seg_on[T] && row < nc) {
+    const v2d* a = (const v2d*)(inv64 + sgio[T] + (size_t)row * ld);
+    const double* ws = w + off;
+    double s = 0.0;
+    for (int j = lane; j < ld / 2; j += 64) {
+      v2d v = ((const v2d*)ws)[j];
+#pragma unroll
+      for (int p = 1; p < kSplit; ++p) v += ((const v2d*)(ws + (size_t)p * vlen))[j];
+      const v2d u = a[j];
+      s += u.x * v.x + u.y * v.y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) y[off + row] = s;
+    double wr = ws[row];
+#pragma unroll
+    for (int p = 1; p < kSplit; ++p) wr += ws[(size_t)p * vlen + row];
+    dot = wr * s;
   }
   if (lane == 0) wy[wave] = dot;
   __syncthreads();
@@ -718,8 +764,12 @@ hipError_t factor_segments(CoarseOp& c, const std::vector<int>& segs, rocblas_ha
         double* X = cur[T] ? S.X2 : S.X;
         if (r < 1e-5) {
           ok[T] = 1;
-          hipLaunchKernelGGL(coarse_convert_kernel, dim3((nc + 31) / 32, (nc + 31) / 32),
+          hipLaunchKernelGGL(coarse_convert_kernel<float>, dim3((nc + 31) / 32, (nc + 31) / 32),
                              dim3(kBlock), 0, st, nc, c.seg_ld[T], X, c.inv32 + c.seg_ioff[T]);
+          if (c.want64)
+            hipLaunchKernelGGL(coarse_convert_kernel<double>,
+                               dim3((nc + 31) / 32, (nc + 31) / 32), dim3(kBlock), 0, st, nc,
+                               c.seg_ld[T], X, c.inv64 + c.seg_ioff[T]);
           if ((e = hipGetLastError()) != hipSuccess) return e;
           continue;
         }
@@ -740,8 +790,11 @@ hipError_t factor_segments(CoarseOp& c, const std::vector<int>& segs, rocblas_ha
 
 }  // namespace
 
-hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, int64_t n_slots,
-                         const uint8_t* dof_flags, const int32_t* run, hipStream_t st) {
+namespace {
+
+template <typename V>
+hipError_t coarse_factor_impl(CoarseOp& c, const int32_t* col, const V* val32, int64_t n_slots,
+                              const uint8_t* dof_flags, const int32_t* run, hipStream_t st) {
   c.active = false;
   if (!c.built) return hipSuccess;
   if (!c.blas) {
@@ -757,7 +810,7 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, in
   // the segments' dense blocks lie in order: A_c ends with the last one
   const int last = c.n_seg - 1;
   const int64_t total = c.seg_moff[last] + (int64_t)c.seg_nc[last] * c.seg_nc[last];
-  hipLaunchKernelGGL(coarse_pairs_kernel, dim3(std::max(c.n_cpairs, 1)), dim3(kBlock), 0, st,
+  hipLaunchKernelGGL(coarse_pairs_kernel<V>, dim3(std::max(c.n_cpairs, 1)), dim3(kBlock), 0, st,
                      c.n_cpairs, c.cp_ptr, c.cp_slot, c.cp_row, col, val32, n_slots, c.node_d,
                      dof_flags, c.pairblk);
   hipLaunchKernelGGL(coarse_fill_kernel, dim3(grid_nodes(std::max<int64_t>(total, 1))),
@@ -788,12 +841,44 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, in
   return hipStreamSynchronize(st);
 }
 
+}  // namespace
+
+hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, int64_t n_slots,
+                         const uint8_t* dof_flags, const int32_t* run, hipStream_t st) {
+  c.want64 = 0;
+  return coarse_factor_impl<float>(c, col, val32, n_slots, dof_flags, run, st);
+}
+
+hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const double* val64, int64_t n_slots,
+                         const uint8_t* dof_flags, const int32_t* run, hipStream_t st) {
+  if (c.built && !c.inv64) {
+    const int last = c.n_seg - 1;
+    const int64_t n_inv = std::max<int64_t>(c.seg_ioff[last] + (int64_t)c.seg_nc[last] * c.seg_ld[last], 1);
+    hipError_t e;
+    if ((e = upload(c, &c.inv64, std::vector<double>((size_t)n_inv, 0.0))) != hipSuccess) return e;
+    if ((e = upload(c, &c.w64, std::vector<double>(kSplit * (size_t)std::max(c.vlen, 1), 0.0))) != hipSuccess) return e;
+    if ((e = upload(c, &c.y64, std::vector<double>((size_t)std::max(c.vlen, 1), 0.0))) != hipSuccess) return e;
+  }
+  c.want64 = 1;
+  hipError_t e = coarse_factor_impl<double>(c, col, val64, n_slots, dof_flags, run, st);
+  c.want64 = 0;
+  return e;
+}
+
 void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, hipStream_t st) {
-  hipLaunchKernelGGL(coarse_restrict_kernel, dim3(c.n_agg * kSplit), dim3(kBlock), 0, st,
+  hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.n_agg * kSplit), dim3(kBlock), 0, st,
                      c.n_agg, c.vlen, c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on,
                      c.list_d, c.list_mask, r, state, c.w);
   hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
                      c.sgio, c.seg_on, c.vlen, c.inv32, c.w, state, c.y, c.cpart);
+}
+
+void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, hipStream_t st) {
+  hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.n_agg * kSplit), dim3(kBlock), 0, st,
+                     c.n_agg, c.vlen, c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on,
+                     c.list_d, c.list_mask, r, state, c.w64);
+  hipLaunchKernelGGL(coarse_gemv64_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
+                     c.sgio, c.seg_on, c.vlen, c.inv64, c.w64, state, c.y64, c.cpart);
 }
 
 }  // namespace tf

@@ -133,6 +133,11 @@ struct CoarseOp {
   float* inv32 = nullptr;       // per segment A_c^-1, row-major FP32
   float* w = nullptr;           // P^T r as partial vectors of vlen entries (zero padding)
   float* y = nullptr;           // A_c^-1 P^T r
+  // the same in FP64, for the FP64 CG (allocated by its first factorisation)
+  double* inv64 = nullptr;
+  double* w64 = nullptr;
+  double* y64 = nullptr;
+  int want64 = 0;               // the factorisation in progress also writes inv64
   double* cpart = nullptr;      // per workgroup of the product: partial (w, y)
   std::vector<void*> allocs;
   void* blas = nullptr;         // rocblas_handle
@@ -653,11 +658,17 @@ void coarse_release(CoarseOp& c);
 // some segment's inverse is in use.
 hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const float* val32, int64_t n_slots,
                          const uint8_t* dof_flags, const int32_t* run, hipStream_t st);
+// The same from the FP64 operator, keeping also an FP64 copy of the inverse
+// (for the FP64 CG).
+hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const double* val64, int64_t n_slots,
+                         const uint8_t* dof_flags, const int32_t* run, hipStream_t st);
 // y = A_c^-1 P^T r (FP32) in c.y, with the partial (w, y) per product
 // workgroup in c.cpart; nothing for a tissue whose state (state[2 T]) is not
 // running, and zero partials for a segment whose factorisation failed.  The
 // correction P y is added where the search direction is formed.
 void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, hipStream_t st);
+// FP64: y64 = A_c^-1 P^T r with the FP64 inverse, w64 and y64 in FP64.
+void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, hipStream_t st);
 
 void set_error(const std::string& s);
 

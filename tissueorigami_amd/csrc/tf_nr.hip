@@ -902,7 +902,27 @@ KrylovArgs krylov_args(tf_handle* h) {
 // iteration of a step only and kept for the step's later iterations: the
 // preconditioner stays a fixed SPD operator within every solve, and a step
 // repeated from the same state repeats its bits.
-tf_status coarse_prepare(tf_handle* h, int nr_iteration, const int32_t* run) {
+// The coarse space of the CG solve and its factorisation from the step's
+// operator (FP32 for the mixed step, FP64 otherwise).
+tf_status coarse_factor_op(tf_handle* h, const int32_t* run, int mixed) {
+  CoarseOp& c = h->coarse;
+  if (h->bound.ready) {
+    if (mixed)
+      TF_CHECK(coarse_factor(c, h->bound.pat.col, h->bound.val32, h->bound.pat.n_slots,
+                             h->dof_flags, run, h->st));
+    else
+      TF_CHECK(coarse_factor(c, h->bound.pat.col, (const double*)h->bound.val,
+                             h->bound.pat.n_slots, h->dof_flags, run, h->st));
+  } else if (mixed) {
+    TF_CHECK(coarse_factor(c, h->pat.col, h->ws.val32, h->pat.n_slots, h->dof_flags, run, h->st));
+  } else {
+    TF_CHECK(coarse_factor(c, h->pat.col, (const double*)h->ws.val, h->pat.n_slots, h->dof_flags,
+                           run, h->st));
+  }
+  return TF_OK;
+}
+
+tf_status coarse_prepare(tf_handle* h, int nr_iteration, const int32_t* run, int mixed) {
   CoarseOp& c = h->coarse;
   const bool bound = h->bound.ready;
   if (h->coarse_lag && nr_iteration > 0 && c.built && c.for_bound == bound &&
@@ -918,13 +938,7 @@ tf_status coarse_prepare(tf_handle* h, int nr_iteration, const int32_t* run) {
       any |= retry[T] != 0;
     }
     if (!any) return TF_OK;
-    if (bound)
-      TF_CHECK(coarse_factor(c, h->bound.pat.col, h->bound.val32, h->bound.pat.n_slots,
-                             h->dof_flags, retry.data(), h->st));
-    else
-      TF_CHECK(coarse_factor(c, h->pat.col, h->ws.val32, h->pat.n_slots, h->dof_flags,
-                             retry.data(), h->st));
-    return TF_OK;
+    return coarse_factor_op(h, retry.data(), mixed);
   }
   c.active = false;
   if (h->coarse_cell == 0.0) return TF_OK;
@@ -971,13 +985,7 @@ tf_status coarse_prepare(tf_handle* h, int nr_iteration, const int32_t* run) {
     c.for_bound = bound;
     ++c.gen;
   }
-  if (bound)
-    TF_CHECK(coarse_factor(c, h->bound.pat.col, h->bound.val32, h->bound.pat.n_slots,
-                           h->dof_flags, run, h->st));
-  else
-    TF_CHECK(coarse_factor(c, h->pat.col, h->ws.val32, h->pat.n_slots, h->dof_flags, run,
-                           h->st));
-  return TF_OK;
+  return coarse_factor_op(h, run, mixed);
 }
 
 // Lays the pattern out again with `cap` contact columns in every slice that
@@ -1542,8 +1550,8 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     tf_status s = assemble(h, dt, mixed);
     if (s != TF_OK) return s;
     const int method = h->krylov_method[mixed];
-    if (mixed && method == KRYLOV_CG) {
-      s = coarse_prepare(h, pass, act.data());
+    if (method == KRYLOV_CG) {
+      s = coarse_prepare(h, pass, act.data(), mixed);
       if (s != TF_OK) return s;
     } else {
       h->coarse.active = false;

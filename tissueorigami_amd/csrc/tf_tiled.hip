@@ -536,21 +536,22 @@ __global__ __launch_bounds__(kBlock) void tzc_update(TC c, T* x, T* r, T* z, con
   }
 }
 
-// Coarse data of the search-direction update (FP32 CG with a coarse space).
+// Coarse data of the search-direction update (CG with a coarse space).
+template <typename T>
 struct CoarseP {
   const int32_t* node_agg;
   const int32_t* agg_cbase;
   const int32_t* seg_on;
   const float* node_d;
   const uint8_t* flags;
-  const float* y;
+  const T* y;
 };
 
 // S1 (with the coarse partials (w, y)); p = (z_D + P y) + beta p, in place;
 // the coarse term t + omega x d of the node's aggregate on the DOF that take
 // part in the coarse space.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void tzc_pupdate(TC c, CgS1 s1, CoarseP cp,
+__global__ __launch_bounds__(kBlock) void tzc_pupdate(TC c, CgS1 s1, CoarseP<T> cp,
                                                       const T* __restrict__ z,
                                                       T* __restrict__ p) {
   TileCtx t;
@@ -565,13 +566,13 @@ __global__ __launch_bounds__(kBlock) void tzc_pupdate(TC c, CgS1 s1, CoarseP cp,
     T zi[3] = {z[3 * i], z[3 * i + 1], z[3 * i + 2]};
     const int a = on ? cp.node_agg[i] : -1;
     if (a >= 0) {
-      const float* ya = cp.y + cp.agg_cbase[a];
-      const float d0 = cp.node_d[3 * i], d1 = cp.node_d[3 * i + 1], d2 = cp.node_d[3 * i + 2];
-      const float v[3] = {ya[0] + (ya[4] * d2 - ya[5] * d1), ya[1] + (ya[5] * d0 - ya[3] * d2),
-                          ya[2] + (ya[3] * d1 - ya[4] * d0)};
+      const T* ya = cp.y + cp.agg_cbase[a];
+      const T d0 = cp.node_d[3 * i], d1 = cp.node_d[3 * i + 1], d2 = cp.node_d[3 * i + 2];
+      const T v[3] = {ya[0] + (ya[4] * d2 - ya[5] * d1), ya[1] + (ya[5] * d0 - ya[3] * d2),
+                      ya[2] + (ya[3] * d1 - ya[4] * d0)};
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        if (!(cp.flags && (cp.flags[3 * i + d] & kCoarseExcluded))) zi[d] += (T)v[d];
+        if (!(cp.flags && (cp.flags[3 * i + d] & kCoarseExcluded))) zi[d] += v[d];
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d) p[3 * i + d] = fma_t(tb, p[3 * i + d], zi[d]);
@@ -852,7 +853,7 @@ struct CV {
   T *r, *z, *x, *p0, *p1, *q;
   const T* val;
   const T* minv;
-  const CoarseOp* co = nullptr;  // FP32 only: z = D^-1 r + P A_c^-1 P^T r
+  const CoarseOp* co = nullptr;  // z = D^-1 r + P A_c^-1 P^T r (FP64: with the FP64 inverse)
   const uint8_t* flags = nullptr;
   int32_t n_nodes = 0;
 };
@@ -868,20 +869,20 @@ CgS1 host_s1(const TC& c) {
   return s;
 }
 
-// The coarse half of a CG iteration (FP32 with a coarse space): restriction,
+// The coarse half of a CG iteration (with a coarse space): restriction,
 // product, then S1 and p = (z_D + P y) + beta p.
 template <typename T>
 void coarse_half(const TC& c, const CV<T>& V, int G, hipStream_t st) {
-  if constexpr (std::is_same<T, float>::value) {
-    coarse_apply(*V.co, c.state, V.r, st);
-    CgS1 s1 = host_s1(c);
-    s1.cpart = V.co->cpart;
-    s1.cpart_off = V.co->cpart_off;
-    const CoarseP cp{V.co->node_agg, V.co->agg_cbase, V.co->seg_on, V.co->node_d, V.flags,
-                     V.co->y};
-    hipLaunchKernelGGL(tzc_pupdate<T>, dim3(G), dim3(kBlock), 0, st, c, s1, cp, (const T*)V.z,
-                       V.p0);
-  }
+  coarse_apply(*V.co, c.state, (const T*)V.r, st);
+  CgS1 s1 = host_s1(c);
+  s1.cpart = V.co->cpart;
+  s1.cpart_off = V.co->cpart_off;
+  const T* y;
+  if constexpr (std::is_same<T, float>::value) y = V.co->y;
+  else y = V.co->y64;
+  const CoarseP<T> cp{V.co->node_agg, V.co->agg_cbase, V.co->seg_on, V.co->node_d, V.flags, y};
+  hipLaunchKernelGGL(tzc_pupdate<T>, dim3(G), dim3(kBlock), 0, st, c, s1, cp, (const T*)V.z,
+                     V.p0);
 }
 
 // r = b, x = 0, p = 0, z; with the coarse space also the first S1 and p.
@@ -938,7 +939,9 @@ hipError_t tiled_cg_impl(const KrylovArgs& k, const CV<T>& V, const double* b, d
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const void* ptrs[12] = {V.r, V.z, V.x, V.p0, V.p1, V.q, V.val, V.minv,
-                          V.co ? V.co->inv32 : nullptr, V.co ? V.co->node_d : nullptr,
+                          V.co ? (sizeof(T) == 4 ? (const void*)V.co->inv32 : (const void*)V.co->inv64)
+                               : nullptr,
+                          V.co ? V.co->node_d : nullptr,
                           V.co ? V.co->agg_nodes : nullptr,
                           V.co ? (const void*)(intptr_t)V.co->gen : nullptr};
   int done = 0;
@@ -1026,6 +1029,11 @@ hipError_t tiled_cg(const KrylovArgs& k, const double* b, double* x, double rtol
                     int sync_every, TiledResult* res) {
   Workspace* ws = k.ws;
   CV<double> V{ws->r, ws->s, x, ws->p, ws->ph, ws->v, k.val, k.minv};
+  if (k.coarse && k.coarse->active && k.coarse->inv64) {
+    V.co = k.coarse;
+    V.flags = k.dof_flags;
+    V.n_nodes = k.n_nodes;
+  }
   return tiled_cg_impl<double>(k, V, b, rtol, maxit, sync_every, res);
 }
 
