@@ -214,6 +214,16 @@ tf_status tf_nr_get_csr(tf_handle* h, double threshold, int32_t* ia, int32_t* ja
  * reference's storage orientation (n_prisms x 324).  Any output may be NULL. */
 tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke, double* Kv);
 
+/* Per-prism Green-Lagrange strain E = (Fe^T Fe - I) / 2 of the last Gauss
+ * point, as Prism::calculateCurrNodalForces keeps it in ShapeBase::Strain
+ * (Prism.cpp:826-837) and Simulation::writeTensionCompression saves it
+ * (Simulation.cpp:8521-8531): n_prisms x 6 in the order E00, E11, E22,
+ * 2 E01, 2 E21, 2 E02, taken at the positions of the last assembly (in an
+ * NR step, the last iterate before the final increment, as in the
+ * reference).  In a batch each tissue keeps the strain of its own last NR
+ * iteration; ablated prisms give zero. */
+tf_status tf_nr_element_strain(tf_handle* h, double* strain);
+
 /* Device-resident time stepping with the pre-step producers on the device.
  * tf_nr_set_growth_rates: per-prism growth rates (n_prisms x 3, x/y/z, 1/s)
  *   and F_g reset to identity.  Every step of tf_nr_advance applies

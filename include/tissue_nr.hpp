@@ -113,6 +113,10 @@ class ShapeBase {
   bool isActinMimicing = false;
   std::vector<int> elementsIdsOnSameColumn;
   double initialRelativePosInBoundingBox[2] = {0.0, 0.0};
+  // ShapeBase::Strain (ShapeBase.h:159): Green-Lagrange strain of the last
+  // Gauss point of the last force evaluation, E00, E11, E22, 2E01, 2E21, 2E02
+  // (Prism.cpp:826-837); set by the solver after each assembly and step
+  double Strain[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   int getNodeNumber() const { return 6; }
   const int* getNodeIds() const { return NodeIds.data(); }
   // ShapeBase::updatePositions (ShapeBase.cpp:3279-3285): element positions
@@ -184,6 +188,14 @@ class NewtonRaphsonSolver {
     bind(Nodes, Elements);
     check(tf_nr_assemble(h, uk.data(), un.data(), dt, gSum.data(), nullptr));
     lastDt = dt;
+    readStrains(Elements);
+  }
+  // The elements' ShapeBase::Strain from the device's last assembly.
+  void readStrains(const std::vector<std::unique_ptr<ShapeBase>>& Elements) {
+    std::vector<double> s(6 * Elements.size());
+    check(tf_nr_element_strain(h, s.data()));
+    for (size_t e = 0; e < Elements.size(); ++e)
+      for (int q = 0; q < 6; ++q) Elements[e]->Strain[q] = s[6 * e + q];
   }
   // checkJacobianForAblatedNodes (NewtonRaphsonSolver.cpp:425-439): a zero
   // Jacobian diagonal entry of a listed node becomes 1.  The device assembly
@@ -230,6 +242,7 @@ class NewtonRaphsonSolver {
     if (s != TF_OK && s != TF_ERR_NOT_CONVERGED) check(s);
     updateNodePositionsNR(Nodes);
     for (auto& e : Elements) e->updatePositions(Nodes);
+    readStrains(Elements);
     return st;
   }
 

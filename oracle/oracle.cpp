@@ -1071,6 +1071,56 @@ void orc_prism_geometry(const double* refpos18, double* invdXde54, double* detdX
   }
 }
 
+// ShapeBase::Strain after ShapeBase::calculateForces3D: the loop over the six
+// Gauss points (ShapeBase.cpp:2407-2416) leaves the strain of the last one,
+// E = (C - I) / 2 with C = Fe^T Fe (Prism.cpp:782-799, 826-837;
+// ShapeBase.cpp:2448-2472), in the Voigt order E00, E11, E22, 2E01, 2E21, 2E02.
+// pos and refpos: 18 per prism; invFg: 9 per prism; strain: 6 per prism;
+// gp: the Gauss point (5 for the reference's output, others for tests that
+// show the saved strain tells the points apart).
+void orc_element_strain(int n_el, const double* pos, const double* refpos, const double* invFg,
+                        int gp, double* strain) {
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < n_el; ++e) {
+    double p[6][3], r[6][3], f[3][3];
+    for (int a = 0; a < 6; ++a)
+      for (int i = 0; i < 3; ++i) {
+        p[a][i] = pos[18 * e + 3 * a + i];
+        r[a][i] = refpos[18 * e + 3 * a + i];
+      }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) f[i][j] = invFg[9 * e + 3 * i + j];
+    PrismGeom G;
+    prism_geometry(r, G);
+    const int g = gp;
+    double Jac[3][3], F[3][3], Fe[3][3], C[3][3], E[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int a = 0; a < 6; ++a) s += G.SFD[g][j][a] * p[a][i];
+        Jac[i][j] = s;
+      }
+    matmul3(Jac, G.InvdXde[g], F);
+    matmul3(F, f, Fe);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += Fe[k][i] * Fe[k][j];
+        C[i][j] = s;
+      }
+    // calculateEForNodalForcesKirshoff: copy C, subtract I, scale by 0.5
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) E[i][j] = (C[i][j] - (i == j ? 1.0 : 0.0)) * 0.5;
+    double* out = strain + 6 * (size_t)e;
+    out[0] = E[0][0];
+    out[1] = E[1][1];
+    out[2] = E[2][2];
+    out[3] = 2.0 * E[0][1];
+    out[4] = 2.0 * E[2][1];
+    out[5] = 2.0 * E[0][2];
+  }
+}
+
 void orc_inv3(const double* A9, double* X9) {
   double A[3][3], X[3][3];
   std::memcpy(A, A9, sizeof(A));

@@ -33,6 +33,7 @@ def lib():
         L.orc_inv3.argtypes = [_d, _d]
         L.orc_elements.argtypes = [ctypes.c_int, _d, _d, _d, _d, _d, _d, _d, ctypes.c_double,
                                    ctypes.c_int, _d, _d, _d, _d]
+        L.orc_element_strain.argtypes = [ctypes.c_int, _d, _d, _d, ctypes.c_int, _d]
         L.orc_assemble_dense.argtypes = [ctypes.c_int, ctypes.c_int, _i, _d, _d, _d, _d, _d, _d,
                                          _d, _u8, _i, ctypes.c_int, _d, _d, ctypes.c_double,
                                          ctypes.c_int, _d, _d, _d]
@@ -137,6 +138,20 @@ def prism_geometry(ref18):
     inv = np.empty(54); det = np.empty(6)
     lib().orc_prism_geometry(_p(ref18, _d), _p(inv, _d), _p(det, _d))
     return inv.reshape(6, 3, 3), det
+
+
+def element_strain(pos, refpos, invFg, gp=5):
+    """Green-Lagrange strain of each prism's last Gauss point (n_el x 6, Voigt
+    E00, E11, E22, 2E01, 2E21, 2E02): ShapeBase::Strain, Prism.cpp:826-837.
+    pos and refpos: per-prism node positions (n_el x 18); invFg: n_el x 9;
+    gp: the Gauss point evaluated (the reference keeps the last, 5)."""
+    pos = _f64(pos).reshape(-1, 18); ne = pos.shape[0]
+    refpos = _f64(refpos).reshape(ne, 18); invFg = _f64(invFg).reshape(ne, 9)
+    out = np.empty((ne, 6))
+    if not 0 <= int(gp) < 6:
+        raise ValueError("gp must be 0..5")
+    lib().orc_element_strain(ne, _p(pos, _d), _p(refpos, _d), _p(invFg, _d), int(gp), _p(out, _d))
+    return out
 
 
 def elements(pos, refpos, invFg, lam, mu, eta, vel=None, dt=1.0, tangent_mode=1):

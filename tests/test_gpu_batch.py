@@ -66,6 +66,10 @@ def test_batch_step_is_bitwise_each_tissue_alone(orc, nr, precision, rtol, spt, 
     ub, stats = b.step(u0, dt, 1e-8, 20, precision, rtol)
     assert len(stats) == len(cs)
     parts = b.split_nodes(ub)
+    # each tissue's strain is that of its own last NR iterate
+    strain = b.element_strain()
+    po = b.prism_offsets
+    print("NR iterations per tissue:", [st["nr_iterations"] for st in stats])
     for t, c in enumerate(cs):
         s = _alone_physical(nr, orc, c, dt, spt)
         s.set_krylov(precision, method)
@@ -74,6 +78,7 @@ def test_batch_step_is_bitwise_each_tissue_alone(orc, nr, precision, rtol, spt, 
         assert st["krylov_iterations"] == stats[t]["krylov_iterations"], t
         assert st["du_norms"] == stats[t]["du_norms"], t
         assert np.array_equal(parts[t].reshape(-1), ua), t
+        assert np.array_equal(strain[po[t]:po[t + 1]], s.element_strain()), t
         s.close()
     b.close()
 

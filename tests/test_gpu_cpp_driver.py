@@ -45,6 +45,11 @@ def read_output(path, nn):
     return u, du
 
 
+def read_strain(path):
+    return np.array([[float(x) for x in l.split()[1:]] for l in open(path).read().splitlines()
+                     if l.startswith("strain")])
+
+
 @pytest.mark.parametrize("mode", ["loop", "step"])
 @pytest.mark.parametrize("run,k", [("run07007", 0), ("run07007", 4), ("run07007", 9),
                                    ("run07008", -1), ("run07008", 4), ("run07008", 9)])
@@ -61,6 +66,10 @@ def test_cpp_driver_replays_pardiso_runs(orc, tmp_path, mode, run, k):
     big = s["du_log"] > 1e-10
     assert len(du) == len(s["du_log"])
     assert np.allclose(du[big], s["du_log"][big], rtol=1e-5, atol=0)
+    # ShapeBase::Strain of every prism against Save_TensionCompression
+    strain = read_strain(out)
+    assert strain.shape == (len(s["conn"]), 6)
+    assert np.abs(strain - d["strain"][k + 1]).max() < {"run07007": 5e-9, "run07008": 6e-8}[run]
 
 
 @pytest.mark.parametrize("mode", ["loop", "step"])

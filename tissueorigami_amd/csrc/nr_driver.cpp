@@ -31,7 +31,8 @@
 //       (calculateRelativePositions, calculateGrowth at t, updateStepNR).
 // Output: one line per node with the converged x y z (%.17g), then a line
 //   "iterations <k>" and one "du <norm>" line per NR iteration (of the last
-//   step), and with growth one "Fg" line of 9 values per prism.
+//   step), with growth one "Fg" line of 9 values per prism, and one "strain"
+//   line of 6 values per prism (ShapeBase::Strain after the last step).
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -254,6 +255,11 @@ int main(int argc, char** argv) {
         for (int j = 0; j < 3; ++j) std::fprintf(out, " %.17g", el->Fg[i][j]);
       std::fprintf(out, "\n");
     }
+  for (auto& el : Elements) {
+    std::fprintf(out, "strain");
+    for (int q = 0; q < 6; ++q) std::fprintf(out, " %.17g", el->Strain[q]);
+    std::fprintf(out, "\n");
+  }
   std::fclose(out);
   return 0;
 }

@@ -519,7 +519,7 @@ hipError_t launch_grow(int32_t n_prisms, const double* rates, double dt, double*
 
 namespace {
 // One workgroup per ablated prism: its 324 tangent entries (in the buffers the
-// element mode writes) and its 18 + 18 force entries are set to zero.
+// element mode writes), its 18 + 18 force entries and its strain are set to zero.
 __global__ __launch_bounds__(kBlock) void ablate_zero_kernel(ElementArgs a, int mode,
                                                              const int32_t* list, int n) {
   const int i = blockIdx.x;
@@ -538,6 +538,9 @@ __global__ __launch_bounds__(kBlock) void ablate_zero_kernel(ElementArgs a, int 
     a.Fe[e * 18 + q] = 0.0;
     a.Fv[e * 18 + q] = 0.0;
   }
+  // an ablated prism is never evaluated by the reference, whose strain of it
+  // stays at its initial zero
+  if (a.strain && threadIdx.x < 6) a.strain[e * 6 + threadIdx.x] = 0.0;
 }
 }  // namespace
 

@@ -276,6 +276,21 @@ __device__ inline void mixed_row_blocks(const GPData* sd, const GPVisc* svd, int
   }
 }
 
+// Green-Lagrange strain E = (C - I) / 2 of Prism::calculateCurrNodalForces
+// (Prism.cpp:826-837, ShapeBase.cpp:2459-2472), from C = Fe^T Fe: the
+// reference keeps the value of the last Gauss point it evaluates, in the Voigt
+// order E00, E11, E22, 2 E01, 2 E21, 2 E02 (2 (C01 / 2) = C01 exactly).
+__device__ inline void store_strain(const ElementArgs& a, int e, const double C[3][3]) {
+  if (!a.strain || (a.ptissue && !a.tactive[a.ptissue[e]])) return;
+  double* s = a.strain + 6 * (size_t)e;
+  s[0] = (C[0][0] - 1.0) * 0.5;
+  s[1] = (C[1][1] - 1.0) * 0.5;
+  s[2] = (C[2][2] - 1.0) * 0.5;
+  s[3] = C[0][1];
+  s[4] = C[2][1];
+  s[5] = C[0][2];
+}
+
 // MODE 0: FP64 tangents (oracle operation order); 1: FP32 elastic and
 // viscous tangents stored apart (Ke32, Kv32, for element-level parity);
 // 2: FP32 blocks of both summed per K block into Ke32 (mixed-precision NR).
@@ -347,6 +362,7 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
           for (int k = 0; k < 3; ++k) s += Fe[k][i] * Fe[k][j];
           C[i][j] = s;
         }
+      if (g == 5) store_strain(a, e, C);
       const double detFe = det3(Fe);
       const double lnJ = log(detFe);
       double InvC[3][3];
@@ -770,6 +786,17 @@ __global__ __launch_bounds__(kMThreads) void element_kernel_mix(ElemConst c, Ele
       double F[3][3], Fe[3][3];
       matmul3(Jac, InvdXde, F);
       matmul3(F, iFg, Fe);
+      if (g == 5 && a.strain) {
+        double C[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = i; j < 3; ++j) {
+            C[i][j] = Fe[0][i] * Fe[0][j] + Fe[1][i] * Fe[1][j] + Fe[2][i] * Fe[2][j];
+            C[j][i] = C[i][j];
+          }
+        store_strain(a, e, C);
+      }
       const double detFe = det3(Fe);
       const double lnJ = log(detFe);
       const double rJ = 1.0 / detFe;

@@ -181,6 +181,7 @@ struct Workspace {
   float* elemKv32 = nullptr;
   double* elemFe = nullptr;  // n_prisms * 18, unmasked -sum_gp w g
   double* elemFv = nullptr;
+  double* strain = nullptr;  // n_prisms * 6, last Gauss point of the last assembly
   // assembled operator
   double* val = nullptr;    // 9 * n_slots
   float* val32 = nullptr;   // 9 * n_slots (mixed precision)
@@ -330,6 +331,12 @@ struct ElementArgs {
   float* Kv32;
   double* Fe;
   double* Fv;
+  // optional: the Green-Lagrange strain of the last Gauss point, 6 per prism
+  // in Voigt order (E00, E11, E22, 2E01, 2E21, 2E02); with ptissue set, only
+  // prisms whose tissue is flagged in tactive are written
+  double* strain = nullptr;
+  const int32_t* ptissue = nullptr;
+  const int32_t* tactive = nullptr;
 };
 // mode 0: FP64 tangents; 1: FP32 elastic and viscous tangents apart;
 // 2: FP32 elastic + viscous summed per K block into Ke32, and the elastic +

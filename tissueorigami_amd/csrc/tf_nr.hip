@@ -141,6 +141,10 @@ struct tf_handle {
   // and last_dt its time step: tf_nr_element_outputs recomputes mixed tangents from them
   bool assembly_inputs_kept = false;
   double last_dt = 0.0;
+  // ws.strain holds the last Gauss point's strain of the last assembly of
+  // every prism; ptissue maps prisms to tissues in a batch (else null)
+  bool have_strain = false;
+  int32_t* ptissue = nullptr;
   Workspace ws;
   Launch L;
   Prof prof;
@@ -438,6 +442,14 @@ tf_status alloc_workspace(tf_handle* h) {
   TF_CHECK(dalloc(h, &w.elemKe, (size_t)h->Ne * 324));
   TF_CHECK(dalloc(h, &w.elemFe, (size_t)h->Ne * 18));
   TF_CHECK(dalloc(h, &w.elemFv, (size_t)h->Ne * 18));
+  TF_CHECK(dalloc(h, &w.strain, (size_t)h->Ne * 6));
+  if (h->n_tissues > 1) {
+    std::vector<int32_t> pt((size_t)h->Ne);
+    for (int T = 0; T < h->n_tissues; ++T)
+      for (int e = h->elem_off[T]; e < h->elem_off[T + 1]; ++e) pt[e] = T;
+    TF_CHECK(dalloc(h, &h->ptissue, (size_t)h->Ne));
+    TF_CHECK(hipMemcpy(h->ptissue, pt.data(), sizeof(int32_t) * pt.size(), hipMemcpyHostToDevice));
+  }
   TF_CHECK(dalloc(h, &w.val, (size_t)h->pat.n_slots * 9));
   TF_CHECK(dalloc(h, &w.minv, (size_t)h->Nn * 9));
   double** vecs[] = {&w.un, &w.uk, &w.g, &w.du, &w.internal, &w.r, &w.rh, &w.p, &w.v,
@@ -1052,7 +1064,10 @@ tf_status packing_detect(tf_handle* h, const double* u) {
 }
 
 // Element kernel + slot-map assembly + residual and binding of the residual.
-tf_status assemble(tf_handle* h, double dt, int mixed) {
+// in_step: an NR pass, whose strain output is written only for the tissues
+// flagged in ws.tactive (in a batch, a tissue's strain then stays that of its
+// own last assembly while the others iterate on)
+tf_status assemble(tf_handle* h, double dt, int mixed, bool in_step = false) {
   if (!h->have_material || !h->have_growth || !h->have_area) {
     set_error("tf_nr_set_material, tf_nr_set_growth and a viscosity surface must precede assembly");
     return TF_ERR_ARG;
@@ -1069,6 +1084,11 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
   ElementArgs ea{h->Ne, h->conn, h->refpos, h->invFg, h->lam, h->mu, h->eta, w.uk, w.un, dt,
                  h->any_visc ? 1 : 0, w.elemKe, w.elemKv, w.elemKe32, w.elemKv32, w.elemFe,
                  w.elemFv};
+  ea.strain = w.strain;
+  if (in_step) {
+    ea.ptissue = h->ptissue;
+    ea.tactive = w.tactive;
+  }
   TF_CHECK(hipEventRecord(h->ev[0], h->st));
   // mixed: FP32 elastic + viscous blocks summed per K block (one gather each)
   TF_CHECK(launch_element(ea, mixed ? 2 : 0, h->st));
@@ -1103,6 +1123,7 @@ tf_status assemble(tf_handle* h, double dt, int mixed) {
     TF_CHECK(launch_bound_gather(h->bound, h->Nn, h->pat.n_slots, w.val, w.val32, mixed, h->st));
   TF_CHECK(hipEventRecord(h->ev[2], h->st));
   h->assembled = true;
+  h->have_strain = true;
   h->last_mixed = mixed;
   h->last_dt = dt;
   h->assembly_inputs_kept = false;
@@ -1547,7 +1568,7 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     if (!any) break;
     TF_CHECK(hipMemcpyAsync(w.tactive, act.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
                             h->st));
-    tf_status s = assemble(h, dt, mixed);
+    tf_status s = assemble(h, dt, mixed, true);
     if (s != TF_OK) return s;
     const int method = h->krylov_method[mixed];
     if (method == KRYLOV_CG) {
@@ -1602,7 +1623,7 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
       TF_CHECK(hipMemcpyAsync(w.tactive, f.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice,
                               h->st));
       if (stage > 0 && !have_fp64) {
-        tf_status as = assemble(h, dt, 0);
+        tf_status as = assemble(h, dt, 0, true);
         if (as != TF_OK) return as;
         have_fp64 = true;
       }
@@ -2955,6 +2976,21 @@ tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke
       std::memset(Kv, 0, sizeof(double) * Ne * 324);
     }
   }
+  return TF_OK;
+}
+
+tf_status tf_nr_element_strain(tf_handle* h, double* strain) {
+  if (!h || !strain) {
+    set_error("tf_nr_element_strain: invalid arguments");
+    return TF_ERR_ARG;
+  }
+  if (!h->have_strain) {
+    set_error("tf_nr_element_strain needs a preceding assembly or step");
+    return TF_ERR_ARG;
+  }
+  TF_CHECK(hipStreamSynchronize(h->st));
+  TF_CHECK(hipMemcpy(strain, h->ws.strain, sizeof(double) * 6 * (size_t)h->Ne,
+                     hipMemcpyDeviceToHost));
   return TF_OK;
 }
 

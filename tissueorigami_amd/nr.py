@@ -21,7 +21,8 @@ MAX_NR = 32
 
 EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_visc_surface",
            "tf_nr_update_visc_surface", "tf_nr_update_visc_surface_noalloc", "tf_nr_set_ablation", "tf_nr_step", "tf_nr_assemble", "tf_nr_solve",
-           "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_profile",
+           "tf_csr_pattern", "tf_nr_get_csr", "tf_nr_element_outputs", "tf_nr_element_strain",
+           "tf_nr_profile",
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
            "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs", "tf_nr_set_krylov", "tf_nr_assemble_ex",
@@ -101,6 +102,7 @@ def load():
     L.tf_csr_pattern.argtypes = [_h, _i, _i, _i64]
     L.tf_nr_get_csr.argtypes = [_h, ctypes.c_double, _i, _i, _d, _i64]
     L.tf_nr_element_outputs.argtypes = [_h, _d, _d, _d, _d]
+    L.tf_nr_element_strain.argtypes = [_h, _d]
     L.tf_nr_profile.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_kernel_times.argtypes = [_h, ctypes.POINTER(KernelTimes)]
     L.tf_nr_pattern_info.argtypes = [_h, _i64, _i64]
@@ -580,6 +582,15 @@ class NewtonRaphsonSolver:
         _check(load().tf_nr_element_outputs(self._h, _p(ge, _d), _p(gv, _d), _p(Ke, _d),
                                             _p(Kv, _d)))
         return ge, gv, Ke, Kv
+
+    def element_strain(self):
+        """Green-Lagrange strain of each prism's last Gauss point at the
+        positions of the last assembly, (n_prisms, 6) in the Voigt order
+        E00, E11, E22, 2E01, 2E21, 2E02 of ShapeBase::Strain (Prism.cpp:826-837),
+        the rows Simulation::writeTensionCompression saves (Simulation.cpp:8521)."""
+        out = np.empty((self.n_prisms, 6))
+        _check(load().tf_nr_element_strain(self._h, _p(out, _d)))
+        return out
 
 
 class TissueBatch(NewtonRaphsonSolver):
