@@ -143,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
   // gathered concurrently
   constexpr int kCols = kBlock / kSlice;
   const int lane = threadIdx.x % kSlice, kq = threadIdx.x / kSlice;
-  for (int sl = blockIdx.x; sl < a.pat.n_slices; sl += gridDim.x) {
+  for (int sl = xcd_item(blockIdx.x, gridDim.x); sl < a.pat.n_slices; sl += gridDim.x) {
     const int R = sl * kSlice + lane;
     const int64_t sbase = a.pat.slice_base[sl];
     const int width = (int)((a.pat.slice_base[sl + 1] - sbase) / kSlice);
@@ -407,7 +407,7 @@ int grid_for(int64_t n) {
 }  // namespace
 
 hipError_t launch_assemble(const AssembleArgs& a, int mixed, hipStream_t st) {
-  const int g1 = (int)std::min<int64_t>(a.pat.n_slices, 65535 * 4);
+  const int g1 = xcd_grid(a.pat.n_slices);
   if (a.any_visc && !(mixed && a.combined)) {
     if (mixed) hipLaunchKernelGGL((assemble_kernel<true, true>), dim3(g1), dim3(kBlock), 0, st, a);
     else hipLaunchKernelGGL((assemble_kernel<true, false>), dim3(g1), dim3(kBlock), 0, st, a);
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void bound_gather_kernel(BoundOp b, int32_t
   constexpr int kCols = kBlock / kSlice;
   const int64_t nsB = b.pat.n_slots;
   const int lane = threadIdx.x % kSlice, kq = threadIdx.x / kSlice;
-  for (int sl = blockIdx.x; sl < b.pat.n_slices; sl += gridDim.x) {
+  for (int sl = xcd_item(blockIdx.x, gridDim.x); sl < b.pat.n_slices; sl += gridDim.x) {
     const int R = sl * kSlice + lane;
     const int64_t sbase = b.pat.slice_base[sl];
     const int width = (int)((b.pat.slice_base[sl + 1] - sbase) / kSlice);
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void bound_gather_kernel(BoundOp b, int32_t
 hipError_t launch_bound_gather(const BoundOp& b, int32_t n_nodes, int64_t n_slots_J,
                                const double* valJ, const float* valJ32, int mixed,
                                hipStream_t st) {
-  const int g1 = (int)std::min<int64_t>(b.pat.n_slices, 65535 * 4);
+  const int g1 = xcd_grid(b.pat.n_slices);
   if (mixed)
     hipLaunchKernelGGL(bound_gather_kernel<float>, dim3(g1), dim3(kBlock), 0, st, b, n_nodes,
                        n_slots_J, valJ32, b.val32, b.minv32);

@@ -472,11 +472,12 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
       if (e >= a.n_prisms) break;
       float K[6][9];
       mixed_row_blocks(sd, svd, el, ia, VISC, (float)w, K);
-      float* out = a.Ke32 + ((size_t)e * 36 + ia * 6) * 9;
 #pragma unroll
-      for (int b = 0; b < 6; ++b)
+      for (int b = 0; b < 6; ++b) {
+        float* out = a.Ke32 + ((size_t)e * 36 + b * 6 + ia) * 9;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) out[9 * b + q] = K[b][q];
+        for (int q = 0; q < 9; ++q) out[q] = K[b][q];
+      }
     }
   } else if (MIXED) {
     // ---------------- phase 2 (mixed): FP32 elastic and viscous blocks -----
@@ -486,14 +487,14 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
       const int e = e0 + el;
       if (e >= a.n_prisms) break;
       const int ia = ab / 6, ib = ab % 6;
-      const size_t base = ((size_t)e * 36 + ab) * 9;
+      const size_t base = ((size_t)e * 36 + ib * 6 + ia) * 9;
       float Ke[9], Kv[9];
       mixed_blocks(sd, svd, el, ia, ib, VISC, (float)w, Ke, Kv);
 #pragma unroll
       for (int q = 0; q < 9; ++q) a.Ke32[base + q] = Ke[q];
       if (VISC) {
         // reference storage: TriPointKv[3b+j][3a+i] -> block (b, a), entry (j, i)
-        const size_t bv = ((size_t)e * 36 + ib * 6 + ia) * 9;
+        const size_t bv = ((size_t)e * 36 + ia * 6 + ib) * 9;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
           K[3 * i + k] += v * tw;
         }
     }
-    const size_t base = ((size_t)e * 36 + ab) * 9;
+    const size_t base = ((size_t)e * 36 + ib * 6 + ia) * 9;
 #pragma unroll
     for (int q = 0; q < 9; ++q) a.Ke[base + q] = K[q];
   }
@@ -633,7 +634,7 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
         }
       }
       // reference storage: TriPointKv[3b+j][3a+i] -> block (b, a), entry (j, i)
-      const size_t base = ((size_t)e * 36 + ib * 6 + ia) * 9;
+      const size_t base = ((size_t)e * 36 + ia * 6 + ib) * 9;
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -967,11 +968,12 @@ __global__ __launch_bounds__(kMThreads) void element_kernel_mix(ElemConst c, Ele
       for (int r = 0; r < 3; ++r) K[b][4 * r] += diag;
     }
   }
-  float* out = a.Ke32 + ((size_t)e * 36 + ia * 6) * 9;
 #pragma unroll
-  for (int b = 0; b < 6; ++b)
+  for (int b = 0; b < 6; ++b) {
+    float* out = a.Ke32 + ((size_t)e * 36 + b * 6 + ia) * 9;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) out[9 * b + q] = K[b][q];
+    for (int q = 0; q < 9; ++q) out[q] = K[b][q];
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i) a.Fe[18 * e + 3 * ia + i] = 0.0 - fe[i];
 }

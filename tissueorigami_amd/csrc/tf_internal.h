@@ -28,7 +28,7 @@ struct DevPattern {
   int32_t* row_len = nullptr;     // n_nodes: stored blocks of each row
   int32_t* tslot = nullptr;       // n_slots: slot of the transposed block (C, R)
   int32_t* contrib_ptr = nullptr; // n_slots + 1
-  int32_t* contrib_src = nullptr; // n_prisms * 36: e*36 + A*6 + B (K block A,B)
+  int32_t* contrib_src = nullptr; // n_prisms * 36: e*36 + B*6 + A (K block A,B)
   int32_t* node_ptr = nullptr;    // n_nodes + 1
   int32_t* node_src = nullptr;    // n_prisms * 6: e*6 + a
   // contact slots of node-node packing (null when the handle does not pack):
@@ -174,7 +174,9 @@ struct TiledResult {
 
 struct Workspace {
   // per-prism outputs (block-major, reference K orientation):
-  // elemK[(e*36 + A*6 + B)*9 + r*3 + c] = TriPointKe[3A+r][3B+c]
+  // elemK[(e*36 + B*6 + A)*9 + r*3 + c] = TriPointKe[3A+r][3B+c]: the six
+  // blocks that row B of J gathers (J(R, C) = K(C, R)^T) are contiguous, and
+  // those of the three nodes of one face fill one half of the prism's record
   double* elemKe = nullptr;
   double* elemKv = nullptr;
   float* elemKe32 = nullptr;
@@ -232,6 +234,18 @@ constexpr int kGmresRestart = 100;
 enum KState { K_RUN = 0, K_CONVERGED = 1, K_BREAKDOWN = 2 };
 // A Krylov residual above kDivergence |b| ends the solve as a breakdown.
 constexpr double kDivergence = 1e8;
+
+// Work item of workgroup bid in a grid that is a multiple of 8.  The eight
+// XCDs take workgroups round-robin; this map gives each XCD a contiguous
+// eighth of the items, so neighbouring items (slices and tiles that share
+// prisms and gathered vector entries) meet in one XCD's L2.
+__device__ inline int xcd_item(int bid, int grid) { return (bid & 7) * (grid >> 3) + (bid >> 3); }
+
+// A grid of workgroups for n items, a multiple of 8 below the launch limit.
+inline int xcd_grid(int64_t n) {
+  const int64_t g = (n + 7) / 8 * 8;
+  return (int)(g < 8 ? 8 : (g > 262136 ? 262136 : g));
+}
 
 // ---- per-tissue scalars of the tiled solvers ----
 // Ordered sum of part[beg..end), identical in every thread of the workgroup.

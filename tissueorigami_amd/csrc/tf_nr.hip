@@ -384,7 +384,7 @@ tf_status layout_pattern(tf_handle* h, const int32_t* conn) {
       for (int B = 0; B < 6; ++B) {
         const int R = conn[6 * e + B], C = conn[6 * e + A];
         const int64_t slot = slot_of(R, C);
-        slot_cache[(size_t)e * 36 + A * 6 + B] = (int32_t)slot;
+        slot_cache[(size_t)e * 36 + B * 6 + A] = (int32_t)slot;
         cptr[slot + 1]++;
       }
   for (int64_t i = 0; i < total; ++i) cptr[i + 1] += cptr[i];
@@ -2962,7 +2962,7 @@ tf_status tf_nr_element_outputs(tf_handle* h, double* ge, double* gv, double* Ke
         for (int B = 0; B < 6; ++B)
           for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c)
-              out[e * 324 + (3 * A + r) * 18 + 3 * B + c] = blk[((e * 36) + A * 6 + B) * 9 + 3 * r + c];
+              out[e * 324 + (3 * A + r) * 18 + 3 * B + c] = blk[((e * 36) + B * 6 + A) * 9 + 3 * r + c];
   };
   if (Ke) {
     TF_CHECK(fetch(h->ws.elemKe, h->ws.elemKe32));

@@ -33,11 +33,6 @@ namespace {
 
 constexpr int kW = kBlock / 64;
 
-__device__ inline int xcd_tile(int bid, int grid) {
-  const int per = grid >> 3;
-  return (bid & 7) * per + (bid >> 3);
-}
-
 __device__ inline void reduce_store(double* vals, int n, double* const* outs, int slot) {
   __shared__ double red[4][kW];
   for (int k = 0; k < n; ++k) {
@@ -83,7 +78,7 @@ struct TileCtx {
 };
 
 __device__ inline bool tile_ctx(const TC& c, TileCtx& t) {
-  t.L = xcd_tile(blockIdx.x, gridDim.x);
+  t.L = xcd_item(blockIdx.x, gridDim.x);
   if (t.L >= c.tm.n_tiles) return false;
   t.T = c.tm.tile_tissue[t.L];
   t.s0 = c.tm.tile_slice[t.L];
