@@ -143,7 +143,8 @@ __global__ __launch_bounds__(kBlock) void assemble_kernel(AssembleArgs a) {
   // gathered concurrently
   constexpr int kCols = kBlock / kSlice;
   const int lane = threadIdx.x % kSlice, kq = threadIdx.x / kSlice;
-  for (int sl = xcd_item(blockIdx.x, gridDim.x); sl < a.pat.n_slices; sl += gridDim.x) {
+  for (int it = xcd_item(blockIdx.x, gridDim.x); it < a.pat.n_slices; it += gridDim.x) {
+    const int sl = a.pat.slice_order ? a.pat.slice_order[it] : it;
     const int R = sl * kSlice + lane;
     const int64_t sbase = a.pat.slice_base[sl];
     const int width = (int)((a.pat.slice_base[sl + 1] - sbase) / kSlice);

@@ -193,89 +193,6 @@ __device__ inline void mixed_blocks(const GPData* sd, const GPVisc* svd, int el,
   }
 }
 
-// The six K blocks (a, 0..5) of one prism for the mixed-precision operator,
-// in one pass: each block is the elastic block (a, b) plus the viscous
-// contribution that the reference's transposed storage of TriPointKv places
-// at block (a, b), the viscous block (b, a) of mixed_blocks transposed.  Per
-// Gauss point, with Na = dN_a, Nb = dN_b, f = vol w, S' = f S and
-// P' = (f eta / 2) P, entry (r, c) of the sum is
-//   Na_r U_c + Nb_r V_c + Na_c W_r + (Na.Nb) P'_cr + d_rc (cm w Na.Nb + vol w Na.sigma.Nb)
-// with U = cl w Nb + S' Nb, V = cm w Na - S' Na, W = P'^T Nb; V and sigma^T Na
-// depend on a alone and are formed once per point.  The assembly then
-// gathers one block per element instead of two.
-__device__ inline void mixed_row_blocks(const GPData* sd, const GPVisc* svd, int el, int ia,
-                                        bool visc, float w, float K[6][9]) {
-  // every fused multiply-add is explicit (contraction stays off), so each
-  // instantiation of the kernel rounds alike and a prism's blocks do not
-  // depend on the other prisms of the launch
-#pragma unroll
-  for (int b = 0; b < 6; ++b)
-#pragma unroll
-    for (int q = 0; q < 9; ++q) K[b][q] = 0.f;
-  const bool has_eta = visc && svd[el * 6].eta != 0;
-  auto dot3 = [](const float* x, int sx, const float* y) {
-    return fmaf(x[2 * sx], y[2], fmaf(x[sx], y[1], x[0] * y[0]));
-  };
-#pragma unroll 1
-  for (int g = 0; g < 6; ++g) {
-    const GPData& d = sd[el * 6 + g];
-    float Na[3], sg[9];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) Na[j] = (float)d.dN[ia][j];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) sg[q] = (float)d.sig[q];
-    const float vol = (float)(d.detF * d.detdXde);
-    const float clw = (float)d.cl * w, cmw = (float)d.cm * w, volw = vol * w;
-    float sTNa[3], V[3], Sp[9], Pp[9];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      sTNa[j] = dot3(sg + j, 3, Na);  // (sigma^T Na)_j
-      V[j] = cmw * Na[j];
-    }
-    if (has_eta) {
-      const GPVisc& dv = svd[el * 6 + g];
-      const float fe2 = 0.5f * volw * (float)dv.eta;
-#pragma unroll
-      for (int q = 0; q < 9; ++q) { Sp[q] = volw * (float)dv.sv[q]; Pp[q] = fe2 * (float)dv.P[q]; }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) V[c] = V[c] - dot3(Sp + 3 * c, 1, Na);
-    }
-#pragma unroll
-    for (int b = 0; b < 6; ++b) {
-      float Nb[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) Nb[j] = (float)d.dN[b][j];
-      const float dot = dot3(Na, 1, Nb);
-      const float diag = fmaf(volw, dot3(sTNa, 1, Nb), cmw * dot);
-      float U[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) U[c] = clw * Nb[c];
-      if (has_eta) {
-        float W[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          U[c] = U[c] + dot3(Sp + 3 * c, 1, Nb);
-          W[c] = dot3(Pp + c, 3, Nb);  // (P'^T Nb)_c
-        }
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            K[b][3 * r + c] = fmaf(Na[r], U[c], fmaf(Nb[r], V[c],
-                              fmaf(Na[c], W[r], fmaf(dot, Pp[3 * c + r], K[b][3 * r + c]))));
-      } else {
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            K[b][3 * r + c] = fmaf(Na[r], U[c], fmaf(Nb[r], V[c], K[b][3 * r + c]));
-      }
-#pragma unroll
-      for (int r = 0; r < 3; ++r) K[b][4 * r] = K[b][4 * r] + diag;
-    }
-  }
-}
-
 // Green-Lagrange strain E = (C - I) / 2 of Prism::calculateCurrNodalForces
 // (Prism.cpp:826-837, ShapeBase.cpp:2459-2472), from C = Fe^T Fe: the
 // reference keeps the value of the last Gauss point it evaluates, in the Voigt
@@ -292,8 +209,8 @@ __device__ inline void store_strain(const ElementArgs& a, int e, const double C[
 }
 
 // MODE 0: FP64 tangents (oracle operation order); 1: FP32 elastic and
-// viscous tangents stored apart (Ke32, Kv32, for element-level parity);
-// 2: FP32 blocks of both summed per K block into Ke32 (mixed-precision NR).
+// viscous tangents stored apart (Ke32, Kv32, for element-level parity).
+// The mixed-precision NR step uses element_kernel_mix.
 template <bool VISC, int MODE>
 __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementArgs a) {
   constexpr bool MIXED = MODE != 0;
@@ -464,22 +381,7 @@ __global__ __launch_bounds__(kThreads) void element_kernel(ElemConst c, ElementA
   __syncthreads();
 
   const double w = c.w;
-  if (MODE == 2) {
-    // ---------------- phase 2 (mixed, summed): one thread per (prism, a) ----
-    for (int task = tid; task < kGP; task += kThreads) {
-      const int el = task / 6, ia = task % 6;
-      const int e = e0 + el;
-      if (e >= a.n_prisms) break;
-      float K[6][9];
-      mixed_row_blocks(sd, svd, el, ia, VISC, (float)w, K);
-#pragma unroll
-      for (int b = 0; b < 6; ++b) {
-        float* out = a.Ke32 + ((size_t)e * 36 + b * 6 + ia) * 9;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) out[q] = K[b][q];
-      }
-    }
-  } else if (MIXED) {
+  if (MIXED) {
     // ---------------- phase 2 (mixed): FP32 elastic and viscous blocks -----
     for (int task = tid; task < kEPB * 36; task += kThreads) {
       const int el = task / 36;

@@ -29,6 +29,11 @@ struct DevPattern {
   int32_t* tslot = nullptr;       // n_slots: slot of the transposed block (C, R)
   int32_t* contrib_ptr = nullptr; // n_slots + 1
   int32_t* contrib_src = nullptr; // n_prisms * 36: e*36 + B*6 + A (K block A,B)
+  // n_slices: the order in which the assembly visits slices (Morton order of
+  // the slices' reference (x, y) centroids within each tissue), so that the
+  // slices of one column of prisms, which gather from the same prisms' tangent
+  // records, are assembled close together in time
+  int32_t* slice_order = nullptr;
   int32_t* node_ptr = nullptr;    // n_nodes + 1
   int32_t* node_src = nullptr;    // n_prisms * 6: e*6 + a
   // contact slots of node-node packing (null when the handle does not pack):

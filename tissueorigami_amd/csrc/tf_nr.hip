@@ -822,6 +822,57 @@ tf_status build_bound(tf_handle* h) {
 
 // Common tail of tf_nr_create and tf_batch_create: stream, pattern, slot map,
 // device buffers, tiles.  conn holds internal node ids.  Frees h on failure.
+// Interleaves the bits of two 16-bit coordinates (Z order).
+uint32_t morton2(uint32_t x, uint32_t y) {
+  auto spread = [](uint32_t v) {
+    v &= 0xffff;
+    v = (v | (v << 8)) & 0x00ff00ff;
+    v = (v | (v << 4)) & 0x0f0f0f0f;
+    v = (v | (v << 2)) & 0x33333333;
+    v = (v | (v << 1)) & 0x55555555;
+    return v;
+  };
+  return spread(x) | (spread(y) << 1);
+}
+
+// DevPattern::slice_order from the nodes' reference positions (h_X): within
+// each tissue, slices by the Morton code of their (x, y) centroid on a
+// 2^16 x 2^16 grid over the tissue's extent, ties by slice index.
+tf_status slice_order(tf_handle* h) {
+  const int ns = h->pat.n_slices;
+  std::vector<uint64_t> key(ns);
+  for (int T = 0; T < h->n_tissues; ++T) {
+    const int r0 = h->int_off[T], r1 = h->int_off[T] + (h->user_off[T + 1] - h->user_off[T]);
+    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+    for (int i = r0; i < r1; ++i)
+      for (int k = 0; k < 2; ++k) {
+        lo[k] = std::min(lo[k], h->h_X[3 * (size_t)i + k]);
+        hi[k] = std::max(hi[k], h->h_X[3 * (size_t)i + k]);
+      }
+    for (int sl = h->int_off[T] / kSlice; sl < (h->int_off[T + 1] + kSlice - 1) / kSlice; ++sl) {
+      double c[2] = {0.0, 0.0};
+      int n = 0;
+      for (int i = sl * kSlice; i < std::min((sl + 1) * kSlice, r1); ++i, ++n)
+        for (int k = 0; k < 2; ++k) c[k] += h->h_X[3 * (size_t)i + k];
+      uint32_t q[2] = {0, 0};
+      for (int k = 0; k < 2 && n > 0; ++k) {
+        const double ext = hi[k] - lo[k];
+        const double f = ext > 0.0 ? (c[k] / n - lo[k]) / ext : 0.0;
+        q[k] = (uint32_t)std::min(65535.0, std::max(0.0, f * 65535.0));
+      }
+      key[sl] = ((uint64_t)T << 32) | morton2(q[0], q[1]);
+    }
+  }
+  std::vector<int32_t> order(ns);
+  for (int i = 0; i < ns; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return key[a] < key[b]; });
+  TF_CHECK(dalloc(h, &h->pat.slice_order, ns));
+  TF_CHECK(hipMemcpy(h->pat.slice_order, order.data(), sizeof(int32_t) * ns,
+                     hipMemcpyHostToDevice));
+  return TF_OK;
+}
+
 tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos, tf_handle** out) {
   tf_status s = TF_OK;
   auto fail = [&](tf_status st) {
@@ -871,6 +922,7 @@ tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos
         for (int k = 0; k < 3; ++k) h->h_X[3 * (size_t)nd + k] = ref_pos[18 * e + 3 * a + k];
       }
   }
+  if ((s = slice_order(h)) != TF_OK) return fail(s);
   if ((s = alloc_tissue_state(h)) != TF_OK) return fail(s);
   if ((s = build_tiles(h, 8)) != TF_OK) return fail(s);
   *out = h;
