@@ -700,6 +700,10 @@ hipError_t refactor(CoarseOp& c, const std::vector<int>& segs, rocblas_handle hb
   return hipSuccess;
 }
 
+// Largest residual max |I - A_c X| from which one Newton-Schulz step is taken
+// without checking the result.
+constexpr double kNsAccept = 1e-3;
+
 // The inverse of the A_c of every listed segment: Newton-Schulz steps from
 // its X_ref, which is refactorised when they do not converge; ok[T] when the
 // segment's inverse is usable.  The segments step together, with one
@@ -762,15 +766,18 @@ hipError_t factor_segments(CoarseOp& c, const std::vector<int>& segs, rocblas_ha
         double r = 0.0;
         for (int q = 0; q < nc; ++q) r = std::max(r, cm[c.seg_poff[T] + q]);
         double* X = cur[T] ? S.X2 : S.X;
-        if (r < 1e-5) {
+        auto accept = [&](const double* Xa) {
           ok[T] = 1;
           hipLaunchKernelGGL(coarse_convert_kernel<float>, dim3((nc + 31) / 32, (nc + 31) / 32),
-                             dim3(kBlock), 0, st, nc, c.seg_ld[T], X, c.inv32 + c.seg_ioff[T]);
+                             dim3(kBlock), 0, st, nc, c.seg_ld[T], Xa, c.inv32 + c.seg_ioff[T]);
           if (c.want64)
             hipLaunchKernelGGL(coarse_convert_kernel<double>,
                                dim3((nc + 31) / 32, (nc + 31) / 32), dim3(kBlock), 0, st, nc,
-                               c.seg_ld[T], X, c.inv64 + c.seg_ioff[T]);
-          if ((e = hipGetLastError()) != hipSuccess) return e;
+                               c.seg_ld[T], Xa, c.inv64 + c.seg_ioff[T]);
+          return hipGetLastError();
+        };
+        if (r < 1e-5) {
+          if ((e = accept(X)) != hipSuccess) return e;
           continue;
         }
         if (!(r < 0.25) || it == 6) continue;  // too far from X_ref (or NaN)
@@ -780,6 +787,15 @@ hipError_t factor_segments(CoarseOp& c, const std::vector<int>& segs, rocblas_ha
                           nc, S.T, nc, &zero, cur[T] ? S.X : S.X2, nc) != rocblas_status_success)
           return hipErrorUnknown;
         cur[T] ^= 1;
+        // the step squares the residual, I - A_c X' = (I - A_c X)^2, so from
+        // max |I - A_c X| = r every new entry is below nc r^2; from r below
+        // kNsAccept the step is accepted without a residual product (a
+        // preconditioner needs no more, and the step stays a function of
+        // A_c and X_ref alone)
+        if (r < kNsAccept) {
+          if ((e = accept(cur[T] ? S.X2 : S.X)) != hipSuccess) return e;
+          continue;
+        }
         next.push_back(T);
       }
       run.swap(next);
