@@ -24,7 +24,7 @@
 // preconditioner needs no more).  When the steps do not converge (residual
 // above 0.25, or seven steps), X_ref is refactorised from the current A_c.
 // The inverse is kept as a dense FP32 matrix.  Per CG iteration: w = P^T r
-// (kSplit workgroups per aggregate), y = A_c^-1 w (one wavefront per row) and
+// (one workgroup per aggregate), y = A_c^-1 w (one wavefront per row) and
 // the partial products (w, y) that complete (r, z); the CG's search-direction
 // update adds P y (tzc_pupdate).  Every sum has a fixed order and rocBLAS runs
 // with atomics off, so the solve stays bitwise reproducible.
@@ -44,7 +44,6 @@ namespace {
 constexpr int kMinAggNodes = 16;   // smaller cells are merged into the nearest larger one
 constexpr int kMaxAggregates = 8192;  // per segment
 constexpr int kGemvRows = kBlock / 64;  // rows of A_c^-1 per product workgroup
-constexpr int kSplit = 4;               // workgroups per aggregate in the restriction
 
 template <typename T>
 hipError_t upload(CoarseOp& c, T** p, const std::vector<T>& v, size_t count = 0) {
@@ -218,45 +217,86 @@ __global__ __launch_bounds__(kBlock) void coarse_twoI_kernel(int32_t n_c, double
 
 
 
-// w = P^T r as kSplit partial vectors (w_s of stride ld, summed in order by
-// the product kernel): kSplit workgroups per aggregate, each over a fixed
-// part of the aggregate's node list; the offsets and masks are stored in list
-// order, only r is gathered.
+// max |V - W| of every probe column (one workgroup per column), where W =
+// A_c X_ref V: the probe estimate of how far X_ref is from the inverse of the
+// current A_c.  V has entries +-1, so a row i contributes |sum_j R_ij v_j|,
+// about the 2-norm of row i of R = I - A_c X_ref, never less than its largest
+// entry on average.
+__global__ __launch_bounds__(kBlock) void coarse_probe_kernel(int32_t n_c, const double* V,
+                                                              const double* W, double* out) {
+  __shared__ double red[kBlock / 64];
+  const int j = blockIdx.x;
+  double m = 0.0;
+  for (int r = threadIdx.x; r < n_c; r += kBlock) {
+    const double v = fabs(V[(size_t)j * n_c + r] - W[(size_t)j * n_c + r]);
+    m = fmax(m, v == v ? v : 1e300);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+#pragma unroll
+    for (int k = 1; k < kBlock / 64; ++k) t = fmax(t, red[k]);
+    out[j] = t;
+  }
+}
+
+// w = P^T r: one workgroup per aggregate over its node list.  The offsets and
+// masks are stored in list order, only r is gathered; every thread has kRU
+// list entries in flight (indices first, then offsets, masks and the gathered
+// r) before it adds any of them, in list order.
+constexpr int kRU = 4;
 template <typename R>
 __global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
-    int32_t n_agg, int32_t vlen, const int32_t* __restrict__ agg_ptr,
-    const int32_t* __restrict__ agg_nodes, const int32_t* __restrict__ agg_tissue,
-    const int32_t* __restrict__ agg_cbase, const int32_t* __restrict__ seg_on,
-    const float* __restrict__ list_d, const uint8_t* __restrict__ list_mask,
-    const R* __restrict__ r, const int32_t* __restrict__ state, R* __restrict__ w) {
+    int32_t n_agg, const int32_t* __restrict__ agg_ptr, const int32_t* __restrict__ agg_nodes,
+    const int32_t* __restrict__ agg_tissue, const int32_t* __restrict__ agg_cbase,
+    const int32_t* __restrict__ seg_on, const float* __restrict__ list_d,
+    const uint8_t* __restrict__ list_mask, const R* __restrict__ r,
+    const int32_t* __restrict__ state, R* __restrict__ w) {
   __shared__ double red[6][kBlock / 64];
-  const int a = blockIdx.x / kSplit, part = blockIdx.x % kSplit;
+  const int a = blockIdx.x;
   if (a >= n_agg) return;
   const int T = agg_tissue[a];
   if (state[2 * T] != K_RUN || !seg_on[T]) return;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const int a0 = agg_ptr[a], len = agg_ptr[a + 1] - a0;
-  const int beg = a0 + (int)((int64_t)len * part / kSplit);
-  const int end = a0 + (int)((int64_t)len * (part + 1) / kSplit);
-  for (int t = beg + threadIdx.x; t < end; t += kBlock) {
-    const int i = agg_nodes[t];
-    const uint8_t mk = list_mask[t];
-    const double d0 = list_d[3 * t], d1 = list_d[3 * t + 1], d2 = list_d[3 * t + 2];
-    const double r0 = (mk & 1) ? 0.0 : (double)r[3 * i];
-    const double r1 = (mk & 2) ? 0.0 : (double)r[3 * i + 1];
-    const double r2 = (mk & 4) ? 0.0 : (double)r[3 * i + 2];
-    acc[0] += r0;
-    acc[1] += r1;
-    acc[2] += r2;
-    // rotation rows of P^T: d x r
-    acc[3] += d1 * r2 - d2 * r1;
-    acc[4] += d2 * r0 - d0 * r2;
-    acc[5] += d0 * r1 - d1 * r0;
+  const int beg = agg_ptr[a], end = agg_ptr[a + 1];
+  for (int t0 = beg + threadIdx.x; t0 < end; t0 += kRU * kBlock) {
+    int idx[kRU];
+#pragma unroll
+    for (int u = 0; u < kRU; ++u) {
+      const int t = t0 + u * kBlock;
+      idx[u] = t < end ? agg_nodes[t] : -1;
+    }
+    double d[kRU][3], rv[kRU][3];
+#pragma unroll
+    for (int u = 0; u < kRU; ++u) {
+      const int t = t0 + u * kBlock;
+      const uint8_t mk = idx[u] >= 0 ? list_mask[t] : (uint8_t)7;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        d[u][k] = idx[u] >= 0 ? (double)list_d[3 * t + k] : 0.0;
+        rv[u][k] = (mk >> k) & 1 ? 0.0 : (double)r[3 * idx[u] + k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kRU; ++u) {
+      const double r0 = rv[u][0], r1 = rv[u][1], r2 = rv[u][2];
+      const double d0 = d[u][0], d1 = d[u][1], d2 = d[u][2];
+      acc[0] += r0;
+      acc[1] += r1;
+      acc[2] += r2;
+      // rotation rows of P^T: d x r
+      acc[3] += d1 * r2 - d2 * r1;
+      acc[4] += d2 * r0 - d0 * r2;
+      acc[5] += d0 * r1 - d1 * r0;
+    }
   }
   block_sum<6>(acc, red);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) w[(size_t)part * vlen + agg_cbase[a] + q] = (R)acc[q];
+    for (int q = 0; q < 6; ++q) w[agg_cbase[a] + q] = (R)acc[q];
 }
 
 // Fixed / slave bits of every listed node (list order), from the DOF flags.
@@ -276,12 +316,37 @@ __global__ __launch_bounds__(kBlock) void coarse_mask_kernel(int32_t n_list,
   }
 }
 
+// Per node: the first coarse DOF of its aggregate, shifted left by three,
+// with bit p set when DOF p takes no part in the coarse space (fixed or
+// slave); -1 for a node without an aggregate.  The search-direction update
+// reads this one word instead of the aggregate, its base and the DOF flags.
+__global__ __launch_bounds__(kBlock) void coarse_nodecb_kernel(int32_t n_nodes,
+                                                               const int32_t* __restrict__ node_agg,
+                                                               const int32_t* __restrict__ agg_cbase,
+                                                               const uint8_t* __restrict__ flags,
+                                                               int32_t* __restrict__ node_cb) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n_nodes; i += gridDim.x * kBlock) {
+    const int a = node_agg[i];
+    int32_t cb = -1;
+    if (a >= 0) {
+      int32_t m = 0;
+      if (flags) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+          if (flags[3 * i + p] & kCoarseExcluded) m |= 1 << p;
+      }
+      cb = (agg_cbase[a] << 3) | m;
+    }
+    node_cb[i] = cb;
+  }
+}
+
 // y = A_c^-1 w, one wavefront per row (16-byte loads over the padded row
 // stride ld); partial (w, y) per workgroup.
 typedef float v4f __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
     const int32_t* __restrict__ gw, const int32_t* __restrict__ sgi,
-    const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on, int32_t vlen,
+    const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on,
     const float* __restrict__ inv32, const float* __restrict__ w,
     const int32_t* __restrict__ state, float* __restrict__ y, double* __restrict__ cpart) {
   __shared__ double wy[kGemvRows];
@@ -296,19 +361,14 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
     const float* ws = w + off;
     float s = 0.f;
     for (int j = lane; j < ld / 4; j += 64) {
-      v4f v = ((const v4f*)ws)[j];
-#pragma unroll
-      for (int p = 1; p < kSplit; ++p) v += ((const v4f*)(ws + (size_t)p * vlen))[j];
+      const v4f v = ((const v4f*)ws)[j];
       const v4f u = a[j];
       s += u.x * v.x + u.y * v.y + u.z * v.z + u.w * v.w;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) y[off + row] = s;
-    float wr = ws[row];
-#pragma unroll
-    for (int p = 1; p < kSplit; ++p) wr += ws[(size_t)p * vlen + row];
-    dot = (double)wr * (double)s;
+    dot = (double)ws[row] * (double)s;
   }
   if (lane == 0) wy[wave] = dot;
   __syncthreads();
@@ -324,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
 typedef double v2d __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(kBlock) void coarse_gemv64_kernel(
     const int32_t* __restrict__ gw, const int32_t* __restrict__ sgi,
-    const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on, int32_t vlen,
+    const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on,
     const double* __restrict__ inv64, const double* __restrict__ w,
     const int32_t* __restrict__ state, double* __restrict__ y, double* __restrict__ cpart) {
   __shared__ double wy[kGemvRows];
@@ -339,19 +399,14 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv64_kernel(
     const double* ws = w + off;
     double s = 0.0;
     for (int j = lane; j < ld / 2; j += 64) {
-      v2d v = ((const v2d*)ws)[j];
-#pragma unroll
-      for (int p = 1; p < kSplit; ++p) v += ((const v2d*)(ws + (size_t)p * vlen))[j];
+      const v2d v = ((const v2d*)ws)[j];
       const v2d u = a[j];
       s += u.x * v.x + u.y * v.y;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) y[off + row] = s;
-    double wr = ws[row];
-#pragma unroll
-    for (int p = 1; p < kSplit; ++p) wr += ws[(size_t)p * vlen + row];
-    dot = wr * s;
+    dot = ws[row] * s;
   }
   if (lane == 0) wy[wave] = dot;
   __syncthreads();
@@ -597,6 +652,7 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
   for (int t = 0; t < n_list; ++t)
     for (int k = 0; k < 3; ++k) ldv[3 * (size_t)t + k] = dn[3 * (size_t)anodes[t] + k];
   c.n_agg = n_agg;
+  c.n_nodes = n_nodes;
   c.n_c = 6 * n_agg;
   c.n_cpairs = n_pairs;
   c.n_gemv = (int32_t)(gw.size() / 2);
@@ -604,6 +660,7 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
   c.agg_total = n_list;
   hipError_t e;
   if ((e = upload(c, &c.node_agg, agg)) != hipSuccess) return e;
+  if ((e = upload(c, &c.node_cb, agg)) != hipSuccess) return e;
   if ((e = upload(c, &c.node_d, dn)) != hipSuccess) return e;
   if ((e = upload(c, &c.agg_ptr, aptr)) != hipSuccess) return e;
   if ((e = upload(c, &c.agg_nodes, anodes)) != hipSuccess) return e;
@@ -629,11 +686,29 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
   if ((e = upload(c, &c.X2, std::vector<double>(), (size_t)moff)) != hipSuccess) return e;
   if ((e = upload(c, &c.T, std::vector<double>(), (size_t)moff)) != hipSuccess) return e;
   c.colmax_len = std::max(poff, 1);
+  {
+    // probe block V of every segment: kCoarseProbes columns of +-1 entries
+    // from a fixed hash of (row, column), the same for every build
+    std::vector<double> pv((size_t)kCoarseProbes * c.colmax_len, 0.0);
+    for (int T = 0; T < n_seg; ++T)
+      for (int j = 0; j < kCoarseProbes; ++j)
+        for (int r = 0; r < c.seg_nc[T]; ++r) {
+          const uint32_t hsh = (uint32_t)r * 2654435761u ^ (uint32_t)(j + 1) * 40503u;
+          pv[(size_t)kCoarseProbes * c.seg_poff[T] + (size_t)j * c.seg_nc[T] + r] =
+              ((hsh >> 15) & 1u) ? 1.0 : -1.0;
+        }
+    if ((e = upload(c, &c.probe_v, pv)) != hipSuccess) return e;
+    if ((e = upload(c, &c.probe_u, std::vector<double>(), pv.size())) != hipSuccess) return e;
+    if ((e = upload(c, &c.probe_w, std::vector<double>(), pv.size())) != hipSuccess) return e;
+    if ((e = upload(c, &c.probe_max, std::vector<double>((size_t)kCoarseProbes * n_seg, 0.0))) !=
+        hipSuccess)
+      return e;
+  }
   if ((e = upload(c, &c.colmax, std::vector<double>((size_t)c.colmax_len, 0.0))) != hipSuccess) return e;
   if ((e = upload(c, &c.ipiv, std::vector<int32_t>(), (size_t)std::max(poff, 1))) != hipSuccess) return e;
   if ((e = upload(c, &c.info, std::vector<int32_t>(), 2 * (size_t)n_seg)) != hipSuccess) return e;
   if ((e = upload(c, &c.inv32, std::vector<float>((size_t)std::max<int64_t>(ioff, 1), 0.f))) != hipSuccess) return e;
-  if ((e = upload(c, &c.w, std::vector<float>(kSplit * (size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
+  if ((e = upload(c, &c.w, std::vector<float>((size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
   if ((e = upload(c, &c.y, std::vector<float>((size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
   if ((e = upload(c, &c.cpart, std::vector<double>((size_t)std::max(c.n_gemv, 1), 0.0))) != hipSuccess) return e;
   c.built = true;
@@ -703,6 +778,64 @@ hipError_t refactor(CoarseOp& c, const std::vector<int>& segs, rocblas_handle hb
 // Largest residual max |I - A_c X| from which one Newton-Schulz step is taken
 // without checking the result.
 constexpr double kNsAccept = 1e-3;
+// Largest probe estimate of |I - A_c X_ref| (largest row of the probe
+// residual) at which X_ref is used as it is, with no Newton-Schulz step.
+// X_ref is the exact inverse of an earlier A_c, so it is symmetric positive
+// definite and the preconditioned CG stays valid; a coarse inverse within a
+// few per cent of the current one changes the CG iteration count little.
+constexpr double kProbeAccept = 2e-2;
+
+// FP32 (and with want64 FP64) row-major copies of the segment's inverse Xa.
+hipError_t accept_inverse(CoarseOp& c, int T, const double* Xa, hipStream_t st) {
+  const int nc = c.seg_nc[T];
+  hipLaunchKernelGGL(coarse_convert_kernel<float>, dim3((nc + 31) / 32, (nc + 31) / 32),
+                     dim3(kBlock), 0, st, nc, c.seg_ld[T], Xa, c.inv32 + c.seg_ioff[T]);
+  if (c.want64)
+    hipLaunchKernelGGL(coarse_convert_kernel<double>, dim3((nc + 31) / 32, (nc + 31) / 32),
+                       dim3(kBlock), 0, st, nc, c.seg_ld[T], Xa, c.inv64 + c.seg_ioff[T]);
+  return hipGetLastError();
+}
+
+// Every listed segment that holds a reference inverse: W = A_c (X_ref V) for
+// its probe block V and the largest |V - W| per probe column; one
+// synchronisation for all.  probe[T] is the largest over the columns (1e300
+// for a segment without a reference inverse).
+hipError_t probe_segments(CoarseOp& c, const std::vector<int>& segs, rocblas_handle hb,
+                          hipStream_t st, std::vector<double>& probe) {
+  probe.assign(c.n_seg, 1e300);
+  const double one = 1.0, zero = 0.0;
+  bool any = false;
+  for (int T : segs) {
+    if (!c.seg_have_inv[T]) continue;
+    const SegMats S = seg_mats(c, T);
+    const int nc = S.nc;
+    const size_t po = (size_t)kCoarseProbes * c.seg_poff[T];
+    if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, nc, kCoarseProbes, nc,
+                      &one, S.Xref, nc, c.probe_v + po, nc, &zero, c.probe_u + po, nc) !=
+            rocblas_status_success ||
+        rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, nc, kCoarseProbes, nc,
+                      &one, S.Ac, nc, c.probe_u + po, nc, &zero, c.probe_w + po, nc) !=
+            rocblas_status_success)
+      return hipErrorUnknown;
+    hipLaunchKernelGGL(coarse_probe_kernel, dim3(kCoarseProbes), dim3(kBlock), 0, st, nc,
+                       c.probe_v + po, c.probe_w + po, c.probe_max + kCoarseProbes * T);
+    any = true;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !any) return e;
+  std::vector<double> pm((size_t)kCoarseProbes * c.n_seg);
+  if ((e = hipMemcpyAsync(pm.data(), c.probe_max, sizeof(double) * pm.size(),
+                          hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  for (int T : segs) {
+    if (!c.seg_have_inv[T]) continue;
+    double m = 0.0;
+    for (int j = 0; j < kCoarseProbes; ++j) m = std::max(m, pm[(size_t)kCoarseProbes * T + j]);
+    probe[T] = m;
+  }
+  return hipSuccess;
+}
 
 // The inverse of the A_c of every listed segment: Newton-Schulz steps from
 // its X_ref, which is refactorised when they do not converge; ok[T] when the
@@ -719,6 +852,18 @@ hipError_t factor_segments(CoarseOp& c, const std::vector<int>& segs, rocblas_ha
                        nc, c.Ac + c.seg_moff[T]);
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  // a reference inverse still close to the inverse of the current A_c is
+  // used as it is (no Newton-Schulz GEMM)
+  {
+    std::vector<double> probe;
+    if ((e = probe_segments(c, segs, hb, st, probe)) != hipSuccess) return e;
+    for (int T : segs)
+      if (probe[T] < kProbeAccept) {
+        if ((e = accept_inverse(c, T, c.Xref + c.seg_moff[T], st)) != hipSuccess) return e;
+        ok[T] = 1;
+        ++c.probe_accepts;
+      }
+  }
   // X from the reference inverse by Newton-Schulz steps against the current
   // A_c.  The result depends only on (A_c, X_ref), never on earlier solves,
   // so a repeated step repeats its bits; X_ref is refactorised (rocSOLVER)
@@ -768,13 +913,7 @@ hipError_t factor_segments(CoarseOp& c, const std::vector<int>& segs, rocblas_ha
         double* X = cur[T] ? S.X2 : S.X;
         auto accept = [&](const double* Xa) {
           ok[T] = 1;
-          hipLaunchKernelGGL(coarse_convert_kernel<float>, dim3((nc + 31) / 32, (nc + 31) / 32),
-                             dim3(kBlock), 0, st, nc, c.seg_ld[T], Xa, c.inv32 + c.seg_ioff[T]);
-          if (c.want64)
-            hipLaunchKernelGGL(coarse_convert_kernel<double>,
-                               dim3((nc + 31) / 32, (nc + 31) / 32), dim3(kBlock), 0, st, nc,
-                               c.seg_ld[T], Xa, c.inv64 + c.seg_ioff[T]);
-          return hipGetLastError();
+          return accept_inverse(c, T, Xa, st);
         };
         if (r < 1e-5) {
           if ((e = accept(X)) != hipSuccess) return e;
@@ -852,6 +991,8 @@ hipError_t coarse_factor_impl(CoarseOp& c, const int32_t* col, const V* val32, i
     return e;
   hipLaunchKernelGGL(coarse_mask_kernel, dim3(grid_nodes(std::max(c.agg_total, 1))), dim3(kBlock),
                      0, st, c.agg_total, c.agg_nodes, dof_flags, c.list_mask);
+  hipLaunchKernelGGL(coarse_nodecb_kernel, dim3(grid_nodes(std::max(c.n_nodes, 1))), dim3(kBlock),
+                     0, st, c.n_nodes, c.node_agg, c.agg_cbase, dof_flags, c.node_cb);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // the activity flags are pageable host memory: wait for their copy
   return hipStreamSynchronize(st);
@@ -872,7 +1013,7 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const double* val64, i
     const int64_t n_inv = std::max<int64_t>(c.seg_ioff[last] + (int64_t)c.seg_nc[last] * c.seg_ld[last], 1);
     hipError_t e;
     if ((e = upload(c, &c.inv64, std::vector<double>((size_t)n_inv, 0.0))) != hipSuccess) return e;
-    if ((e = upload(c, &c.w64, std::vector<double>(kSplit * (size_t)std::max(c.vlen, 1), 0.0))) != hipSuccess) return e;
+    if ((e = upload(c, &c.w64, std::vector<double>((size_t)std::max(c.vlen, 1), 0.0))) != hipSuccess) return e;
     if ((e = upload(c, &c.y64, std::vector<double>((size_t)std::max(c.vlen, 1), 0.0))) != hipSuccess) return e;
   }
   c.want64 = 1;
@@ -882,19 +1023,19 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const double* val64, i
 }
 
 void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, hipStream_t st) {
-  hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.n_agg * kSplit), dim3(kBlock), 0, st,
-                     c.n_agg, c.vlen, c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on,
-                     c.list_d, c.list_mask, r, state, c.w);
+  hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.n_agg), dim3(kBlock), 0, st, c.n_agg,
+                     c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on, c.list_d,
+                     c.list_mask, r, state, c.w);
   hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
-                     c.sgio, c.seg_on, c.vlen, c.inv32, c.w, state, c.y, c.cpart);
+                     c.sgio, c.seg_on, c.inv32, c.w, state, c.y, c.cpart);
 }
 
 void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, hipStream_t st) {
-  hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.n_agg * kSplit), dim3(kBlock), 0, st,
-                     c.n_agg, c.vlen, c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on,
-                     c.list_d, c.list_mask, r, state, c.w64);
+  hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.n_agg), dim3(kBlock), 0, st, c.n_agg,
+                     c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on, c.list_d,
+                     c.list_mask, r, state, c.w64);
   hipLaunchKernelGGL(coarse_gemv64_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
-                     c.sgio, c.seg_on, c.vlen, c.inv64, c.w64, state, c.y64, c.cpart);
+                     c.sgio, c.seg_on, c.inv64, c.w64, state, c.y64, c.cpart);
 }
 
 }  // namespace tf

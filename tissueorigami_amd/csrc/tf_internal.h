@@ -86,6 +86,9 @@ struct BoundOp {
 // started from a reference inverse (rocSOLVER getrf / getri, refactorised
 // only when the steps do not converge); it is applied as a dense FP32
 // product.
+// Probe columns of the check of a reference coarse inverse against the
+// current coarse operator.
+constexpr int kCoarseProbes = 4;
 struct CoarseOp {
   bool built = false;    // aggregates and block lists match the operator pattern
   bool for_bound = false;
@@ -105,7 +108,9 @@ struct CoarseOp {
   std::vector<int32_t> seg_poff;  // offset of the segment's pivots
   std::vector<char> seg_have_inv;  // Xref holds an inverse of the segment
   std::vector<char> seg_active;    // the segment's last factorisation succeeded
+  int32_t n_nodes = 0;          // nodes covered by node_agg / node_cb
   int32_t* node_agg = nullptr;  // per node (-1: no aggregate)
+  int32_t* node_cb = nullptr;   // per node: agg_cbase << 3 | coarse-excluded DOF bits (-1: none)
   float* node_d = nullptr;      // per node: (X - centroid) / radius, 3 floats
   int32_t* agg_ptr = nullptr;   // nodes of each aggregate, ascending
   int32_t* agg_nodes = nullptr;
@@ -133,10 +138,15 @@ struct CoarseOp {
   double* colmax = nullptr;     // per column of every segment: max |I - A_c X|
   int32_t colmax_len = 0;
   int32_t refactors = 0;        // rocSOLVER factorisations so far
+  int32_t probe_accepts = 0;    // setups that kept X_ref on its probe residual
+  double* probe_v = nullptr;    // per segment kCoarseProbes columns of +-1 (column-major)
+  double* probe_u = nullptr;    // X_ref V
+  double* probe_w = nullptr;    // A_c X_ref V
+  double* probe_max = nullptr;  // kCoarseProbes per segment: max |V - W| per column
   int32_t* ipiv = nullptr;
   int32_t* info = nullptr;      // 2 per segment: getrf, getri
   float* inv32 = nullptr;       // per segment A_c^-1, row-major FP32
-  float* w = nullptr;           // P^T r as partial vectors of vlen entries (zero padding)
+  float* w = nullptr;           // P^T r (vlen entries, zero padding)
   float* y = nullptr;           // A_c^-1 P^T r
   // the same in FP64, for the FP64 CG (allocated by its first factorisation)
   double* inv64 = nullptr;

@@ -534,11 +534,9 @@ __global__ __launch_bounds__(kBlock) void tzc_update(TC c, T* x, T* r, T* z, con
 // Coarse data of the search-direction update (CG with a coarse space).
 template <typename T>
 struct CoarseP {
-  const int32_t* node_agg;
-  const int32_t* agg_cbase;
+  const int32_t* node_cb;  // per node: first coarse DOF << 3 | excluded DOF bits (-1: none)
   const int32_t* seg_on;
   const float* node_d;
-  const uint8_t* flags;
   const T* y;
 };
 
@@ -559,18 +557,19 @@ __global__ __launch_bounds__(kBlock) void tzc_pupdate(TC c, CgS1 s1, CoarseP<T> 
   const int end = min(t.s1 * 64, c.n_nodes);
   for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) {
     T zi[3] = {z[3 * i], z[3 * i + 1], z[3 * i + 2]};
-    const int a = on ? cp.node_agg[i] : -1;
-    if (a >= 0) {
-      const T* ya = cp.y + cp.agg_cbase[a];
+    const T pi[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    const int cb = on ? cp.node_cb[i] : -1;
+    if (cb >= 0) {
+      const T* ya = cp.y + (cb >> 3);
       const T d0 = cp.node_d[3 * i], d1 = cp.node_d[3 * i + 1], d2 = cp.node_d[3 * i + 2];
       const T v[3] = {ya[0] + (ya[4] * d2 - ya[5] * d1), ya[1] + (ya[5] * d0 - ya[3] * d2),
                       ya[2] + (ya[3] * d1 - ya[4] * d0)};
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        if (!(cp.flags && (cp.flags[3 * i + d] & kCoarseExcluded))) zi[d] += v[d];
+        if (!((cb >> d) & 1)) zi[d] += v[d];
     }
 #pragma unroll
-    for (int d = 0; d < 3; ++d) p[3 * i + d] = fma_t(tb, p[3 * i + d], zi[d]);
+    for (int d = 0; d < 3; ++d) p[3 * i + d] = fma_t(tb, pi[d], zi[d]);
   }
 }
 
@@ -875,7 +874,7 @@ void coarse_half(const TC& c, const CV<T>& V, int G, hipStream_t st) {
   const T* y;
   if constexpr (std::is_same<T, float>::value) y = V.co->y;
   else y = V.co->y64;
-  const CoarseP<T> cp{V.co->node_agg, V.co->agg_cbase, V.co->seg_on, V.co->node_d, V.flags, y};
+  const CoarseP<T> cp{V.co->node_cb, V.co->seg_on, V.co->node_d, y};
   hipLaunchKernelGGL(tzc_pupdate<T>, dim3(G), dim3(kBlock), 0, st, c, s1, cp, (const T*)V.z,
                      V.p0);
 }

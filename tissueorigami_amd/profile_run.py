@@ -32,6 +32,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--rtol", type=float, default=None)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="direct kernel launches instead of captured hipGraph chunks")
     args = ap.parse_args()
     rtol = args.rtol if args.rtol else (1e-3 if args.precision == "mixed" else 1e-10)
     n = args.copy_mib * (1 << 20)
@@ -45,6 +47,8 @@ def main():
     s.set_material(w["E"], w["nu"], w["eta"], w["ext"])
     s.set_growth_rates(w["rates"])
     s.set_surface_flags(w["place"], w["etype"], w["spans"])
+    if args.no_graphs:
+        s.set_graphs(False)
     n_blocks, n_slots = s.pattern_info()
     print(f"[profile_run] {args.config}: {ne} prisms, {nn} nodes, {n_blocks} blocks",
           file=sys.stderr, flush=True)
