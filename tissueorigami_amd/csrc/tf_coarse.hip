@@ -75,8 +75,8 @@ __device__ inline void p_rows(const float* d, const uint8_t* flags, int i, doubl
 
 // Workgroup sum of n values per thread in a fixed order (wave trees, then
 // waves in order); the result is valid in thread 0.
-template <int N>
-__device__ inline void block_sum(double (&v)[N], double (*red)[kBlock / 64]) {
+template <int N, int B = kBlock>
+__device__ inline void block_sum(double (&v)[N], double (*red)[B / 64]) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
@@ -91,7 +91,7 @@ __device__ inline void block_sum(double (&v)[N], double (*red)[kBlock / 64]) {
     for (int k = 0; k < N; ++k) {
       double s = 0.0;
 #pragma unroll
-      for (int w = 0; w < kBlock / 64; ++w) s += red[k][w];
+      for (int w = 0; w < B / 64; ++w) s += red[k][w];
       v[k] = s;
     }
   }
@@ -248,31 +248,32 @@ __global__ __launch_bounds__(kBlock) void coarse_probe_kernel(int32_t n_c, const
 // list entries in flight (indices first, then offsets, masks and the gathered
 // r) before it adds any of them, in list order.
 constexpr int kRU = 4;
+constexpr int kRBlock = 512;  // threads per aggregate: one pass over 2048 list entries
 template <typename R>
-__global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
+__global__ __launch_bounds__(kRBlock) void coarse_restrict_kernel(
     int32_t n_agg, const int32_t* __restrict__ agg_ptr, const int32_t* __restrict__ agg_nodes,
     const int32_t* __restrict__ agg_tissue, const int32_t* __restrict__ agg_cbase,
     const int32_t* __restrict__ seg_on, const float* __restrict__ list_d,
     const uint8_t* __restrict__ list_mask, const R* __restrict__ r,
     const int32_t* __restrict__ state, R* __restrict__ w) {
-  __shared__ double red[6][kBlock / 64];
+  __shared__ double red[6][kRBlock / 64];
   const int a = blockIdx.x;
   if (a >= n_agg) return;
   const int T = agg_tissue[a];
   if (state[2 * T] != K_RUN || !seg_on[T]) return;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const int beg = agg_ptr[a], end = agg_ptr[a + 1];
-  for (int t0 = beg + threadIdx.x; t0 < end; t0 += kRU * kBlock) {
+  for (int t0 = beg + threadIdx.x; t0 < end; t0 += kRU * kRBlock) {
     int idx[kRU];
 #pragma unroll
     for (int u = 0; u < kRU; ++u) {
-      const int t = t0 + u * kBlock;
+      const int t = t0 + u * kRBlock;
       idx[u] = t < end ? agg_nodes[t] : -1;
     }
     double d[kRU][3], rv[kRU][3];
 #pragma unroll
     for (int u = 0; u < kRU; ++u) {
-      const int t = t0 + u * kBlock;
+      const int t = t0 + u * kRBlock;
       const uint8_t mk = idx[u] >= 0 ? list_mask[t] : (uint8_t)7;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void coarse_restrict_kernel(
       acc[5] += d0 * r1 - d1 * r0;
     }
   }
-  block_sum<6>(acc, red);
+  block_sum<6, kRBlock>(acc, red);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int q = 0; q < 6; ++q) w[agg_cbase[a] + q] = (R)acc[q];
@@ -341,6 +342,29 @@ __global__ __launch_bounds__(kBlock) void coarse_nodecb_kernel(int32_t n_nodes,
   }
 }
 
+// The first product workgroup of every segment also forms its tissue's
+// ordered sums of the (r, z_D) and (r, r) tile partials (the same sums the
+// search-direction update would form), so that the update's workgroups read
+// two numbers instead of every tile partial.
+__device__ inline void s1_prereduce(const S1Pre& pre, int T, int block) {
+  if (!pre.out) return;
+  const int g0 = pre.cpart_off[T], g1 = pre.cpart_off[T + 1];
+  // the segment's first workgroup forms the (r, z_D) sum, its second the
+  // (r, r) sum (the first both when the segment has one product workgroup)
+  const bool two = g1 - g0 > 1;
+  const bool do_rz = block == g0, do_rr = two ? block == g0 + 1 : block == g0;
+  if (!do_rz && !do_rr) return;
+  const int t0 = pre.tissue_tile[T], t1 = pre.tissue_tile[T + 1];
+  if (do_rz) {
+    const double v = wg_ordered_sum(pre.rz, t0, t1);
+    if (threadIdx.x == 0) pre.out[2 * T] = v;
+  }
+  if (do_rr) {
+    const double v = wg_ordered_sum(pre.rr, t0, t1);
+    if (threadIdx.x == 0) pre.out[2 * T + 1] = v;
+  }
+}
+
 // y = A_c^-1 w, one wavefront per row (16-byte loads over the padded row
 // stride ld); partial (w, y) per workgroup.
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -348,10 +372,12 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
     const int32_t* __restrict__ gw, const int32_t* __restrict__ sgi,
     const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on,
     const float* __restrict__ inv32, const float* __restrict__ w,
-    const int32_t* __restrict__ state, float* __restrict__ y, double* __restrict__ cpart) {
+    const int32_t* __restrict__ state, float* __restrict__ y, double* __restrict__ cpart,
+    S1Pre pre) {
   __shared__ double wy[kGemvRows];
   const int T = gw[2 * blockIdx.x], row0 = gw[2 * blockIdx.x + 1];
   if (state[2 * T] != K_RUN) return;
+  s1_prereduce(pre, T, (int)blockIdx.x);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nc = sgi[3 * T], ld = sgi[3 * T + 1], off = sgi[3 * T + 2];
   const int row = row0 + wave;
@@ -386,10 +412,12 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv64_kernel(
     const int32_t* __restrict__ gw, const int32_t* __restrict__ sgi,
     const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on,
     const double* __restrict__ inv64, const double* __restrict__ w,
-    const int32_t* __restrict__ state, double* __restrict__ y, double* __restrict__ cpart) {
+    const int32_t* __restrict__ state, double* __restrict__ y, double* __restrict__ cpart,
+    S1Pre pre) {
   __shared__ double wy[kGemvRows];
   const int T = gw[2 * blockIdx.x], row0 = gw[2 * blockIdx.x + 1];
   if (state[2 * T] != K_RUN) return;
+  s1_prereduce(pre, T, (int)blockIdx.x);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nc = sgi[3 * T], ld = sgi[3 * T + 1], off = sgi[3 * T + 2];
   const int row = row0 + wave;
@@ -711,6 +739,7 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
   if ((e = upload(c, &c.w, std::vector<float>((size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
   if ((e = upload(c, &c.y, std::vector<float>((size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
   if ((e = upload(c, &c.cpart, std::vector<double>((size_t)std::max(c.n_gemv, 1), 0.0))) != hipSuccess) return e;
+  if ((e = upload(c, &c.s1pre, std::vector<double>(2 * (size_t)n_seg, 0.0))) != hipSuccess) return e;
   c.built = true;
   c.active = false;
   return hipSuccess;
@@ -1022,20 +1051,22 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const double* val64, i
   return e;
 }
 
-void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, hipStream_t st) {
-  hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.n_agg), dim3(kBlock), 0, st, c.n_agg,
+void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, const S1Pre& pre,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.n_agg), dim3(kRBlock), 0, st, c.n_agg,
                      c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on, c.list_d,
                      c.list_mask, r, state, c.w);
   hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
-                     c.sgio, c.seg_on, c.inv32, c.w, state, c.y, c.cpart);
+                     c.sgio, c.seg_on, c.inv32, c.w, state, c.y, c.cpart, pre);
 }
 
-void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, hipStream_t st) {
-  hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.n_agg), dim3(kBlock), 0, st, c.n_agg,
+void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, const S1Pre& pre,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(coarse_restrict_kernel<double>, dim3(c.n_agg), dim3(kRBlock), 0, st, c.n_agg,
                      c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on, c.list_d,
                      c.list_mask, r, state, c.w64);
   hipLaunchKernelGGL(coarse_gemv64_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
-                     c.sgio, c.seg_on, c.inv64, c.w64, state, c.y64, c.cpart);
+                     c.sgio, c.seg_on, c.inv64, c.w64, state, c.y64, c.cpart, pre);
 }
 
 }  // namespace tf

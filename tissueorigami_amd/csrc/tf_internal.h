@@ -154,6 +154,7 @@ struct CoarseOp {
   double* y64 = nullptr;
   int want64 = 0;               // the factorisation in progress also writes inv64
   double* cpart = nullptr;      // per workgroup of the product: partial (w, y)
+  double* s1pre = nullptr;      // 2 per segment: ordered sums of the (r, z_D), (r, r) partials
   std::vector<void*> allocs;
   void* blas = nullptr;         // rocblas_handle
   int32_t gen = 0;              // incremented by every build (captured graphs key on it)
@@ -286,6 +287,41 @@ __device__ inline double wg_ordered_sum(const double* part, int beg, int end) {
   return result;
 }
 
+// N ordered sums at once, each bit for bit the sum wg_ordered_sum gives, with
+// one load round and one chain of barriers for all of them.
+template <int N>
+__device__ inline void wg_ordered_sums(const double* const (&part)[N], const int (&beg)[N],
+                                       const int (&end)[N], double (&out)[N]) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ double red[N][kWaves];
+  __shared__ double result[N];
+  double s[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) {
+    s[q] = 0.0;
+    for (int i = beg[q] + (int)threadIdx.x; i < end[q]; i += kBlock) s[q] += part[q][i];
+  }
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s[q] += __shfl_xor(s[q], off, 64);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < N; ++q) red[q][wave] = s[q];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) t += red[threadIdx.x][w];
+    result[threadIdx.x] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < N; ++q) out[q] = result[q];
+}
+
 // Slots of the per-tissue scalars (8 per tissue) of the tiled solvers.  A
 // slot is written only by the first workgroup of its tissue, and only in a
 // launch whose workgroups do not read it.  CG: TS_RHO holds rho of the last
@@ -302,9 +338,22 @@ struct CgS1 {
   const int32_t* tissue_tile = nullptr;
   const double* cpart = nullptr;   // null: no coarse partials
   const int32_t* cpart_off = nullptr;
+  // with the coarse partials: the ordered sums of rz and rr per tissue, formed
+  // by the coarse product (two per tissue); null: sum the tile partials here
+  const double* pre = nullptr;
   double* scal = nullptr;
   int32_t* state = nullptr;
   double rtol = 0.0;
+};
+
+// What the coarse product needs to form the ordered sums of the (r, z_D) and
+// (r, r) tile partials for the next CG scalar step (out: two per tissue).
+struct S1Pre {
+  const double* rz = nullptr;
+  const double* rr = nullptr;
+  const int32_t* tissue_tile = nullptr;
+  const int32_t* cpart_off = nullptr;  // first product workgroup of each segment
+  double* out = nullptr;               // null: nothing to form
 };
 
 // rho = (r, z) [+ (w, y)], the convergence test on |r| (against |b| = |r| of
@@ -314,9 +363,25 @@ struct CgS1 {
 // solve.
 __device__ inline int cg_s1(const CgS1& s, int T, bool write, double* beta_out) {
   const int t0 = s.tissue_tile[T], t1 = s.tissue_tile[T + 1];
-  double rho = wg_ordered_sum(s.rz, t0, t1);
-  if (s.cpart) rho += wg_ordered_sum(s.cpart, s.cpart_off[T], s.cpart_off[T + 1]);
-  const double rr = wg_ordered_sum(s.rr, t0, t1);
+  double rho, rr;
+  if (s.cpart && s.pre) {
+    rho = s.pre[2 * T] + wg_ordered_sum(s.cpart, s.cpart_off[T], s.cpart_off[T + 1]);
+    rr = s.pre[2 * T + 1];
+  } else if (s.cpart) {
+    const double* const parts[3] = {s.rz, s.cpart, s.rr};
+    const int beg[3] = {t0, s.cpart_off[T], t0}, end[3] = {t1, s.cpart_off[T + 1], t1};
+    double sums[3];
+    wg_ordered_sums<3>(parts, beg, end, sums);
+    rho = sums[0] + sums[1];
+    rr = sums[2];
+  } else {
+    const double* const parts[2] = {s.rz, s.rr};
+    const int beg[2] = {t0, t0}, end[2] = {t1, t1};
+    double sums[2];
+    wg_ordered_sums<2>(parts, beg, end, sums);
+    rho = sums[0];
+    rr = sums[1];
+  }
   double* sc = s.scal + 8 * T;
   const bool first = s.state[2 * T + 1] == 0;
   const double rnorm = sqrt(rr);
@@ -702,9 +767,11 @@ hipError_t coarse_factor(CoarseOp& c, const int32_t* col, const double* val64, i
 // workgroup in c.cpart; nothing for a tissue whose state (state[2 T]) is not
 // running, and zero partials for a segment whose factorisation failed.  The
 // correction P y is added where the search direction is formed.
-void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, hipStream_t st);
+void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, const S1Pre& pre,
+                  hipStream_t st);
 // FP64: y64 = A_c^-1 P^T r with the FP64 inverse, w64 and y64 in FP64.
-void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, hipStream_t st);
+void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, const S1Pre& pre,
+                  hipStream_t st);
 
 void set_error(const std::string& s);
 

@@ -491,13 +491,56 @@ __global__ __launch_bounds__(kBlock) void tzc_spmv(TC c, const T* __restrict__ v
   reduce_store(acc, 1, outs, t.L);
 }
 
-// S2; x += alpha p; r -= alpha q; z = M^-1 r; partials (r, z), (r, r).
+// S2; x += alpha p; r -= alpha q; z = M^-1 r; partials (r, z), (r, r).  A
+// thread's first kUpre rows (p, q, x, r and the block-Jacobi inverse) are
+// loaded before (p, q) is reduced, so that their loads overlap the reduction.
+constexpr int kUpre = 2;
+
+template <typename T>
+__device__ inline void update_row(T ta, const T (&pv)[3], const T (&qv)[3], T (&xv)[3],
+                                  T (&rv)[3], const T (&m)[9], T* x, T* r, T* z, int i,
+                                  double* acc) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    xv[d] = fma_t(ta, pv[d], xv[d]);
+    rv[d] = fma_t(-ta, qv[d], rv[d]);
+    x[3 * i + d] = xv[d];
+    r[3 * i + d] = rv[d];
+  }
+  const T zv[3] = {m[0] * rv[0] + m[1] * rv[1] + m[2] * rv[2],
+                   m[3] * rv[0] + m[4] * rv[1] + m[5] * rv[2],
+                   m[6] * rv[0] + m[7] * rv[1] + m[8] * rv[2]};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    z[3 * i + d] = zv[d];
+    acc[0] += (double)rv[d] * (double)zv[d];
+    acc[1] += (double)rv[d] * (double)rv[d];
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void tzc_update(TC c, T* x, T* r, T* z, const T* p,
                                                      const T* q, const T* minv) {
   TileCtx t;
   if (!tile_ctx(c, t)) return;
   if (c.state[2 * t.T] != K_RUN) return;
+  const int beg = t.s0 * 64 + threadIdx.x;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  T pv[kUpre][3], qv[kUpre][3], xv[kUpre][3], rv[kUpre][3], m[kUpre][9];
+#pragma unroll
+  for (int k = 0; k < kUpre; ++k) {
+    const int i = beg + k * kBlock;
+    const bool in = i < end;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pv[k][d] = in ? p[3 * i + d] : (T)0;
+      qv[k][d] = in ? q[3 * i + d] : (T)0;
+      xv[k][d] = in ? x[3 * i + d] : (T)0;
+      rv[k][d] = in ? r[3 * i + d] : (T)0;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m[k][e] = in ? minv[9 * (size_t)i + e] : (T)0;
+  }
   const double rho = c.scal[8 * t.T + TS_RHON];
   const double pq = tissue_sum(c, TC_PQ, t.T);
   const double alpha = rho / pq;
@@ -506,21 +549,21 @@ __global__ __launch_bounds__(kBlock) void tzc_update(TC c, T* x, T* r, T* z, con
     return;
   }
   const T ta = (T)alpha;
-  const int end = min(t.s1 * 64, c.n_nodes);
   double acc[2] = {0.0, 0.0};
-  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const int k = 3 * i + d;
-      x[k] = fma_t(ta, p[k], x[k]);
-      r[k] = fma_t(-ta, q[k], r[k]);
-    }
-    minv_apply(minv, r, z, i);
+  for (int k = 0; k < kUpre; ++k) {
+    const int i = beg + k * kBlock;
+    if (i < end) update_row<T>(ta, pv[k], qv[k], xv[k], rv[k], m[k], x, r, z, i, acc);
+  }
+  for (int i = beg + kUpre * kBlock; i < end; i += kBlock) {
+    const T p1[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    const T q1[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+    T x1[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    T r1[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+    T m1[9];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      acc[0] += (double)r[3 * i + d] * (double)z[3 * i + d];
-      acc[1] += (double)r[3 * i + d] * (double)r[3 * i + d];
-    }
+    for (int e = 0; e < 9; ++e) m1[e] = minv[9 * (size_t)i + e];
+    update_row<T>(ta, p1, q1, x1, r1, m1, x, r, z, i, acc);
   }
   double* outs[2] = {TP(c, TC_RZ0), TP(c, TC_RR0)};
   reduce_store(acc, 2, outs, t.L);
@@ -542,7 +585,26 @@ struct CoarseP {
 
 // S1 (with the coarse partials (w, y)); p = (z_D + P y) + beta p, in place;
 // the coarse term t + omega x d of the node's aggregate on the DOF that take
-// part in the coarse space.
+// part in the coarse space.  A thread's first kPre rows are loaded before the
+// tissue's scalars are reduced, so that their loads overlap the reduction.
+template <typename T>
+__device__ inline void pupdate_row(const CoarseP<T>& cp, bool on, T tb, const T (&zi)[3],
+                                   const T (&pi)[3], const T (&d)[3], int cb, T* p, int i) {
+  T zz[3] = {zi[0], zi[1], zi[2]};
+  if (on && cb >= 0) {
+    const T* ya = cp.y + (cb >> 3);
+    const T v[3] = {ya[0] + (ya[4] * d[2] - ya[5] * d[1]), ya[1] + (ya[5] * d[0] - ya[3] * d[2]),
+                    ya[2] + (ya[3] * d[1] - ya[4] * d[0])};
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (!((cb >> q) & 1)) zz[q] += v[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) p[3 * i + q] = fma_t(tb, pi[q], zz[q]);
+}
+
+constexpr int kPre = 2;
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void tzc_pupdate(TC c, CgS1 s1, CoarseP<T> cp,
                                                       const T* __restrict__ z,
@@ -550,26 +612,36 @@ __global__ __launch_bounds__(kBlock) void tzc_pupdate(TC c, CgS1 s1, CoarseP<T> 
   TileCtx t;
   if (!tile_ctx(c, t)) return;
   if (c.state[2 * t.T] != K_RUN) return;
+  const bool on = cp.seg_on[t.T] != 0;
+  const int beg = t.s0 * 64 + threadIdx.x;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  T zi[kPre][3], pi[kPre][3], d[kPre][3];
+  int cb[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int i = beg + k * kBlock;
+    const bool in = i < end;
+    cb[k] = in ? cp.node_cb[i] : -1;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      zi[k][q] = in ? z[3 * i + q] : (T)0;
+      pi[k][q] = in ? p[3 * i + q] : (T)0;
+      d[k][q] = in ? (T)cp.node_d[3 * i + q] : (T)0;
+    }
+  }
   double beta;
   if (cg_s1(s1, t.T, t.first, &beta) != K_RUN) return;
   const T tb = (T)beta;
-  const bool on = cp.seg_on[t.T] != 0;
-  const int end = min(t.s1 * 64, c.n_nodes);
-  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock) {
-    T zi[3] = {z[3 * i], z[3 * i + 1], z[3 * i + 2]};
-    const T pi[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
-    const int cb = on ? cp.node_cb[i] : -1;
-    if (cb >= 0) {
-      const T* ya = cp.y + (cb >> 3);
-      const T d0 = cp.node_d[3 * i], d1 = cp.node_d[3 * i + 1], d2 = cp.node_d[3 * i + 2];
-      const T v[3] = {ya[0] + (ya[4] * d2 - ya[5] * d1), ya[1] + (ya[5] * d0 - ya[3] * d2),
-                      ya[2] + (ya[3] * d1 - ya[4] * d0)};
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        if (!((cb >> d) & 1)) zi[d] += v[d];
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) p[3 * i + d] = fma_t(tb, pi[d], zi[d]);
+  for (int k = 0; k < kPre; ++k) {
+    const int i = beg + k * kBlock;
+    if (i < end) pupdate_row<T>(cp, on, tb, zi[k], pi[k], d[k], cb[k], p, i);
+  }
+  for (int i = beg + kPre * kBlock; i < end; i += kBlock) {
+    const T z1[3] = {z[3 * i], z[3 * i + 1], z[3 * i + 2]};
+    const T p1[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    const T d1[3] = {(T)cp.node_d[3 * i], (T)cp.node_d[3 * i + 1], (T)cp.node_d[3 * i + 2]};
+    pupdate_row<T>(cp, on, tb, z1, p1, d1, cp.node_cb[i], p, i);
   }
 }
 
@@ -867,10 +939,17 @@ CgS1 host_s1(const TC& c) {
 // product, then S1 and p = (z_D + P y) + beta p.
 template <typename T>
 void coarse_half(const TC& c, const CV<T>& V, int G, hipStream_t st) {
-  coarse_apply(*V.co, c.state, (const T*)V.r, st);
   CgS1 s1 = host_s1(c);
+  S1Pre pre;
+  pre.rz = s1.rz;
+  pre.rr = s1.rr;
+  pre.tissue_tile = s1.tissue_tile;
+  pre.cpart_off = V.co->cpart_off;
+  pre.out = V.co->s1pre;
+  coarse_apply(*V.co, c.state, (const T*)V.r, pre, st);
   s1.cpart = V.co->cpart;
   s1.cpart_off = V.co->cpart_off;
+  s1.pre = V.co->s1pre;
   const T* y;
   if constexpr (std::is_same<T, float>::value) y = V.co->y;
   else y = V.co->y64;
