@@ -433,7 +433,7 @@ def run_packing(steps, warmup, mixed_rtol, mixed_krylov, name="c2", gap=0.6):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=list(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -517,6 +517,18 @@ tf_status tf_nr_set_coarse_bands(tf_handle* h, int32_t z_bands);
  * mixed assembly.  No reference counterpart. */
 tf_status tf_nr_set_coarse_lag(tf_handle* h, int32_t lag);
 tf_status tf_nr_coarse_info(tf_handle* h, int32_t* aggregates, int32_t* active);
+/* enable != 0 (the default): within one tf_nr_advance trajectory, the first
+ * Krylov solve of every step after the first starts from the displacement
+ * that the tissue's last one or two converged steps extrapolate to (linear
+ * extrapolation of the step displacements), and is handed that guess's
+ * residual with the tolerance still measured against the full residual
+ * (rtol |g|), so the increment meets the same test as a solve from zero.  A
+ * tissue whose guess does not halve |g| solves from zero.  Positions handed
+ * in (tf_nr_advance with u0, tf_nr_step, tf_adj_forward) start a new
+ * trajectory without history.  0: every solve starts from zero.  No
+ * reference counterpart: Pardiso, the reference's direct solver, takes no
+ * initial guess. */
+tf_status tf_nr_set_warm_start(tf_handle* h, int32_t enable);
 
 /* Size of the Jacobian pattern: stored 3x3 blocks (n_blocks, the structural
  * nonzeros / 9) and padded SELL-64 slots.  No reference counterpart. */

@@ -222,6 +222,13 @@ struct Workspace {
   double* host_tnorm = nullptr;
   double* gnorm = nullptr;       // per tissue: |g|_2 of the Newton residual
   double* host_gnorm = nullptr;
+  // warm start of a step's first solve: the guess x0, its residual r0, the
+  // per-tissue rtol factors (device and pinned host), the displacements of
+  // the last two steps (hist0 the latest), allocated on first use
+  double *x0 = nullptr, *r0 = nullptr, *rscale = nullptr, *host_rscale = nullptr;
+  double *hist0 = nullptr, *hist1 = nullptr;
+  int32_t* guess_use = nullptr;  // per tissue: the guess is used
+  int32_t* hist_order = nullptr; // per tissue: steps of history in use (0, 1, 2)
   // restarted GMRES (FP64): basis of kGmresRestart + 1 vectors, per-tile
   // partials, per-tissue sums and host-computed coefficients
   double* gV = nullptr;
@@ -341,6 +348,7 @@ struct CgS1 {
   // with the coarse partials: the ordered sums of rz and rr per tissue, formed
   // by the coarse product (two per tissue); null: sum the tile partials here
   const double* pre = nullptr;
+  const double* rtol_scale = nullptr;  // per tissue factor on rtol (null: 1)
   double* scal = nullptr;
   int32_t* state = nullptr;
   double rtol = 0.0;
@@ -389,7 +397,7 @@ __device__ inline int cg_s1(const CgS1& s, int T, bool write, double* beta_out) 
   const double beta = rho / sc[TS_RHO];
   int state = K_RUN;
   if (!isfinite(rnorm) || rnorm > kDivergence * bnorm) state = K_BREAKDOWN;
-  else if (rnorm <= s.rtol * bnorm) state = K_CONVERGED;
+  else if (rnorm <= (s.rtol_scale ? s.rtol * s.rtol_scale[T] : s.rtol) * bnorm) state = K_CONVERGED;
   else if (!(rho > 0.0) || !isfinite(beta)) state = K_BREAKDOWN;
   if (write && threadIdx.x == 0) {
     if (first) sc[TS_BNORM] = bnorm;
@@ -669,6 +677,11 @@ struct KrylovArgs {
   TileMap tm;
   const CoarseOp* coarse = nullptr;  // two-level FP32 CG preconditioner when active
   const uint8_t* dof_flags = nullptr;
+  // per-tissue factor on rtol (device, and its host copy for host-side
+  // tests): a solve from a warm start targets rtol |b_T| of the original
+  // right-hand side while it is handed the smaller residual of the guess
+  const double* rtol_scale = nullptr;
+  const double* rtol_scale_h = nullptr;
 };
 // Solves J x = b (FP64).  Returns the iteration count in *iters, the Krylov
 // state (K_CONVERGED/K_BREAKDOWN/K_RUN=cap reached) in *state.
@@ -709,6 +722,22 @@ hipError_t launch_copy(const void* src, void* dst, int64_t n4, int variant, hipS
 void tiled_release(Workspace* ws);
 // dst = src on the tissues whose tactive entry is 1 (others untouched).
 hipError_t tiled_copy_active(const KrylovArgs& k, const double* src, double* dst);
+// Warm start of a solve of J x = b: r0 = b - J x0 with the FP32 operator
+// (mixed) or the FP64 one, per tissue.  A tissue whose guess leaves at least
+// half of |b_T| (or a non-finite residual) takes x0 = 0 and r0 = b instead.
+// ws->gnorm must hold |b_T|^2 (tiled_residual_norms).  scale_h / scale_d get
+// |b_T| / |r0_T| (1 where the guess is not used): the factor on rtol that
+// keeps the solve's target at rtol |b_T|.  Synchronises the stream.
+hipError_t tiled_warm_start(const KrylovArgs& k, int mixed, const double* b, double* x0,
+                            double* r0, double* scale_d, double* scale_h);
+// dst += src on the tissues whose tactive entry is 1.
+hipError_t tiled_add_active(const KrylovArgs& k, const double* src, double* dst);
+// Per tissue T: x0 = 0 (order[T] 0), h0 (1) or 2 h0 - h1 (2); order on the device.
+hipError_t tiled_extrapolate(const KrylovArgs& k, const int32_t* order, const double* h0,
+                             const double* h1, double* x0);
+// d = a - b, n entries.
+hipError_t launch_difference(int64_t n, const double* a, const double* b, double* d,
+                             hipStream_t st);
 
 // ---- adjoint (tf_adjoint.hip) ----
 struct AdjointLaunch {

@@ -95,6 +95,11 @@ struct tf_handle {
   int32_t n_abl = 0;
   bool any_abl_node = false;
   hipStream_t st = nullptr;
+  // warm start of a step's first Krylov solve (tf_nr_set_warm_start): per
+  // tissue, the number of the handle's own last steps (0..2) whose
+  // displacements tf_nr_advance keeps; positions handed in reset it
+  bool warm_start = true;
+  std::vector<int32_t> n_hist;
   DevPattern pat;
   std::vector<int32_t> h_slice_base, h_col, h_row_len, h_rowptr, h_nbr;
   // contact capacity and element width per slice (node-node packing)
@@ -500,6 +505,23 @@ tf_status ensure_gmres(tf_handle* h) {
 
 // Tiles of spt slices that never cross a tissue boundary (tissues start on
 // slice boundaries).  Rebuilding replaces the device tile table.
+tf_status ensure_warm(tf_handle* h) {
+  Workspace& w = h->ws;
+  if (w.x0) return TF_OK;
+  const size_t n = 3 * (size_t)h->Nn, nt = (size_t)h->n_tissues;
+  TF_CHECK(dalloc(h, &w.x0, n));
+  TF_CHECK(dalloc(h, &w.r0, n));
+  TF_CHECK(dalloc(h, &w.hist0, n));
+  TF_CHECK(dalloc(h, &w.hist1, n));
+  TF_CHECK(dalloc(h, &w.rscale, nt));
+  TF_CHECK(dalloc(h, &w.guess_use, nt));
+  TF_CHECK(dalloc(h, &w.hist_order, nt));
+  TF_CHECK(hipHostMalloc((void**)&w.host_rscale, sizeof(double) * nt, hipHostMallocDefault));
+  return TF_OK;
+}
+
+void reset_history(tf_handle* h) { h->n_hist.assign(h->n_tissues, 0); }
+
 tf_status build_tiles(tf_handle* h, int spt) {
   std::vector<int32_t> ts, tt, tissue_tile(h->n_tissues + 1, 0);
   ts.push_back(0);
@@ -923,6 +945,7 @@ tf_status finish_create(tf_handle* h, const int32_t* conn, const double* ref_pos
       }
   }
   if ((s = slice_order(h)) != TF_OK) return fail(s);
+  reset_history(h);
   if ((s = alloc_tissue_state(h)) != TF_OK) return fail(s);
   if ((s = build_tiles(h, 8)) != TF_OK) return fail(s);
   *out = h;
@@ -1635,16 +1658,38 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
     // ends that tissue's step with TF_ERR_NAN
     TF_CHECK(tiled_residual_norms(k, w.g));
     const double inner = std::max(rtol, kMixedInnerRtol);
+    // The step's first solve starts from the displacement that the tissue's
+    // last steps extrapolate to (no counterpart in the reference, whose
+    // direct solve needs no guess): the solve is handed the guess's residual
+    // and still ends at rtol |g_T|, so the increment meets the same test.
+    bool warm = false;
+    if (pass == 0 && h->warm_start && method != KRYLOV_GMRES && h->ws.x0 &&
+        h->n_hist.size() == (size_t)nt)
+      for (int T = 0; T < nt; ++T) warm |= h->n_hist[T] > 0;
+    const double* rhs = w.g;
+    if (warm) {
+      TF_CHECK(hipMemcpyAsync(w.hist_order, h->n_hist.data(), sizeof(int32_t) * nt,
+                              hipMemcpyHostToDevice, h->st));
+      TF_CHECK(tiled_extrapolate(k, w.hist_order, w.hist0, w.hist1, w.x0));
+      TF_CHECK(tiled_warm_start(k, mixed, w.g, w.x0, w.r0, w.rscale, w.host_rscale));
+      k.rtol_scale = w.rscale;
+      k.rtol_scale_h = w.host_rscale;
+      rhs = w.r0;
+    }
     if (mixed) {
-      TF_CHECK(tiled_mixed(k, w.g, w.du, rtol, inner, kMixedCap, 16, method, &res));
+      TF_CHECK(tiled_mixed(k, rhs, w.du, rtol, inner, kMixedCap, 16, method, &res));
     } else if (method == KRYLOV_GMRES) {
       s = ensure_gmres(h);
       if (s != TF_OK) return s;
       TF_CHECK(tiled_gmres(k, w.g, w.du, rtol, kGmresCap, &res));
     } else if (method == KRYLOV_CG) {
-      TF_CHECK(tiled_cg(k, w.g, w.du, rtol, kmax, 16, &res));
+      TF_CHECK(tiled_cg(k, rhs, w.du, rtol, kmax, 16, &res));
     } else {
-      TF_CHECK(tiled_bicgstab(k, w.g, w.du, rtol, kmax, 16, &res));
+      TF_CHECK(tiled_bicgstab(k, rhs, w.du, rtol, kmax, 16, &res));
+    }
+    if (warm) {
+      TF_CHECK(tiled_add_active(k, w.x0, w.du));
+      k = krylov_args(h);
     }
     TF_CHECK(hipEventRecord(h->ev[3], h->st));
     TF_CHECK(hipEventSynchronize(h->ev[3]));
@@ -1815,6 +1860,7 @@ tf_status tf_nr_step(tf_handle* h, const double* u_n, double dt, double tol, int
   if (up != TF_OK) return up;
   TF_CHECK(hipMemcpyAsync(w.uk, w.un, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
   const double rtol = krylov_rtol > 0 ? krylov_rtol : 1e-10;
+  reset_history(h);
   tf_status result = nr_loop(h, dt, tol, maxit, mixed, rtol, S);
   if (result != TF_OK && result != TF_ERR_NOT_CONVERGED && result != TF_ERR_KRYLOV &&
       result != TF_ERR_NAN)
@@ -2345,6 +2391,12 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
   if (u0) {
     tf_status up = upload_nodes(h, w.uk, u0);
     if (up != TF_OK) return up;
+    reset_history(h);
+  }
+  if (h->n_hist.size() != (size_t)h->n_tissues) reset_history(h);
+  if (h->warm_start) {
+    tf_status ws = ensure_warm(h);
+    if (ws != TF_OK) return ws;
   }
   h->have_positions = true;
   const int mixed = precision_mode == TF_PREC_MIXED ? 1 : 0;
@@ -2388,8 +2440,18 @@ tf_status tf_nr_advance(tf_handle* h, const double* u0, int32_t n_steps, double 
     TF_CHECK(hipMemcpyAsync(w.un, w.uk, sizeof(double) * n, hipMemcpyDeviceToDevice, h->st));
     tf_status r = nr_loop(h, dt, tol, maxit, mixed, rtol, S.data());
     if (stats) std::memcpy(stats + (size_t)s * h->n_tissues, S.data(), sizeof(tf_step_stats) * h->n_tissues);
-    if (r != TF_OK && r != TF_ERR_NOT_CONVERGED) return r;
+    if (r != TF_OK && r != TF_ERR_NOT_CONVERGED) {
+      reset_history(h);
+      return r;
+    }
     if (r != TF_OK) worst = r;
+    // the step's displacement joins each converged tissue's history
+    if (h->warm_start) {
+      std::swap(w.hist0, w.hist1);
+      TF_CHECK(launch_difference((int64_t)n, w.uk, w.un, w.hist0, h->st));
+      for (int T = 0; T < h->n_tissues; ++T)
+        h->n_hist[T] = S[T].status == TF_OK && S[T].converged ? std::min(h->n_hist[T] + 1, 2) : 0;
+    }
   }
   if (u_out) {
     tf_status dn = download_nodes(h, u_out, w.uk);
@@ -2466,6 +2528,7 @@ tf_status tf_adj_forward(tf_handle* h, int32_t n_steps, const double* u0,
   }
   const size_t n = 3 * (size_t)h->Nn, Ne = h->Ne;
   Workspace& w = h->ws;
+  reset_history(h);  // every forward solve starts from zero
   if (h->adj_cap < n_steps + 1) {
     if (h->adj_ckpt) TF_CHECK(hipFree(h->adj_ckpt));
     h->adj_ckpt = nullptr;
@@ -3204,6 +3267,16 @@ tf_status tf_nr_set_coarse_lag(tf_handle* h, int32_t lag) {
   return TF_OK;
 }
 
+tf_status tf_nr_set_warm_start(tf_handle* h, int32_t enable) {
+  if (!h) {
+    set_error("tf_nr_set_warm_start: invalid handle");
+    return TF_ERR_ARG;
+  }
+  h->warm_start = enable != 0;
+  reset_history(h);
+  return TF_OK;
+}
+
 tf_status tf_nr_set_coarse_bands(tf_handle* h, int32_t z_bands) {
   if (!h || z_bands < 1 || z_bands > 64) {
     set_error("tf_nr_set_coarse_bands: invalid arguments");
@@ -3264,6 +3337,7 @@ tf_status tf_nr_destroy(tf_handle* h) {
   if (h->ws.host_tstate) (void)hipHostFree(h->ws.host_tstate);
   if (h->ws.host_tnorm) (void)hipHostFree(h->ws.host_tnorm);
   if (h->ws.host_gnorm) (void)hipHostFree(h->ws.host_gnorm);
+  if (h->ws.host_rscale) (void)hipHostFree(h->ws.host_rscale);
   for (auto& e : h->ev)
     if (e) (void)hipEventDestroy(e);
   if (h->st) (void)hipStreamDestroy(h->st);

@@ -68,7 +68,12 @@ struct TC {
   int stride;
   const int32_t* active; // per tissue: 0 = skip the solve (x = 0)
   double rtol;
+  const double* rtol_scale;  // per tissue factor on rtol (null: 1)
 };
+
+__device__ inline double tc_rtol(const TC& c, int T) {
+  return c.rtol_scale ? c.rtol * c.rtol_scale[T] : c.rtol;
+}
 
 
 __device__ inline double* TP(const TC& c, int k) { return c.part + (size_t)k * c.stride; }
@@ -261,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void tz_spmv_v(TC c, const T* __restrict__ 
   // solve on an ill-conditioned tissue) is a breakdown, never convergence
   int state = K_RUN;
   if (!isfinite(rnorm) || rnorm > kDivergence * bnorm) state = K_BREAKDOWN;
-  else if (rnorm <= c.rtol * bnorm) state = K_CONVERGED;
+  else if (rnorm <= tc_rtol(c, t.T) * bnorm) state = K_CONVERGED;
   else if (!first_it && sc[TS_OMEGA] == 0.0) state = K_BREAKDOWN;
   else if (rho == 0.0 || wp == 0.0 || !isfinite(alpha)) state = K_BREAKDOWN;
   if (t.first && threadIdx.x == 0) {
@@ -405,6 +410,7 @@ __device__ inline CgS1 make_s1(const TC& c) {
   s.scal = c.scal;
   s.state = c.state;
   s.rtol = c.rtol;
+  s.rtol_scale = c.rtol_scale;
   return s;
 }
 
@@ -711,7 +717,8 @@ __global__ __launch_bounds__(kBlock) void tz_axpy(TC c, const float* z, const do
 }
 
 // r = b - J x (FP32 values promoted, FP64 accumulation), per-tile |r|^2 in slot 1.
-__global__ __launch_bounds__(kBlock) void tz_residual(TC c, const float* __restrict__ val,
+template <typename V>
+__global__ __launch_bounds__(kBlock) void tz_residual(TC c, const V* __restrict__ val,
                                                       const double* __restrict__ x,
                                                       const double* __restrict__ b,
                                                       double* __restrict__ r) {
@@ -754,6 +761,53 @@ __global__ __launch_bounds__(kBlock) void tz_copy_active(TC c, const double* src
     for (int d = 0; d < 3; ++d) dst[3 * i + d] = src[3 * i + d];
 }
 
+// dst += src on the tissues being solved.
+__global__ __launch_bounds__(kBlock) void tz_add_active(TC c, const double* src, double* dst) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (!c.active[t.T]) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) dst[3 * i + d] += src[3 * i + d];
+}
+
+// Tissues that do not use their guess: x0 = 0, r0 = b.
+__global__ __launch_bounds__(kBlock) void tz_guess_select(TC c, const int32_t* use,
+                                                          const double* b, double* x0,
+                                                          double* r0) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  if (use[t.T]) return;
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = t.s0 * 64 + threadIdx.x; i < end; i += kBlock)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      x0[3 * i + d] = 0.0;
+      r0[3 * i + d] = b[3 * i + d];
+    }
+}
+
+// x0 from the displacements of a tissue's last steps: 0 (no history), h0
+// (one step) or 2 h0 - h1 (two steps).
+__global__ __launch_bounds__(kBlock) void tz_extrapolate(TC c, const int32_t* order,
+                                                         const double* h0, const double* h1,
+                                                         double* x0) {
+  TileCtx t;
+  if (!tile_ctx(c, t)) return;
+  const int o = order[t.T];
+  const int end = min(t.s1 * 64, c.n_nodes);
+  for (int i = 3 * (t.s0 * 64) + threadIdx.x; i < 3 * end; i += kBlock)
+    x0[i] = o >= 2 ? 2.0 * h0[i] - h1[i] : (o == 1 ? h0[i] : 0.0);
+}
+
+__global__ __launch_bounds__(kBlock) void difference_kernel(int64_t n, const double* a,
+                                                            const double* b, double* d) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    d[i] = a[i] - b[i];
+}
+
 __global__ __launch_bounds__(kBlock) void tz_fill(TC c, double* v, double value) {
   TileCtx t;
   if (!tile_ctx(c, t)) return;
@@ -778,6 +832,7 @@ TC make_tc(const KrylovArgs& k, double rtol) {
   c.stride = k.ws->tpart_stride;
   c.active = k.ws->tactive;
   c.rtol = rtol;
+  c.rtol_scale = k.rtol_scale;
   return c;
 }
 
@@ -932,6 +987,7 @@ CgS1 host_s1(const TC& c) {
   s.scal = c.scal;
   s.state = c.state;
   s.rtol = c.rtol;
+  s.rtol_scale = c.rtol_scale;
   return s;
 }
 
@@ -1179,7 +1235,7 @@ hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double r
         res->state[T] = inner.state[T];
       }
     if (rtol >= inner_rtol) break;  // the NR loop (FP64 residual) is the refinement
-    hipLaunchKernelGGL(tz_residual, dim3(G), dim3(kBlock), 0, st, c, k.val32, x, b, ws->r);
+    hipLaunchKernelGGL(tz_residual<float>, dim3(G), dim3(kBlock), 0, st, c, k.val32, x, b, ws->r);
     hipLaunchKernelGGL(tz_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, 1, ws->tnorm);
     if ((e = hipMemcpyAsync(ws->host_tnorm, ws->tnorm, sizeof(double) * nt,
                             hipMemcpyDeviceToHost, st)) != hipSuccess)
@@ -1189,7 +1245,8 @@ hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double r
     for (int T = 0; T < nt; ++T) {
       if (!act[T]) continue;
       rnorm[T] = sqrt(ws->host_tnorm[T]);
-      if (!(rnorm[T] > rtol * bnorm[T])) { res->state[T] = K_CONVERGED; act[T] = 0; }
+      const double rt = k.rtol_scale_h ? rtol * k.rtol_scale_h[T] : rtol;
+      if (!(rnorm[T] > rt * bnorm[T])) { res->state[T] = K_CONVERGED; act[T] = 0; }
       else if (!std::isfinite(rnorm[T]) || inner.state[T] == K_BREAKDOWN) {
         res->state[T] = K_BREAKDOWN; act[T] = 0;
       } else if (res->iters[T] >= maxit) { res->state[T] = K_RUN; act[T] = 0; }
@@ -1206,6 +1263,73 @@ hipError_t tiled_mixed(const KrylovArgs& k, const double* b, double* x, double r
 hipError_t tiled_copy_active(const KrylovArgs& k, const double* src, double* dst) {
   TC c = make_tc(k, 0.0);
   hipLaunchKernelGGL(tz_copy_active, dim3(k.tm.grid), dim3(kBlock), 0, k.L.stream, c, src, dst);
+  return hipGetLastError();
+}
+
+hipError_t tiled_add_active(const KrylovArgs& k, const double* src, double* dst) {
+  TC c = make_tc(k, 0.0);
+  hipLaunchKernelGGL(tz_add_active, dim3(k.tm.grid), dim3(kBlock), 0, k.L.stream, c, src, dst);
+  return hipGetLastError();
+}
+
+hipError_t tiled_warm_start(const KrylovArgs& k, int mixed, const double* b, double* x0,
+                            double* r0, double* scale_d, double* scale_h) {
+  Workspace* ws = k.ws;
+  hipStream_t st = k.L.stream;
+  const int G = k.tm.grid;
+  const int nt = k.tm.n_tissues;
+  TC c = make_tc(k, 0.0);
+  if (mixed)
+    hipLaunchKernelGGL(tz_residual<float>, dim3(G), dim3(kBlock), 0, st, c, k.val32, x0, b, r0);
+  else
+    hipLaunchKernelGGL(tz_residual<double>, dim3(G), dim3(kBlock), 0, st, c, k.val, x0, b, r0);
+  hipLaunchKernelGGL(tz_tissue_sums, dim3(nt), dim3(kBlock), 0, st, c, 1, ws->tnorm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(ws->host_tnorm, ws->tnorm, sizeof(double) * nt, hipMemcpyDeviceToHost,
+                          st)) != hipSuccess)
+    return e;
+  if ((e = hipMemcpyAsync(ws->host_gnorm, ws->gnorm, sizeof(double) * nt, hipMemcpyDeviceToHost,
+                          st)) != hipSuccess)
+    return e;
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+  std::vector<int32_t> use(nt, 0);
+  bool all = true;
+  for (int T = 0; T < nt; ++T) {
+    const double bb = ws->host_gnorm[T], rr = ws->host_tnorm[T];
+    // the guess must leave at most half of |b_T| (|r0|^2 <= |b|^2 / 4)
+    use[T] = std::isfinite(rr) && std::isfinite(bb) && bb > 0.0 && rr > 0.0 && 4.0 * rr <= bb;
+    scale_h[T] = use[T] ? std::sqrt(bb / rr) : 1.0;
+    all &= use[T] != 0;
+  }
+  if ((e = hipMemcpyAsync(scale_d, scale_h, sizeof(double) * nt, hipMemcpyHostToDevice, st)) !=
+      hipSuccess)
+    return e;
+  if (!all) {
+    if ((e = hipMemcpyAsync(ws->guess_use, use.data(), sizeof(int32_t) * nt,
+                            hipMemcpyHostToDevice, st)) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL(tz_guess_select, dim3(G), dim3(kBlock), 0, st, c,
+                       (const int32_t*)ws->guess_use, b, x0, r0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  // use is pageable and local: wait for its copy
+  return hipStreamSynchronize(st);
+}
+
+hipError_t tiled_extrapolate(const KrylovArgs& k, const int32_t* order, const double* h0,
+                             const double* h1, double* x0) {
+  TC c = make_tc(k, 0.0);
+  hipLaunchKernelGGL(tz_extrapolate, dim3(k.tm.grid), dim3(kBlock), 0, k.L.stream, c, order, h0,
+                     h1, x0);
+  return hipGetLastError();
+}
+
+hipError_t launch_difference(int64_t n, const double* a, const double* b, double* d,
+                             hipStream_t st) {
+  const int64_t g = std::min<int64_t>((n + kBlock - 1) / kBlock, 65536);
+  hipLaunchKernelGGL(difference_kernel, dim3((unsigned)std::max<int64_t>(g, 1)), dim3(kBlock), 0,
+                     st, n, a, b, d);
   return hipGetLastError();
 }
 

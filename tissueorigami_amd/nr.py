@@ -26,7 +26,7 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_nr_kernel_times", "tf_nr_pattern_info", "tf_adj_forward", "tf_adj_backward",
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
            "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs", "tf_nr_set_krylov", "tf_nr_assemble_ex",
-           "tf_copy_bandwidth", "tf_nr_set_coarse", "tf_nr_set_coarse_bands", "tf_nr_set_coarse_lag", "tf_nr_coarse_info",
+           "tf_copy_bandwidth", "tf_nr_set_coarse", "tf_nr_set_coarse_bands", "tf_nr_set_coarse_lag", "tf_nr_coarse_info", "tf_nr_set_warm_start",
            "tf_nr_set_packing", "tf_nr_set_node_mass", "tf_nr_update_node_mass",
            "tf_nr_detect_packing", "tf_nr_get_packing",
            "tf_nr_add_stiffness_series", "tf_nr_apply_stiffness_series", "tf_nr_set_time",
@@ -125,6 +125,7 @@ def load():
     L.tf_nr_set_coarse.argtypes = [_h, ctypes.c_double]
     L.tf_nr_set_coarse_bands.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_set_coarse_lag.argtypes = [_h, ctypes.c_int32]
+    L.tf_nr_set_warm_start.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_coarse_info.argtypes = [_h, _i, _i]
     L.tf_nr_set_packing.argtypes = [_h, _i, _u8, ctypes.c_int32, ctypes.c_int32]
     L.tf_nr_set_node_mass.argtypes = [_h, _d]
@@ -521,6 +522,11 @@ class NewtonRaphsonSolver:
     def set_coarse_lag(self, lag=True):
         """Form the coarse inverse once per step (True) or at every mixed assembly."""
         _check(load().tf_nr_set_coarse_lag(self._h, int(bool(lag))))
+
+    def set_warm_start(self, enable=True):
+        """Start each step's first solve of an advance trajectory from the
+        extrapolated displacement of the last steps (True) or from zero."""
+        _check(load().tf_nr_set_warm_start(self._h, int(bool(enable))))
 
     def coarse_info(self):
         """(aggregates of the last coarse build, whether the last mixed solve used it)."""
