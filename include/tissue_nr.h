@@ -529,6 +529,17 @@ tf_status tf_nr_coarse_info(tf_handle* h, int32_t* aggregates, int32_t* active);
  * reference counterpart: Pardiso, the reference's direct solver, takes no
  * initial guess. */
 tf_status tf_nr_set_warm_start(tf_handle* h, int32_t enable);
+/* enable != 0 (the default): with a Krylov rtol of 1e-6 or more (inexact
+ * Newton, rtol the forcing term), a solve that the step's last two increments
+ * predict to be its last (next increment e = |du_{k-1}|^2 / |du_{k-2}|) runs
+ * to the relative tolerance min(0.1, max(rtol, 0.5 tol / e)) instead of rtol:
+ * the safeguard against oversolving of Eisenstat and Walker's forcing terms.
+ * Its increment still decides convergence to within ten per cent, and leaves
+ * at most about a tenth of an increment already below tol.  Tight solves
+ * (rtol below 1e-6, the default 1e-10 of the FP64 step) are never loosened.
+ * 0: every solve runs to rtol.  No reference counterpart: Pardiso solves
+ * exactly. */
+tf_status tf_nr_set_forcing_safeguard(tf_handle* h, int32_t enable);
 
 /* Size of the Jacobian pattern: stored 3x3 blocks (n_blocks, the structural
  * nonzeros / 9) and padded SELL-64 slots.  No reference counterpart. */

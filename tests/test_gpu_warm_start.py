@@ -1,9 +1,16 @@
-"""Warm start of a step's first Krylov solve (tf_nr_set_warm_start): the
+"""Solve policies of the inexact Newton loop.
+
+Warm start of a step's first Krylov solve (tf_nr_set_warm_start): the
 increment still meets rtol |g|, so a trajectory stepped with the warm start
 lands where the same trajectory stepped from zero lands, within the Newton
 tolerance; the first solves of later steps take fewer Krylov iterations;
 positions handed in start a trajectory without history, so repeating a call
-repeats its bits.  Every test here needs an MI355X."""
+repeats its bits.
+
+Forcing-term safeguard (tf_nr_set_forcing_safeguard): a solve predicted to be
+a step's last runs to a looser tolerance; the step converges at the same NR
+iteration to within a tenth of its last increment, and tight solves are left
+alone.  Every test here needs an MI355X."""
 import numpy as np
 import pytest
 
@@ -23,18 +30,19 @@ def c2():
     return make_workload("c2")
 
 
-def _solver(nr, w, warm):
+def _solver(nr, w, warm, safeguard=True):
     m = w["mesh"]
     s = nr.NewtonRaphsonSolver(len(m["nodes"]), m["conn"], m["refpos"])
     s.set_material(w["E"], w["nu"], w["eta"], w["ext"])
     s.set_growth_rates(w["rates"])
     s.set_surface_flags(w["place"], w["etype"], w["spans"])
     s.set_warm_start(warm)
+    s.set_forcing_safeguard(safeguard)
     return s
 
 
-def _trajectory(nr, w, warm, precision, rtol, steps=4):
-    s = _solver(nr, w, warm)
+def _trajectory(nr, w, warm, precision, rtol, steps=4, safeguard=True):
+    s = _solver(nr, w, warm, safeguard)
     u, st = s.advance(steps, w["dt"], u0=w["mesh"]["nodes"].reshape(-1), precision=precision,
                       krylov_rtol=rtol)
     s.close()
@@ -113,3 +121,27 @@ def test_batch_warm_start_is_bitwise_each_tissue_alone(nr):
         ua, _ = a.advance(4, dt, u0=m["nodes"].reshape(-1), precision="mixed", krylov_rtol=1e-3)
         a.close()
         assert np.array_equal(ua, ubt.reshape(-1))
+
+
+def test_forcing_safeguard_loosens_only_the_last_solve(nr, c2):
+    """C2 mixed (forcing term 1e-3), three steps from zero with and without
+    the safeguard: the same NR iteration counts, the same first two solves,
+    no solve with more Krylov iterations and a last solve with clearly fewer,
+    and positions within 1e-9 relative; an FP64 step at 1e-10 (a tight solve)
+    gives the same bits."""
+    uo, so = _trajectory(nr, c2, False, "mixed", 1e-3, steps=3, safeguard=True)
+    uf, sf = _trajectory(nr, c2, False, "mixed", 1e-3, steps=3, safeguard=False)
+    print("safeguard on ", [st["krylov_iterations"] for st in so])
+    print("safeguard off", [st["krylov_iterations"] for st in sf])
+    assert all(st["converged"] for st in so + sf)
+    assert np.abs(uo - uf).max() <= 1e-9 * np.abs(uf).max()
+    for a, b in zip(so, sf):
+        assert a["nr_iterations"] == b["nr_iterations"]
+        ka, kb = a["krylov_iterations"], b["krylov_iterations"]
+        assert ka[:2] == kb[:2]
+        assert all(x <= y for x, y in zip(ka, kb))
+        assert ka[-1] <= 0.6 * kb[-1]
+    ud, sd = _trajectory(nr, c2, False, "fp64", 1e-10, steps=1, safeguard=True)
+    ue, se = _trajectory(nr, c2, False, "fp64", 1e-10, steps=1, safeguard=False)
+    assert np.array_equal(ud, ue)
+    assert sd[0]["krylov_iterations"] == se[0]["krylov_iterations"]

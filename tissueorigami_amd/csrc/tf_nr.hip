@@ -100,6 +100,8 @@ struct tf_handle {
   // displacements tf_nr_advance keeps; positions handed in reset it
   bool warm_start = true;
   std::vector<int32_t> n_hist;
+  // forcing-term safeguard of the NR loop (tf_nr_set_forcing_safeguard)
+  bool forcing_safeguard = true;
   DevPattern pat;
   std::vector<int32_t> h_slice_base, h_col, h_row_len, h_rowptr, h_nbr;
   // contact capacity and element width per slice (node-node packing)
@@ -1623,6 +1625,16 @@ constexpr double kMixedInnerRtol = 1e-5;
 constexpr int kMixedCap = 5000;
 // GMRES iterations per solve (each synchronises with the host once)
 constexpr int kGmresCap = 5000;
+// Inexact Newton (a Krylov rtol of at least kForcingMin is a forcing term):
+// a solve that the last two increments predict to be the step's last is not
+// oversolved (the safeguard of Eisenstat and Walker's forcing terms).  The
+// next increment is predicted as e = |du_{k-1}|^2 / |du_{k-2}| (the observed
+// linear contraction), and the solve's relative tolerance is raised to
+// min(kForcingCap, max(rtol, 0.5 tol / e)): its increment then still decides
+// convergence to within the cap, and a position error of about kForcingCap
+// times an increment already below tol is left.
+constexpr double kForcingMin = 1e-6;
+constexpr double kForcingCap = 0.1;
 
 // The NR loop of Simulation::updateStepNR on device-resident state, run for
 // every tissue of the handle with the tissue's own convergence test: w.un
@@ -1634,7 +1646,7 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
   const int nt = h->n_tissues;
   const int kmax = 50000;
   std::vector<int32_t> act(nt, 1), it(nt, 0), cap(nt, 0), fb(nt, 0);
-  std::vector<double> norms(nt, 0.0);
+  std::vector<double> norms(nt, 0.0), dprev1(nt, 0.0), dprev2(nt, 0.0);
   for (int T = 0; T < nt; ++T) std::memset(&S[T], 0, sizeof(tf_step_stats));
   static thread_local TiledResult res, fres;
   for (int pass = 0;; ++pass) {
@@ -1675,6 +1687,28 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
       k.rtol_scale = w.rscale;
       k.rtol_scale_h = w.host_rscale;
       rhs = w.r0;
+    }
+    // a solve predicted to be the step's last is not oversolved
+    if (pass >= 2 && h->forcing_safeguard && rtol >= kForcingMin && method != KRYLOV_GMRES) {
+      bool loose = false;
+      std::vector<double> sc(nt, 1.0);
+      for (int T = 0; T < nt; ++T) {
+        if (!act[T] || !(dprev1[T] > 0.0) || !(dprev2[T] > 0.0)) continue;
+        const double e = dprev1[T] * dprev1[T] / dprev2[T];
+        if (!(e > 0.0) || !std::isfinite(e)) continue;
+        const double eta = std::min(kForcingCap, std::max(rtol, 0.5 * tol / e));
+        sc[T] = eta / rtol;
+        loose |= sc[T] > 1.0;
+      }
+      if (loose) {
+        s = ensure_warm(h);
+        if (s != TF_OK) return s;
+        std::copy(sc.begin(), sc.end(), w.host_rscale);
+        TF_CHECK(hipMemcpyAsync(w.rscale, w.host_rscale, sizeof(double) * nt,
+                                hipMemcpyHostToDevice, h->st));
+        k.rtol_scale = w.rscale;
+        k.rtol_scale_h = w.host_rscale;
+      }
     }
     if (mixed) {
       TF_CHECK(tiled_mixed(k, rhs, w.du, rtol, inner, kMixedCap, 16, method, &res));
@@ -1807,6 +1841,8 @@ tf_status nr_loop(tf_handle* h, double dt, double tol, int32_t maxit, int mixed,
       // NewtonRaphsonSolver::checkConvergenceViaDeltaU (NewtonRaphsonSolver.cpp:807-825);
       // the increment is applied whether or not it converged (Simulation.cpp:6716-6720)
       const bool conv = !(dnorm > tol);
+      dprev2[T] = dprev1[T];
+      dprev1[T] = dnorm;
       it[T]++;
       if (result != TF_OK) {
         St.status = result;
@@ -3274,6 +3310,15 @@ tf_status tf_nr_set_warm_start(tf_handle* h, int32_t enable) {
   }
   h->warm_start = enable != 0;
   reset_history(h);
+  return TF_OK;
+}
+
+tf_status tf_nr_set_forcing_safeguard(tf_handle* h, int32_t enable) {
+  if (!h) {
+    set_error("tf_nr_set_forcing_safeguard: invalid handle");
+    return TF_ERR_ARG;
+  }
+  h->forcing_safeguard = enable != 0;
   return TF_OK;
 }
 

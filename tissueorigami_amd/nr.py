@@ -27,6 +27,7 @@ EXPORTS = ["tf_nr_create", "tf_nr_set_material", "tf_nr_set_growth", "tf_nr_set_
            "tf_nr_set_growth_rates", "tf_nr_growth_tensor", "tf_nr_set_surface_flags", "tf_nr_advance",
            "tf_batch_create", "tf_nr_autotune", "tf_nr_set_tile_size", "tf_nr_set_graphs", "tf_nr_set_krylov", "tf_nr_assemble_ex",
            "tf_copy_bandwidth", "tf_nr_set_coarse", "tf_nr_set_coarse_bands", "tf_nr_set_coarse_lag", "tf_nr_coarse_info", "tf_nr_set_warm_start",
+           "tf_nr_set_forcing_safeguard",
            "tf_nr_set_packing", "tf_nr_set_node_mass", "tf_nr_update_node_mass",
            "tf_nr_detect_packing", "tf_nr_get_packing",
            "tf_nr_add_stiffness_series", "tf_nr_apply_stiffness_series", "tf_nr_set_time",
@@ -126,6 +127,7 @@ def load():
     L.tf_nr_set_coarse_bands.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_set_coarse_lag.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_set_warm_start.argtypes = [_h, ctypes.c_int32]
+    L.tf_nr_set_forcing_safeguard.argtypes = [_h, ctypes.c_int32]
     L.tf_nr_coarse_info.argtypes = [_h, _i, _i]
     L.tf_nr_set_packing.argtypes = [_h, _i, _u8, ctypes.c_int32, ctypes.c_int32]
     L.tf_nr_set_node_mass.argtypes = [_h, _d]
@@ -527,6 +529,10 @@ class NewtonRaphsonSolver:
         """Start each step's first solve of an advance trajectory from the
         extrapolated displacement of the last steps (True) or from zero."""
         _check(load().tf_nr_set_warm_start(self._h, int(bool(enable))))
+
+    def set_forcing_safeguard(self, enable=True):
+        """Do not oversolve a solve predicted to be a step's last (inexact Newton)."""
+        _check(load().tf_nr_set_forcing_safeguard(self._h, int(bool(enable))))
 
     def coarse_info(self):
         """(aggregates of the last coarse build, whether the last mixed solve used it)."""
