@@ -457,6 +457,12 @@ def main():
                     help="bands through the thickness per aggregate cell of the coarse space")
     ap.add_argument("--coarse-every-iteration", action="store_true",
                     help="form the coarse inverse at every NR iteration (default: once per step)")
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="start every Krylov solve from zero (default: a step's first solve "
+                         "starts from the displacement of the last steps)")
+    ap.add_argument("--no-forcing-safeguard", action="store_true",
+                    help="run every inexact-Newton solve to its forcing term (default: a solve "
+                         "predicted to end the step is not oversolved)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -482,6 +488,8 @@ def main():
     solver.set_coarse(args.coarse_cell)
     solver.set_coarse_bands(args.coarse_bands)
     solver.set_coarse_lag(not args.coarse_every_iteration)
+    solver.set_warm_start(not args.no_warm_start)
+    solver.set_forcing_safeguard(not args.no_forcing_safeguard)
     n_blocks, n_slots = solver.pattern_info()
     print(f"[bench] setup {setup_s:.2f} s, {n_blocks} blocks, {n_slots} SELL slots",
           file=sys.stderr, flush=True)
@@ -557,6 +565,14 @@ def main():
                                            else "once per time step)")
                                         if coarse_on else "node-block Jacobi"),
                          krylov_mixed=args.mixed_krylov, krylov_fp64="bicgstab",
+                         solve_policies=dict(
+                             warm_start=("off" if args.no_warm_start else
+                                         "every leg: a step's first solve starts from the "
+                                         "displacement the last two steps extrapolate to"),
+                             forcing_safeguard=("off" if args.no_forcing_safeguard else
+                                                "legs with rtol >= 1e-6 (mixed, fp64_cg): a solve "
+                                                "predicted to end the step runs to min(0.1, "
+                                                "max(rtol, 0.5 tol / e))")),
                          growth_seed=w["seed"], setup_s=setup_s, slices_per_tile=spt)
     # roofline of the dominant kernel: the FP32 SpMV v = J ph of the mixed
     # solve (tz_spmv_v<float>), timed with HIP events on the solver's stream
