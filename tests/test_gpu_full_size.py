@@ -148,6 +148,25 @@ def test_c3_mixed_equals_fp64_and_is_reproducible(c3):
     assert stm["krylov_iterations"] == stm2["krylov_iterations"]
 
 
+def test_c3_trajectory_with_solve_policies_matches_fp64(c3):
+    """1.62 M prisms, four steps: the mixed step with the warm start and the
+    forcing-term safeguard (the bench's configuration) ends within 1e-8
+    relative of the FP64 BiCGStab trajectory at 1e-10, and the later steps'
+    first solves take fewer CG iterations than the first step's."""
+    w, s = c3
+    ne = len(w["mesh"]["conn"])
+    u0 = w["mesh"]["nodes"].reshape(-1)
+    fg0 = np.tile(np.eye(3).reshape(9), (ne, 1))
+    s.growth_tensor(fg0)
+    um, stm = s.advance(4, w["dt"], u0=u0, precision="mixed", krylov_rtol=1e-3)
+    s.growth_tensor(fg0)
+    ud, std = s.advance(4, w["dt"], u0=u0, precision="fp64", krylov_rtol=1e-10)
+    assert all(st["converged"] for st in stm + std)
+    assert np.abs(um - ud).max() <= 1e-8 * np.abs(ud).max()
+    first = [st["krylov_iterations"][0] for st in stm]
+    assert max(first[2:]) < first[0], first
+
+
 def test_c3_converged_positions_are_a_root(c3):
     """The FP64 residual assembled at the converged positions of the mixed
     step is below 1e-6 of the residual at the start of the step."""

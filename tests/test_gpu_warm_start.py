@@ -145,3 +145,40 @@ def test_forcing_safeguard_loosens_only_the_last_solve(nr, c2):
     ue, se = _trajectory(nr, c2, False, "fp64", 1e-10, steps=1, safeguard=False)
     assert np.array_equal(ud, ue)
     assert sd[0]["krylov_iterations"] == se[0]["krylov_iterations"]
+
+
+def test_policies_c1_advance_matches_oracle_steps(orc, nr):
+    """C1 (plain rectangle), ten device-resident mixed steps with the warm
+    start and the safeguard on (forcing term 1e-3) against the oracle's dense
+    NR steps with the same growth and surfaces: every step within 1e-8
+    relative."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import problems
+    from tissueorigami_amd import tissue
+    c = problems.c1_config()
+    m = c["mesh"]
+    ne, nn = len(m["conn"]), len(m["nodes"])
+    rates = np.zeros((ne, 3))
+    rates[:, 0] = rates[:, 1] = c["rate"] / 3600.0
+    s = nr.NewtonRaphsonSolver(nn, m["conn"], m["refpos"])
+    s.set_material(c["E"], c["nu"], c["eta"], c["ext_visc"])
+    s.set_growth_rates(rates)
+    s.set_surface_flags(c["place"], c["etype"], c["spans"])
+    u0 = m["nodes"].reshape(-1).copy()
+    lam, mu = orc.lame(c["E"], c["nu"])
+    inc = tissue.growth_increment(c["rate"], c["dt"])
+    Fg = np.tile(np.eye(3), (ne, 1, 1))
+    u_o = u0.copy()
+    u_g = None
+    for k in range(10):
+        u_g, st = s.advance(1, c["dt"], u0=u0 if k == 0 else None, precision="mixed",
+                            krylov_rtol=1e-3)
+        Fg = inc @ Fg
+        A_o = orc.visc_surfaces(nn, m["conn"], u_o, c["place"], c["etype"], c["spans"])
+        P = orc.Problem(m["conn"], m["refpos"], orc.inv3_many(Fg), lam, mu, c["eta"],
+                        c["ext_visc"], A_o)
+        u_o, du, conv = P.nr_step_dense(u_o, c["dt"], 1e-8, 20)
+        assert st[0]["converged"] and conv
+        assert np.abs(u_g - u_o).max() <= 1e-8 * np.abs(u_o).max(), k
+    s.close()
