@@ -342,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void coarse_nodecb_kernel(int32_t n_nodes,
   }
 }
 
-// The first product workgroup of every segment also forms its tissue's
+// The first two product workgroups of every segment also form its tissue's
 // ordered sums of the (r, z_D) and (r, r) tile partials (the same sums the
 // search-direction update would form), so that the update's workgroups read
 // two numbers instead of every tile partial.

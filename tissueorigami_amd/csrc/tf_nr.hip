@@ -505,8 +505,7 @@ tf_status ensure_gmres(tf_handle* h) {
   return TF_OK;
 }
 
-// Tiles of spt slices that never cross a tissue boundary (tissues start on
-// slice boundaries).  Rebuilding replaces the device tile table.
+// Buffers of the warm start and of the forcing-term safeguard (first use).
 tf_status ensure_warm(tf_handle* h) {
   Workspace& w = h->ws;
   if (w.x0) return TF_OK;
@@ -524,6 +523,8 @@ tf_status ensure_warm(tf_handle* h) {
 
 void reset_history(tf_handle* h) { h->n_hist.assign(h->n_tissues, 0); }
 
+// Tiles of spt slices that never cross a tissue boundary (tissues start on
+// slice boundaries).  Rebuilding replaces the device tile table.
 tf_status build_tiles(tf_handle* h, int spt) {
   std::vector<int32_t> ts, tt, tissue_tile(h->n_tissues + 1, 0);
   ts.push_back(0);
