@@ -354,7 +354,7 @@ __device__ inline void s1_prereduce(const S1Pre& pre, int T, int block) {
   const bool two = g1 - g0 > 1;
   const bool do_rz = block == g0, do_rr = two ? block == g0 + 1 : block == g0;
   if (!do_rz && !do_rr) return;
-  const int t0 = pre.tissue_tile[T], t1 = pre.tissue_tile[T + 1];
+  const int t0 = pre.part_off[T], t1 = pre.part_off[T + 1];
   if (do_rz) {
     const double v = wg_ordered_sum(pre.rz, t0, t1);
     if (threadIdx.x == 0) pre.out[2 * T] = v;
@@ -740,6 +740,9 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
   if ((e = upload(c, &c.y, std::vector<float>((size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
   if ((e = upload(c, &c.cpart, std::vector<double>((size_t)std::max(c.n_gemv, 1), 0.0))) != hipSuccess) return e;
   if ((e = upload(c, &c.s1pre, std::vector<double>(2 * (size_t)n_seg, 0.0))) != hipSuccess) return e;
+  if ((e = upload(c, &c.seg_agg_d, seg_agg)) != hipSuccess) return e;
+  if ((e = upload(c, &c.agg_rz, std::vector<double>((size_t)std::max(n_agg, 1), 0.0))) != hipSuccess) return e;
+  if ((e = upload(c, &c.agg_rr, std::vector<double>((size_t)std::max(n_agg, 1), 0.0))) != hipSuccess) return e;
   c.built = true;
   c.active = false;
   return hipSuccess;
@@ -1058,6 +1061,18 @@ void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, const
                      c.list_mask, r, state, c.w);
   hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
                      c.sgio, c.seg_on, c.inv32, c.w, state, c.y, c.cpart, pre);
+}
+
+void coarse_product(const CoarseOp& c, const int32_t* state, const S1Pre& pre, float* /*tag*/,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
+                     c.sgio, c.seg_on, c.inv32, c.w, state, c.y, c.cpart, pre);
+}
+
+void coarse_product(const CoarseOp& c, const int32_t* state, const S1Pre& pre, double* /*tag*/,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(coarse_gemv64_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
+                     c.sgio, c.seg_on, c.inv64, c.w64, state, c.y64, c.cpart, pre);
 }
 
 void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, const S1Pre& pre,

@@ -155,6 +155,9 @@ struct CoarseOp {
   int want64 = 0;               // the factorisation in progress also writes inv64
   double* cpart = nullptr;      // per workgroup of the product: partial (w, y)
   double* s1pre = nullptr;      // 2 per segment: ordered sums of the (r, z_D), (r, r) partials
+  int32_t* seg_agg_d = nullptr; // n_seg + 1: first aggregate of each segment (device)
+  double* agg_rz = nullptr;     // per aggregate: partial (r, z_D) of the update
+  double* agg_rr = nullptr;     // per aggregate: partial (r, r)
   std::vector<void*> allocs;
   void* blas = nullptr;         // rocblas_handle
   int32_t gen = 0;              // incremented by every build (captured graphs key on it)
@@ -272,12 +275,14 @@ inline int xcd_grid(int64_t n) {
 
 // ---- per-tissue scalars of the tiled solvers ----
 // Ordered sum of part[beg..end), identical in every thread of the workgroup.
+// B is the workgroup size.
+template <int B = kBlock>
 __device__ inline double wg_ordered_sum(const double* part, int beg, int end) {
-  constexpr int kWaves = kBlock / 64;
+  constexpr int kWaves = B / 64;
   __shared__ double red[kWaves];
   __shared__ double result;
   double s = 0.0;
-  for (int i = beg + (int)threadIdx.x; i < end; i += kBlock) s += part[i];
+  for (int i = beg + (int)threadIdx.x; i < end; i += B) s += part[i];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -359,7 +364,9 @@ struct CgS1 {
 struct S1Pre {
   const double* rz = nullptr;
   const double* rr = nullptr;
-  const int32_t* tissue_tile = nullptr;
+  // the partials of tissue T are rz/rr[part_off[T] .. part_off[T+1]): one per
+  // tile, or one per aggregate after the update that forms the restriction
+  const int32_t* part_off = nullptr;
   const int32_t* cpart_off = nullptr;  // first product workgroup of each segment
   double* out = nullptr;               // null: nothing to form
 };
@@ -801,6 +808,12 @@ void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, const
 // FP64: y64 = A_c^-1 P^T r with the FP64 inverse, w64 and y64 in FP64.
 void coarse_apply(const CoarseOp& c, const int32_t* state, const double* r, const S1Pre& pre,
                   hipStream_t st);
+// y = A_c^-1 w and the (w, y) partials only, for a w that the CG update formed
+// (the FP32 or the FP64 product, chosen by the tag's type).
+void coarse_product(const CoarseOp& c, const int32_t* state, const S1Pre& pre, float* tag,
+                    hipStream_t st);
+void coarse_product(const CoarseOp& c, const int32_t* state, const S1Pre& pre, double* tag,
+                    hipStream_t st);
 
 void set_error(const std::string& s);
 

@@ -580,6 +580,147 @@ __global__ __launch_bounds__(kBlock) void tzc_update(TC c, T* x, T* r, T* z, con
   }
 }
 
+// Coarse data of the CG update that also forms the restriction w = P^T r.
+template <typename T>
+struct CoarseU {
+  const int32_t* agg_ptr;
+  const int32_t* agg_nodes;
+  const int32_t* agg_tissue;
+  const int32_t* agg_cbase;
+  const int32_t* seg_agg;  // first aggregate of each tissue
+  const float* list_d;
+  const uint8_t* list_mask;
+  T* w;
+  double* rz;  // per aggregate: partial (r, z_D)
+  double* rr;  // per aggregate: partial (r, r)
+  int32_t n_agg;
+};
+
+constexpr int kUBlock = 512;
+
+// Workgroup sum of n values per thread in a fixed order (wave trees, then the
+// waves in order), valid in thread 0.
+template <int N, int B>
+__device__ inline void wg_sum(double (&v)[N]) {
+  __shared__ double red[N][B / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) red[k][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < B / 64; ++w) t += red[k][w];
+      v[k] = t;
+    }
+  }
+}
+
+// The restriction terms of one node (r after the update) into acc[2..7]:
+// translations r, rotations d x r, on the DOF that take part in the coarse space.
+template <typename T>
+__device__ inline void restrict_row(const T (&rv)[3], const float* d3, uint8_t mk, double* acc) {
+  const double r0 = (mk & 1) ? 0.0 : (double)rv[0];
+  const double r1 = (mk & 2) ? 0.0 : (double)rv[1];
+  const double r2 = (mk & 4) ? 0.0 : (double)rv[2];
+  const double d0 = d3[0], d1 = d3[1], d2 = d3[2];
+  acc[2] += r0;
+  acc[3] += r1;
+  acc[4] += r2;
+  acc[5] += d1 * r2 - d2 * r1;
+  acc[6] += d2 * r0 - d0 * r2;
+  acc[7] += d0 * r1 - d1 * r0;
+}
+
+// S2 and the update of tzc_update over the node list of one aggregate (one
+// workgroup per aggregate), which also forms the aggregate's w = P^T r from the
+// updated r: the restriction needs no launch of its own.  The partials
+// (r, z_D), (r, r) are per aggregate.  Every node of a tissue belongs to one
+// aggregate; rows of no aggregate (the padding rows of a batch) hold
+// x = r = z = 0 and are left alone.
+template <typename T>
+__global__ __launch_bounds__(kUBlock) void tzc_update_agg(TC c, CoarseU<T> cu, T* x, T* r, T* z,
+                                                         const T* p, const T* q,
+                                                         const T* minv) {
+  const int a = xcd_item(blockIdx.x, gridDim.x);
+  if (a >= cu.n_agg) return;
+  const int T_ = cu.agg_tissue[a];
+  if (c.state[2 * T_] != K_RUN) return;
+  const bool first = a == cu.seg_agg[T_];
+  const int beg = cu.agg_ptr[a] + threadIdx.x, end = cu.agg_ptr[a + 1];
+  int idx[kUpre];
+  T pv[kUpre][3], qv[kUpre][3], xv[kUpre][3], rv[kUpre][3], m[kUpre][9];
+#pragma unroll
+  for (int k = 0; k < kUpre; ++k) {
+    const int t = beg + k * kUBlock;
+    idx[k] = t < end ? cu.agg_nodes[t] : -1;
+  }
+#pragma unroll
+  for (int k = 0; k < kUpre; ++k) {
+    const int i = idx[k];
+    const bool in = i >= 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pv[k][d] = in ? p[3 * i + d] : (T)0;
+      qv[k][d] = in ? q[3 * i + d] : (T)0;
+      xv[k][d] = in ? x[3 * i + d] : (T)0;
+      rv[k][d] = in ? r[3 * i + d] : (T)0;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m[k][e] = in ? minv[9 * (size_t)i + e] : (T)0;
+  }
+  const double rho = c.scal[8 * T_ + TS_RHON];
+  const double pq =
+      wg_ordered_sum<kUBlock>(TP(c, TC_PQ), c.tm.tissue_tile[T_], c.tm.tissue_tile[T_ + 1]);
+  const double alpha = rho / pq;
+  if (!(pq > 0.0) || !isfinite(alpha)) {
+    if (first && threadIdx.x == 0) c.state[2 * T_] = K_BREAKDOWN;
+    return;
+  }
+  const T ta = (T)alpha;
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kUpre; ++k) {
+    const int i = idx[k];
+    if (i < 0) continue;
+    const int t = beg + k * kUBlock;
+    update_row<T>(ta, pv[k], qv[k], xv[k], rv[k], m[k], x, r, z, i, acc);
+    restrict_row<T>(rv[k], cu.list_d + 3 * (size_t)t, cu.list_mask[t], acc);
+  }
+  for (int t = beg + kUpre * kUBlock; t < end; t += kUBlock) {
+    const int i = cu.agg_nodes[t];
+    const T p1[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    const T q1[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+    T x1[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    T r1[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+    T m1[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m1[e] = minv[9 * (size_t)i + e];
+    update_row<T>(ta, p1, q1, x1, r1, m1, x, r, z, i, acc);
+    restrict_row<T>(r1, cu.list_d + 3 * (size_t)t, cu.list_mask[t], acc);
+  }
+  wg_sum<8, kUBlock>(acc);
+  if (threadIdx.x == 0) {
+    cu.rz[a] = acc[0];
+    cu.rr[a] = acc[1];
+    const int cb = cu.agg_cbase[a];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) cu.w[cb + k] = (T)acc[2 + k];
+    if (first) {
+      c.scal[8 * T_ + TS_ALPHA] = alpha;
+      c.scal[8 * T_ + TS_RHO] = rho;
+      c.state[2 * T_ + 1] = c.state[2 * T_ + 1] + 1;
+    }
+  }
+}
+
 // Coarse data of the search-direction update (CG with a coarse space).
 template <typename T>
 struct CoarseP {
@@ -999,7 +1140,7 @@ void coarse_half(const TC& c, const CV<T>& V, int G, hipStream_t st) {
   S1Pre pre;
   pre.rz = s1.rz;
   pre.rr = s1.rr;
-  pre.tissue_tile = s1.tissue_tile;
+  pre.part_off = s1.tissue_tile;
   pre.cpart_off = V.co->cpart_off;
   pre.out = V.co->s1pre;
   coarse_apply(*V.co, c.state, (const T*)V.r, pre, st);
@@ -1012,6 +1153,47 @@ void coarse_half(const TC& c, const CV<T>& V, int G, hipStream_t st) {
   const CoarseP<T> cp{V.co->node_cb, V.co->seg_on, V.co->node_d, y};
   hipLaunchKernelGGL(tzc_pupdate<T>, dim3(G), dim3(kBlock), 0, st, c, s1, cp, (const T*)V.z,
                      V.p0);
+}
+
+// The coarse half of a CG iteration whose update formed w: product, then S1
+// (from the per-aggregate partials) and p = (z_D + P y) + beta p.
+template <typename T>
+void coarse_half_fused(const TC& c, const CV<T>& V, int G, hipStream_t st) {
+  CgS1 s1 = host_s1(c);
+  S1Pre pre;
+  pre.rz = V.co->agg_rz;
+  pre.rr = V.co->agg_rr;
+  pre.part_off = V.co->seg_agg_d;
+  pre.cpart_off = V.co->cpart_off;
+  pre.out = V.co->s1pre;
+  coarse_product(*V.co, c.state, pre, (T*)nullptr, st);
+  s1.cpart = V.co->cpart;
+  s1.cpart_off = V.co->cpart_off;
+  s1.pre = V.co->s1pre;
+  const T* y;
+  if constexpr (std::is_same<T, float>::value) y = V.co->y;
+  else y = V.co->y64;
+  const CoarseP<T> cp{V.co->node_cb, V.co->seg_on, V.co->node_d, y};
+  hipLaunchKernelGGL(tzc_pupdate<T>, dim3(G), dim3(kBlock), 0, st, c, s1, cp, (const T*)V.z,
+                     V.p0);
+}
+
+template <typename T>
+CoarseU<T> coarse_u(const CoarseOp& co) {
+  CoarseU<T> u;
+  u.agg_ptr = co.agg_ptr;
+  u.agg_nodes = co.agg_nodes;
+  u.agg_tissue = co.agg_tissue;
+  u.agg_cbase = co.agg_cbase;
+  u.seg_agg = co.seg_agg_d;
+  u.list_d = co.list_d;
+  u.list_mask = co.list_mask;
+  if constexpr (std::is_same<T, float>::value) u.w = co.w;
+  else u.w = co.w64;
+  u.rz = co.agg_rz;
+  u.rr = co.agg_rr;
+  u.n_agg = co.n_agg;
+  return u;
 }
 
 // r = b, x = 0, p = 0, z; with the coarse space also the first S1 and p.
@@ -1041,9 +1223,10 @@ void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chu
     if (V.co) {
       hipExtLaunchKernelGGL(tzc_spmv<T, false>, dim3(G), dim3(kBlock), 0, st, ev0, ev1, 0, c,
                             V.val, (const T*)V.z, (const T*)V.p0, (T*)nullptr, V.q);
-      hipLaunchKernelGGL(tzc_update<T>, dim3(G), dim3(kBlock), 0, st, c, V.x, V.r, V.z,
-                         (const T*)V.p0, (const T*)V.q, V.minv);
-      coarse_half<T>(c, V, G, st);
+      hipLaunchKernelGGL(tzc_update_agg<T>, dim3(xcd_grid(V.co->n_agg)), dim3(kUBlock), 0, st, c,
+                         coarse_u<T>(*V.co), V.x, V.r, V.z, (const T*)V.p0, (const T*)V.q,
+                         V.minv);
+      coarse_half_fused<T>(c, V, G, st);
     } else {
       const int par = (it0 + it) & 1;
       const T* pold = par ? V.p1 : V.p0;
