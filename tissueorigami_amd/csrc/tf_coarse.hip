@@ -366,12 +366,13 @@ __device__ inline void s1_prereduce(const S1Pre& pre, int T, int block) {
 }
 
 // y = A_c^-1 w, one wavefront per row (16-byte loads over the padded row
-// stride ld); partial (w, y) per workgroup.
+// stride ld); partial (w, y) per workgroup.  With w2, w is the sum w + w2 of
+// the two half-aggregate terms, formed in that order.
 typedef float v4f __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
     const int32_t* __restrict__ gw, const int32_t* __restrict__ sgi,
     const int64_t* __restrict__ sgio, const int32_t* __restrict__ seg_on,
-    const float* __restrict__ inv32, const float* __restrict__ w,
+    const float* __restrict__ inv32, const float* __restrict__ w, const float* __restrict__ w2,
     const int32_t* __restrict__ state, float* __restrict__ y, double* __restrict__ cpart,
     S1Pre pre) {
   __shared__ double wy[kGemvRows];
@@ -386,15 +387,25 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
     const v4f* a = (const v4f*)(inv32 + sgio[T] + (size_t)row * ld);
     const float* ws = w + off;
     float s = 0.f;
-    for (int j = lane; j < ld / 4; j += 64) {
-      const v4f v = ((const v4f*)ws)[j];
-      const v4f u = a[j];
-      s += u.x * v.x + u.y * v.y + u.z * v.z + u.w * v.w;
+    if (w2) {
+      const float* ws2 = w2 + off;
+      for (int j = lane; j < ld / 4; j += 64) {
+        const v4f v = ((const v4f*)ws)[j] + ((const v4f*)ws2)[j];
+        const v4f u = a[j];
+        s += u.x * v.x + u.y * v.y + u.z * v.z + u.w * v.w;
+      }
+    } else {
+      for (int j = lane; j < ld / 4; j += 64) {
+        const v4f v = ((const v4f*)ws)[j];
+        const v4f u = a[j];
+        s += u.x * v.x + u.y * v.y + u.z * v.z + u.w * v.w;
+      }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) y[off + row] = s;
-    dot = (double)ws[row] * (double)s;
+    const float wr = w2 ? ws[row] + w2[off + row] : ws[row];
+    dot = (double)wr * (double)s;
   }
   if (lane == 0) wy[wave] = dot;
   __syncthreads();
@@ -404,6 +415,13 @@ __global__ __launch_bounds__(kBlock) void coarse_gemv_kernel(
     for (int k = 0; k < kGemvRows; ++k) t += wy[k];
     cpart[blockIdx.x] = t;
   }
+}
+
+// The FP32 product; w2 as above (null: w alone).
+void launch_gemv32(const CoarseOp& c, const int32_t* state, const float* w2, const S1Pre& pre,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
+                     c.sgio, c.seg_on, c.inv32, c.w, w2, state, c.y, c.cpart, pre);
 }
 
 // The FP64 product for the FP64 CG: y = A_c^-1 w with the FP64 inverse.
@@ -741,8 +759,14 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
   if ((e = upload(c, &c.cpart, std::vector<double>((size_t)std::max(c.n_gemv, 1), 0.0))) != hipSuccess) return e;
   if ((e = upload(c, &c.s1pre, std::vector<double>(2 * (size_t)n_seg, 0.0))) != hipSuccess) return e;
   if ((e = upload(c, &c.seg_agg_d, seg_agg)) != hipSuccess) return e;
-  if ((e = upload(c, &c.agg_rz, std::vector<double>((size_t)std::max(n_agg, 1), 0.0))) != hipSuccess) return e;
-  if ((e = upload(c, &c.agg_rr, std::vector<double>((size_t)std::max(n_agg, 1), 0.0))) != hipSuccess) return e;
+  {
+    std::vector<int32_t> seg_half(seg_agg.size());
+    for (size_t k = 0; k < seg_agg.size(); ++k) seg_half[k] = 2 * seg_agg[k];
+    if ((e = upload(c, &c.seg_half_d, seg_half)) != hipSuccess) return e;
+  }
+  if ((e = upload(c, &c.w2, std::vector<float>((size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
+  if ((e = upload(c, &c.agg_rz, std::vector<double>(2 * (size_t)std::max(n_agg, 1), 0.0))) != hipSuccess) return e;
+  if ((e = upload(c, &c.agg_rr, std::vector<double>(2 * (size_t)std::max(n_agg, 1), 0.0))) != hipSuccess) return e;
   c.built = true;
   c.active = false;
   return hipSuccess;
@@ -1059,14 +1083,13 @@ void coarse_apply(const CoarseOp& c, const int32_t* state, const float* r, const
   hipLaunchKernelGGL(coarse_restrict_kernel<float>, dim3(c.n_agg), dim3(kRBlock), 0, st, c.n_agg,
                      c.agg_ptr, c.agg_nodes, c.agg_tissue, c.agg_cbase, c.seg_on, c.list_d,
                      c.list_mask, r, state, c.w);
-  hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
-                     c.sgio, c.seg_on, c.inv32, c.w, state, c.y, c.cpart, pre);
+  launch_gemv32(c, state, nullptr, pre, st);
 }
 
+// After the FP32 update that formed w in two half-aggregate terms (w, w2).
 void coarse_product(const CoarseOp& c, const int32_t* state, const S1Pre& pre, float* /*tag*/,
                     hipStream_t st) {
-  hipLaunchKernelGGL(coarse_gemv_kernel, dim3(c.n_gemv), dim3(kBlock), 0, st, c.gw, c.sgi,
-                     c.sgio, c.seg_on, c.inv32, c.w, state, c.y, c.cpart, pre);
+  launch_gemv32(c, state, c.w2, pre, st);
 }
 
 void coarse_product(const CoarseOp& c, const int32_t* state, const S1Pre& pre, double* /*tag*/,

@@ -147,6 +147,7 @@ struct CoarseOp {
   int32_t* info = nullptr;      // 2 per segment: getrf, getri
   float* inv32 = nullptr;       // per segment A_c^-1, row-major FP32
   float* w = nullptr;           // P^T r (vlen entries, zero padding)
+  float* w2 = nullptr;          // the FP32 update's second half-aggregate terms of P^T r
   float* y = nullptr;           // A_c^-1 P^T r
   // the same in FP64, for the FP64 CG (allocated by its first factorisation)
   double* inv64 = nullptr;
@@ -156,8 +157,9 @@ struct CoarseOp {
   double* cpart = nullptr;      // per workgroup of the product: partial (w, y)
   double* s1pre = nullptr;      // 2 per segment: ordered sums of the (r, z_D), (r, r) partials
   int32_t* seg_agg_d = nullptr; // n_seg + 1: first aggregate of each segment (device)
-  double* agg_rz = nullptr;     // per aggregate: partial (r, z_D) of the update
-  double* agg_rr = nullptr;     // per aggregate: partial (r, r)
+  int32_t* seg_half_d = nullptr; // n_seg + 1: twice seg_agg_d (first half-aggregate)
+  double* agg_rz = nullptr;     // per half-aggregate: partial (r, z_D) of the FP32 update
+  double* agg_rr = nullptr;     // per half-aggregate: partial (r, r)
   std::vector<void*> allocs;
   void* blas = nullptr;         // rocblas_handle
   int32_t gen = 0;              // incremented by every build (captured graphs key on it)

@@ -590,13 +590,16 @@ struct CoarseU {
   const int32_t* seg_agg;  // first aggregate of each tissue
   const float* list_d;
   const uint8_t* list_mask;
-  T* w;
-  double* rz;  // per aggregate: partial (r, z_D)
-  double* rr;  // per aggregate: partial (r, r)
+  T* w;        // P^T r terms of the first half of every aggregate's list
+  T* w2;       // and of the second half
+  double* rz;  // per half-aggregate: partial (r, z_D)
+  double* rr;  // per half-aggregate: partial (r, r)
   int32_t n_agg;
 };
 
-constexpr int kUBlock = 512;
+// One workgroup per half of an aggregate's node list: 630 aggregates at C3
+// make 1260 workgroups, which spread over the CUs more evenly than 630.
+constexpr int kUBlock = 256;
 
 // Workgroup sum of n values per thread in a fixed order (wave trees, then the
 // waves in order), valid in thread 0.
@@ -639,22 +642,25 @@ __device__ inline void restrict_row(const T (&rv)[3], const float* d3, uint8_t m
   acc[7] += d0 * r1 - d1 * r0;
 }
 
-// S2 and the update of tzc_update over the node list of one aggregate (one
-// workgroup per aggregate), which also forms the aggregate's w = P^T r from the
-// updated r: the restriction needs no launch of its own.  The partials
-// (r, z_D), (r, r) are per aggregate.  Every node of a tissue belongs to one
-// aggregate; rows of no aggregate (the padding rows of a batch) hold
-// x = r = z = 0 and are left alone.
+// S2 and the update of tzc_update over the node list of one aggregate, which
+// also forms the aggregate's w = P^T r from the updated r: the restriction
+// needs no launch of its own.  Workgroup h takes half h & 1 of aggregate h >> 1
+// and writes its terms of w to w (first half) or w2 (second half); the product
+// adds the two.  The partials (r, z_D), (r, r) are per half-aggregate.  Every
+// node of a tissue belongs to one aggregate; rows of no aggregate (the padding
+// rows of a batch) hold x = r = z = 0 and are left alone.
 template <typename T>
 __global__ __launch_bounds__(kUBlock) void tzc_update_agg(TC c, CoarseU<T> cu, T* x, T* r, T* z,
                                                          const T* p, const T* q,
                                                          const T* minv) {
-  const int a = xcd_item(blockIdx.x, gridDim.x);
-  if (a >= cu.n_agg) return;
+  const int h = xcd_item(blockIdx.x, gridDim.x);
+  if (h >= 2 * cu.n_agg) return;
+  const int a = h >> 1, half = h & 1;
   const int T_ = cu.agg_tissue[a];
   if (c.state[2 * T_] != K_RUN) return;
-  const bool first = a == cu.seg_agg[T_];
-  const int beg = cu.agg_ptr[a] + threadIdx.x, end = cu.agg_ptr[a + 1];
+  const bool first = a == cu.seg_agg[T_] && half == 0;
+  const int a0 = cu.agg_ptr[a], a1 = cu.agg_ptr[a + 1], mid = a0 + (a1 - a0 + 1) / 2;
+  const int beg = (half ? mid : a0) + threadIdx.x, end = half ? a1 : mid;
   int idx[kUpre];
   T pv[kUpre][3], qv[kUpre][3], xv[kUpre][3], rv[kUpre][3], m[kUpre][9];
 #pragma unroll
@@ -708,11 +714,12 @@ __global__ __launch_bounds__(kUBlock) void tzc_update_agg(TC c, CoarseU<T> cu, T
   }
   wg_sum<8, kUBlock>(acc);
   if (threadIdx.x == 0) {
-    cu.rz[a] = acc[0];
-    cu.rr[a] = acc[1];
+    cu.rz[h] = acc[0];
+    cu.rr[h] = acc[1];
     const int cb = cu.agg_cbase[a];
+    T* wh = half ? cu.w2 : cu.w;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) cu.w[cb + k] = (T)acc[2 + k];
+    for (int k = 0; k < 6; ++k) wh[cb + k] = (T)acc[2 + k];
     if (first) {
       c.scal[8 * T_ + TS_ALPHA] = alpha;
       c.scal[8 * T_ + TS_RHO] = rho;
@@ -1163,7 +1170,7 @@ void coarse_half_fused(const TC& c, const CV<T>& V, int G, hipStream_t st) {
   S1Pre pre;
   pre.rz = V.co->agg_rz;
   pre.rr = V.co->agg_rr;
-  pre.part_off = V.co->seg_agg_d;
+  pre.part_off = V.co->seg_half_d;
   pre.cpart_off = V.co->cpart_off;
   pre.out = V.co->s1pre;
   coarse_product(*V.co, c.state, pre, (T*)nullptr, st);
@@ -1188,8 +1195,13 @@ CoarseU<T> coarse_u(const CoarseOp& co) {
   u.seg_agg = co.seg_agg_d;
   u.list_d = co.list_d;
   u.list_mask = co.list_mask;
-  if constexpr (std::is_same<T, float>::value) u.w = co.w;
-  else u.w = co.w64;
+  if constexpr (std::is_same<T, float>::value) {
+    u.w = co.w;
+    u.w2 = co.w2;
+  } else {
+    u.w = co.w64;
+    u.w2 = nullptr;  // the FP64 CG keeps the tile update and the restriction launch
+  }
   u.rz = co.agg_rz;
   u.rr = co.agg_rr;
   u.n_agg = co.n_agg;
@@ -1225,7 +1237,7 @@ void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chu
                             V.val, (const T*)V.z, (const T*)V.p0, (T*)nullptr, V.q);
       if constexpr (sizeof(T) == 4) {
         // FP32: the update walks the aggregate lists and forms the restriction
-        hipLaunchKernelGGL(tzc_update_agg<T>, dim3(xcd_grid(V.co->n_agg)), dim3(kUBlock), 0, st,
+        hipLaunchKernelGGL(tzc_update_agg<T>, dim3(xcd_grid(2 * V.co->n_agg)), dim3(kUBlock), 0, st,
                            c, coarse_u<T>(*V.co), V.x, V.r, V.z, (const T*)V.p0,
                            (const T*)V.q, V.minv);
         coarse_half_fused<T>(c, V, G, st);
