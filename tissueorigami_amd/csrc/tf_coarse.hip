@@ -765,6 +765,9 @@ hipError_t coarse_build(CoarseOp& c, int32_t n_nodes, const double* X,
     if ((e = upload(c, &c.seg_half_d, seg_half)) != hipSuccess) return e;
   }
   if ((e = upload(c, &c.w2, std::vector<float>((size_t)std::max(vlen, 1), 0.f))) != hipSuccess) return e;
+  if ((e = upload(c, &c.xl, std::vector<float>(), 3 * (size_t)std::max(n_list, 1))) != hipSuccess) return e;
+  if ((e = upload(c, &c.rl, std::vector<float>(), 3 * (size_t)std::max(n_list, 1))) != hipSuccess) return e;
+  if ((e = upload(c, &c.ml, std::vector<float>(), 9 * (size_t)std::max(n_list, 1))) != hipSuccess) return e;
   if ((e = upload(c, &c.agg_rz, std::vector<double>(2 * (size_t)std::max(n_agg, 1), 0.0))) != hipSuccess) return e;
   if ((e = upload(c, &c.agg_rr, std::vector<double>(2 * (size_t)std::max(n_agg, 1), 0.0))) != hipSuccess) return e;
   c.built = true;

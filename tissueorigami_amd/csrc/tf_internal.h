@@ -148,6 +148,11 @@ struct CoarseOp {
   float* inv32 = nullptr;       // per segment A_c^-1, row-major FP32
   float* w = nullptr;           // P^T r (vlen entries, zero padding)
   float* w2 = nullptr;          // the FP32 update's second half-aggregate terms of P^T r
+  // the FP32 CG's x, r (3 per list position) and block-Jacobi inverse (9), in
+  // the order of agg_nodes, which the update streams
+  float* xl = nullptr;
+  float* rl = nullptr;
+  float* ml = nullptr;
   float* y = nullptr;           // A_c^-1 P^T r
   // the same in FP64, for the FP64 CG (allocated by its first factorisation)
   double* inv64 = nullptr;

@@ -649,10 +649,36 @@ __device__ inline void restrict_row(const T (&rv)[3], const float* d3, uint8_t m
 // adds the two.  The partials (r, z_D), (r, r) are per half-aggregate.  Every
 // node of a tissue belongs to one aggregate; rows of no aggregate (the padding
 // rows of a batch) hold x = r = z = 0 and are left alone.
+// update_row with x, r at list position t and z at node i
 template <typename T>
-__global__ __launch_bounds__(kUBlock) void tzc_update_agg(TC c, CoarseU<T> cu, T* x, T* r, T* z,
+__device__ inline void update_row_l(T ta, const T (&pv)[3], const T (&qv)[3], T (&xv)[3],
+                                    T (&rv)[3], const T (&m)[9], T* xl, T* rl, T* z, int t, int i,
+                                    double* acc) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    xv[d] = fma_t(ta, pv[d], xv[d]);
+    rv[d] = fma_t(-ta, qv[d], rv[d]);
+    xl[3 * t + d] = xv[d];
+    rl[3 * t + d] = rv[d];
+  }
+  const T zv[3] = {m[0] * rv[0] + m[1] * rv[1] + m[2] * rv[2],
+                   m[3] * rv[0] + m[4] * rv[1] + m[5] * rv[2],
+                   m[6] * rv[0] + m[7] * rv[1] + m[8] * rv[2]};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    z[3 * i + d] = zv[d];
+    acc[0] += (double)rv[d] * (double)zv[d];
+    acc[1] += (double)rv[d] * (double)rv[d];
+  }
+}
+
+// x, r and the block-Jacobi inverse are read and written in list order (xl,
+// rl, ml: position t of agg_nodes), so that they stream; p and q are gathered
+// and z is written at the node.
+template <typename T>
+__global__ __launch_bounds__(kUBlock) void tzc_update_agg(TC c, CoarseU<T> cu, T* xl, T* rl, T* z,
                                                          const T* p, const T* q,
-                                                         const T* minv) {
+                                                         const T* ml) {
   const int h = xcd_item(blockIdx.x, gridDim.x);
   if (h >= 2 * cu.n_agg) return;
   const int a = h >> 1, half = h & 1;
@@ -670,17 +696,17 @@ __global__ __launch_bounds__(kUBlock) void tzc_update_agg(TC c, CoarseU<T> cu, T
   }
 #pragma unroll
   for (int k = 0; k < kUpre; ++k) {
-    const int i = idx[k];
+    const int i = idx[k], t = beg + k * kUBlock;
     const bool in = i >= 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       pv[k][d] = in ? p[3 * i + d] : (T)0;
       qv[k][d] = in ? q[3 * i + d] : (T)0;
-      xv[k][d] = in ? x[3 * i + d] : (T)0;
-      rv[k][d] = in ? r[3 * i + d] : (T)0;
+      xv[k][d] = in ? xl[3 * t + d] : (T)0;
+      rv[k][d] = in ? rl[3 * t + d] : (T)0;
     }
 #pragma unroll
-    for (int e = 0; e < 9; ++e) m[k][e] = in ? minv[9 * (size_t)i + e] : (T)0;
+    for (int e = 0; e < 9; ++e) m[k][e] = in ? ml[9 * (size_t)t + e] : (T)0;
   }
   const double rho = c.scal[8 * T_ + TS_RHON];
   const double pq =
@@ -697,19 +723,19 @@ __global__ __launch_bounds__(kUBlock) void tzc_update_agg(TC c, CoarseU<T> cu, T
     const int i = idx[k];
     if (i < 0) continue;
     const int t = beg + k * kUBlock;
-    update_row<T>(ta, pv[k], qv[k], xv[k], rv[k], m[k], x, r, z, i, acc);
+    update_row_l<T>(ta, pv[k], qv[k], xv[k], rv[k], m[k], xl, rl, z, t, i, acc);
     restrict_row<T>(rv[k], cu.list_d + 3 * (size_t)t, cu.list_mask[t], acc);
   }
   for (int t = beg + kUpre * kUBlock; t < end; t += kUBlock) {
     const int i = cu.agg_nodes[t];
     const T p1[3] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     const T q1[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
-    T x1[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
-    T r1[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+    T x1[3] = {xl[3 * t], xl[3 * t + 1], xl[3 * t + 2]};
+    T r1[3] = {rl[3 * t], rl[3 * t + 1], rl[3 * t + 2]};
     T m1[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) m1[e] = minv[9 * (size_t)i + e];
-    update_row<T>(ta, p1, q1, x1, r1, m1, x, r, z, i, acc);
+    for (int e = 0; e < 9; ++e) m1[e] = ml[9 * (size_t)t + e];
+    update_row_l<T>(ta, p1, q1, x1, r1, m1, xl, rl, z, t, i, acc);
     restrict_row<T>(r1, cu.list_d + 3 * (size_t)t, cu.list_mask[t], acc);
   }
   wg_sum<8, kUBlock>(acc);
@@ -1208,12 +1234,53 @@ CoarseU<T> coarse_u(const CoarseOp& co) {
   return u;
 }
 
+// The FP32 CG with the coarse space keeps x, r and the block-Jacobi inverse of
+// the nodes of aggregates in list order: after tzc_init, r and the inverse are
+// copied in and x starts at zero; at the end x goes back to node order (the
+// rows of no aggregate keep tzc_init's zero).
+__global__ __launch_bounds__(kBlock) void tzc_list_in(int32_t n_list,
+                                                      const int32_t* __restrict__ nodes,
+                                                      const float* __restrict__ r,
+                                                      const float* __restrict__ minv,
+                                                      float* __restrict__ xl, float* __restrict__ rl,
+                                                      float* __restrict__ ml) {
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n_list; t += gridDim.x * kBlock) {
+    const int i = nodes[t];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      rl[3 * t + d] = r[3 * i + d];
+      xl[3 * t + d] = 0.f;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) ml[9 * (size_t)t + e] = minv[9 * (size_t)i + e];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void tzc_list_out(int32_t n_list,
+                                                       const int32_t* __restrict__ nodes,
+                                                       const float* __restrict__ xl,
+                                                       float* __restrict__ x) {
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n_list; t += gridDim.x * kBlock) {
+    const int i = nodes[t];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[3 * i + d] = xl[3 * t + d];
+  }
+}
+
+inline int list_grid(int32_t n) { return std::max(1, std::min((n + kBlock - 1) / kBlock, 8192)); }
+
 // r = b, x = 0, p = 0, z; with the coarse space also the first S1 and p.
 template <typename T>
 void cg_start(const TC& c, const CV<T>& V, const double* b, int G, hipStream_t st) {
   hipLaunchKernelGGL(tzc_init<T>, dim3(G), dim3(kBlock), 0, st, c, b, V.r, V.z, V.x, V.p0,
                      V.minv);
-  if (V.co) coarse_half<T>(c, V, G, st);
+  if (V.co) {
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL(tzc_list_in, dim3(list_grid(V.co->agg_total)), dim3(kBlock), 0, st,
+                         V.co->agg_total, V.co->agg_nodes, (const float*)V.r,
+                         (const float*)V.minv, V.co->xl, V.co->rl, V.co->ml);
+    coarse_half<T>(c, V, G, st);
+  }
 }
 
 template <typename T>
@@ -1238,8 +1305,8 @@ void launch_cg_chunk(const TC& c, const CV<T>& V, int G, hipStream_t st, int chu
       if constexpr (sizeof(T) == 4) {
         // FP32: the update walks the aggregate lists and forms the restriction
         hipLaunchKernelGGL(tzc_update_agg<T>, dim3(xcd_grid(2 * V.co->n_agg)), dim3(kUBlock), 0, st,
-                           c, coarse_u<T>(*V.co), V.x, V.r, V.z, (const T*)V.p0,
-                           (const T*)V.q, V.minv);
+                           c, coarse_u<T>(*V.co), V.co->xl, V.co->rl, V.z, (const T*)V.p0,
+                           (const T*)V.q, (const T*)V.co->ml);
         coarse_half_fused<T>(c, V, G, st);
       } else {
         // FP64: the list-order gathers of 8-byte rows cost the update more than the
@@ -1300,6 +1367,13 @@ hipError_t tiled_cg_impl(const KrylovArgs& k, const CV<T>& V, const double* b, d
     bool running = false;
     for (int q = 0; q < nt; ++q) running |= (ws->host_tstate[2 * q] == K_RUN);
     if (!running || done >= maxit) break;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (V.co) {
+      hipLaunchKernelGGL(tzc_list_out, dim3(list_grid(V.co->agg_total)), dim3(kBlock), 0, st,
+                         V.co->agg_total, V.co->agg_nodes, (const float*)V.co->xl, (float*)V.x);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
   }
   for (int q = 0; q < nt; ++q) {
     res->state[q] = ws->host_tstate[2 * q];
